@@ -1,0 +1,140 @@
+/*
+ * pphip.h -- C ABI of libpphip.so, the MI355X (gfx950) backend for the
+ * propagate() hot path of PauliPropagation.jl v0.9.2.
+ *
+ * The reference has no FFI for this path: it dispatches in Julia on the
+ * propagation cache type. Each entry point below replaces one Julia method
+ * of that dispatch (paths relative to the reference checkout); a Julia host
+ * package binds them with `ccall` (the binding is shown in INTEGRATION.md).
+ *
+ * Conventions
+ *   - A Pauli string on n qubits is W = 1, 2 or 4 little-endian uint64 limbs
+ *     (n <= 32, 64, 128), qubit q (0-based) at bits 2q, 2q+1, with I=0, X=1,
+ *     Y=2, Z=3 (src/PauliAlgebra/bitoperations.jl:154, utils.jl:2). Host key
+ *     arrays are row-major: term i occupies keys[i*W .. i*W+W-1].
+ *   - Coefficients are FP64, row-major n x B (B = batch columns sharing one
+ *     key set; B = 1 is the reference's scalar case).
+ *   - max_weight < 0 means no weight truncation (the reference's Inf);
+ *     min_abs_coeff is compared against max_b |c_b| (|c| for B = 1).
+ *   - Every function returns PP_OK (0) or a pp_status code; pp_last_error()
+ *     returns the calling thread's message for the last failure.
+ *   - A handle owns all device memory and one HIP stream. Host pointers are
+ *     only borrowed for the duration of a call. Calls on one handle must not
+ *     overlap (as the reference's owner-only workers,
+ *     src/Base/threading_utils.jl:111-122); different handles are independent.
+ */
+#ifndef PPHIP_H
+#define PPHIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pp_handle pp_handle;
+
+enum pp_status {
+    PP_OK = 0,
+    PP_ERR_ARG = 1,          /* invalid argument (qubit out of range, bad noise strength, ...) */
+    PP_ERR_NOMEM = 2,        /* device allocation failed */
+    PP_ERR_HIP = 3,          /* HIP runtime error */
+    PP_ERR_UNSUPPORTED = 4,  /* gate or option this backend does not implement */
+    PP_ERR_CAPACITY = 5,     /* more terms than the backend's index range (2^31 - 1) */
+    PP_ERR_STATE = 6         /* call not valid in the handle's current state */
+};
+
+enum pp_gate_kind {
+    PP_GATE_ROTATION = 0,    /* PauliRotation: mask = generator */
+    PP_GATE_PAULI_NOISE = 1, /* PauliNoise family: damp_codes on qubit */
+    PP_GATE_AMP_DAMPING = 2, /* AmplitudeDampingNoise on qubit */
+    PP_GATE_CLIFFORD = 3     /* CliffordGate: table `lut` on qubits[0..nq) */
+};
+
+/* One gate of a Heisenberg-ordered gate list. Parametrized gates read
+ * theta[param * B + b]; param < 0 uses `value` (a FrozenGate). */
+typedef struct pp_gate {
+    uint64_t mask[4];     /* rotation generator, limb 0 first (unused limbs 0) */
+    int32_t kind;         /* enum pp_gate_kind */
+    int32_t param;        /* parameter row, or -1 */
+    double value;         /* frozen parameter */
+    int32_t qubit;        /* noise / damping qubit, 0-based */
+    uint32_t damp_codes;  /* PauliNoise: bit v set <=> Pauli code v is damped */
+    int32_t nq;           /* Clifford: number of qubits (1 or 2) */
+    int32_t qubits[2];    /* Clifford qubits, 0-based; qubits[0] is the table's low code */
+    int32_t lut;          /* Clifford: table slot */
+} pp_gate;
+
+typedef struct pp_gate_stats {
+    int64_t n_in;         /* live terms before the gate */
+    int64_t n_touched;    /* anticommuting (rotation) / touched terms */
+    int64_t n_out;        /* live terms after the gate */
+} pp_gate_stats;
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int pp_version(void);
+
+/* Message for the calling thread's last failed call ("" if none). */
+const char *pp_last_error(void);
+
+/* Create a device Pauli sum for `nqubits` qubits with key_words limbs
+ * (0 = smallest that fits), `batch` coefficient columns and room for
+ * `capacity` terms (it grows on demand).
+ * Replaces: PropagationCache(::VectorPauliSum) -> VectorPauliPropagationCache
+ *           (src/Base/propagationcache.jl:9-10, src/Propagation/propagationcache.jl:92-101). */
+int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle **out);
+
+/* Free every device buffer of the handle. */
+int pp_destroy(pp_handle *h);
+
+/* nqubits, key_words, batch, live terms, capacity (any pointer may be NULL). */
+int pp_info(const pp_handle *h, int *nqubits, int *key_words, int *batch, size_t *n, size_t *capacity);
+
+/* Replace the sum with n distinct terms (any order; duplicates are an error).
+ * Replaces: VectorPauliSum(::PauliSum) + cache construction
+ *           (src/PauliDataTypes/vectorpaulisum.jl:21-49). */
+int pp_upload(pp_handle *h, const uint64_t *keys, const double *coeffs, size_t n);
+
+/* Copy the live terms, ascending by key, into caller memory with room for
+ * `cap` terms; *n receives the term count (also when cap is too small, in
+ * which case nothing is copied and PP_ERR_ARG is returned).
+ * Replaces: extractsum!(cache, sum) (src/Base/propagationcache.jl:293-328). */
+int pp_download(pp_handle *h, uint64_t *keys, double *coeffs, size_t cap, size_t *n);
+
+/* PauliRotation step: branch, merge and truncate in one call.
+ * cos_b / sin_b are B host values (computed by the caller, as the reference
+ * does once per gate, src/Propagation/specializations.jl:30).
+ * Replaces: applymergetruncate!(::PauliRotation, cache, theta; max_weight, min_abs_coeff)
+ *           (src/Propagation/specializations.jl:53, 365-388). */
+int pp_apply_rotation(pp_handle *h, const uint64_t *mask, const double *cos_b, const double *sin_b,
+                      int max_weight, double min_abs_coeff, pp_gate_stats *stats);
+
+/* PauliNoise step: c *= factor_b[b] for the Pauli codes set in damp_codes
+ * (bit v <=> code v) on `qubit`, then truncate every term.
+ * Replaces: applymergetruncate!(::PauliNoise, cache, lambda; ...) with factor = 1 - lambda
+ *           (src/Propagation/specializations.jl:218-238). */
+int pp_apply_pauli_noise(pp_handle *h, int qubit, uint32_t damp_codes, const double *factor_b,
+                         int max_weight, double min_abs_coeff, pp_gate_stats *stats);
+
+/* Run a Heisenberg-ordered gate list (the reference's _propagate! loop over
+ * applymergetruncate!, src/Base/propagate.jl:44-71). theta is nparams x B
+ * row-major. Clifford tables are nluts slots of 16 entries: lut_changes
+ * (16 x W limbs per slot) and lut_signs (16 per slot). counts, if not NULL,
+ * receives n_in of every gate (ngates entries; @countpaulis,
+ * src/Base/countterms.jl:41-66). */
+int pp_run_circuit(pp_handle *h, const pp_gate *gates, size_t ngates, const double *theta, size_t nparams,
+                   const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
+                   double min_abs_coeff, int64_t *counts);
+
+/* overlapwithzero for every column: sum of the coefficients of terms with no
+ * X or Y, accumulated in double-double and rounded once.
+ * Replaces: overlapwithzero(::VectorPauliSum) (src/stateoverlap.jl:12-35). */
+int pp_overlap_zero(pp_handle *h, double *out_b);
+
+/* Wait for all work on the handle's stream. */
+int pp_synchronize(pp_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,195 @@
+"""ctypes wrapper of the CPU oracle libppo.so (oracle/ppo.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker and as the timed host baseline.
+The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libppo.so")
+
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_DP = ctypes.POINTER(ctypes.c_double)
+_P = ctypes.c_void_p
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-B", "-C", HERE, "libppo.so"], check=True, capture_output=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        sig = [
+            ("ppo_create", _P, [ctypes.c_int, ctypes.c_int]),
+            ("ppo_destroy", None, [_P]),
+            ("ppo_set", ctypes.c_int, [_P, _U64P, _DP, ctypes.c_size_t]),
+            ("ppo_size", ctypes.c_size_t, [_P]),
+            ("ppo_sorted", ctypes.c_int, [_P]),
+            ("ppo_get", None, [_P, _U64P, _DP]),
+            ("ppo_rotation", ctypes.c_long, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_double]),
+            ("ppo_pauli_noise", None, [_P, ctypes.c_int, ctypes.c_uint, _DP, ctypes.c_int, ctypes.c_double]),
+            ("ppo_amplitude_damping", ctypes.c_long, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                                      ctypes.c_double]),
+            ("ppo_clifford", None, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _U64P, _DP, ctypes.c_int,
+                                    ctypes.c_double]),
+            ("ppo_overlap_zero", None, [_P, _DP]),
+            ("ppo_run", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
+                                       ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
+            ("ppo_adjoint", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, ctypes.c_int,
+                                           ctypes.c_double, _DP, _DP]),
+            ("ppo_anticommutes", ctypes.c_int, [_U64P, _U64P, ctypes.c_int]),
+            ("ppo_sign_exponent", ctypes.c_int, [_U64P, _U64P, ctypes.c_int]),
+            ("ppo_weight", ctypes.c_int, [_U64P, ctypes.c_int]),
+            ("ppo_contains_xy", ctypes.c_int, [_U64P, ctypes.c_int]),
+            ("ppo_xor_sort_tail", ctypes.c_int, [_U64P, _DP, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _U64P]),
+        ]
+        for name, res, args in sig:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _mw(max_weight) -> int:
+    if max_weight is None or (isinstance(max_weight, float) and math.isinf(max_weight)):
+        return -1
+    return int(max_weight)
+
+
+def u64p(a):
+    return a.ctypes.data_as(_U64P)
+
+
+def dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class OracleSum:
+    """A VectorPauliSum restated on the CPU: sorted keys, n x B coefficients."""
+
+    def __init__(self, W: int, B: int = 1):
+        self.W, self.B = W, B
+        self._s = lib().ppo_create(W, B)
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().ppo_destroy(self._s)
+            self._s = None
+
+    def set(self, keys, coeffs):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.B)
+        if lib().ppo_set(self._s, u64p(keys), dp(coeffs), keys.shape[0]) != 0:
+            raise ValueError("duplicate key")
+        return self
+
+    def __len__(self):
+        return lib().ppo_size(self._s)
+
+    def get(self):
+        n = len(self)
+        keys = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        coeffs = np.zeros((max(n, 1), self.B), dtype=np.float64)
+        lib().ppo_get(self._s, u64p(keys), dp(coeffs))
+        return keys[:n], coeffs[:n]
+
+    def rotation(self, mask_limbs, cosv, sinv, max_weight=math.inf, min_abs=0.0):
+        m = np.ascontiguousarray(mask_limbs, dtype=np.uint64)
+        c = np.ascontiguousarray(np.broadcast_to(cosv, (self.B,)), dtype=np.float64)
+        s = np.ascontiguousarray(np.broadcast_to(sinv, (self.B,)), dtype=np.float64)
+        return lib().ppo_rotation(self._s, u64p(m), dp(c), dp(s), _mw(max_weight), float(min_abs))
+
+    def pauli_noise(self, qubit0, damp_codes, factor, max_weight=math.inf, min_abs=0.0):
+        f = np.ascontiguousarray(np.broadcast_to(factor, (self.B,)), dtype=np.float64)
+        lib().ppo_pauli_noise(self._s, int(qubit0), int(damp_codes), dp(f), _mw(max_weight), float(min_abs))
+
+    def amplitude_damping(self, qubit0, gamma, max_weight=math.inf, min_abs=0.0):
+        return lib().ppo_amplitude_damping(self._s, int(qubit0), float(gamma), _mw(max_weight), float(min_abs))
+
+    def overlap_zero(self):
+        out = np.zeros(self.B, dtype=np.float64)
+        lib().ppo_overlap_zero(self._s, dp(out))
+        return out
+
+    def run(self, gates, ngates, theta, nparams, lut_changes, lut_signs, max_weight, min_abs):
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if th.size == 0:
+            th = np.zeros(1)
+        lc = np.ascontiguousarray(lut_changes if len(lut_changes) else [0], dtype=np.uint64)
+        ls = np.ascontiguousarray(lut_signs if len(lut_signs) else [0.0], dtype=np.float64)
+        counts = np.zeros(max(ngates, 1), dtype=np.int64)
+        rc = lib().ppo_run(self._s, ctypes.cast(gates, ctypes.c_void_p), ngates, dp(th), nparams, u64p(lc), dp(ls),
+                           _mw(max_weight), float(min_abs), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        if rc != 0:
+            raise ValueError("oracle rejected a gate")
+        return counts[:ngates]
+
+
+def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
+                     batch: int = 1, W: int | None = None):
+    """Propagate a Schrodinger-ordered circuit on the oracle; returns
+    (OracleSum, per-gate n_in counts). thetas: P or P x B."""
+    import sys
+    pkg = os.path.join(os.path.dirname(HERE), "paulipropagation.jl_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from pauliprop_hip.gates import countparameters, pack_circuit, toheisenberg
+    from pauliprop_hip.paulis import key_words, int_to_limbs
+
+    W = W or key_words(nqubits)
+    P = countparameters(circuit)
+    th = np.zeros((0, batch)) if thetas is None else np.asarray(thetas, dtype=np.float64).reshape(P, -1)
+    hc, hth = toheisenberg(circuit, list(th))
+    hth = np.array(hth).reshape(P, -1) if P else np.zeros((0, batch))
+    gates, lc, ls, _ = pack_circuit(hc, nqubits, W)
+    s = OracleSum(W, batch)
+    keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
+    coeffs = np.array([np.broadcast_to(c, (batch,)) for c in init_terms.values()], dtype=np.float64).reshape(-1, batch)
+    s.set(keys, coeffs)
+    counts = s.run(gates, len(hc), hth, P, lc, ls, max_weight, min_abs_coeff)
+    return s, counts
+
+
+def oracle_adjoint(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
+                   batch: int = 1):
+    """(loss[B], grad[P x B]) of overlapwithzero(propagate(...)) w.r.t. the
+    Schrodinger-ordered parameters, by the oracle's reverse sweep."""
+    import sys
+    pkg = os.path.join(os.path.dirname(HERE), "paulipropagation.jl_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from pauliprop_hip.gates import countparameters, pack_circuit, toheisenberg
+    from pauliprop_hip.paulis import key_words, int_to_limbs
+
+    W = key_words(nqubits)
+    P = countparameters(circuit)
+    th = np.asarray(thetas, dtype=np.float64).reshape(P, -1)
+    hc, hth = toheisenberg(circuit, list(th))
+    hth = np.ascontiguousarray(np.array(hth).reshape(P, -1))
+    gates, _, _, _ = pack_circuit(hc, nqubits, W)
+    s = OracleSum(W, batch)
+    keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
+    coeffs = np.array([np.broadcast_to(c, (batch,)) for c in init_terms.values()], dtype=np.float64).reshape(-1, batch)
+    s.set(keys, coeffs)
+    loss = np.zeros(batch)
+    grad = np.zeros((max(P, 1), batch))
+    rc = lib().ppo_adjoint(s._s, ctypes.cast(gates, ctypes.c_void_p), len(hc), dp(hth), P, _mw(max_weight),
+                           float(min_abs_coeff), dp(loss), dp(grad))
+    if rc != 0:
+        raise ValueError("oracle adjoint rejected a gate")
+    # Heisenberg parameter rows run in reverse Schrodinger order
+    return loss, grad[:P][::-1].copy()

@@ -1,0 +1,676 @@
+/*
+ * ppo.c -- CPU oracle: a plain-C restatement of the VectorPauliSum
+ * propagate() path of PauliPropagation.jl v0.9.2.
+ *
+ * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load this library, as the checker and as the timed host
+ * baseline. The product library never links or calls it.
+ *
+ * What each function follows (paths relative to the reference checkout):
+ *   anticommutation          src/PauliAlgebra/wide_bitoperations.jl:16-25
+ *   sign exponent            src/PauliAlgebra/paulioperations.jl:518-564 (restated
+ *                            here from the single-qubit product table, see below)
+ *   rotation rule            src/Propagation/specializations.jl:34-44, 62-73
+ *   branch (append in parent order)   src/Base/xorbranch.jl:214-240
+ *   nothing touched => nothing merged or truncated   src/Base/Merge/xortailmerge.jl:121-123
+ *   XOR block-reversal tail sort      src/Base/Merge/xortailmerge.jl:317-452
+ *   head/tail merge, truncation of every output  src/Base/Merge/sortedtailmerge.jl:140-202
+ *   full sort + run reduction fallback           src/Base/Merge/mergeandtruncate.jl:87-120,
+ *                                                src/Base/Merge/deduplicate.jl:14-104
+ *   truncation predicate     src/Propagation/generics.jl:223-250, src/truncations.jl:42-65
+ *   PauliNoise               src/Propagation/specializations.jl:218-238,
+ *                            src/Base/Primitives/mapandtruncate.jl:115-197
+ *   AmplitudeDampingNoise    src/Propagation/specializations.jl:251-292
+ *   CliffordGate             src/Propagation/specializations.jl:144-192
+ *   overlapwithzero          src/stateoverlap.jl:12-35, src/Base/Primitives/mapreduce.jl:43-65
+ *
+ * Batched columns (B > 1) share one key set; a term is truncated when its
+ * weight exceeds max_weight or when max_b |c_b| < min_abs. For B = 1 this is
+ * exactly the reference predicate.
+ *
+ * Compile with -ffp-contract=off: every coefficient is one product or a sum of
+ * two products, written out in the reference's order.
+ */
+#include "ppo.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ALT 0x5555555555555555ULL
+#define INLINE static inline __attribute__((always_inline))
+
+struct ppo_sum {
+    int W, B;
+    size_t n, cap;
+    uint64_t *keys;
+    double *coeffs;
+    int sorted; /* the whole active set is ascending and duplicate-free */
+};
+
+/* ------------------------------------------------------------------ */
+/* bit algebra                                                        */
+/* ------------------------------------------------------------------ */
+
+INLINE uint64_t swap_pairs(uint64_t w) { return ((w >> 1) & ALT) | ((w & ALT) << 1); }
+
+INLINE int anticommutes_w(const uint64_t *m, const uint64_t *t, int W) {
+    uint64_t f = 0;
+    for (int l = 0; l < W; ++l) f ^= swap_pairs(m[l]) & t[l];
+    return __builtin_popcountll(f) & 1;
+}
+
+/* Phase exponent e of P1*P2 = i^e (P1 xor P2), from the per-qubit table.
+ * With x = low ^ high, z = high for each 2-bit code, sigma_{x,z} = i^{xz} X^x Z^z,
+ * so sigma_a sigma_b = i^{x1 z1 + x2 z2 + 2 z1 x2 - x3 z3} sigma_{x3,z3}. */
+INLINE int sign_exponent_w(const uint64_t *p1, const uint64_t *p2, int W) {
+    int e = 0;
+    for (int l = 0; l < W; ++l) {
+        uint64_t a = p1[l], b = p2[l];
+        uint64_t z1 = (a >> 1) & ALT, x1 = (a ^ (a >> 1)) & ALT;
+        uint64_t z2 = (b >> 1) & ALT, x2 = (b ^ (b >> 1)) & ALT;
+        uint64_t x3 = x1 ^ x2, z3 = z1 ^ z2;
+        e += __builtin_popcountll(x1 & z1) + __builtin_popcountll(x2 & z2) +
+             2 * __builtin_popcountll(z1 & x2) - __builtin_popcountll(x3 & z3);
+    }
+    return e & 3;
+}
+
+INLINE int weight_w(const uint64_t *a, int W) {
+    int c = 0;
+    for (int l = 0; l < W; ++l) c += __builtin_popcountll((a[l] | (a[l] >> 1)) & ALT);
+    return c;
+}
+
+INLINE int contains_xy_w(const uint64_t *a, int W) {
+    for (int l = 0; l < W; ++l)
+        if ((a[l] ^ (a[l] >> 1)) & ALT) return 1;
+    return 0;
+}
+
+INLINE int key_cmp(const uint64_t *a, const uint64_t *b, int W) {
+    for (int l = W - 1; l >= 0; --l)
+        if (a[l] != b[l]) return a[l] < b[l] ? -1 : 1;
+    return 0;
+}
+
+INLINE int key_eq(const uint64_t *a, const uint64_t *b, int W) {
+    for (int l = 0; l < W; ++l)
+        if (a[l] != b[l]) return 0;
+    return 1;
+}
+
+/* equal on every bit at position >= bit */
+INLINE int eq_from(const uint64_t *a, const uint64_t *b, int bit, int W) {
+    int L = bit >> 6, off = bit & 63;
+    if (L >= W) return 1;
+    for (int l = W - 1; l > L; --l)
+        if (a[l] != b[l]) return 0;
+    return (a[L] >> off) == (b[L] >> off);
+}
+
+INLINE int get_pauli(const uint64_t *a, int q) { return (int)((a[q >> 5] >> (2 * (q & 31))) & 3); }
+
+INLINE int truncated(const uint64_t *key, const double *c, int W, int B, int max_weight,
+                     double min_abs) {
+    if (max_weight >= 0 && weight_w(key, W) > max_weight) return 1;
+    double m = fabs(c[0]);
+    for (int b = 1; b < B; ++b) {
+        double a = fabs(c[b]);
+        if (a > m) m = a;
+    }
+    return m < min_abs;
+}
+
+int ppo_anticommutes(const uint64_t *a, const uint64_t *b, int W) { return anticommutes_w(a, b, W); }
+int ppo_sign_exponent(const uint64_t *p1, const uint64_t *p2, int W) { return sign_exponent_w(p1, p2, W); }
+int ppo_weight(const uint64_t *a, int W) { return weight_w(a, W); }
+int ppo_contains_xy(const uint64_t *a, int W) { return contains_xy_w(a, W); }
+
+/* ------------------------------------------------------------------ */
+/* storage                                                            */
+/* ------------------------------------------------------------------ */
+
+static void *xmalloc(size_t bytes) {
+    void *p = malloc(bytes ? bytes : 8);
+    if (!p) abort();
+    return p;
+}
+
+static void reserve(ppo_sum *s, size_t cap) {
+    if (cap <= s->cap) return;
+    size_t nc = s->cap ? s->cap : 16;
+    while (nc < cap) nc = nc + nc / 2 + 16;
+    s->keys = realloc(s->keys, nc * s->W * sizeof(uint64_t));
+    s->coeffs = realloc(s->coeffs, nc * s->B * sizeof(double));
+    if (!s->keys || !s->coeffs) abort();
+    s->cap = nc;
+}
+
+ppo_sum *ppo_create(int W, int B) {
+    if (!(W == 1 || W == 2 || W == 4) || B < 1) return NULL;
+    ppo_sum *s = calloc(1, sizeof(ppo_sum));
+    s->W = W;
+    s->B = B;
+    s->sorted = 1;
+    return s;
+}
+
+void ppo_destroy(ppo_sum *s) {
+    if (!s) return;
+    free(s->keys);
+    free(s->coeffs);
+    free(s);
+}
+
+size_t ppo_size(const ppo_sum *s) { return s->n; }
+int ppo_sorted(const ppo_sum *s) { return s->sorted; }
+
+void ppo_get(const ppo_sum *s, uint64_t *keys, double *coeffs) {
+    if (keys) memcpy(keys, s->keys, s->n * s->W * sizeof(uint64_t));
+    if (coeffs) memcpy(coeffs, s->coeffs, s->n * s->B * sizeof(double));
+}
+
+/* index sort by key (ties by index, so equal keys keep their order) */
+static const uint64_t *g_sort_keys;
+static int g_sort_W;
+static int idx_cmp(const void *pa, const void *pb) {
+    size_t a = *(const size_t *)pa, b = *(const size_t *)pb;
+    int c = key_cmp(g_sort_keys + a * g_sort_W, g_sort_keys + b * g_sort_W, g_sort_W);
+    if (c) return c;
+    return a < b ? -1 : (a > b);
+}
+
+static size_t *sorted_index(const uint64_t *keys, size_t n, int W) {
+    size_t *idx = xmalloc(n * sizeof(size_t));
+    for (size_t i = 0; i < n; ++i) idx[i] = i;
+    g_sort_keys = keys;
+    g_sort_W = W;
+    qsort(idx, n, sizeof(size_t), idx_cmp);
+    return idx;
+}
+
+int ppo_set(ppo_sum *s, const uint64_t *keys, const double *coeffs, size_t n) {
+    int W = s->W, B = s->B;
+    size_t *idx = sorted_index(keys, n, W);
+    for (size_t i = 1; i < n; ++i)
+        if (key_eq(keys + idx[i] * W, keys + idx[i - 1] * W, W)) {
+            free(idx);
+            return -1;
+        }
+    reserve(s, n);
+    for (size_t i = 0; i < n; ++i) {
+        memcpy(s->keys + i * W, keys + idx[i] * W, W * sizeof(uint64_t));
+        memcpy(s->coeffs + i * B, coeffs + idx[i] * B, B * sizeof(double));
+    }
+    free(idx);
+    s->n = n;
+    s->sorted = 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* XOR block-reversal tail sort                                        */
+/* ------------------------------------------------------------------ */
+
+/* maximal runs of adjacent set bits of the mask, highest first */
+static int mask_groups(const uint64_t *mask, int W, int *lo, int *hi, int maxg) {
+    int g = 0;
+    int b = 64 * W - 1;
+    while (b >= 0) {
+        if ((mask[b >> 6] >> (b & 63)) & 1) {
+            int top = b;
+            while (b >= 0 && ((mask[b >> 6] >> (b & 63)) & 1)) --b;
+            if (g < maxg) {
+                hi[g] = top;
+                lo[g] = b + 1;
+            }
+            ++g;
+        } else {
+            --b;
+        }
+    }
+    return g;
+}
+
+/* One pass: within every run agreeing above `hi`, write the sub-blocks that
+ * agree from `lo` up in reverse order; order inside a sub-block is kept. */
+static void xor_pass(const uint64_t *ik, const double *ic, uint64_t *ok, double *oc, size_t n,
+                     int W, int B, int lo, int hi) {
+    size_t rs = 0;
+    while (rs < n) {
+        size_t re = rs + 1;
+        while (re < n && eq_from(ik + re * W, ik + rs * W, hi + 1, W)) ++re;
+        /* sub-blocks of [rs, re), written back to front */
+        size_t w = rs;
+        size_t be = re;
+        while (be > rs) {
+            size_t bs = be - 1;
+            while (bs > rs && eq_from(ik + (bs - 1) * W, ik + (be - 1) * W, lo, W)) --bs;
+            memcpy(ok + w * W, ik + bs * W, (be - bs) * W * sizeof(uint64_t));
+            memcpy(oc + w * B, ic + bs * B, (be - bs) * B * sizeof(double));
+            w += be - bs;
+            be = bs;
+        }
+        rs = re;
+    }
+}
+
+int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, const uint64_t *mask) {
+    int lo[256], hi[256];
+    int g = mask_groups(mask, W, lo, hi, 256);
+    if (g > 256) return -1;
+    uint64_t *tk = xmalloc(n * W * sizeof(uint64_t));
+    double *tc = xmalloc(n * B * sizeof(double));
+    uint64_t *ak = keys, *bk = tk;
+    double *ac = coeffs, *bc = tc;
+    for (int i = 0; i < g; ++i) {
+        xor_pass(ak, ac, bk, bc, n, W, B, lo[i], hi[i]);
+        uint64_t *t1 = ak; ak = bk; bk = t1;
+        double *t2 = ac; ac = bc; bc = t2;
+    }
+    if (ak != keys) {
+        memcpy(keys, ak, n * W * sizeof(uint64_t));
+        memcpy(coeffs, ac, n * B * sizeof(double));
+    }
+    free(tk);
+    free(tc);
+    return g;
+}
+
+/* ------------------------------------------------------------------ */
+/* merge with truncation                                               */
+/* ------------------------------------------------------------------ */
+
+/* Two-pointer merge of the sorted head (the sum itself) with a sorted,
+ * duplicate-free tail. Equal keys merge as head + tail; every output term is
+ * truncated. The result replaces the sum. */
+static void merge_head_tail(ppo_sum *s, const uint64_t *tk, const double *tc, size_t nt,
+                            int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    size_t nh = s->n;
+    uint64_t *ok = xmalloc((nh + nt) * W * sizeof(uint64_t));
+    double *oc = xmalloc((nh + nt) * B * sizeof(double));
+    double *tmp = xmalloc(B * sizeof(double));
+    size_t i = 0, j = 0, o = 0;
+    while (i < nh || j < nt) {
+        const uint64_t *key;
+        int c = (i < nh && j < nt) ? key_cmp(s->keys + i * W, tk + j * W, W) : (i < nh ? -1 : 1);
+        if (c == 0) {
+            key = s->keys + i * W;
+            for (int b = 0; b < B; ++b) tmp[b] = s->coeffs[i * B + b] + tc[j * B + b];
+            ++i;
+            ++j;
+        } else if (c < 0) {
+            key = s->keys + i * W;
+            memcpy(tmp, s->coeffs + i * B, B * sizeof(double));
+            ++i;
+        } else {
+            key = tk + j * W;
+            memcpy(tmp, tc + j * B, B * sizeof(double));
+            ++j;
+        }
+        if (truncated(key, tmp, W, B, max_weight, min_abs)) continue;
+        memcpy(ok + o * W, key, W * sizeof(uint64_t));
+        memcpy(oc + o * B, tmp, B * sizeof(double));
+        ++o;
+    }
+    free(tmp);
+    free(s->keys);
+    free(s->coeffs);
+    s->keys = ok;
+    s->coeffs = oc;
+    s->cap = nh + nt;
+    s->n = o;
+    s->sorted = 1;
+}
+
+/* Fallback for an unsorted head: sort head and tail together, reduce runs of
+ * equal keys left to right (head before tail), truncate every output. */
+static void sort_reduce_truncate(ppo_sum *s, const uint64_t *tk, const double *tc, size_t nt,
+                                 int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    size_t nh = s->n, n = nh + nt;
+    uint64_t *ak = xmalloc(n * W * sizeof(uint64_t));
+    double *ac = xmalloc(n * B * sizeof(double));
+    memcpy(ak, s->keys, nh * W * sizeof(uint64_t));
+    memcpy(ac, s->coeffs, nh * B * sizeof(double));
+    if (nt) {
+        memcpy(ak + nh * W, tk, nt * W * sizeof(uint64_t));
+        memcpy(ac + nh * B, tc, nt * B * sizeof(double));
+    }
+    size_t *idx = sorted_index(ak, n, W);
+    uint64_t *ok = xmalloc((n ? n : 1) * W * sizeof(uint64_t));
+    double *oc = xmalloc((n ? n : 1) * B * sizeof(double));
+    double *tmp = xmalloc(B * sizeof(double));
+    size_t o = 0, r = 0;
+    while (r < n) {
+        const uint64_t *key = ak + idx[r] * W;
+        memcpy(tmp, ac + idx[r] * B, B * sizeof(double));
+        size_t q = r + 1;
+        while (q < n && key_eq(ak + idx[q] * W, key, W)) {
+            for (int b = 0; b < B; ++b) tmp[b] = tmp[b] + ac[idx[q] * B + b];
+            ++q;
+        }
+        r = q;
+        if (truncated(key, tmp, W, B, max_weight, min_abs)) continue;
+        memcpy(ok + o * W, key, W * sizeof(uint64_t));
+        memcpy(oc + o * B, tmp, B * sizeof(double));
+        ++o;
+    }
+    free(tmp);
+    free(idx);
+    free(ak);
+    free(ac);
+    free(s->keys);
+    free(s->coeffs);
+    s->keys = ok;
+    s->coeffs = oc;
+    s->cap = n ? n : 1;
+    s->n = o;
+    s->sorted = 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* gates                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Shared tail of a branching gate: an empty touch leaves everything as is;
+ * a sorted head merges a tail sorted by XOR passes; otherwise sort + reduce. */
+static void finish_branch(ppo_sum *s, uint64_t *tk, double *tc, size_t nt, const uint64_t *mask,
+                          int max_weight, double min_abs) {
+    if (s->sorted) {
+        ppo_xor_sort_tail(tk, tc, nt, s->W, s->B, mask);
+        merge_head_tail(s, tk, tc, nt, max_weight, min_abs);
+    } else {
+        sort_reduce_truncate(s, tk, tc, nt, max_weight, min_abs);
+    }
+}
+
+INLINE long rotation_impl(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv,
+                          int max_weight, double min_abs, const int W) {
+    const int B = s->B;
+    const size_t n = s->n;
+    size_t na = 0;
+    for (size_t i = 0; i < n; ++i) na += anticommutes_w(mask, s->keys + i * W, W);
+    if (na == 0) return 0;
+    uint64_t *tk = xmalloc(na * W * sizeof(uint64_t));
+    double *tc = xmalloc(na * B * sizeof(double));
+    size_t j = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        if (!anticommutes_w(mask, k, W)) continue;
+        int e = sign_exponent_w(mask, k, W);
+        double sign = (double)((e & 2) - 1);
+        for (int l = 0; l < W; ++l) tk[j * W + l] = k[l] ^ mask[l];
+        double *c = s->coeffs + i * B;
+        for (int b = 0; b < B; ++b) {
+            tc[j * B + b] = (c[b] * sv[b]) * sign;
+            c[b] = c[b] * cv[b];
+        }
+        ++j;
+    }
+    finish_branch(s, tk, tc, na, mask, max_weight, min_abs);
+    free(tk);
+    free(tc);
+    return (long)na;
+}
+
+long ppo_rotation(ppo_sum *s, const uint64_t *mask, const double *cosv, const double *sinv,
+                  int max_weight, double min_abs) {
+    switch (s->W) {
+    case 1: return rotation_impl(s, mask, cosv, sinv, max_weight, min_abs, 1);
+    case 2: return rotation_impl(s, mask, cosv, sinv, max_weight, min_abs, 2);
+    default: return rotation_impl(s, mask, cosv, sinv, max_weight, min_abs, 4);
+    }
+}
+
+void ppo_pauli_noise(ppo_sum *s, int qubit, unsigned damp_codes, const double *factor,
+                     int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    size_t o = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        double *c = s->coeffs + i * B;
+        if ((damp_codes >> get_pauli(k, qubit)) & 1)
+            for (int b = 0; b < B; ++b) c[b] = c[b] * factor[b];
+        if (truncated(k, c, W, B, max_weight, min_abs)) continue;
+        if (o != i) {
+            memmove(s->keys + o * W, k, W * sizeof(uint64_t));
+            memmove(s->coeffs + o * B, c, B * sizeof(double));
+        }
+        ++o;
+    }
+    s->n = o;
+}
+
+long ppo_amplitude_damping(ppo_sum *s, int qubit, double gamma, int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    uint64_t mask[4] = {0, 0, 0, 0};
+    mask[qubit >> 5] = 3ULL << (2 * (qubit & 31));
+    double damp_val = sqrt(1 - gamma);
+    size_t nz = 0, touched = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        int p = get_pauli(s->keys + i * W, qubit);
+        nz += (p == 3);
+        touched += (p != 0);
+    }
+    if (touched == 0) return 0;
+    uint64_t *tk = xmalloc((nz ? nz : 1) * W * sizeof(uint64_t));
+    double *tc = xmalloc((nz ? nz : 1) * B * sizeof(double));
+    size_t j = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        double *c = s->coeffs + i * B;
+        int p = get_pauli(k, qubit);
+        if (p == 3) {
+            for (int l = 0; l < W; ++l) tk[j * W + l] = k[l] ^ mask[l];
+            for (int b = 0; b < B; ++b) {
+                tc[j * B + b] = gamma * c[b];
+                c[b] = (1 - gamma) * c[b];
+            }
+            ++j;
+        } else if (p != 0) {
+            for (int b = 0; b < B; ++b) c[b] = damp_val * c[b];
+        }
+    }
+    finish_branch(s, tk, tc, nz, mask, max_weight, min_abs);
+    free(tk);
+    free(tc);
+    return (long)touched;
+}
+
+void ppo_clifford(ppo_sum *s, int nq, const int *qubits, const uint64_t *changes,
+                  const double *signs, int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    size_t o = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        double *c = s->coeffs + i * B;
+        int idx = 0;
+        for (int a = 0; a < nq; ++a) idx |= get_pauli(k, qubits[a]) << (2 * a);
+        for (int l = 0; l < W; ++l) k[l] ^= changes[idx * W + l];
+        for (int b = 0; b < B; ++b) c[b] = c[b] * signs[idx];
+        if (max_weight >= 0 && truncated(k, c, W, B, max_weight, min_abs)) continue;
+        if (o != i) {
+            memmove(s->keys + o * W, k, W * sizeof(uint64_t));
+            memmove(s->coeffs + o * B, c, B * sizeof(double));
+        }
+        ++o;
+    }
+    s->n = o;
+    s->sorted = 0;
+}
+
+void ppo_overlap_zero(const ppo_sum *s, double *out) {
+    int W = s->W, B = s->B;
+    for (int b = 0; b < B; ++b) out[b] = 0.0;
+    for (size_t i = 0; i < s->n; ++i)
+        if (!contains_xy_w(s->keys + i * W, W))
+            for (int b = 0; b < B; ++b) out[b] += s->coeffs[i * B + b];
+}
+
+/* ------------------------------------------------------------------ */
+/* gate lists                                                          */
+/* ------------------------------------------------------------------ */
+
+static double gate_value(const ppo_gate *g, const double *theta, int B, int b) {
+    return g->param < 0 ? g->value : theta[(size_t)g->param * B + b];
+}
+
+static int apply_gate(ppo_sum *s, const ppo_gate *g, const double *theta, const uint64_t *lut_changes,
+                      const double *lut_signs, int max_weight, double min_abs, double *cv, double *sv) {
+    int B = s->B, W = s->W;
+    switch (g->kind) {
+    case PPO_ROTATION:
+        for (int b = 0; b < B; ++b) {
+            double t = gate_value(g, theta, B, b);
+            cv[b] = cos(t);
+            sv[b] = sin(t);
+        }
+        ppo_rotation(s, g->mask, cv, sv, max_weight, min_abs);
+        return 0;
+    case PPO_PAULI_NOISE:
+        for (int b = 0; b < B; ++b) cv[b] = 1 - gate_value(g, theta, B, b);
+        ppo_pauli_noise(s, g->qubit, g->damp_codes, cv, max_weight, min_abs);
+        return 0;
+    case PPO_AMP_DAMPING:
+        if (g->param >= 0 && B > 1) return -1;
+        ppo_amplitude_damping(s, g->qubit, gate_value(g, theta, B, 0), max_weight, min_abs);
+        return 0;
+    case PPO_CLIFFORD: {
+        int nent = g->nq == 1 ? 4 : 16;
+        size_t base = 0;
+        /* tables are packed back to back, 16 entries per table slot */
+        base = (size_t)g->lut * 16;
+        ppo_clifford(s, g->nq, g->qubits, lut_changes + base * W, lut_signs + base, max_weight, min_abs);
+        (void)nent;
+        return 0;
+    }
+    default:
+        return -1;
+    }
+}
+
+int ppo_run(ppo_sum *s, const ppo_gate *gates, size_t ngates, const double *theta, size_t nparams,
+            const uint64_t *lut_changes, const double *lut_signs, int max_weight, double min_abs,
+            int64_t *counts) {
+    (void)nparams;
+    double *cv = xmalloc(s->B * sizeof(double));
+    double *sv = xmalloc(s->B * sizeof(double));
+    int rc = 0;
+    for (size_t g = 0; g < ngates && rc == 0; ++g) {
+        if (counts) counts[g] = (int64_t)s->n;
+        rc = apply_gate(s, gates + g, theta, lut_changes, lut_signs, max_weight, min_abs, cv, sv);
+    }
+    free(cv);
+    free(sv);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* reverse-mode gradient (every intermediate sum kept)                 */
+/* ------------------------------------------------------------------ */
+
+static long find_key(const uint64_t *keys, size_t n, const uint64_t *k, int W) {
+    size_t lo = 0, hi = n;
+    while (lo < hi) {
+        size_t mid = (lo + hi) / 2;
+        int c = key_cmp(keys + mid * W, k, W);
+        if (c == 0) return (long)mid;
+        if (c < 0) lo = mid + 1;
+        else hi = mid;
+    }
+    return -1;
+}
+
+static ppo_sum *clone_sum(const ppo_sum *a) {
+    ppo_sum *s = ppo_create(a->W, a->B);
+    reserve(s, a->n ? a->n : 1);
+    memcpy(s->keys, a->keys, a->n * a->W * sizeof(uint64_t));
+    memcpy(s->coeffs, a->coeffs, a->n * a->B * sizeof(double));
+    s->n = a->n;
+    s->sorted = a->sorted;
+    return s;
+}
+
+int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const double *theta,
+                size_t nparams, int max_weight, double min_abs, double *loss, double *grad) {
+    int W = s0->W, B = s0->B;
+    if (!s0->sorted) return -1;
+    for (size_t g = 0; g < ngates; ++g)
+        if (gates[g].kind != PPO_ROTATION && gates[g].kind != PPO_PAULI_NOISE) return -1;
+    ppo_sum **st = xmalloc((ngates + 1) * sizeof(ppo_sum *));
+    double *cv = xmalloc(B * sizeof(double));
+    double *sv = xmalloc(B * sizeof(double));
+    st[0] = clone_sum(s0);
+    for (size_t g = 0; g < ngates; ++g) {
+        st[g + 1] = clone_sum(st[g]);
+        apply_gate(st[g + 1], gates + g, theta, NULL, NULL, max_weight, min_abs, cv, sv);
+    }
+    ppo_overlap_zero(st[ngates], loss);
+    memset(grad, 0, nparams * B * sizeof(double));
+    /* adjoint of the final sum: d overlap / d c_t */
+    ppo_sum *fin = st[ngates];
+    double *adj = xmalloc((fin->n ? fin->n : 1) * B * sizeof(double));
+    for (size_t i = 0; i < fin->n; ++i)
+        for (int b = 0; b < B; ++b) adj[i * B + b] = contains_xy_w(fin->keys + i * W, W) ? 0.0 : 1.0;
+    uint64_t pk[4];
+    for (size_t gg = ngates; gg-- > 0;) {
+        const ppo_gate *g = gates + gg;
+        ppo_sum *in = st[gg], *out = st[gg + 1];
+        double *nadj = xmalloc((in->n ? in->n : 1) * B * sizeof(double));
+        if (g->kind == PPO_ROTATION) {
+            size_t na = 0;
+            for (size_t i = 0; i < in->n; ++i) na += anticommutes_w(g->mask, in->keys + i * W, W);
+            for (int b = 0; b < B; ++b) {
+                double t = gate_value(g, theta, B, b);
+                cv[b] = cos(t);
+                sv[b] = sin(t);
+            }
+            for (size_t i = 0; i < in->n; ++i) {
+                const uint64_t *k = in->keys + i * W;
+                const double *c = in->coeffs + i * B;
+                if (na == 0) { /* untouched gate: identity, no truncation */
+                    for (int b = 0; b < B; ++b) nadj[i * B + b] = adj[i * B + b];
+                    continue;
+                }
+                long ps = find_key(out->keys, out->n, k, W);
+                if (!anticommutes_w(g->mask, k, W)) {
+                    for (int b = 0; b < B; ++b) nadj[i * B + b] = ps >= 0 ? adj[ps * B + b] : 0.0;
+                    continue;
+                }
+                for (int l = 0; l < W; ++l) pk[l] = k[l] ^ g->mask[l];
+                long pp = find_key(out->keys, out->n, pk, W);
+                double sign = (double)((sign_exponent_w(g->mask, k, W) & 2) - 1);
+                for (int b = 0; b < B; ++b) {
+                    double as = ps >= 0 ? adj[ps * B + b] : 0.0;
+                    double ap = pp >= 0 ? adj[pp * B + b] : 0.0;
+                    nadj[i * B + b] = as * cv[b] + ap * sv[b] * sign;
+                    if (g->param >= 0)
+                        grad[(size_t)g->param * B + b] += as * (-c[b] * sv[b]) + ap * (c[b] * cv[b] * sign);
+                }
+            }
+        } else { /* PauliNoise: diagonal */
+            for (size_t i = 0; i < in->n; ++i) {
+                const uint64_t *k = in->keys + i * W;
+                long ps = find_key(out->keys, out->n, k, W);
+                int damped = (g->damp_codes >> get_pauli(k, g->qubit)) & 1;
+                for (int b = 0; b < B; ++b) {
+                    double f = damped ? 1 - gate_value(g, theta, B, b) : 1.0;
+                    nadj[i * B + b] = ps >= 0 ? adj[ps * B + b] * f : 0.0;
+                    if (g->param >= 0 && damped && ps >= 0)
+                        grad[(size_t)g->param * B + b] += adj[ps * B + b] * (-in->coeffs[i * B + b]);
+                }
+            }
+        }
+        free(adj);
+        adj = nadj;
+    }
+    free(adj);
+    for (size_t g = 0; g <= ngates; ++g) ppo_destroy(st[g]);
+    free(st);
+    free(cv);
+    free(sv);
+    return 0;
+}

@@ -1,0 +1,284 @@
+// pp_device.h -- device-side building blocks for the gfx950 Pauli-propagation kernels.
+//
+// Keys are W uint64 limbs (W = 1, 2, 4) stored as limb planes: limb l of term i
+// lives at keys[l * stride + i]. Qubit q occupies bits 2q, 2q+1 of the
+// 64W-bit integer (I=0, X=1, Y=2, Z=3; src/PauliAlgebra/bitoperations.jl:154).
+// Order is unsigned integer order, highest limb first
+// (src/Base/NTupleInteger/datatype.jl:280-309).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pp {
+
+constexpr uint64_t ALT = 0x5555555555555555ull;
+constexpr int WG = 256;          // threads per workgroup: four wave64
+constexpr int NWAVES = WG / 64;
+constexpr int MAX_LEVELS = 16;   // bit levels one tail-permutation launch handles
+
+template <int W>
+struct Key {
+    uint64_t l[W];
+};
+
+// ---------------------------------------------------------------- bit algebra
+
+__device__ __forceinline__ uint64_t swap_pairs(uint64_t w) { return ((w >> 1) & ALT) | ((w & ALT) << 1); }
+
+// anticommutation: parity of popcount(swap(mask) & key) over the limbs
+// (src/PauliAlgebra/wide_bitoperations.jl:16-25)
+template <int W>
+__device__ __forceinline__ bool anticommutes(const Key<W>& m, const Key<W>& k) {
+    uint64_t f = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) f ^= swap_pairs(m.l[l]) & k.l[l];
+    return __popcll(f) & 1;
+}
+
+// exponent e of P1 * P2 = i^e (P1 xor P2), from the single-qubit table:
+// with x = low ^ high and z = high, e = x1 z1 + x2 z2 + 2 z1 x2 - x3 z3 summed mod 4
+template <int W>
+__device__ __forceinline__ int sign_exponent(const Key<W>& p1, const Key<W>& p2) {
+    int e = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) {
+        uint64_t a = p1.l[l], b = p2.l[l];
+        uint64_t z1 = (a >> 1) & ALT, x1 = (a ^ (a >> 1)) & ALT;
+        uint64_t z2 = (b >> 1) & ALT, x2 = (b ^ (b >> 1)) & ALT;
+        uint64_t x3 = x1 ^ x2, z3 = z1 ^ z2;
+        e += __popcll(x1 & z1) + __popcll(x2 & z2) + 2 * __popcll(z1 & x2) - __popcll(x3 & z3);
+    }
+    return e & 3;
+}
+
+template <int W>
+__device__ __forceinline__ int weight(const Key<W>& k) {
+    int c = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) c += __popcll((k.l[l] | (k.l[l] >> 1)) & ALT);
+    return c;
+}
+
+template <int W>
+__device__ __forceinline__ bool contains_xy(const Key<W>& k) {
+    uint64_t f = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) f |= (k.l[l] ^ (k.l[l] >> 1)) & ALT;
+    return f != 0;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_le(const Key<W>& a, const Key<W>& b) {
+#pragma unroll
+    for (int l = W - 1; l >= 0; --l)
+        if (a.l[l] != b.l[l]) return a.l[l] < b.l[l];
+    return true;
+}
+
+template <int W>
+__device__ __forceinline__ bool key_eq(const Key<W>& a, const Key<W>& b) {
+    uint64_t d = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) d |= a.l[l] ^ b.l[l];
+    return d == 0;
+}
+
+// equal on every bit at position >= bit (bit may be 64W: always equal)
+template <int W>
+__device__ __forceinline__ bool eq_from(const Key<W>& a, const Key<W>& b, int bit) {
+    bool eq = true;
+#pragma unroll
+    for (int l = 0; l < W; ++l) {
+        uint64_t d = a.l[l] ^ b.l[l];
+        int lo = l * 64;
+        if (bit >= lo + 64) continue;
+        if (bit > lo) d >>= (bit - lo);
+        eq = eq && (d == 0);
+    }
+    return eq;
+}
+
+template <int W>
+__device__ __forceinline__ Key<W> load_key(const uint64_t* __restrict__ keys, size_t stride, size_t i) {
+    Key<W> k;
+#pragma unroll
+    for (int l = 0; l < W; ++l) k.l[l] = keys[l * stride + i];
+    return k;
+}
+
+// only the limbs at or above limb `from` (the rest read as zero)
+template <int W>
+__device__ __forceinline__ Key<W> load_key_from(const uint64_t* __restrict__ keys, size_t stride, size_t i,
+                                                int from) {
+    Key<W> k;
+#pragma unroll
+    for (int l = 0; l < W; ++l) k.l[l] = (l >= from) ? keys[l * stride + i] : 0ull;
+    return k;
+}
+
+template <int W>
+__device__ __forceinline__ void store_key(uint64_t* __restrict__ keys, size_t stride, size_t i, const Key<W>& k) {
+#pragma unroll
+    for (int l = 0; l < W; ++l) keys[l * stride + i] = k.l[l];
+}
+
+template <int W>
+__device__ __forceinline__ Key<W> key_xor(const Key<W>& a, const Key<W>& b) {
+    Key<W> r;
+#pragma unroll
+    for (int l = 0; l < W; ++l) r.l[l] = a.l[l] ^ b.l[l];
+    return r;
+}
+
+// ---------------------------------------------------------------- wave helpers
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max_f64(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// inclusive scan over the wave of an int with max / min
+__device__ __forceinline__ int wave_incl_max(int v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o) v = max(v, u);
+    }
+    return v;
+}
+
+// inclusive suffix scan (lane l gets the min over lanes >= l)
+__device__ __forceinline__ int wave_suffix_min(int v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_down(v, o, 64);
+        if (lane + o < 64) v = min(v, u);
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- double-double sums
+
+struct DD {
+    double hi, lo;
+};
+
+// error-free sum of two doubles (Knuth's TwoSum)
+__device__ __forceinline__ DD two_sum(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double err = (a - (s - bb)) + (b - bb);
+    return {s, err};
+}
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+    DD s = two_sum(a.hi, b.hi);
+    double lo = s.lo + (a.lo + b.lo);
+    return two_sum(s.hi, lo);
+}
+
+__device__ __forceinline__ DD dd_add_d(DD a, double b) {
+    DD s = two_sum(a.hi, b);
+    return two_sum(s.hi, s.lo + a.lo);
+}
+
+__device__ __forceinline__ DD wave_sum_dd(DD v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        DD u{__shfl_xor(v.hi, o, 64), __shfl_xor(v.lo, o, 64)};
+        v = dd_add(v, u);
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------- decoupled look-back
+//
+// One 64-bit word per tile: [63:62] status (1 = tile aggregate, 2 = inclusive
+// prefix), [61:40] launch epoch, [39:0] value. A word from an older launch
+// carries another epoch and reads as "not ready", so the array is never
+// cleared between launches. All accesses are relaxed agent-scope atomics:
+// only the words themselves are communicated.
+
+constexpr uint64_t LB_VAL_MASK = (1ull << 40) - 1;
+constexpr uint32_t LB_EPOCH_MASK = (1u << 22) - 1;
+
+__device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Workgroups take tile ids in the order they start, so a tile only waits on
+// tiles that are already running. The workgroup that takes the last id resets
+// the counter for the next launch on the stream.
+__device__ __forceinline__ uint32_t lb_tile_id(uint32_t* counter, uint32_t* s_slot) {
+    if (threadIdx.x == 0) {
+        uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (t == gridDim.x - 1) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_slot = t;
+    }
+    __syncthreads();
+    return *s_slot;
+}
+
+// Called by all 64 lanes of one wave. Publishes `agg` for `tile` and returns
+// the exclusive prefix of all earlier tiles.
+__device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch, uint64_t agg) {
+    const int lane = lane_id();
+    const uint64_t ep = (uint64_t)(epoch & LB_EPOCH_MASK) << 40;
+    if (tile == 0) {
+        if (lane == 0) lb_store(states, (2ull << 62) | ep | agg);
+        return 0;
+    }
+    if (lane == 0) lb_store(states + tile, (1ull << 62) | ep | agg);
+    uint64_t excl = 0;
+    int64_t base = (int64_t)tile - 1;
+    while (true) {
+        const int64_t idx = base - lane;
+        uint64_t word = 0;
+        uint32_t status = 2;  // below tile 0: never reached past tile 0's prefix
+        uint64_t prefix_mask, need;
+        while (true) {
+            if (idx >= 0) {
+                word = lb_load(states + idx);
+                status = (((word >> 40) & LB_EPOCH_MASK) == (epoch & LB_EPOCH_MASK)) ? (uint32_t)(word >> 62) : 0u;
+            }
+            prefix_mask = __ballot(status == 2);
+            const uint64_t zero_mask = __ballot(status == 0);
+            const int p = prefix_mask ? (__ffsll((unsigned long long)prefix_mask) - 1) : 63;
+            need = (p == 63) ? ~0ull : ((2ull << p) - 1ull);
+            if ((zero_mask & need) == 0) break;
+            __builtin_amdgcn_s_sleep(1);
+        }
+        const uint64_t mine = ((need >> lane) & 1ull) && idx >= 0 ? (word & LB_VAL_MASK) : 0ull;
+        excl += wave_sum_u64(mine);
+        if (prefix_mask & need) break;
+        base -= 64;
+    }
+    if (lane == 0) lb_store(states + tile, (2ull << 62) | ep | (excl + agg));
+    return excl;
+}
+
+}  // namespace pp

@@ -1,0 +1,79 @@
+"""ctypes binding of libpphip.so (include/pphip.h).
+
+The library must be built in-tree (`__graft_entry__.build()` or
+`make -C paulipropagation.jl_amd/csrc`); a missing library raises at load
+time -- there is no host fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .gates import pp_gate
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpphip.so")
+
+PP_OK, PP_ERR_ARG, PP_ERR_NOMEM, PP_ERR_HIP, PP_ERR_UNSUPPORTED, PP_ERR_CAPACITY, PP_ERR_STATE = range(7)
+
+
+class PPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pphip {code}] {msg}")
+        self.code = code
+
+
+class pp_gate_stats(ctypes.Structure):
+    _fields_ = [("n_in", ctypes.c_int64), ("n_touched", ctypes.c_int64), ("n_out", ctypes.c_int64)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every symbol the header declares
+_P = ctypes.c_void_p
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_DP = ctypes.POINTER(ctypes.c_double)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_SZ = ctypes.c_size_t
+SIGNATURES = [
+    ("pp_version", ctypes.c_int, []),
+    ("pp_last_error", ctypes.c_char_p, []),
+    ("pp_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _SZ, ctypes.POINTER(_P)]),
+    ("pp_destroy", ctypes.c_int, [_P]),
+    ("pp_info", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+    ("pp_upload", ctypes.c_int, [_P, _U64P, _DP, _SZ]),
+    ("pp_download", ctypes.c_int, [_P, _U64P, _DP, _SZ, ctypes.POINTER(_SZ)]),
+    ("pp_apply_rotation", ctypes.c_int, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_double,
+                                         ctypes.POINTER(pp_gate_stats)]),
+    ("pp_apply_pauli_noise", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, _DP, ctypes.c_int, ctypes.c_double,
+                                            ctypes.POINTER(pp_gate_stats)]),
+    ("pp_run_circuit", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
+                                      ctypes.c_double, _I64P]),
+    ("pp_overlap_zero", ctypes.c_int, [_P, _DP]),
+    ("pp_synchronize", ctypes.c_int, [_P]),
+]
+
+
+def lib():
+    """Load libpphip.so once; raise if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"libpphip.so not found at {LIB_PATH}; build it with __graft_entry__.build() "
+                "or `make -C paulipropagation.jl_amd/csrc`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != PP_OK:
+        msg = lib().pp_last_error().decode(errors="replace")
+        if rc == PP_ERR_ARG:
+            raise ValueError(f"[pphip] {msg}")
+        raise PPError(rc, msg)

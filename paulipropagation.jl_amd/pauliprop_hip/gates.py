@@ -1,0 +1,266 @@
+"""Gate types of the reference, unchanged in meaning, plus their packing into
+the C-ABI gate descriptor `pp_gate` (include/pphip.h).
+
+Reference: src/Gates/paulirotations.jl:11-70 (PauliRotation),
+src/Gates/cliffordgates.jl:4-72 (CliffordGate and the default Clifford map),
+src/Gates/noisechannels.jl:24-278 (Pauli noise channels, AmplitudeDampingNoise,
+the strength check), src/Gates/frozengates.jl:12-45 (FrozenGate, freeze),
+src/Gates/heisenbergschrodinger.jl:20-49 (toheisenberg).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from .paulis import SYMBOLS, symboltoint
+
+# pp_gate kinds (include/pphip.h)
+ROTATION, PAULI_NOISE, AMP_DAMPING, CLIFFORD = 0, 1, 2, 3
+
+
+class pp_gate(ctypes.Structure):
+    """Mirror of `pp_gate` in include/pphip.h."""
+
+    _fields_ = [
+        ("mask", ctypes.c_uint64 * 4),
+        ("kind", ctypes.c_int32),
+        ("param", ctypes.c_int32),
+        ("value", ctypes.c_double),
+        ("qubit", ctypes.c_int32),
+        ("damp_codes", ctypes.c_uint32),
+        ("nq", ctypes.c_int32),
+        ("qubits", ctypes.c_int32 * 2),
+        ("lut", ctypes.c_int32),
+    ]
+
+
+class Gate:
+    parametrized = False
+
+
+class PauliRotation(Gate):
+    """exp(-i theta/2 P) for the Pauli string P = `symbols` on `qinds` (1-based)."""
+
+    parametrized = True
+
+    def __init__(self, symbols, qinds):
+        if isinstance(symbols, str) and len(symbols) == 1:
+            symbols = [symbols]
+        if isinstance(qinds, int):
+            qinds = [qinds]
+        self.symbols = [str(s).upper() for s in symbols]
+        self.qinds = [int(q) for q in qinds]
+        if len(self.symbols) != len(self.qinds):
+            raise ValueError("symbols and qinds must have equal length")
+        for s in self.symbols:
+            if s not in SYMBOLS:
+                raise ValueError(f"unknown Pauli symbol {s}")
+
+    def mask(self, nqubits: int) -> int:
+        return symboltoint(nqubits, self.symbols, self.qinds)
+
+    def __repr__(self):
+        return f"PauliRotation({self.symbols}, {self.qinds})"
+
+
+# The default Clifford map: for each gathered Pauli index (qubit 1 in the low
+# two bits) the new partial Pauli string and the sign
+# (src/Gates/cliffordgates.jl:47-59). The table is data the gate semantics
+# fix, so it is kept entry for entry.
+CLIFFORD_MAP = {
+    "H": [(0x0, 1), (0x3, 1), (0x2, -1), (0x1, 1)],
+    "X": [(0x0, 1), (0x1, 1), (0x2, -1), (0x3, -1)],
+    "Y": [(0x0, 1), (0x1, -1), (0x2, 1), (0x3, -1)],
+    "Z": [(0x0, 1), (0x1, -1), (0x2, -1), (0x3, 1)],
+    "SX": [(0x0, 1), (0x1, 1), (0x3, -1), (0x2, 1)],
+    "SY": [(0x0, 1), (0x3, 1), (0x2, 1), (0x1, -1)],
+    "S": [(0x0, 1), (0x2, -1), (0x1, 1), (0x3, 1)],
+    "CNOT": [(0x0, 1), (0x5, 1), (0x6, 1), (0x3, 1), (0x4, 1), (0x1, 1), (0x2, 1), (0x7, 1),
+             (0xB, 1), (0xE, 1), (0xD, -1), (0x8, 1), (0xF, 1), (0xA, -1), (0x9, 1), (0xC, 1)],
+    "CZ": [(0x0, 1), (0xD, 1), (0xE, 1), (0x3, 1), (0x7, 1), (0xA, 1), (0x9, -1), (0x4, 1),
+           (0xB, 1), (0x6, -1), (0x5, 1), (0x8, 1), (0xC, 1), (0x1, 1), (0x2, 1), (0xF, 1)],
+    "SWAP": [(0x0, 1), (0x4, 1), (0x8, 1), (0xC, 1), (0x1, 1), (0x5, 1), (0x9, 1), (0xD, 1),
+             (0x2, 1), (0x6, 1), (0xA, 1), (0xE, 1), (0x3, 1), (0x7, 1), (0xB, 1), (0xF, 1)],
+    "ZZPIHALF": [(0x0, 1), (0xE, 1), (0xD, -1), (0x3, 1), (0xB, 1), (0x5, 1), (0x6, 1), (0x8, 1),
+                 (0x7, -1), (0x9, 1), (0xA, 1), (0x4, -1), (0xC, 1), (0x2, 1), (0x1, -1), (0xF, 1)],
+}
+CLIFFORD_MAP["SZ"] = CLIFFORD_MAP["S"]
+
+
+class CliffordGate(Gate):
+    def __init__(self, symbol: str, qinds):
+        if isinstance(qinds, int):
+            qinds = [qinds]
+        self.symbol = str(symbol).upper()
+        self.qinds = [int(q) for q in qinds]
+        if self.symbol not in CLIFFORD_MAP:
+            raise ValueError(f"unknown Clifford gate {symbol}")
+        table = CLIFFORD_MAP[self.symbol]
+        if len(table) != 4 ** len(self.qinds):
+            need = 1 if len(table) == 4 else 2
+            raise ValueError(f"Clifford gate {symbol} acts on {need} qubit(s), got {len(self.qinds)}")
+
+    def __repr__(self):
+        return f"CliffordGate({self.symbol}, {self.qinds})"
+
+
+def _check_noise_strength(name: str, lam: float) -> None:
+    if not (0.0 <= lam <= 1.0):
+        raise ValueError(f"{name} parameter must be between 0 and 1. Got {lam}.")
+
+
+class PauliNoise(Gate):
+    """Damps the Pauli codes in `damped` on one qubit by 1 - lambda
+    (src/Propagation/specializations.jl:218-238)."""
+
+    parametrized = True
+    damped: tuple = ()
+
+    def __init__(self, qind: int):
+        self.qind = int(qind)
+
+    @property
+    def damp_codes(self) -> int:
+        return sum(1 << c for c in self.damped)
+
+    def __repr__(self):
+        return f"{type(self).__name__}({self.qind})"
+
+
+def _noise_type(name, damped):
+    cls = type(name, (PauliNoise,), {"damped": damped})
+
+    def make(qind: int, lam: float | None = None):
+        g = cls(qind)
+        if lam is None:
+            return g
+        _check_noise_strength(name, lam)
+        return FrozenGate(g, lam)
+
+    make.__name__ = name
+    make.gate_type = cls
+    return make
+
+
+# isdamped tables (src/Gates/noisechannels.jl:48-243)
+DepolarizingNoise = _noise_type("DepolarizingNoise", (1, 2, 3))
+PauliXNoise = _noise_type("PauliXNoise", (2, 3))
+PauliYNoise = _noise_type("PauliYNoise", (1, 3))
+PauliZNoise = _noise_type("PauliZNoise", (1, 2))
+DephasingNoise = PauliZNoise
+PauliXDamping = _noise_type("PauliXDamping", (1,))
+PauliYDamping = _noise_type("PauliYDamping", (2,))
+PauliZDamping = _noise_type("PauliZDamping", (3,))
+
+
+class AmplitudeDampingNoise(Gate):
+    parametrized = True
+
+    def __init__(self, qind: int):
+        self.qind = int(qind)
+
+    def __repr__(self):
+        return f"AmplitudeDampingNoise({self.qind})"
+
+
+class FrozenGate(Gate):
+    """A parametrized gate with its parameter fixed (src/Gates/frozengates.jl:12)."""
+
+    parametrized = False
+
+    def __init__(self, gate: Gate, parameter: float):
+        if not gate.parametrized:
+            raise ValueError("only parametrized gates can be frozen")
+        self.gate = gate
+        self.parameter = float(parameter)
+
+    def __repr__(self):
+        return f"FrozenGate({self.gate!r}, {self.parameter})"
+
+
+def freeze(gate: Gate, parameter: float) -> FrozenGate:
+    return FrozenGate(gate, parameter)
+
+
+def countparameters(circuit) -> int:
+    return sum(1 for g in circuit if g.parametrized)
+
+
+def toheisenberg(circuit, thetas):
+    """Reverse gates and parameters; angles are unchanged
+    (src/Gates/heisenbergschrodinger.jl:20-49)."""
+    n = countparameters(circuit)
+    if thetas is not None and len(thetas) != n:
+        raise ValueError(f"the circuit has {n} parameters, got {len(thetas)}")
+    return list(reversed(circuit)), (None if thetas is None else list(reversed(list(thetas))))
+
+
+def _qubit0(nqubits: int, q: int) -> int:
+    if not 1 <= q <= nqubits:
+        raise ValueError(f"qubit index {q} is out of range 1..{nqubits}")
+    return q - 1
+
+
+def pack_circuit(circuit, nqubits: int, W: int, param_offset=None):
+    """Pack a Heisenberg-ordered gate list into pp_gate descriptors.
+
+    Parametrized gates read parameter rows in order (row k is the k-th
+    parametrized gate of the list). Returns (gates array, Clifford table
+    changes as flat uint64 list, signs list, number of tables).
+    """
+    arr = (pp_gate * max(len(circuit), 1))()
+    lut_changes: list[int] = []
+    lut_signs: list[float] = []
+    lut_slots: dict[tuple, int] = {}
+    k = 0
+    for gi, gate in enumerate(circuit):
+        d = arr[gi]
+        d.param = -1
+        inner, frozen = (gate.gate, gate.parameter) if isinstance(gate, FrozenGate) else (gate, None)
+        if frozen is None and inner.parametrized:
+            d.param = k
+            k += 1
+        else:
+            d.value = 0.0 if frozen is None else frozen
+        if isinstance(inner, PauliRotation):
+            d.kind = ROTATION
+            m = inner.mask(nqubits)
+            for l in range(W):
+                d.mask[l] = (m >> (64 * l)) & ((1 << 64) - 1)
+        elif isinstance(inner, PauliNoise):
+            d.kind = PAULI_NOISE
+            d.qubit = _qubit0(nqubits, inner.qind)
+            d.damp_codes = inner.damp_codes
+            if frozen is not None:
+                _check_noise_strength(type(inner).__name__, frozen)
+        elif isinstance(inner, AmplitudeDampingNoise):
+            d.kind = AMP_DAMPING
+            d.qubit = _qubit0(nqubits, inner.qind)
+            if frozen is not None:
+                _check_noise_strength("AmplitudeDampingNoise", frozen)
+        elif isinstance(inner, CliffordGate):
+            d.kind = CLIFFORD
+            d.nq = len(inner.qinds)
+            qs = [_qubit0(nqubits, q) for q in inner.qinds]
+            for a, q in enumerate(qs):
+                d.qubits[a] = q
+            slot_key = (inner.symbol, tuple(qs))
+            if slot_key not in lut_slots:
+                lut_slots[slot_key] = len(lut_slots)
+                table = CLIFFORD_MAP[inner.symbol]
+                for idx in range(16):
+                    if idx < len(table):
+                        new, sign = table[idx]
+                        change = 0
+                        for a, q in enumerate(qs):
+                            old_p = (idx >> (2 * a)) & 3
+                            new_p = (new >> (2 * a)) & 3
+                            change |= (old_p ^ new_p) << (2 * q)
+                        lut_changes.extend((change >> (64 * l)) & ((1 << 64) - 1) for l in range(W))
+                        lut_signs.append(float(sign))
+                    else:
+                        lut_changes.extend([0] * W)
+                        lut_signs.append(1.0)
+            d.lut = lut_slots[slot_key]
+        else:
+            raise NotImplementedError(f"gate {gate!r} is not supported by this backend")
+    return arr, lut_changes, lut_signs, len(lut_slots)

@@ -1,0 +1,193 @@
+"""HIPPauliSum: a Pauli sum resident in MI355X memory, and the propagate()
+entry points that run on it.
+
+Mirrors the reference's user API (src/Propagation/generics.jl:54-95,
+src/stateoverlap.jl:35): `propagate(circuit, psum, thetas; max_weight,
+min_abs_coeff)` with the circuit in Schrodinger order, reversed here into
+Heisenberg order (src/Gates/heisenbergschrodinger.jl:20-49), and
+`overlapwithzero(psum)`. Every step runs in libpphip.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from .gates import countparameters, pack_circuit, toheisenberg
+from .paulis import PauliString, PauliSum, key_words, limbs_to_int
+
+_U64P = ctypes.POINTER(ctypes.c_uint64)
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def _u64p(a):
+    return a.ctypes.data_as(_U64P)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _max_weight_arg(max_weight) -> int:
+    if max_weight is None or (isinstance(max_weight, float) and math.isinf(max_weight)):
+        return -1
+    mw = int(max_weight)
+    if mw < 0:
+        raise ValueError("max_weight must be non-negative")
+    return mw
+
+
+class HIPPauliSum:
+    """Device Pauli sum: sorted, duplicate-free keys and an n x B FP64
+    coefficient matrix (B = batch columns sharing one key set)."""
+
+    def __init__(self, nqubits: int, batch: int = 1, capacity: int = 1024, key_words_: int = 0):
+        L = _lib.lib()
+        self.nqubits = int(nqubits)
+        self.batch = int(batch)
+        h = ctypes.c_void_p()
+        _lib.check(L.pp_create(self.nqubits, int(key_words_), self.batch, int(capacity), ctypes.byref(h)))
+        self._h = h
+        W = ctypes.c_int()
+        L.pp_info(self._h, None, ctypes.byref(W), None, None, None)
+        self.W = W.value
+
+    # -- construction --------------------------------------------------------------
+    @classmethod
+    def from_paulisum(cls, psum, batch: int = 1, capacity: int = 1024) -> "HIPPauliSum":
+        if isinstance(psum, PauliString):
+            psum = PauliSum.from_string(psum)
+        s = cls(psum.nqubits, batch=batch, capacity=max(capacity, 2 * len(psum)))
+        keys, coeffs = psum.to_arrays(s.W, batch)
+        s.upload(keys, coeffs)
+        return s
+
+    def upload(self, keys: np.ndarray, coeffs: np.ndarray) -> None:
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.batch)
+        if keys.shape[0] != coeffs.shape[0]:
+            raise ValueError("keys and coeffs have different lengths")
+        _lib.check(_lib.lib().pp_upload(self._h, _u64p(keys), _dp(coeffs), keys.shape[0]))
+
+    def download(self):
+        """(keys n x W uint64 ascending, coeffs n x B float64)"""
+        n = len(self)
+        keys = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        coeffs = np.zeros((max(n, 1), self.batch), dtype=np.float64)
+        got = ctypes.c_size_t()
+        _lib.check(_lib.lib().pp_download(self._h, _u64p(keys), _dp(coeffs), max(n, 1), ctypes.byref(got)))
+        return keys[:n], coeffs[:n]
+
+    def to_paulisum(self) -> PauliSum:
+        keys, coeffs = self.download()
+        return PauliSum.from_arrays(self.nqubits, keys, coeffs)
+
+    def __len__(self) -> int:
+        n = ctypes.c_size_t()
+        _lib.check(_lib.lib().pp_info(self._h, None, None, None, ctypes.byref(n), None))
+        return n.value
+
+    @property
+    def capacity(self) -> int:
+        c = ctypes.c_size_t()
+        _lib.check(_lib.lib().pp_info(self._h, None, None, None, None, ctypes.byref(c)))
+        return c.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().pp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- single gates ----------------------------------------------------------------
+    def apply_rotation(self, mask: int, theta, max_weight=math.inf, min_abs_coeff=1e-10):
+        """applymergetruncate!(PauliRotation, cache, theta) for an integer generator mask."""
+        th = np.broadcast_to(np.asarray(theta, dtype=np.float64), (self.batch,))
+        c = np.ascontiguousarray([math.cos(t) for t in th], dtype=np.float64)
+        s = np.ascontiguousarray([math.sin(t) for t in th], dtype=np.float64)
+        m = np.array([(mask >> (64 * l)) & ((1 << 64) - 1) for l in range(self.W)], dtype=np.uint64)
+        st = _lib.pp_gate_stats()
+        _lib.check(_lib.lib().pp_apply_rotation(self._h, _u64p(m), _dp(c), _dp(s), _max_weight_arg(max_weight),
+                                                float(min_abs_coeff), ctypes.byref(st)))
+        return st
+
+    def apply_pauli_noise(self, qubit: int, damp_codes: int, lam, max_weight=math.inf, min_abs_coeff=1e-10):
+        lam = np.broadcast_to(np.asarray(lam, dtype=np.float64), (self.batch,))
+        f = np.ascontiguousarray(1 - lam, dtype=np.float64)
+        st = _lib.pp_gate_stats()
+        _lib.check(_lib.lib().pp_apply_pauli_noise(self._h, int(qubit) - 1, int(damp_codes), _dp(f),
+                                                   _max_weight_arg(max_weight), float(min_abs_coeff),
+                                                   ctypes.byref(st)))
+        return st
+
+    def overlapwithzero(self):
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_overlap_zero(self._h, _dp(out)))
+        return float(out[0]) if self.batch == 1 else out
+
+
+def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, min_abs_coeff=1e-10,
+                   counts: bool = False):
+    """Run an already Heisenberg-ordered circuit; hthetas is P x B (or P) in that order."""
+    P = countparameters(hcircuit)
+    B = psum.batch
+    th = np.zeros((max(P, 1), B), dtype=np.float64)
+    if P:
+        th[:P] = np.asarray(hthetas, dtype=np.float64).reshape(P, -1)
+    gates, lut_c, lut_s, nluts = pack_circuit(hcircuit, psum.nqubits, psum.W)
+    lc = np.ascontiguousarray(lut_c or [0], dtype=np.uint64)
+    ls = np.ascontiguousarray(lut_s or [0.0], dtype=np.float64)
+    cnt = np.zeros(max(len(hcircuit), 1), dtype=np.int64)
+    _lib.check(_lib.lib().pp_run_circuit(
+        psum._h, gates, len(hcircuit), _dp(th), P, _u64p(lc), _dp(ls), nluts, _max_weight_arg(max_weight),
+        float(min_abs_coeff), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    return cnt[:len(hcircuit)] if counts else None
+
+
+def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-10, batch: int = None,
+              counts: bool = False):
+    """propagate(circuit, psum, thetas; max_weight, min_abs_coeff) on the GPU.
+
+    `psum` is a PauliString, a PauliSum or a HIPPauliSum (propagated in place
+    and returned). `thetas` holds one parameter per parametrized gate in
+    Schrodinger order, or a P x B array for B batch columns. The reference's
+    default min_abs_coeff is 1e-10 and max_weight Inf
+    (src/Propagation/generics.jl:54).
+    """
+    P = countparameters(circuit)
+    th = np.zeros((0, 1)) if thetas is None else np.asarray(thetas, dtype=np.float64)
+    if th.ndim == 1:
+        th = th.reshape(-1, 1)
+    if th.shape[0] != P:
+        raise ValueError(f"the circuit has {P} parameters, got {th.shape[0]}")
+    B = batch or (th.shape[1] if P else 1)
+    if not isinstance(psum, HIPPauliSum):
+        psum = HIPPauliSum.from_paulisum(psum, batch=B)
+    if P and th.shape[1] != psum.batch:
+        raise ValueError(f"thetas have {th.shape[1]} columns, the sum has {psum.batch}")
+    hcirc, hth = toheisenberg(circuit, list(th) if P else [])
+    hth = np.array(hth) if P else None
+    cnt = run_heisenberg(psum, hcirc, hth, max_weight=max_weight, min_abs_coeff=min_abs_coeff, counts=counts)
+    return (psum, cnt) if counts else psum
+
+
+def overlapwithzero(psum):
+    """<0|psum|0>: the sum of the coefficients of strings with no X or Y
+    (src/stateoverlap.jl:12-35)."""
+    if isinstance(psum, HIPPauliSum):
+        return psum.overlapwithzero()
+    raise TypeError("overlapwithzero runs on a HIPPauliSum; upload the sum first")
+
+
+def key_to_int(row) -> int:
+    return limbs_to_int(row)
+
+
+__all__ = ["HIPPauliSum", "propagate", "overlapwithzero", "run_heisenberg", "key_words"]

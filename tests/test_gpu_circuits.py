@@ -1,0 +1,115 @@
+"""GPU parity, whole circuits: propagate() on the GPU against the oracle
+(bit for bit) and against the reference's published values."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import assert_bitwise, fsum_overlap, rel_close
+from oracle import oracle_propagate
+import pauliprop_hip as pp
+
+pytestmark = pytest.mark.gpu
+
+
+def _config1(nlayers=32):
+    n = 32
+    circ = pp.tfitrottercircuit(n, nlayers, topology=pp.bricklayertopology(n, periodic=True))
+    th = np.full(pp.countparameters(circ), 0.1)
+    obs = pp.PauliSum.from_string(pp.PauliString(n, "Z", 16))
+    return n, circ, th, obs
+
+
+def test_config1_readme_value():
+    # README.md:114 of the reference: overlapwithzero -> 0.154596728241...
+    n, circ, th, obs = _config1()
+    dev = pp.propagate(circ, obs, th, max_weight=6, min_abs_coeff=1e-4)
+    ref, _ = oracle_propagate(circ, n, th, obs.terms, max_weight=6, min_abs_coeff=1e-4)
+    assert_bitwise(*dev.download(), *ref.get(), "config 1")
+    ov = dev.overlapwithzero()
+    assert abs(ov - 0.154596728241) < 1e-12
+    assert rel_close(ov, float(ref.overlap_zero()[0]))
+
+
+def test_config1_weight_only_counts():
+    n, circ, th, obs = _config1()
+    dev, counts = pp.propagate(circ, obs, th, max_weight=6, min_abs_coeff=0.0, counts=True)
+    ref, rcounts = oracle_propagate(circ, n, th, obs.terms, max_weight=6, min_abs_coeff=0.0)
+    assert np.array_equal(counts, rcounts)
+    assert_bitwise(*dev.download(), *ref.get(), "config 1 weight-only")
+    assert rel_close(dev.overlapwithzero(), fsum_overlap(*ref.get())[0])
+
+
+def test_tfim_6x6_notebook_anchor():
+    # examples/advanced_performance.ipynb:92-118,182,224 of the reference:
+    # 1,036,142 terms and <Z21 Z22> = 0.6696851623009762
+    nq = 36
+    layer = []
+    pp.rxlayer(layer, nq)
+    pp.rzlayer(layer, nq)
+    pp.rzzlayer(layer, pp.rectangletopology(6, 6))
+    th = np.ones(len(layer)) * 0.05 * 2
+    th[2 * nq:] *= 0.5
+    circ, cth = layer * 14, np.tile(th, 14)
+    obs = pp.PauliSum.from_string(pp.PauliString(nq, ["Z", "Z"], [21, 22]))
+    dev = pp.propagate(circ, obs, cth, min_abs_coeff=2.0 ** -20)
+    assert len(dev) == 1036142
+    assert abs(dev.overlapwithzero() - 0.6696851623009762) < 1e-13
+
+
+@pytest.mark.parametrize("B", [2, 5, 64])
+def test_batched_heisenberg_matches_columns(B):
+    # one shared key set, B parameter columns: weight-only truncation gives the
+    # same keys for every column, and each column equals its own B=1 run
+    nx = 3
+    nq = nx * nx
+    topo = pp.rectangletopology(nx, nx, periodic=True)
+    circ = []
+    for _ in range(2):
+        pp.rxxlayer(circ, topo)
+        pp.ryylayer(circ, topo)
+        pp.rzzlayer(circ, topo)
+        pp.rzlayer(circ, nq)
+    P = pp.countparameters(circ)
+    rng = np.random.default_rng(B)
+    th = rng.uniform(0, 0.2, size=(P, B))
+    obs = pp.PauliSum(nq)
+    for a, b in topo:
+        obs.add(["Z", "Z"], [a, b], 1.0)
+    dev = pp.propagate(circ, obs, th, max_weight=5, min_abs_coeff=0.0)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=5, min_abs_coeff=0.0, batch=B)
+    assert_bitwise(*dev.download(), *ref.get(), f"batched B={B}")
+    ov = dev.overlapwithzero()
+    exact = fsum_overlap(*ref.get())
+    for b in range(B):
+        assert rel_close(ov[b], exact[b])
+
+
+def test_batched_coefficient_truncation_uses_batch_max():
+    nq = 16
+    circ = pp.tfitrottercircuit(nq, 4, topology=pp.bricklayertopology(nq, periodic=True))
+    P = pp.countparameters(circ)
+    B = 8
+    th = np.random.default_rng(1).uniform(0, 0.6, size=(P, B))
+    obs = pp.PauliSum.from_string(pp.PauliString(nq, "Z", 8))
+    dev = pp.propagate(circ, obs, th, min_abs_coeff=1e-3)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, min_abs_coeff=1e-3, batch=B)
+    assert_bitwise(*dev.download(), *ref.get(), "batched truncation")
+
+
+def test_noisy_circuit():
+    nq = 12
+    circ = []
+    for _ in range(3):
+        pp.rylayer(circ, nq)
+        pp.rzlayer(circ, nq)
+        pp.rzzlayer(circ, pp.staircasetopology(nq))
+        pp.depolarizinglayer(circ, nq, 0.01)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(5).normal(0, 0.3, size=P)
+    obs = pp.PauliSum(nq)
+    for i in range(1, nq):
+        obs.add(["Z", "Z"], [i, i + 1])
+    dev = pp.propagate(circ, obs, th, max_weight=7, min_abs_coeff=1e-5)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=7, min_abs_coeff=1e-5)
+    assert_bitwise(*dev.download(), *ref.get(), "noisy")

@@ -1,0 +1,106 @@
+"""GPU parity, single gates: libpphip.so through the C ABI against the CPU
+oracle on identical inputs, bit for bit (keys and FP64 coefficients)."""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import assert_bitwise, mask_of, random_terms, terms_to_arrays, fsum_overlap
+from oracle import OracleSum
+from pauliprop_hip import HIPPauliSum
+from pauliprop_hip.paulis import key_words, int_to_limbs
+
+pytestmark = pytest.mark.gpu
+
+# (nqubits, {qubit: code}) -- low, middle, top, limb straddle, wrap-around, multi-run masks
+MASKS = [
+    (32, {1: 1}), (32, {16: 3}), (32, {32: 2}), (32, {1: 3, 2: 3}), (32, {32: 3, 1: 3}),
+    (32, {5: 1, 9: 1}), (32, {31: 2, 32: 2}), (32, {1: 1, 10: 2, 20: 3}),
+    (64, {33: 1}), (64, {32: 3, 33: 3}), (64, {1: 2, 64: 2}), (64, {17: 1, 40: 2, 63: 3}),
+    (127, {127: 1}), (127, {63: 3, 65: 3}), (127, {64: 3}), (127, {1: 3, 127: 3}), (127, {30: 1, 95: 2}),
+    (127, {2: 1, 4: 2, 6: 3, 8: 1, 70: 2, 100: 3, 126: 1}),
+]
+
+
+def _pair(nq, n, seed, p=0.3, batch=1):
+    rng = np.random.default_rng(seed)
+    terms = random_terms(nq, n, rng, p)
+    coeffs = rng.standard_normal((len(terms), batch))
+    keys, c = terms_to_arrays(terms, nq, coeffs)
+    dev = HIPPauliSum(nq, batch=batch, capacity=max(1024, len(terms)))
+    dev.upload(keys, c)
+    ref = OracleSum(key_words(nq), batch).set(keys, c)
+    return dev, ref
+
+
+@pytest.mark.parametrize("nq,codes", MASKS)
+@pytest.mark.parametrize("n", [1, 5, 3000, 120000])
+def test_rotation_weight_only(nq, codes, n):
+    print(f"rotation nq={nq} codes={codes} n={n}", flush=True)
+    dev, ref = _pair(nq, n, seed=n + nq)
+    m = mask_of(nq, codes)
+    theta = 0.37
+    W = key_words(nq)
+    for step in range(3):
+        dev.apply_rotation(m, theta, max_weight=12, min_abs_coeff=0.0)
+        ref.rotation(int_to_limbs(m, W), math.cos(theta), math.sin(theta), max_weight=12, min_abs=0.0)
+        k, c = dev.download()
+        rk, rc = ref.get()
+        assert_bitwise(k, c, rk, rc, f"step {step}")
+
+
+@pytest.mark.parametrize("nq,codes", [(32, {16: 1}), (64, {20: 3, 21: 3}), (127, {63: 1, 64: 2})])
+@pytest.mark.parametrize("min_abs", [1e-3, 0.3])
+def test_rotation_coefficient_truncation(nq, codes, min_abs):
+    print(f"truncation nq={nq} codes={codes} min_abs={min_abs}", flush=True)
+    dev, ref = _pair(nq, 60000, seed=7)
+    m = mask_of(nq, codes)
+    W = key_words(nq)
+    for theta in (0.2, 1.1, -0.4):
+        dev.apply_rotation(m, theta, max_weight=math.inf, min_abs_coeff=min_abs)
+        ref.rotation(int_to_limbs(m, W), math.cos(theta), math.sin(theta), min_abs=min_abs)
+        k, c = dev.download()
+        rk, rc = ref.get()
+        assert_bitwise(k, c, rk, rc, f"theta {theta}")
+
+
+@pytest.mark.parametrize("nq", [32, 64, 127])
+def test_noise_and_overlap(nq):
+    print(f"noise nq={nq}", flush=True)
+    dev, ref = _pair(nq, 50000, seed=3, p=0.2)
+    for q, codes, lam in [(1, 0b1110, 0.1), (nq, 0b0110, 0.5), (nq // 2, 0b1000, 0.9)]:
+        dev.apply_pauli_noise(q, codes, lam, max_weight=10, min_abs_coeff=0.05)
+        ref.pauli_noise(q - 1, codes, 1 - lam, max_weight=10, min_abs=0.05)
+        k, c = dev.download()
+        rk, rc = ref.get()
+        assert_bitwise(k, c, rk, rc, f"noise q={q}")
+    ov = dev.overlapwithzero()
+    exact = fsum_overlap(*dev.download())[0]
+    assert abs(ov - exact) <= 1e-14 * max(1.0, abs(exact))
+
+
+def test_untouched_gate_leaves_sum_alone():
+    # a gate touching no term neither merges nor truncates (src/Base/Merge/xortailmerge.jl:121-123)
+    nq = 8
+    dev = HIPPauliSum(nq)
+    keys, c = terms_to_arrays([mask_of(nq, {1: 3}), mask_of(nq, {2: 3, 3: 3})], nq, [1e-9, 0.5])
+    dev.upload(keys, c)
+    st = dev.apply_rotation(mask_of(nq, {8: 1}), 0.3, min_abs_coeff=1e-3)
+    assert st.n_touched == 0 and len(dev) == 2
+    st = dev.apply_rotation(mask_of(nq, {1: 1}), 0.3, min_abs_coeff=1e-3)
+    assert st.n_touched == 1
+    k, cc = dev.download()
+    # the 1e-9 term (kept through the untouched gate) is truncated with its branch now
+    assert len(k) == 1 and int(k[0, 0]) == mask_of(nq, {2: 3, 3: 3})
+
+
+def test_empty_sum_and_errors():
+    dev = HIPPauliSum(40)
+    assert len(dev) == 0
+    dev.apply_rotation(mask_of(40, {3: 1}), 0.1)
+    assert dev.overlapwithzero() == 0.0
+    with pytest.raises(ValueError):
+        dev.apply_rotation(1 << 90, 0.1)  # qubit 46 of a 40-qubit sum
+    keys, c = terms_to_arrays([5, 5], 40, [1.0, 2.0])
+    with pytest.raises(ValueError):
+        dev.upload(keys, c)
