@@ -134,6 +134,35 @@ int pp_overlap_zero(pp_handle *h, double *out_b);
 /* Wait for all work on the handle's stream. */
 int pp_synchronize(pp_handle *h);
 
+/* Kernel kinds of the per-handle profile. */
+enum pp_kernel_kind {
+    PP_KERNEL_BRANCH = 0,        /* anticommuting terms, compacted in order */
+    PP_KERNEL_TAIL_PERMUTE = 1,  /* tail into ascending order of term ^ mask */
+    PP_KERNEL_MERGE = 2,         /* merge with truncation */
+    PP_KERNEL_NOISE = 3,         /* Pauli noise with truncation */
+    PP_KERNEL_OVERLAP = 4,       /* overlapwithzero partial sums */
+    PP_KERNEL_KINDS = 5
+};
+
+/* Counters since the last reset: gates applied, term-gate applications
+ * (sum of n_in over those gates), and per kernel kind the launches, the time
+ * between HIP events recorded around them on the handle's stream (only while
+ * timing is enabled), and their algorithmic HBM bytes: one read and one write
+ * of the live set for a merge, (K + 8B)(n_in + n_out). */
+typedef struct pp_profile {
+    int64_t gates;
+    int64_t term_applications;
+    int64_t launches[PP_KERNEL_KINDS];
+    double milliseconds[PP_KERNEL_KINDS];
+    double algorithmic_bytes[PP_KERNEL_KINDS];
+} pp_profile;
+
+/* Turn event timing on (1) or off (0), or leave it (-1); reset != 0 zeroes
+ * the counters. No counterpart in the reference; its analogue is the per-gate
+ * term counter of @countpaulis (src/Base/countterms.jl:41-66). */
+int pp_profile_control(pp_handle *h, int enable_timing, int reset);
+int pp_profile_read(const pp_handle *h, pp_profile *out);
+
 #ifdef __cplusplus
 }
 #endif

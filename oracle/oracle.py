@@ -36,6 +36,7 @@ def lib():
             ("ppo_create", _P, [ctypes.c_int, ctypes.c_int]),
             ("ppo_destroy", None, [_P]),
             ("ppo_set", ctypes.c_int, [_P, _U64P, _DP, ctypes.c_size_t]),
+            ("ppo_load_merge", ctypes.c_int, [_P, _U64P, _DP, ctypes.c_size_t, ctypes.c_int, ctypes.c_double]),
             ("ppo_size", ctypes.c_size_t, [_P]),
             ("ppo_sorted", ctypes.c_int, [_P]),
             ("ppo_get", None, [_P, _U64P, _DP]),
@@ -95,6 +96,12 @@ class OracleSum:
         coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.B)
         if lib().ppo_set(self._s, u64p(keys), dp(coeffs), keys.shape[0]) != 0:
             raise ValueError("duplicate key")
+        return self
+
+    def load_merge(self, keys, coeffs, max_weight=math.inf, min_abs=0.0):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.B)
+        lib().ppo_load_merge(self._s, u64p(keys), dp(coeffs), keys.shape[0], _mw(max_weight), float(min_abs))
         return self
 
     def __len__(self):

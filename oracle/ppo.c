@@ -371,6 +371,17 @@ static void sort_reduce_truncate(ppo_sum *s, const uint64_t *tk, const double *t
     s->sorted = 1;
 }
 
+/* Load unsorted terms that may repeat: sort, reduce equal keys in input
+ * order and truncate (mergeandtruncate! on an unsorted array,
+ * src/Base/Merge/mergeandtruncate.jl:104-120). */
+int ppo_load_merge(ppo_sum *s, const uint64_t *keys, const double *coeffs, size_t n, int max_weight,
+                   double min_abs) {
+    s->n = 0;
+    s->sorted = 0;
+    sort_reduce_truncate(s, keys, coeffs, n, max_weight, min_abs);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* gates                                                               */
 /* ------------------------------------------------------------------ */

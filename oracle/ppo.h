@@ -41,6 +41,9 @@ ppo_sum *ppo_create(int W, int B);
 void ppo_destroy(ppo_sum *s);
 /* Load n terms; they are sorted here. Returns -1 on a duplicate key. */
 int ppo_set(ppo_sum *s, const uint64_t *keys, const double *coeffs, size_t n);
+/* Load terms that may repeat: sort, sum equal keys in input order, truncate. */
+int ppo_load_merge(ppo_sum *s, const uint64_t *keys, const double *coeffs, size_t n, int max_weight,
+                   double min_abs);
 size_t ppo_size(const ppo_sum *s);
 int ppo_sorted(const ppo_sum *s);
 void ppo_get(const ppo_sum *s, uint64_t *keys, double *coeffs);

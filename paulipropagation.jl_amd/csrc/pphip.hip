@@ -59,6 +59,13 @@ struct pp_handle {
     size_t partial_cap = 0;
     double* d_out = nullptr;       // B results
     uint32_t epoch = 0;
+    // profile
+    bool prof_on = false;
+    hipEvent_t ev[8] = {};
+    int64_t gates = 0, term_apps = 0;
+    int64_t launches[PP_KERNEL_KINDS] = {};
+    double ms[PP_KERNEL_KINDS] = {};
+    double alg_bytes[PP_KERNEL_KINDS] = {};
 };
 
 namespace {
@@ -151,11 +158,12 @@ uint32_t next_epoch(pp_handle* h) {
     return h->epoch;
 }
 
-int read_scalar(pp_handle* h, int idx, uint32_t* out) {
-    HIP_TRY(hipMemcpyAsync(h->h_scalar + idx, h->d_scalar + idx, sizeof(uint32_t), hipMemcpyDeviceToHost,
+// copy `count` device scalars from index idx and wait for the stream
+int read_scalars(pp_handle* h, int idx, int count, uint32_t* out) {
+    HIP_TRY(hipMemcpyAsync(h->h_scalar + idx, h->d_scalar + idx, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    *out = h->h_scalar[idx];
+    for (int i = 0; i < count; ++i) out[i] = h->h_scalar[idx + i];
     return PP_OK;
 }
 
@@ -221,20 +229,43 @@ int check_mask(const pp_handle* h, const uint64_t* mask) {
     return PP_OK;
 }
 
+// ---------------------------------------------------------------- profile
+
+void prof_mark(pp_handle* h, int ev) {
+    if (h->prof_on) hipEventRecord(h->ev[ev], h->stream);
+}
+
+// after a stream synchronization: the events of slot (2*slot, 2*slot+1) are complete
+void prof_add(pp_handle* h, int slot, int kind, double bytes) {
+    h->launches[kind] += 1;
+    h->alg_bytes[kind] += bytes;
+    if (h->prof_on) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev[2 * slot], h->ev[2 * slot + 1]) == hipSuccess) h->ms[kind] += ms;
+    }
+}
+
 template <int W>
 int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, const double* sin_b, int max_weight,
                   double min_abs, pp_gate_stats* st) {
     const size_t n = h->n;
     if (st) *st = pp_gate_stats{(int64_t)n, 0, (int64_t)n};
+    h->gates += 1;
+    h->term_apps += (int64_t)n;
     if (n == 0) return PP_OK;
     const Key<W> mask = to_key<W>(mask_in);
-    const int cur = h->cur;
+    int mask_limbs = 0;
+    for (int l = 0; l < W; ++l) mask_limbs += mask_in[l] != 0;
+    const double row = 8.0 * W + 8.0 * h->B;
     // K1: anticommuting terms, in order
-    hipLaunchKernelGGL(k_branch<W>, dim3(blocks(n, K1_TILE)), dim3(WG), 0, h->stream, h->keys[cur], h->cap,
+    prof_mark(h, 0);
+    hipLaunchKernelGGL(k_branch<W>, dim3(blocks(n, K1_TILE)), dim3(WG), 0, h->stream, h->keys[h->cur], h->cap,
                        (uint32_t)n, mask, h->tail[0], h->lb_states, h->counter, next_epoch(h), h->d_scalar);
     HIP_TRY(hipGetLastError());
+    prof_mark(h, 1);
     uint32_t na = 0;
-    if (int rc = read_scalar(h, 0, &na)) return rc;
+    if (int rc = read_scalars(h, 0, 1, &na)) return rc;
+    prof_add(h, 0, PP_KERNEL_BRANCH, 8.0 * mask_limbs * n + 4.0 * na);
     if (st) st->n_touched = na;
     if (na == 0) return PP_OK;  // nothing touched: nothing merged or truncated
     if (n + na > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
@@ -253,13 +284,16 @@ int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, co
     int from_limb;
     if (mask_levels(mask_in, W, na, &lv, &const_off, &from_limb))
         return fail(PP_ERR_UNSUPPORTED, "rotation generator with more than 8 runs of set bits (16 levels)");
+    prof_mark(h, 2);
     hipLaunchKernelGGL(k_xorperm<W>, dim3(blocks(na, K2_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
                        h->tail[0], na, lv, const_off, from_limb, h->tail[1], h->d_scalar + 2);
     HIP_TRY(hipGetLastError());
+    prof_mark(h, 3);
     // K3: merge with truncation into the other buffer
     const size_t ntot = n + na;
     if (h->B == 1) {
         RotParams rp{cos_b[0], sin_b[0], max_weight, min_abs};
+        prof_mark(h, 4);
         hipLaunchKernelGGL(k_merge1<W>, dim3(blocks(ntot, K3_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
                            h->coeffs[c], (uint32_t)n, h->tail[1], na, mask, rp, h->keys[1 - c], h->cap,
                            h->coeffs[1 - c], h->lb_states, h->counter, next_epoch(h), h->d_scalar + 1,
@@ -272,6 +306,7 @@ int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, co
         const int TB = tile_rows(B);
         const size_t shm = (size_t)TB * B * sizeof(double);
         const unsigned g = blocks(ntot, TB);
+        prof_mark(h, 4);
 #define LAUNCH_MB(TBV)                                                                                          \
     hipLaunchKernelGGL((k_mergeB<W, TBV>), dim3(g), dim3(WG), shm, h->stream, h->keys[c], h->cap, h->coeffs[c], \
                        (uint32_t)n, B, h->tail[1], na, mask, h->d_param, h->d_param + B, max_weight, min_abs,   \
@@ -287,14 +322,17 @@ int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, co
 #undef LAUNCH_MB
     }
     HIP_TRY(hipGetLastError());
-    uint32_t nout = 0, err = 0;
-    if (int rc = read_scalar(h, 1, &nout)) return rc;
-    if (int rc = read_scalar(h, 2, &err)) return rc;
-    if (err) {
+    prof_mark(h, 5);
+    uint32_t res[2] = {0, 0};  // output count, error bits
+    if (int rc = read_scalars(h, 1, 2, res)) return rc;
+    if (res[1]) {
         HIP_TRY(hipMemsetAsync(h->d_scalar + 2, 0, sizeof(uint32_t), h->stream));
         return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
-                                      std::to_string(err) + ")");
+                                      std::to_string(res[1]) + ")");
     }
+    const uint32_t nout = res[0];
+    prof_add(h, 1, PP_KERNEL_TAIL_PERMUTE, na * (8.0 + 8.0 * (W - from_limb)));
+    prof_add(h, 2, PP_KERNEL_MERGE, row * ((double)n + (double)nout));
     h->cur = 1 - c;
     h->n = nout;
     if (st) st->n_out = nout;
@@ -306,9 +344,12 @@ int noise_impl(pp_handle* h, int qubit, uint32_t damp_codes, const double* facto
                double min_abs, pp_gate_stats* st) {
     const size_t n = h->n;
     if (st) *st = pp_gate_stats{(int64_t)n, (int64_t)n, (int64_t)n};
+    h->gates += 1;
+    h->term_apps += (int64_t)n;
     if (n == 0) return PP_OK;
     const int c = h->cur;
     if (h->B == 1) {
+        prof_mark(h, 4);
         hipLaunchKernelGGL(k_noise1<W>, dim3(blocks(n, KN_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
                            h->coeffs[c], (uint32_t)n, qubit, damp_codes, factor_b[0], max_weight, min_abs,
                            h->keys[1 - c], h->cap, h->coeffs[1 - c], h->lb_states, h->counter, next_epoch(h),
@@ -318,14 +359,17 @@ int noise_impl(pp_handle* h, int qubit, uint32_t damp_codes, const double* facto
         std::memcpy(h->h_param + 2 * B, factor_b, B * sizeof(double));
         HIP_TRY(hipMemcpyAsync(h->d_param + 2 * B, h->h_param + 2 * B, B * sizeof(double), hipMemcpyHostToDevice,
                                h->stream));
+        prof_mark(h, 4);
         hipLaunchKernelGGL(k_noiseB<W>, dim3(blocks(n, 64)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
                            h->coeffs[c], (uint32_t)n, B, qubit, damp_codes, h->d_param + 2 * B, max_weight,
                            min_abs, h->keys[1 - c], h->cap, h->coeffs[1 - c], h->lb_states, h->counter,
                            next_epoch(h), h->d_scalar + 1);
     }
     HIP_TRY(hipGetLastError());
+    prof_mark(h, 5);
     uint32_t nout = 0;
-    if (int rc = read_scalar(h, 1, &nout)) return rc;
+    if (int rc = read_scalars(h, 1, 1, &nout)) return rc;
+    prof_add(h, 2, PP_KERNEL_NOISE, (8.0 * h->W + 8.0 * h->B) * ((double)n + (double)nout));
     h->cur = 1 - c;
     h->n = nout;
     if (st) st->n_out = nout;
@@ -349,6 +393,7 @@ int overlap_impl(pp_handle* h, double* out) {
         HIP_TRY(hipMalloc(&h->d_partial, need * sizeof(double)));
         h->partial_cap = need;
     }
+    prof_mark(h, 6);
     if (B == 1) {
         hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->keys[c], h->cap, h->coeffs[c],
                            (uint32_t)n, h->d_partial);
@@ -357,10 +402,12 @@ int overlap_impl(pp_handle* h, double* out) {
                            h->keys[c], h->cap, h->coeffs[c], (uint32_t)n, B, h->d_partial);
     }
     HIP_TRY(hipGetLastError());
+    prof_mark(h, 7);
     hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, h->d_partial, (int)g, B, h->d_out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, h->d_out, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    prof_add(h, 3, PP_KERNEL_OVERLAP, (8.0 * h->W + 8.0 * B) * (double)n);
     return PP_OK;
 }
 
@@ -422,6 +469,8 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
 int pp_destroy(pp_handle* h) {
     if (!h) return PP_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
+    for (int e = 0; e < 8; ++e)
+        if (h->ev[e]) hipEventDestroy(h->ev[e]);
     free_buffers(h);
     if (h->counter) hipFree(h->counter);
     if (h->d_scalar) hipFree(h->d_scalar);
@@ -584,6 +633,35 @@ int pp_overlap_zero(pp_handle* h, double* out_b) {
 int pp_synchronize(pp_handle* h) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     HIP_TRY(hipStreamSynchronize(h->stream));
+    return PP_OK;
+}
+
+int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    if (enable_timing == 1 && !h->ev[0]) {
+        for (int e = 0; e < 8; ++e) HIP_TRY(hipEventCreate(&h->ev[e]));
+    }
+    if (enable_timing >= 0) h->prof_on = enable_timing == 1;
+    if (reset) {
+        h->gates = h->term_apps = 0;
+        for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
+            h->launches[k] = 0;
+            h->ms[k] = 0.0;
+            h->alg_bytes[k] = 0.0;
+        }
+    }
+    return PP_OK;
+}
+
+int pp_profile_read(const pp_handle* h, pp_profile* out) {
+    if (!h || !out) return fail(PP_ERR_ARG, "NULL argument");
+    out->gates = h->gates;
+    out->term_applications = h->term_apps;
+    for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
+        out->launches[k] = h->launches[k];
+        out->milliseconds[k] = h->ms[k];
+        out->algorithmic_bytes[k] = h->alg_bytes[k];
+    }
     return PP_OK;
 }
 

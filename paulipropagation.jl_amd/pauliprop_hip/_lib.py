@@ -26,6 +26,15 @@ class pp_gate_stats(ctypes.Structure):
     _fields_ = [("n_in", ctypes.c_int64), ("n_touched", ctypes.c_int64), ("n_out", ctypes.c_int64)]
 
 
+KERNEL_KINDS = ("branch", "tail_permute", "merge", "noise", "overlap")
+
+
+class pp_profile(ctypes.Structure):
+    _fields_ = [("gates", ctypes.c_int64), ("term_applications", ctypes.c_int64),
+                ("launches", ctypes.c_int64 * 5), ("milliseconds", ctypes.c_double * 5),
+                ("algorithmic_bytes", ctypes.c_double * 5)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every symbol the header declares
@@ -51,6 +60,8 @@ SIGNATURES = [
                                       ctypes.c_double, _I64P]),
     ("pp_overlap_zero", ctypes.c_int, [_P, _DP]),
     ("pp_synchronize", ctypes.c_int, [_P]),
+    ("pp_profile_control", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
+    ("pp_profile_read", ctypes.c_int, [_P, ctypes.POINTER(pp_profile)]),
 ]
 
 

@@ -132,6 +132,54 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_overlap_zero(self._h, _dp(out)))
         return float(out[0]) if self.batch == 1 else out
 
+    def synchronize(self) -> None:
+        _lib.check(_lib.lib().pp_synchronize(self._h))
+
+    def profile_control(self, timing: int = -1, reset: bool = False) -> None:
+        """timing: 1 records HIP events around every kernel, 0 stops, -1 leaves it."""
+        _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
+
+    def profile(self) -> dict:
+        p = _lib.pp_profile()
+        _lib.check(_lib.lib().pp_profile_read(self._h, ctypes.byref(p)))
+        return {
+            "gates": p.gates,
+            "term_applications": p.term_applications,
+            "kernels": {k: {"launches": p.launches[i], "ms": p.milliseconds[i], "bytes": p.algorithmic_bytes[i]}
+                        for i, k in enumerate(_lib.KERNEL_KINDS)},
+        }
+
+
+class PackedCircuit:
+    """A Heisenberg-ordered circuit packed once into pp_gate descriptors and a
+    P x B parameter matrix, for repeated runs (the benchmark's inner loop)."""
+
+    def __init__(self, circuit, thetas, nqubits: int, batch: int = 1):
+        P = countparameters(circuit)
+        th = np.zeros((0, batch)) if thetas is None else np.asarray(thetas, dtype=np.float64).reshape(P, -1)
+        hc, hth = toheisenberg(circuit, list(th))
+        self.ngates = len(hc)
+        self.nparams = P
+        self.theta = np.zeros((max(P, 1), th.shape[1] if P else batch), dtype=np.float64)
+        if P:
+            self.theta[:P] = np.array(hth).reshape(P, -1)
+        self.W = key_words(nqubits)
+        self.gates, lc, ls, self.nluts = pack_circuit(hc, nqubits, self.W)
+        self.lut_c = np.ascontiguousarray(lc or [0], dtype=np.uint64)
+        self.lut_s = np.ascontiguousarray(ls or [0.0], dtype=np.float64)
+        self.counts = np.zeros(max(self.ngates, 1), dtype=np.int64)
+
+    def run(self, psum: HIPPauliSum, max_weight=math.inf, min_abs_coeff=1e-10, first: int = 0, last=None):
+        """Run gates [first, last) on psum; self.counts[first:last] receives n_in per gate."""
+        last = self.ngates if last is None else last
+        g0 = ctypes.cast(ctypes.addressof(self.gates) + first * ctypes.sizeof(self.gates._type_),
+                         ctypes.POINTER(type(self.gates[0])))
+        c0 = self.counts[first:].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        _lib.check(_lib.lib().pp_run_circuit(
+            psum._h, g0, last - first, _dp(self.theta), self.nparams, _u64p(self.lut_c), _dp(self.lut_s),
+            self.nluts, _max_weight_arg(max_weight), float(min_abs_coeff), c0))
+        return self.counts[first:last]
+
 
 def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, min_abs_coeff=1e-10,
                    counts: bool = False):
