@@ -86,10 +86,14 @@ def main() -> int:
         t = time.perf_counter()
         step()
         log(f"warmup {w}: {time.perf_counter() - t:.3f} s, {len(psum)} terms")
-    psum.profile_control(timing=1, reset=True)
+    # HIP events bracket the merge kernels of the last timed step only: recording
+    # events around every launch of every step would slow the steps themselves
+    psum.profile_control(timing=0, reset=True)
     psum.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
+        if k == args.steps - 1:
+            psum.profile_control(timing=1, reset=True)
         step()
         log(f"step {k}: {time.perf_counter() - t0:.3f} s cumulative")
     psum.synchronize()
@@ -97,9 +101,10 @@ def main() -> int:
     prof = psum.profile()
     psum.profile_control(timing=0)
     counts = pc.counts[:pc.ngates].copy()
-    apps = int(prof["term_applications"]) * B
+    apps = int(prof["term_applications"]) * B * args.steps
     value = apps / elapsed
     overlap = psum.overlapwithzero()
+    final_terms = len(psum)
 
     # dominant kernel and its roofline
     kern = max(prof["kernels"].items(), key=lambda kv: kv[1]["ms"])
@@ -110,9 +115,9 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_{kname}",
                 "avg_launch_us": round(avg_ms * 1e3, 2), "avg_algorithmic_bytes": round(avg_bytes),
-                "launches_per_step": kst["launches"] // max(args.steps, 1)}
-    kernels = {k: {"launches": v["launches"] // max(args.steps, 1), "ms_per_step": round(v["ms"] / args.steps, 2),
-                   "GBps": round(v["bytes"] / max(v["ms"], 1e-12) / 1e6, 1)}
+                "launches_per_step": kst["launches"]}
+    kernels = {k: {"launches": v["launches"], "ms_timed_step": round(v["ms"], 2),
+                   "GBps": round(v["bytes"] / v["ms"] / 1e6, 1) if v["ms"] > 0 else None}
                for k, v in prof["kernels"].items() if v["launches"]}
 
     cpu = None
@@ -138,7 +143,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {"term_applications_per_step": apps // args.steps, "peak_terms": int(counts.max()),
-                   "final_terms": len(psum), "overlapwithzero": overlap if B == 1 else list(overlap)[:4],
+                   "final_terms": final_terms, "overlapwithzero": overlap if B == 1 else list(overlap)[:4],
                    "gates_touching_terms": kernels.get("merge", {}).get("launches"), "kernels": kernels,
                    "layer_gates": L},
     }

@@ -88,7 +88,7 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle 
 int pp_destroy(pp_handle *h);
 
 /* nqubits, key_words, batch, live terms, capacity (any pointer may be NULL). */
-int pp_info(const pp_handle *h, int *nqubits, int *key_words, int *batch, size_t *n, size_t *capacity);
+int pp_info(pp_handle *h, int *nqubits, int *key_words, int *batch, size_t *n, size_t *capacity);
 
 /* Replace the sum with n distinct terms (any order; duplicates are an error).
  * Replaces: VectorPauliSum(::PauliSum) + cache construction
@@ -157,8 +157,8 @@ typedef struct pp_profile {
     double algorithmic_bytes[PP_KERNEL_KINDS];
 } pp_profile;
 
-/* Turn event timing on (1) or off (0), or leave it (-1); reset != 0 zeroes
- * the counters. No counterpart in the reference; its analogue is the per-gate
+/* Event timing: 0 off, 1 around the merge and noise kernels, 2 around every
+ * kernel, -1 leave as is; reset != 0 zeroes the counters. No counterpart in the reference; its analogue is the per-gate
  * term counter of @countpaulis (src/Base/countterms.jl:41-66). */
 int pp_profile_control(pp_handle *h, int enable_timing, int reset);
 int pp_profile_read(const pp_handle *h, pp_profile *out);

@@ -243,11 +243,19 @@ __device__ __forceinline__ uint32_t lb_tile_id(uint32_t* counter, uint32_t* s_sl
     return *s_slot;
 }
 
+// predecessor words each lane reads per poll of the look-back
+#ifndef PP_LB_WORDS
+#define PP_LB_WORDS 1
+#endif
+
 // Called by all 64 lanes of one wave. Publishes `agg` for `tile` and returns
-// the exclusive prefix of all earlier tiles.
+// the exclusive prefix of all earlier tiles. Each poll reads 64 * PP_LB_WORDS
+// predecessors (nearest first) and stops at the nearest inclusive prefix.
 __device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch, uint64_t agg) {
+    constexpr int NW = PP_LB_WORDS;
     const int lane = lane_id();
-    const uint64_t ep = (uint64_t)(epoch & LB_EPOCH_MASK) << 40;
+    const uint32_t ep32 = epoch & LB_EPOCH_MASK;
+    const uint64_t ep = (uint64_t)ep32 << 40;
     if (tile == 0) {
         if (lane == 0) lb_store(states, (2ull << 62) | ep | agg);
         return 0;
@@ -256,26 +264,43 @@ __device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch
     uint64_t excl = 0;
     int64_t base = (int64_t)tile - 1;
     while (true) {
-        const int64_t idx = base - lane;
-        uint64_t word = 0;
-        uint32_t status = 2;  // below tile 0: never reached past tile 0's prefix
-        uint64_t prefix_mask, need;
+        uint64_t val[NW];
+        uint32_t st[NW];
+        int P, lim;
         while (true) {
-            if (idx >= 0) {
-                word = lb_load(states + idx);
-                status = (((word >> 40) & LB_EPOCH_MASK) == (epoch & LB_EPOCH_MASK)) ? (uint32_t)(word >> 62) : 0u;
+            int first_pref = NW;
+            // all loads first (clamped, unconditional) so that they are in flight together
+            uint64_t w[NW];
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+                const int64_t idx = base - (NW * lane + j);
+                w[j] = lb_load(states + (idx >= 0 ? idx : 0));
             }
-            prefix_mask = __ballot(status == 2);
-            const uint64_t zero_mask = __ballot(status == 0);
-            const int p = prefix_mask ? (__ffsll((unsigned long long)prefix_mask) - 1) : 63;
-            need = (p == 63) ? ~0ull : ((2ull << p) - 1ull);
-            if ((zero_mask & need) == 0) break;
+#pragma unroll
+            for (int j = 0; j < NW; ++j) {
+                const int64_t idx = base - (NW * lane + j);
+                // before tile 0: never needed, tile 0 holds a prefix
+                const uint32_t sj = idx < 0 ? 2u : ((((w[j] >> 40) & LB_EPOCH_MASK) == ep32) ? (uint32_t)(w[j] >> 62) : 0u);
+                st[j] = sj;
+                val[j] = idx < 0 ? 0ull : (w[j] & LB_VAL_MASK);
+                if (sj == 2 && first_pref == NW) first_pref = j;
+            }
+            const uint64_t pm = __ballot(first_pref < NW);
+            P = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
+            const int jp = __shfl(first_pref, P < 64 ? P : 0, 64);
+            lim = (lane < P) ? NW : (lane == P ? jp + 1 : 0);
+            bool ready = true;
+#pragma unroll
+            for (int j = 0; j < NW; ++j) ready = ready && (j >= lim || st[j] != 0u);
+            if (__ballot(!ready) == 0) break;
             __builtin_amdgcn_s_sleep(1);
         }
-        const uint64_t mine = ((need >> lane) & 1ull) && idx >= 0 ? (word & LB_VAL_MASK) : 0ull;
+        uint64_t mine = 0;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) mine += (j < lim) ? val[j] : 0ull;
         excl += wave_sum_u64(mine);
-        if (prefix_mask & need) break;
-        base -= 64;
+        if (P < 64) break;
+        base -= 64 * NW;
     }
     if (lane == 0) lb_store(states + tile, (2ull << 62) | ep | (excl + agg));
     return excl;

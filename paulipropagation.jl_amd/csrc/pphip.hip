@@ -4,9 +4,12 @@
 //   keys[2]    W limb planes of cap uint64 each (limb l of term i at l*cap + i)
 //   coeffs[2]  cap x B doubles, row-major
 //   tail[2]    cap uint32 entries: index of an anticommuting term, sign in bit 31
-//   lb_states  one look-back word per tile of the largest compaction launch
-// Buffer `cur` holds the live sum; each merging gate writes the other buffer
-// and flips `cur`.
+//   states     one look-back word per tile of the largest compaction launch
+//   ctl        the live size and buffer of the sum, double-buffered by gate
+//              parity, plus the overflow record (pp_kernels.h)
+// The host enqueues gates without reading sizes back; it synchronizes once
+// per chunk of gates to catch an overflow, grows the buffers and resumes at
+// the gate that overflowed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,31 +40,37 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr size_t MAX_TERMS = 0x7fffffffu;  // indices carry a sign bit
+constexpr size_t RUN_CHUNK = 256;          // gates enqueued between overflow checks
 
 }  // namespace
 
 struct pp_handle {
     int nqubits = 0, W = 1, B = 1;
-    size_t cap = 0, n = 0;
-    int cur = 0;
+    size_t cap = 0;
     hipStream_t stream = nullptr;
-    uint64_t* keys[2] = {nullptr, nullptr};
-    double* coeffs[2] = {nullptr, nullptr};
-    uint32_t* tail[2] = {nullptr, nullptr};
-    uint64_t* lb_states = nullptr;
+    Bufs bufs{};
     size_t lb_cap = 0;
-    uint32_t* counter = nullptr;   // look-back tile counter
-    uint32_t* d_scalar = nullptr;  // [0] touched count, [1] output count, [2] error bits
-    uint32_t* h_scalar = nullptr;  // pinned mirror
-    double* d_param = nullptr;     // 3 x B: cos, sin / factors of the current gate
-    double* h_param = nullptr;     // pinned staging for d_param
-    double* d_partial = nullptr;   // overlap partials
-    size_t partial_cap = 0;
-    double* d_out = nullptr;       // B results
+    unsigned gmax = 2048;          // persistent grid size
+    Ctl* d_ctl = nullptr;
+    Ctl* h_ctl = nullptr;          // pinned mirror
+    uint32_t seq = 0;              // next gate's sequence number; the sum's state is st[seq & 1]
+    size_t n_est = 0;              // live terms at the last synchronization (sizes the grids)
+    uint32_t lb_count = 0;         // look-back launches so far (counter slot parity)
     uint32_t epoch = 0;
+    // per-run device arrays
+    uint32_t* d_log = nullptr;     // 3 words per gate: n_in, n_touched, n_out
+    uint32_t* h_log = nullptr;
+    size_t log_cap = 0;            // gates
+    double* d_trig = nullptr;      // per gate: B cos, B sin (rotation) or B factors (noise)
+    double* h_trig = nullptr;
+    size_t trig_cap = 0;           // doubles
+    // overlap
+    double* d_partial = nullptr;
+    size_t partial_cap = 0;
+    double* d_out = nullptr;
     // profile
-    bool prof_on = false;
-    hipEvent_t ev[8] = {};
+    int prof_mode = 0;             // 0 off, 1 merge/noise kernels, 2 every kernel
+    std::vector<hipEvent_t> ev;    // 6 per gate slot
     int64_t gates = 0, term_apps = 0;
     int64_t launches[PP_KERNEL_KINDS] = {};
     double ms[PP_KERNEL_KINDS] = {};
@@ -70,18 +79,19 @@ struct pp_handle {
 
 namespace {
 
-int free_buffers(pp_handle* h) {
+void free_terms(pp_handle* h) {
     for (int b = 0; b < 2; ++b) {
-        if (h->keys[b]) hipFree(h->keys[b]);
-        if (h->coeffs[b]) hipFree(h->coeffs[b]);
-        if (h->tail[b]) hipFree(h->tail[b]);
-        h->keys[b] = nullptr;
-        h->coeffs[b] = nullptr;
-        h->tail[b] = nullptr;
+        if (h->bufs.keys[b]) hipFree(h->bufs.keys[b]);
+        if (h->bufs.coeffs[b]) hipFree(h->bufs.coeffs[b]);
+        if (h->bufs.tail[b]) hipFree(h->bufs.tail[b]);
+        h->bufs.keys[b] = nullptr;
+        h->bufs.coeffs[b] = nullptr;
+        h->bufs.tail[b] = nullptr;
     }
-    if (h->lb_states) hipFree(h->lb_states);
-    h->lb_states = nullptr;
-    return PP_OK;
+    if (h->bufs.states) hipFree(h->bufs.states);
+    if (h->bufs.tkeys) hipFree(h->bufs.tkeys);
+    h->bufs.states = nullptr;
+    h->bufs.tkeys = nullptr;
 }
 
 int tile_rows(int B) {
@@ -92,11 +102,40 @@ int tile_rows(int B) {
     return 16;
 }
 
-// look-back words for the largest compaction launch over up to 2*cap entries:
-// the smallest tile is the batched merge's or the batched noise's 64 rows
+// look-back words for the largest compaction launch over up to cap entries
 size_t tiles_needed(size_t cap, int B) {
-    const size_t tile = B > 1 ? (size_t)std::min(64, tile_rows(B)) : 1024;
-    return (2 * cap) / tile + 64;
+    const size_t tile = B > 1 ? (size_t)std::min(KNB_ROWS, tile_rows(B)) : (size_t)K3_TILE;
+    return cap / tile + 64;
+}
+
+int sync_ctl(pp_handle* h) {
+    HIP_TRY(hipMemcpyAsync(h->h_ctl, h->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->n_est = h->h_ctl->st[h->seq & 1].n;
+    return PP_OK;
+}
+
+int put_ctl(pp_handle* h) {
+    HIP_TRY(hipMemcpyAsync(h->d_ctl, h->h_ctl, sizeof(Ctl), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PP_OK;
+}
+
+// live size and buffer of the sum
+int live(pp_handle* h, uint32_t* n, uint32_t* cur) {
+    if (int rc = sync_ctl(h)) return rc;
+    const GateState& s = h->h_ctl->st[h->seq & 1];
+    *n = s.n;
+    *cur = s.cur;
+    return PP_OK;
+}
+
+// Set the sum's state to (n, cur) and clear the overflow record and counters.
+int reset_ctl(pp_handle* h, uint32_t n, uint32_t cur) {
+    std::memset(h->h_ctl, 0, sizeof(Ctl));
+    h->h_ctl->st[h->seq & 1] = GateState{n, cur, 0, 0};
+    h->n_est = n;
+    return put_ctl(h);
 }
 
 // (Re)allocate every term buffer for `newcap` terms, keeping the live sum.
@@ -104,10 +143,15 @@ int grow(pp_handle* h, size_t newcap) {
     if (newcap <= h->cap) return PP_OK;
     if (newcap > MAX_TERMS + 1) newcap = MAX_TERMS + 1;
     const int W = h->W, B = h->B;
+    uint32_t n = 0, cur = 0;
+    if (h->cap) {
+        if (int rc = live(h, &n, &cur)) return rc;
+    }
     uint64_t* nk[2] = {nullptr, nullptr};
     double* nc[2] = {nullptr, nullptr};
     uint32_t* nt[2] = {nullptr, nullptr};
     uint64_t* ns = nullptr;
+    uint64_t* ntk = nullptr;
     const size_t nlb = tiles_needed(newcap, B);
     bool ok = true;
     for (int b = 0; b < 2 && ok; ++b) {
@@ -116,6 +160,7 @@ int grow(pp_handle* h, size_t newcap) {
         ok = ok && hipMalloc(&nt[b], newcap * sizeof(uint32_t)) == hipSuccess;
     }
     ok = ok && hipMalloc(&ns, nlb * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
     if (!ok) {
         for (int b = 0; b < 2; ++b) {
             if (nk[b]) hipFree(nk[b]);
@@ -123,51 +168,91 @@ int grow(pp_handle* h, size_t newcap) {
             if (nt[b]) hipFree(nt[b]);
         }
         if (ns) hipFree(ns);
+        if (ntk) hipFree(ntk);
         (void)hipGetLastError();
         return fail(PP_ERR_NOMEM, "device allocation of " + std::to_string(newcap) + " terms failed");
     }
     HIP_TRY(hipMemsetAsync(ns, 0, nlb * sizeof(uint64_t), h->stream));
-    if (h->n > 0) {
+    if (n > 0) {
         for (int l = 0; l < W; ++l)
-            HIP_TRY(hipMemcpyAsync(nk[0] + l * newcap, h->keys[h->cur] + l * h->cap, h->n * sizeof(uint64_t),
-                                   hipMemcpyDeviceToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(nc[0], h->coeffs[h->cur], h->n * B * sizeof(double), hipMemcpyDeviceToDevice,
+            HIP_TRY(hipMemcpyAsync(nk[0] + (size_t)l * newcap, h->bufs.keys[cur] + (size_t)l * h->cap,
+                                   (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(nc[0], h->bufs.coeffs[cur], (size_t)n * B * sizeof(double), hipMemcpyDeviceToDevice,
                                h->stream));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
-    free_buffers(h);
+    free_terms(h);
     for (int b = 0; b < 2; ++b) {
-        h->keys[b] = nk[b];
-        h->coeffs[b] = nc[b];
-        h->tail[b] = nt[b];
+        h->bufs.keys[b] = nk[b];
+        h->bufs.coeffs[b] = nc[b];
+        h->bufs.tail[b] = nt[b];
     }
-    h->lb_states = ns;
+    h->bufs.states = ns;
+    h->bufs.tkeys = ntk;
+    h->bufs.cap = (uint32_t)newcap;
     h->lb_cap = nlb;
     h->cap = newcap;
-    h->cur = 0;
-    return PP_OK;
+    h->epoch = 0;
+    return reset_ctl(h, n, 0);
 }
 
 uint32_t next_epoch(pp_handle* h) {
     h->epoch = (h->epoch + 1) & LB_EPOCH_MASK;
     if (h->epoch == 0) {
         // wrapped: words of old launches could match again, so clear them
-        hipMemsetAsync(h->lb_states, 0, h->lb_cap * sizeof(uint64_t), h->stream);
+        hipMemsetAsync(h->bufs.states, 0, h->lb_cap * sizeof(uint64_t), h->stream);
         h->epoch = 1;
     }
     return h->epoch;
 }
 
-// copy `count` device scalars from index idx and wait for the stream
-int read_scalars(pp_handle* h, int idx, int count, uint32_t* out) {
-    HIP_TRY(hipMemcpyAsync(h->h_scalar + idx, h->d_scalar + idx, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < count; ++i) out[i] = h->h_scalar[idx + i];
-    return PP_OK;
+Launch lb_launch(pp_handle* h, uint32_t* glog) {
+    Launch L{};
+    L.seq = h->seq;
+    L.epoch = next_epoch(h);
+    L.slot = (h->lb_count++) & 1u;
+    L.glog = glog;
+    return L;
 }
 
-inline unsigned blocks(size_t n, size_t per) { return (unsigned)((n + per - 1) / per); }
+// Grid of a persistent kernel: enough workgroups for `expect` entries at the
+// size seen at the last synchronization, at most what the chip holds at once.
+// Too few only means more tiles per workgroup; every workgroup beyond the
+// work still costs a dispatch.
+unsigned grid_for(pp_handle* h, size_t expect, size_t per_tile) {
+    const size_t tiles = (expect + per_tile - 1) / per_tile;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(tiles, h->gmax));
+}
+
+int ensure_run(pp_handle* h, size_t ngates) {
+    if (ngates > h->log_cap) {
+        if (h->d_log) hipFree(h->d_log);
+        if (h->h_log) hipHostFree(h->h_log);
+        h->d_log = nullptr;
+        h->h_log = nullptr;
+        const size_t c = std::max<size_t>(ngates, 1024);
+        HIP_TRY(hipMalloc(&h->d_log, 3 * c * sizeof(uint32_t)));
+        HIP_TRY(hipHostMalloc(&h->h_log, 3 * c * sizeof(uint32_t), hipHostMallocDefault));
+        h->log_cap = c;
+    }
+    const size_t need = ngates * 2 * (size_t)h->B;
+    if (h->B > 1 && need > h->trig_cap) {
+        if (h->d_trig) hipFree(h->d_trig);
+        if (h->h_trig) hipHostFree(h->h_trig);
+        h->d_trig = nullptr;
+        h->h_trig = nullptr;
+        const size_t c = std::max<size_t>(need, 4096);
+        HIP_TRY(hipMalloc(&h->d_trig, c * sizeof(double)));
+        HIP_TRY(hipHostMalloc(&h->h_trig, c * sizeof(double), hipHostMallocDefault));
+        h->trig_cap = c;
+    }
+    if (h->prof_mode && h->ev.size() < 6 * ngates) {
+        const size_t old = h->ev.size();
+        h->ev.resize(6 * ngates, nullptr);
+        for (size_t e = old; e < h->ev.size(); ++e) HIP_TRY(hipEventCreate(&h->ev[e]));
+    }
+    return PP_OK;
+}
 
 template <int W>
 Key<W> to_key(const uint64_t* mask) {
@@ -179,9 +264,9 @@ Key<W> to_key(const uint64_t* mask) {
 // Levels of the tail permutation for mask m: every maximal run [lo, hi] of set
 // bits contributes +(s+e) at level hi+1 and -(s+e) at level lo; a level at or
 // above 64W spans the whole tail, so s+e = na there.
-int mask_levels(const uint64_t* m, int W, uint32_t na, Levels* lv, int64_t* const_off, int* from_limb) {
+int mask_levels(const uint64_t* m, int W, Levels* lv) {
     lv->n = 0;
-    *const_off = 0;
+    lv->top_runs = 0;
     int minbit = 64 * W;
     int b = 64 * W - 1;
     while (b >= 0) {
@@ -190,7 +275,7 @@ int mask_levels(const uint64_t* m, int W, uint32_t na, Levels* lv, int64_t* cons
             while (b >= 0 && ((m[b >> 6] >> (b & 63)) & 1ull)) --b;
             const int lo = b + 1;
             if (hi + 1 >= 64 * W) {
-                *const_off += (int64_t)na;
+                lv->top_runs += 1;
             } else {
                 if (lv->n >= MAX_LEVELS) return -1;
                 lv->bit[lv->n] = hi + 1;
@@ -206,112 +291,70 @@ int mask_levels(const uint64_t* m, int W, uint32_t na, Levels* lv, int64_t* cons
             --b;
         }
     }
-    *from_limb = minbit >> 6;
+    lv->from_limb = minbit >> 6;
     return 0;
 }
 
 int check_mask(const pp_handle* h, const uint64_t* mask) {
     bool any = false;
-    for (int l = 0; l < h->W; ++l) {
-        uint64_t m = mask[l];
+    const int qmax = 2 * h->nqubits;
+    for (int l = 0; l < 4; ++l) {
+        const uint64_t m = mask[l];
         any = any || m != 0;
-        const int lo = l * 64, hi = lo + 64;
-        const int qmax = 2 * h->nqubits;
-        if (qmax < hi) {
+        if (l >= h->W) {
+            if (m) return fail(PP_ERR_ARG, "rotation mask acts on a qubit beyond the key width");
+            continue;
+        }
+        const int lo = l * 64;
+        if (qmax < lo + 64) {
             const int keep = qmax - lo;
-            const uint64_t allowed = keep <= 0 ? 0ull : (keep >= 64 ? ~0ull : ((1ull << keep) - 1));
+            const uint64_t allowed = keep <= 0 ? 0ull : ((1ull << keep) - 1);
             if (m & ~allowed) return fail(PP_ERR_ARG, "rotation mask acts on a qubit beyond nqubits");
         }
     }
-    for (int l = h->W; l < 4; ++l)
-        if (mask[l]) return fail(PP_ERR_ARG, "rotation mask acts on a qubit beyond the key width");
     if (!any) return fail(PP_ERR_ARG, "rotation mask is the identity");
+    Levels lv;
+    if (mask_levels(mask, h->W, &lv))
+        return fail(PP_ERR_UNSUPPORTED, "rotation generator with more than 8 runs of set bits");
     return PP_OK;
 }
 
-// ---------------------------------------------------------------- profile
+// ---------------------------------------------------------------- one gate, enqueued
 
-void prof_mark(pp_handle* h, int ev) {
-    if (h->prof_on) hipEventRecord(h->ev[ev], h->stream);
-}
-
-// after a stream synchronization: the events of slot (2*slot, 2*slot+1) are complete
-void prof_add(pp_handle* h, int slot, int kind, double bytes) {
-    h->launches[kind] += 1;
-    h->alg_bytes[kind] += bytes;
-    if (h->prof_on) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, h->ev[2 * slot], h->ev[2 * slot + 1]) == hipSuccess) h->ms[kind] += ms;
-    }
-}
-
+// Rotation: cos/sin by value for B = 1, or at d_trig (B cos then B sin) for B > 1.
 template <int W>
-int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, const double* sin_b, int max_weight,
-                  double min_abs, pp_gate_stats* st) {
-    const size_t n = h->n;
-    if (st) *st = pp_gate_stats{(int64_t)n, 0, (int64_t)n};
-    h->gates += 1;
-    h->term_apps += (int64_t)n;
-    if (n == 0) return PP_OK;
+void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
+                  double min_abs, uint32_t* glog, hipEvent_t* ev) {
     const Key<W> mask = to_key<W>(mask_in);
-    int mask_limbs = 0;
-    for (int l = 0; l < W; ++l) mask_limbs += mask_in[l] != 0;
-    const double row = 8.0 * W + 8.0 * h->B;
-    // K1: anticommuting terms, in order
-    prof_mark(h, 0);
-    hipLaunchKernelGGL(k_branch<W>, dim3(blocks(n, K1_TILE)), dim3(WG), 0, h->stream, h->keys[h->cur], h->cap,
-                       (uint32_t)n, mask, h->tail[0], h->lb_states, h->counter, next_epoch(h), h->d_scalar);
-    HIP_TRY(hipGetLastError());
-    prof_mark(h, 1);
-    uint32_t na = 0;
-    if (int rc = read_scalars(h, 0, 1, &na)) return rc;
-    prof_add(h, 0, PP_KERNEL_BRANCH, 8.0 * mask_limbs * n + 4.0 * na);
-    if (st) st->n_touched = na;
-    if (na == 0) return PP_OK;  // nothing touched: nothing merged or truncated
-    if (n + na > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
-    if (n + na > h->cap) {
-        // the new buffers do not carry the tail over: branch again into them
-        if (int rc = grow(h, std::max(n + na, h->cap + h->cap / 2))) return rc;
-        hipLaunchKernelGGL(k_branch<W>, dim3(blocks(n, K1_TILE)), dim3(WG), 0, h->stream, h->keys[h->cur],
-                           h->cap, (uint32_t)n, mask, h->tail[0], h->lb_states, h->counter, next_epoch(h),
-                           h->d_scalar);
-        HIP_TRY(hipGetLastError());
-    }
-    const int c = h->cur;
-    // K2: tail into ascending order of term ^ mask
     Levels lv;
-    int64_t const_off;
-    int from_limb;
-    if (mask_levels(mask_in, W, na, &lv, &const_off, &from_limb))
-        return fail(PP_ERR_UNSUPPORTED, "rotation generator with more than 8 runs of set bits (16 levels)");
-    prof_mark(h, 2);
-    hipLaunchKernelGGL(k_xorperm<W>, dim3(blocks(na, K2_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
-                       h->tail[0], na, lv, const_off, from_limb, h->tail[1], h->d_scalar + 2);
-    HIP_TRY(hipGetLastError());
-    prof_mark(h, 3);
-    // K3: merge with truncation into the other buffer
-    const size_t ntot = n + na;
+    mask_levels(mask_in, W, &lv);
+    const bool all = h->prof_mode == 2 && ev;
+    const bool mrg = h->prof_mode >= 1 && ev;
+    if (all) hipEventRecord(ev[0], h->stream);
+    hipLaunchKernelGGL(k_branch<W>, dim3(grid_for(h, h->n_est, K1_TILE)), dim3(WG), 0, h->stream, h->bufs,
+                       h->d_ctl, lb_launch(h, glog), mask);
+    if (all) hipEventRecord(ev[1], h->stream);
+    Launch L2{};
+    L2.seq = h->seq;
+    L2.glog = glog;
+    if (all) hipEventRecord(ev[2], h->stream);
+    hipLaunchKernelGGL(k_xorperm<W>, dim3(grid_for(h, h->n_est / 2, K2_TILE)), dim3(WG), 0, h->stream, h->bufs,
+                       h->d_ctl, L2, lv, mask);
+    if (all) hipEventRecord(ev[3], h->stream);
+    if (mrg) hipEventRecord(ev[4], h->stream);
     if (h->B == 1) {
-        RotParams rp{cos_b[0], sin_b[0], max_weight, min_abs};
-        prof_mark(h, 4);
-        hipLaunchKernelGGL(k_merge1<W>, dim3(blocks(ntot, K3_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
-                           h->coeffs[c], (uint32_t)n, h->tail[1], na, mask, rp, h->keys[1 - c], h->cap,
-                           h->coeffs[1 - c], h->lb_states, h->counter, next_epoch(h), h->d_scalar + 1,
-                           h->d_scalar + 2);
+        RotParams rp{c1, s1, max_weight, min_abs};
+        hipLaunchKernelGGL(k_merge1<W>, dim3(grid_for(h, h->n_est + h->n_est / 2, K3_TILE)), dim3(WG), 0,
+                           h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp);
     } else {
         const int B = h->B;
-        std::memcpy(h->h_param, cos_b, B * sizeof(double));
-        std::memcpy(h->h_param + B, sin_b, B * sizeof(double));
-        HIP_TRY(hipMemcpyAsync(h->d_param, h->h_param, 2 * B * sizeof(double), hipMemcpyHostToDevice, h->stream));
         const int TB = tile_rows(B);
         const size_t shm = (size_t)TB * B * sizeof(double);
-        const unsigned g = blocks(ntot, TB);
-        prof_mark(h, 4);
-#define LAUNCH_MB(TBV)                                                                                          \
-    hipLaunchKernelGGL((k_mergeB<W, TBV>), dim3(g), dim3(WG), shm, h->stream, h->keys[c], h->cap, h->coeffs[c], \
-                       (uint32_t)n, B, h->tail[1], na, mask, h->d_param, h->d_param + B, max_weight, min_abs,   \
-                       h->keys[1 - c], h->cap, h->coeffs[1 - c], h->lb_states, h->counter, next_epoch(h),       \
-                       h->d_scalar + 1, h->d_scalar + 2)
+        const unsigned g = grid_for(h, h->n_est + h->n_est / 2, TB);
+        const Launch L3 = lb_launch(h, glog);
+#define LAUNCH_MB(TBV)                                                                                         \
+    hipLaunchKernelGGL((k_mergeB<W, TBV>), dim3(g), dim3(WG), shm, h->stream, h->bufs, h->d_ctl, L3, mask, B, \
+                       d_trig, max_weight, min_abs)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -321,71 +364,133 @@ int rotation_impl(pp_handle* h, const uint64_t* mask_in, const double* cos_b, co
         }
 #undef LAUNCH_MB
     }
-    HIP_TRY(hipGetLastError());
-    prof_mark(h, 5);
-    uint32_t res[2] = {0, 0};  // output count, error bits
-    if (int rc = read_scalars(h, 1, 2, res)) return rc;
-    if (res[1]) {
-        HIP_TRY(hipMemsetAsync(h->d_scalar + 2, 0, sizeof(uint32_t), h->stream));
-        return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
-                                      std::to_string(res[1]) + ")");
-    }
-    const uint32_t nout = res[0];
-    prof_add(h, 1, PP_KERNEL_TAIL_PERMUTE, na * (8.0 + 8.0 * (W - from_limb)));
-    prof_add(h, 2, PP_KERNEL_MERGE, row * ((double)n + (double)nout));
-    h->cur = 1 - c;
-    h->n = nout;
-    if (st) st->n_out = nout;
-    return PP_OK;
+    if (mrg) hipEventRecord(ev[5], h->stream);
+    h->seq += 1;
 }
 
 template <int W>
-int noise_impl(pp_handle* h, int qubit, uint32_t damp_codes, const double* factor_b, int max_weight,
-               double min_abs, pp_gate_stats* st) {
-    const size_t n = h->n;
-    if (st) *st = pp_gate_stats{(int64_t)n, (int64_t)n, (int64_t)n};
-    h->gates += 1;
-    h->term_apps += (int64_t)n;
-    if (n == 0) return PP_OK;
-    const int c = h->cur;
+void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const double* d_factor, int max_weight,
+               double min_abs, uint32_t* glog, hipEvent_t* ev) {
+    const bool mrg = h->prof_mode >= 1 && ev;
+    if (mrg) hipEventRecord(ev[4], h->stream);
     if (h->B == 1) {
-        prof_mark(h, 4);
-        hipLaunchKernelGGL(k_noise1<W>, dim3(blocks(n, KN_TILE)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
-                           h->coeffs[c], (uint32_t)n, qubit, damp_codes, factor_b[0], max_weight, min_abs,
-                           h->keys[1 - c], h->cap, h->coeffs[1 - c], h->lb_states, h->counter, next_epoch(h),
-                           h->d_scalar + 1);
+        hipLaunchKernelGGL(k_noise1<W>, dim3(grid_for(h, h->n_est, KN_TILE)), dim3(WG), 0, h->stream, h->bufs,
+                           h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
     } else {
-        const int B = h->B;
-        std::memcpy(h->h_param + 2 * B, factor_b, B * sizeof(double));
-        HIP_TRY(hipMemcpyAsync(h->d_param + 2 * B, h->h_param + 2 * B, B * sizeof(double), hipMemcpyHostToDevice,
-                               h->stream));
-        prof_mark(h, 4);
-        hipLaunchKernelGGL(k_noiseB<W>, dim3(blocks(n, 64)), dim3(WG), 0, h->stream, h->keys[c], h->cap,
-                           h->coeffs[c], (uint32_t)n, B, qubit, damp_codes, h->d_param + 2 * B, max_weight,
-                           min_abs, h->keys[1 - c], h->cap, h->coeffs[1 - c], h->lb_states, h->counter,
-                           next_epoch(h), h->d_scalar + 1);
+        hipLaunchKernelGGL(k_noiseB<W>, dim3(grid_for(h, h->n_est, KNB_ROWS)), dim3(WG), 0, h->stream, h->bufs,
+                           h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
+    }
+    if (mrg) hipEventRecord(ev[5], h->stream);
+    h->seq += 1;
+}
+
+// A gate of a run, resolved on the host: what enq_* needs.
+struct HostGate {
+    int kind;              // PP_GATE_ROTATION or PP_GATE_PAULI_NOISE
+    const uint64_t* mask;
+    int qubit;
+    uint32_t damp_codes;
+    double a, b;           // B = 1: cos, sin (rotation) or factor (noise)
+};
+
+int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs) {
+    uint32_t* glog = h->d_log + 3 * slot;
+    hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
+    const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
+    if (g.kind == PP_GATE_ROTATION) {
+        switch (h->W) {
+        case 1: enq_rotation<1>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        case 2: enq_rotation<2>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        default: enq_rotation<4>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        }
+    } else {
+        switch (h->W) {
+        case 1: enq_noise<1>(h, g.qubit, g.damp_codes, g.a, dt, max_weight, min_abs, glog, ev); break;
+        case 2: enq_noise<2>(h, g.qubit, g.damp_codes, g.a, dt, max_weight, min_abs, glog, ev); break;
+        default: enq_noise<4>(h, g.qubit, g.damp_codes, g.a, dt, max_weight, min_abs, glog, ev); break;
+        }
     }
     HIP_TRY(hipGetLastError());
-    prof_mark(h, 5);
-    uint32_t nout = 0;
-    if (int rc = read_scalars(h, 1, 1, &nout)) return rc;
-    prof_add(h, 2, PP_KERNEL_NOISE, (8.0 * h->W + 8.0 * h->B) * ((double)n + (double)nout));
-    h->cur = 1 - c;
-    h->n = nout;
-    if (st) st->n_out = nout;
+    return PP_OK;
+}
+
+// Run resolved gates (trig already staged at d_trig for B > 1), resuming
+// after an overflow; then account the gate log into the profile.
+int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, double min_abs, int64_t* counts) {
+    const size_t ng = gs.size();
+    size_t g = 0;
+    while (g < ng) {
+        const size_t end = std::min(ng, g + RUN_CHUNK);
+        const uint32_t seq0 = h->seq;
+        for (size_t i = g; i < end; ++i)
+            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs)) return rc;
+        if (int rc = sync_ctl(h)) return rc;
+        if (h->h_ctl->err)
+            return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
+                                          std::to_string(h->h_ctl->err) + ")");
+        if (h->h_ctl->overflow) {
+            const uint32_t fseq = h->h_ctl->overflow_seq;
+            const size_t gi = g + (size_t)(fseq - seq0);
+            const size_t need = h->h_ctl->overflow_need;
+            if (need > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
+            h->seq = fseq;  // the state before the failed gate is still in st[fseq & 1]
+            if (int rc = grow(h, std::max(need + need / 4, h->cap + h->cap / 2))) return rc;
+            g = gi;
+            continue;
+        }
+        g = end;
+    }
+    if (ng == 0) return PP_OK;
+    HIP_TRY(hipMemcpyAsync(h->h_log, h->d_log, 3 * ng * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const double row = 8.0 * h->W + 8.0 * h->B;
+    for (size_t i = 0; i < ng; ++i) {
+        const uint32_t nin = h->h_log[3 * i], na = h->h_log[3 * i + 1], nout = h->h_log[3 * i + 2];
+        if (counts) counts[i] = nin;
+        h->gates += 1;
+        h->term_apps += nin;
+        hipEvent_t* ev = h->prof_mode ? &h->ev[6 * i] : nullptr;
+        auto span = [&](int a) -> double {
+            float ms = 0.f;
+            if (ev && hipEventElapsedTime(&ms, ev[a], ev[a + 1]) == hipSuccess) return ms;
+            return 0.0;
+        };
+        if (gs[i].kind == PP_GATE_ROTATION) {
+            int mask_limbs = 0;
+            for (int l = 0; l < h->W; ++l) mask_limbs += gs[i].mask[l] != 0;
+            h->launches[PP_KERNEL_BRANCH] += 1;
+            h->alg_bytes[PP_KERNEL_BRANCH] += 8.0 * mask_limbs * nin + 4.0 * na;
+            if (h->prof_mode == 2) h->ms[PP_KERNEL_BRANCH] += span(0);
+            if (na > 0) {
+                Levels lv;
+                mask_levels(gs[i].mask, h->W, &lv);
+                h->launches[PP_KERNEL_TAIL_PERMUTE] += 1;
+                h->alg_bytes[PP_KERNEL_TAIL_PERMUTE] += na * (8.0 + 8.0 * (h->W - lv.from_limb));
+                h->launches[PP_KERNEL_MERGE] += 1;
+                h->alg_bytes[PP_KERNEL_MERGE] += row * ((double)nin + (double)nout);
+                if (h->prof_mode == 2) h->ms[PP_KERNEL_TAIL_PERMUTE] += span(2);
+                if (h->prof_mode >= 1) h->ms[PP_KERNEL_MERGE] += span(4);
+            }
+        } else {
+            h->launches[PP_KERNEL_NOISE] += 1;
+            h->alg_bytes[PP_KERNEL_NOISE] += row * ((double)nin + (double)nout);
+            if (h->prof_mode >= 1) h->ms[PP_KERNEL_NOISE] += span(4);
+        }
+    }
     return PP_OK;
 }
 
 template <int W>
 int overlap_impl(pp_handle* h, double* out) {
     const int B = h->B;
-    const size_t n = h->n;
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
     if (n == 0) {
         for (int b = 0; b < B; ++b) out[b] = 0.0;
         return PP_OK;
     }
-    const int c = h->cur;
-    const unsigned g = std::min<unsigned>(2048u, blocks(n, B == 1 ? WG : NWAVES));
+    const unsigned per = B == 1 ? WG : NWAVES;
+    const unsigned g = std::min<unsigned>(2048u, (n + per - 1) / per);
     const size_t need = (size_t)g * B * 2;
     if (need > h->partial_cap) {
         if (h->d_partial) hipFree(h->d_partial);
@@ -393,21 +498,19 @@ int overlap_impl(pp_handle* h, double* out) {
         HIP_TRY(hipMalloc(&h->d_partial, need * sizeof(double)));
         h->partial_cap = need;
     }
-    prof_mark(h, 6);
     if (B == 1) {
-        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->keys[c], h->cap, h->coeffs[c],
-                           (uint32_t)n, h->d_partial);
+        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, h->d_partial);
     } else {
         hipLaunchKernelGGL(k_overlapB<W>, dim3(g), dim3(WG), (size_t)NWAVES * B * 2 * sizeof(double), h->stream,
-                           h->keys[c], h->cap, h->coeffs[c], (uint32_t)n, B, h->d_partial);
+                           h->bufs, h->d_ctl, h->seq, B, h->d_partial);
     }
     HIP_TRY(hipGetLastError());
-    prof_mark(h, 7);
     hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, h->d_partial, (int)g, B, h->d_out);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, h->d_out, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    prof_add(h, 3, PP_KERNEL_OVERLAP, (8.0 * h->W + 8.0 * B) * (double)n);
+    h->launches[PP_KERNEL_OVERLAP] += 1;
+    h->alg_bytes[PP_KERNEL_OVERLAP] += (8.0 * h->W + 8.0 * B) * (double)n;
     return PP_OK;
 }
 
@@ -417,13 +520,32 @@ int key_cmp(const uint64_t* a, const uint64_t* b, int W) {
     return 0;
 }
 
+// one gate with host values, synchronously, with its statistics
+int single_gate(pp_handle* h, const HostGate& g, const double* hv1, const double* hv2, int max_weight,
+                double min_abs, pp_gate_stats* st) {
+    if (int rc = ensure_run(h, 1)) return rc;
+    if (h->B > 1) {
+        std::memcpy(h->h_trig, hv1, h->B * sizeof(double));
+        if (hv2) std::memcpy(h->h_trig + h->B, hv2, h->B * sizeof(double));
+        HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, 2 * h->B * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    std::vector<HostGate> gs(1, g);
+    if (int rc = run_resolved(h, gs, max_weight, min_abs, nullptr)) return rc;
+    if (st) {
+        st->n_in = h->h_log[0];
+        st->n_touched = h->h_log[1];
+        st->n_out = h->h_log[2];
+    }
+    return PP_OK;
+}
+
 }  // namespace
 
 // ============================================================ C ABI
 
 extern "C" {
 
-int pp_version(void) { return 100; }
+int pp_version(void) { return 200; }
 
 const char* pp_last_error(void) { return g_err.c_str(); }
 
@@ -449,18 +571,16 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
         pp_destroy(h);
         return rc;
     };
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    h->gmax = (unsigned)std::max(64, 4 * cus);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PP_ERR_HIP, "hipStreamCreate failed"));
-    if (hipMalloc(&h->counter, 4 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&h->d_scalar, 4 * sizeof(uint32_t)) != hipSuccess ||
-        hipHostMalloc(&h->h_scalar, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&h->d_param, 3 * batch * sizeof(double)) != hipSuccess ||
-        hipHostMalloc(&h->h_param, 3 * batch * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+    if (hipMalloc(&h->d_ctl, sizeof(Ctl)) != hipSuccess ||
+        hipHostMalloc(&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess)
         return bail(fail(PP_ERR_NOMEM, "device allocation failed"));
-    if (hipMemsetAsync(h->counter, 0, 4 * sizeof(uint32_t), h->stream) != hipSuccess ||
-        hipMemsetAsync(h->d_scalar, 0, 4 * sizeof(uint32_t), h->stream) != hipSuccess)
-        return bail(fail(PP_ERR_HIP, "hipMemset failed"));
+    if (int rc = reset_ctl(h, 0, 0)) return bail(rc);
     if (int rc = grow(h, std::max<size_t>(capacity, 1024))) return bail(rc);
     *out = h;
     return PP_OK;
@@ -469,14 +589,15 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
 int pp_destroy(pp_handle* h) {
     if (!h) return PP_OK;
     if (h->stream) hipStreamSynchronize(h->stream);
-    for (int e = 0; e < 8; ++e)
-        if (h->ev[e]) hipEventDestroy(h->ev[e]);
-    free_buffers(h);
-    if (h->counter) hipFree(h->counter);
-    if (h->d_scalar) hipFree(h->d_scalar);
-    if (h->h_scalar) hipHostFree(h->h_scalar);
-    if (h->d_param) hipFree(h->d_param);
-    if (h->h_param) hipHostFree(h->h_param);
+    for (hipEvent_t e : h->ev)
+        if (e) hipEventDestroy(e);
+    free_terms(h);
+    if (h->d_ctl) hipFree(h->d_ctl);
+    if (h->h_ctl) hipHostFree(h->h_ctl);
+    if (h->d_log) hipFree(h->d_log);
+    if (h->h_log) hipHostFree(h->h_log);
+    if (h->d_trig) hipFree(h->d_trig);
+    if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
     if (h->d_out) hipFree(h->d_out);
     if (h->stream) hipStreamDestroy(h->stream);
@@ -484,13 +605,17 @@ int pp_destroy(pp_handle* h) {
     return PP_OK;
 }
 
-int pp_info(const pp_handle* h, int* nqubits, int* key_words, int* batch, size_t* n, size_t* capacity) {
+int pp_info(pp_handle* h, int* nqubits, int* key_words, int* batch, size_t* n, size_t* capacity) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (nqubits) *nqubits = h->nqubits;
     if (key_words) *key_words = h->W;
     if (batch) *batch = h->B;
-    if (n) *n = h->n;
     if (capacity) *capacity = h->cap;
+    if (n) {
+        uint32_t nn = 0, cur = 0;
+        if (int rc = live(h, &nn, &cur)) return rc;
+        *n = nn;
+    }
     return PP_OK;
 }
 
@@ -514,40 +639,40 @@ int pp_upload(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n
     for (size_t i = 1; i < n; ++i)
         if (key_cmp(keys + idx[i] * W, keys + idx[i - 1] * W, W) == 0)
             return fail(PP_ERR_ARG, "duplicate Pauli string in upload");
-    if (n > h->cap) {
-        h->n = 0;
+    if (int rc = reset_ctl(h, 0, 0)) return rc;
+    if (n > h->cap)
         if (int rc = grow(h, n)) return rc;
-    }
     std::vector<uint64_t> planes((size_t)W * std::max<size_t>(n, 1));
     std::vector<double> rows((size_t)B * std::max<size_t>(n, 1));
     for (size_t i = 0; i < n; ++i) {
         for (int l = 0; l < W; ++l) planes[(size_t)l * n + i] = keys[idx[i] * W + l];
         std::memcpy(&rows[i * B], coeffs + idx[i] * B, B * sizeof(double));
     }
-    h->cur = 0;
     for (int l = 0; l < W && n > 0; ++l)
-        HIP_TRY(hipMemcpyAsync(h->keys[0] + (size_t)l * h->cap, &planes[(size_t)l * n], n * sizeof(uint64_t),
+        HIP_TRY(hipMemcpyAsync(h->bufs.keys[0] + (size_t)l * h->cap, &planes[(size_t)l * n], n * sizeof(uint64_t),
                                hipMemcpyHostToDevice, h->stream));
     if (n > 0)
-        HIP_TRY(hipMemcpyAsync(h->coeffs[0], rows.data(), n * B * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->bufs.coeffs[0], rows.data(), n * B * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->n = n;
-    return PP_OK;
+    return reset_ctl(h, (uint32_t)n, 0);
 }
 
 int pp_download(pp_handle* h, uint64_t* keys, double* coeffs, size_t cap, size_t* n) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
-    if (n) *n = h->n;
-    if (cap < h->n) return fail(PP_ERR_ARG, "output arrays too small");
-    const size_t m = h->n;
+    uint32_t m = 0, cur = 0;
+    if (int rc = live(h, &m, &cur)) return rc;
+    if (n) *n = m;
+    if (cap < m) return fail(PP_ERR_ARG, "output arrays too small");
     const int W = h->W, B = h->B;
     if (m == 0) return PP_OK;
     std::vector<uint64_t> planes((size_t)W * m);
     for (int l = 0; l < W; ++l)
-        HIP_TRY(hipMemcpyAsync(&planes[(size_t)l * m], h->keys[h->cur] + (size_t)l * h->cap, m * sizeof(uint64_t),
+        HIP_TRY(hipMemcpyAsync(&planes[(size_t)l * m], h->bufs.keys[cur] + (size_t)l * h->cap, m * sizeof(uint64_t),
                                hipMemcpyDeviceToHost, h->stream));
     if (coeffs)
-        HIP_TRY(hipMemcpyAsync(coeffs, h->coeffs[h->cur], m * B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipMemcpyAsync(coeffs, h->bufs.coeffs[cur], (size_t)m * B * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (keys)
         for (size_t i = 0; i < m; ++i)
@@ -561,11 +686,8 @@ int pp_apply_rotation(pp_handle* h, const uint64_t* mask, const double* cos_b, c
     uint64_t m4[4] = {0, 0, 0, 0};
     for (int l = 0; l < h->W; ++l) m4[l] = mask[l];
     if (int rc = check_mask(h, m4)) return rc;
-    switch (h->W) {
-    case 1: return rotation_impl<1>(h, m4, cos_b, sin_b, max_weight, min_abs_coeff, stats);
-    case 2: return rotation_impl<2>(h, m4, cos_b, sin_b, max_weight, min_abs_coeff, stats);
-    default: return rotation_impl<4>(h, m4, cos_b, sin_b, max_weight, min_abs_coeff, stats);
-    }
+    HostGate g{PP_GATE_ROTATION, m4, 0, 0, cos_b[0], sin_b[0]};
+    return single_gate(h, g, cos_b, sin_b, max_weight, min_abs_coeff, stats);
 }
 
 int pp_apply_pauli_noise(pp_handle* h, int qubit, uint32_t damp_codes, const double* factor_b, int max_weight,
@@ -573,11 +695,8 @@ int pp_apply_pauli_noise(pp_handle* h, int qubit, uint32_t damp_codes, const dou
     if (!h || !factor_b) return fail(PP_ERR_ARG, "NULL argument");
     if (qubit < 0 || qubit >= h->nqubits) return fail(PP_ERR_ARG, "noise qubit out of range");
     if (damp_codes & ~0xeu) return fail(PP_ERR_ARG, "damp_codes may only hold X, Y, Z (bits 1..3)");
-    switch (h->W) {
-    case 1: return noise_impl<1>(h, qubit, damp_codes, factor_b, max_weight, min_abs_coeff, stats);
-    case 2: return noise_impl<2>(h, qubit, damp_codes, factor_b, max_weight, min_abs_coeff, stats);
-    default: return noise_impl<4>(h, qubit, damp_codes, factor_b, max_weight, min_abs_coeff, stats);
-    }
+    HostGate g{PP_GATE_PAULI_NOISE, nullptr, qubit, damp_codes, factor_b[0], 0.0};
+    return single_gate(h, g, factor_b, nullptr, max_weight, min_abs_coeff, stats);
 }
 
 int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
@@ -585,40 +704,51 @@ int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const doub
                    double min_abs_coeff, int64_t* counts) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (ngates > 0 && !gates) return fail(PP_ERR_ARG, "gates is NULL");
+    (void)lut_changes;
+    (void)lut_signs;
+    (void)nluts;
     const int B = h->B;
-    std::vector<double> a(B), s(B);
+    // validate and resolve every gate before anything runs
+    std::vector<HostGate> gs(ngates);
+    if (int rc = ensure_run(h, ngates)) return rc;
     for (size_t g = 0; g < ngates; ++g) {
         const pp_gate& gt = gates[g];
-        if (counts) counts[g] = (int64_t)h->n;
         if (gt.param >= 0 && ((size_t)gt.param >= nparams || !theta))
             return fail(PP_ERR_ARG, "gate " + std::to_string(g) + " reads a parameter beyond nparams");
         auto val = [&](int b) { return gt.param < 0 ? gt.value : theta[(size_t)gt.param * B + b]; };
-        int rc = PP_OK;
-        switch (gt.kind) {
-        case PP_GATE_ROTATION:
-            for (int b = 0; b < B; ++b) {
-                a[b] = std::cos(val(b));
-                s[b] = std::sin(val(b));
-            }
-            rc = pp_apply_rotation(h, gt.mask, a.data(), s.data(), max_weight, min_abs_coeff, nullptr);
-            break;
-        case PP_GATE_PAULI_NOISE:
+        HostGate& hg = gs[g];
+        hg.kind = gt.kind;
+        if (gt.kind == PP_GATE_ROTATION) {
+            if (int rc = check_mask(h, gt.mask)) return rc;
+            hg.mask = gt.mask;
+            // cos and sin as two separate libm calls, as the reference evaluates them
+            // (src/Propagation/specializations.jl:30)
+            hg.a = std::cos(val(0));
+            hg.b = std::sin(val(0));
+            if (B > 1)
+                for (int b = 0; b < B; ++b) {
+                    h->h_trig[g * 2 * B + b] = std::cos(val(b));
+                    h->h_trig[g * 2 * B + B + b] = std::sin(val(b));
+                }
+        } else if (gt.kind == PP_GATE_PAULI_NOISE) {
+            if (gt.qubit < 0 || gt.qubit >= h->nqubits) return fail(PP_ERR_ARG, "noise qubit out of range");
+            if (gt.damp_codes & ~0xeu) return fail(PP_ERR_ARG, "damp_codes may only hold X, Y, Z (bits 1..3)");
+            hg.qubit = gt.qubit;
+            hg.damp_codes = gt.damp_codes;
             for (int b = 0; b < B; ++b) {
                 const double lam = val(b);
                 if (!(lam >= 0.0 && lam <= 1.0)) return fail(PP_ERR_ARG, "noise strength must be in [0, 1]");
-                a[b] = 1 - lam;
+                if (B > 1) h->h_trig[g * 2 * B + b] = 1 - lam;
             }
-            rc = pp_apply_pauli_noise(h, gt.qubit, gt.damp_codes, a.data(), max_weight, min_abs_coeff, nullptr);
-            break;
-        default:
-            (void)lut_changes;
-            (void)lut_signs;
-            (void)nluts;
+            hg.a = 1 - val(0);
+        } else {
             return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
         }
-        if (rc) return rc;
     }
-    return PP_OK;
+    if (B > 1 && ngates > 0)
+        HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, ngates * 2 * B * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+    return run_resolved(h, gs, max_weight, min_abs_coeff, counts);
 }
 
 int pp_overlap_zero(pp_handle* h, double* out_b) {
@@ -638,10 +768,7 @@ int pp_synchronize(pp_handle* h) {
 
 int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
-    if (enable_timing == 1 && !h->ev[0]) {
-        for (int e = 0; e < 8; ++e) HIP_TRY(hipEventCreate(&h->ev[e]));
-    }
-    if (enable_timing >= 0) h->prof_on = enable_timing == 1;
+    if (enable_timing >= 0) h->prof_mode = std::min(enable_timing, 2);
     if (reset) {
         h->gates = h->term_apps = 0;
         for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
