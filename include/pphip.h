@@ -152,6 +152,7 @@ enum pp_kernel_kind {
 typedef struct pp_profile {
     int64_t gates;
     int64_t term_applications;
+    int64_t fallbacks;           /* gates run again with atomic tile claims (see pp_tune) */
     int64_t launches[PP_KERNEL_KINDS];
     double milliseconds[PP_KERNEL_KINDS];
     double algorithmic_bytes[PP_KERNEL_KINDS];
@@ -162,6 +163,17 @@ typedef struct pp_profile {
  * term counter of @countpaulis (src/Base/countterms.jl:41-66). */
 int pp_profile_control(pp_handle *h, int enable_timing, int reset);
 int pp_profile_read(const pp_handle *h, pp_profile *out);
+
+/* Tuning keys of pp_tune. The compaction kernels hand out tiles statically,
+ * which needs every workgroup resident at once; the host sizes grids from the
+ * kernels' occupancy. Every look-back wait is bounded by PP_TUNE_SPIN_LIMIT
+ * polls; a gate whose wait runs out is run again with tiles claimed through an
+ * atomic counter, which needs no residency (counted in pp_profile.fallbacks).
+ * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always. */
+enum pp_tune_key { PP_TUNE_SPIN_LIMIT = 1, PP_TUNE_DYNAMIC_TILES = 2 };
+
+/* Set a tuning value. No counterpart in the reference. */
+int pp_tune(pp_handle *h, int key, int64_t value);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,14 @@ __device__ __forceinline__ int sign_exponent(const Key<W>& p1, const Key<W>& p2)
     return e & 3;
 }
 
+// one limb's share of sign_exponent (the shares add up mod 4)
+__device__ __forceinline__ int limb_sign_exponent(uint64_t a, uint64_t b) {
+    const uint64_t z1 = (a >> 1) & ALT, x1 = (a ^ (a >> 1)) & ALT;
+    const uint64_t z2 = (b >> 1) & ALT, x2 = (b ^ (b >> 1)) & ALT;
+    const uint64_t x3 = x1 ^ x2, z3 = z1 ^ z2;
+    return __popcll(x1 & z1) + __popcll(x2 & z2) + 2 * __popcll(z1 & x2) - __popcll(x3 & z3);
+}
+
 template <int W>
 __device__ __forceinline__ int weight(const Key<W>& k) {
     int c = 0;
@@ -128,6 +136,24 @@ __device__ __forceinline__ Key<W> key_xor(const Key<W>& a, const Key<W>& b) {
 #pragma unroll
     for (int l = 0; l < W; ++l) r.l[l] = a.l[l] ^ b.l[l];
     return r;
+}
+
+// Limb `l` of a key for a run-time l, selected with constant indices so that the
+// key stays in registers (a variable index would move it to scratch memory).
+// The limbs are combined with masks rather than selected: a select between
+// limbs can be folded into one load from a variable address.
+template <int W>
+__device__ __forceinline__ uint64_t limb_at(const Key<W>& k, int l) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) r |= k.l[i] & (0ull - (uint64_t)(i == l));
+    return r;
+}
+
+// Pauli code (0..3) of qubit q of a key.
+template <int W>
+__device__ __forceinline__ uint32_t code_at(const Key<W>& k, int q) {
+    return (uint32_t)((limb_at<W>(k, q >> 5) >> (2 * (q & 31))) & 3ull);
 }
 
 // ---------------------------------------------------------------- wave helpers
@@ -251,8 +277,14 @@ __device__ __forceinline__ uint32_t lb_tile_id(uint32_t* counter, uint32_t* s_sl
 // Called by all 64 lanes of one wave. Publishes `agg` for `tile` and returns
 // the exclusive prefix of all earlier tiles. Each poll reads 64 * PP_LB_WORDS
 // predecessors (nearest first) and stops at the nearest inclusive prefix.
-__device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch, uint64_t agg) {
+//
+// spin_limit bounds the polls of one wait (0 = unbounded). When it runs out,
+// *aborted is set and the partial sum read so far is returned: it never
+// exceeds the true prefix, so positions built from it stay in range.
+__device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch, uint64_t agg,
+                                 uint32_t spin_limit, bool* aborted) {
     constexpr int NW = PP_LB_WORDS;
+    uint32_t spins = 0;
     const int lane = lane_id();
     const uint32_t ep32 = epoch & LB_EPOCH_MASK;
     const uint64_t ep = (uint64_t)ep32 << 40;
@@ -293,6 +325,10 @@ __device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch
 #pragma unroll
             for (int j = 0; j < NW; ++j) ready = ready && (j >= lim || st[j] != 0u);
             if (__ballot(!ready) == 0) break;
+            if (spin_limit && ++spins > spin_limit) {
+                *aborted = true;
+                return excl;
+            }
             __builtin_amdgcn_s_sleep(1);
         }
         uint64_t mine = 0;

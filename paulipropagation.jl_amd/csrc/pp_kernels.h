@@ -41,6 +41,7 @@ struct Bufs {
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
     uint64_t* states;    // look-back words, one per tile
+    uint32_t* splits;    // merge-path split of every merge tile boundary
     uint32_t cap;
 };
 
@@ -62,36 +63,51 @@ struct Ctl {
 };
 
 struct Launch {
-    uint32_t seq;     // gate sequence number (parity selects the state)
-    uint32_t epoch;   // look-back epoch
-    uint32_t slot;    // tile counter slot
+    uint32_t seq;         // gate sequence number (parity selects the state)
+    uint32_t epoch;       // look-back epoch
+    uint32_t slot;        // tile counter slot (dynamic tiles)
+    uint32_t dynamic;     // 1: tiles from the atomic counter; 0: tile = blockIdx.x + k * gridDim.x
+    uint32_t spin_limit;  // bound of one look-back wait with static tiles
     uint32_t pad;
-    uint32_t* glog;   // n_in, n_touched, n_out of this gate, or null
+    uint32_t* glog;       // n_in, n_touched, n_out of this gate, or null
 };
 
-// tiles a workgroup claims per atomic increment of the tile counter
-#ifndef PP_TILE_CHUNK
-#define PP_TILE_CHUNK 1
-#endif
-
-// Hands out tile ids in the order workgroups ask for them, PP_TILE_CHUNK
-// consecutive ids per atomic; a workgroup works through its ids in order, so
-// every tile's predecessors are held by running workgroups.
+// Tile ids of a look-back kernel. Static tiles (tile = blockIdx.x + k * gridDim.x)
+// need no atomics; they rely on every workgroup of the grid being resident at
+// once, which the host arranges by sizing the grid from the occupancy of the
+// kernel. Each wait is bounded: if it runs out (workgroups were not resident),
+// the gate is abandoned and the host runs it again with dynamic tiles, which
+// are taken from an atomic counter in the order workgroups start and so always
+// find their predecessors running.
 struct TileSource {
-    uint32_t next = 0, end = 0;
-    __device__ __forceinline__ uint32_t take(uint32_t* ctr, uint32_t* s_slot) {
+    uint32_t cur = 0;
+    bool started = false;
+    __device__ __forceinline__ uint32_t take(const Launch& L, uint32_t* ctr, uint32_t* s_slot) {
         __syncthreads();  // the previous tile's shared memory is free again
-        if (next == end) {
-            if (threadIdx.x == 0)
-                *s_slot = __hip_atomic_fetch_add(ctr, (uint32_t)PP_TILE_CHUNK, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            __syncthreads();
-            next = *s_slot;
-            end = next + PP_TILE_CHUNK;
+        if (!L.dynamic) {
+            cur = started ? cur + gridDim.x : blockIdx.x;
+            started = true;
+            return cur;
         }
-        return next++;
+        if (threadIdx.x == 0) *s_slot = __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        return *s_slot;
     }
 };
+
+// Exclusive prefix of this tile through the look-back; a wait that runs out
+// stops every later kernel and records the gate for the host to run again.
+__device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, uint64_t* states, uint32_t tile,
+                                              uint64_t agg) {
+    bool aborted = false;
+    const uint64_t excl = lb_exclusive(states, tile, L.epoch, agg, L.dynamic ? 0u : L.spin_limit, &aborted);
+    if (aborted && lane_id() == 0) {
+        ctl->overflow_seq = L.seq;
+        __threadfence();
+        atomicMax(&ctl->overflow, 2u);
+    }
+    return excl;
+}
 
 // a[i] for a small register array and a runtime index, without spilling it
 template <int N>
@@ -160,24 +176,36 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint64_t s_excl;
     TileSource ts;
     while (true) {
-        const uint32_t tile = ts.take(&ctl->counter[L.slot], &s_tile);
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)K1_TILE;
         uint64_t ballots[K1_ITEMS];
         uint32_t negbits = 0;
-        // loads of all items first, at clamped indices, so they are in flight together
-        Key<W> kv[K1_ITEMS];
+        // Commutation parity and sign exponent add up limb by limb, so only the
+        // limbs the mask touches are read, one limb of all items at a time (all
+        // loads of a limb in flight together, at clamped indices).
+        uint32_t antibits = 0;
+        int ex[K1_ITEMS];
 #pragma unroll
-        for (int k = 0; k < K1_ITEMS; ++k) {
-            const uint32_t i = min(base + k * WG + t, n - 1);
+        for (int k = 0; k < K1_ITEMS; ++k) ex[k] = 0;
 #pragma unroll
-            for (int l = 0; l < W; ++l) kv[k].l[l] = mask.l[l] ? keys[l * stride + i] : 0ull;
+        for (int l = 0; l < W; ++l) {
+            if (mask.l[l] == 0) continue;
+            uint64_t v[K1_ITEMS];
+#pragma unroll
+            for (int k = 0; k < K1_ITEMS; ++k) v[k] = keys[l * stride + min(base + k * WG + t, n - 1)];
+            const uint64_t m = mask.l[l], sm = swap_pairs(m);
+#pragma unroll
+            for (int k = 0; k < K1_ITEMS; ++k) {
+                antibits ^= (uint32_t)(__popcll(sm & v[k]) & 1) << k;
+                ex[k] += limb_sign_exponent(m, v[k]);
+            }
         }
 #pragma unroll
         for (int k = 0; k < K1_ITEMS; ++k) {
             const uint32_t i = base + k * WG + t;
-            const bool f = (i < n) && anticommutes(mask, kv[k]);
-            if (f && (sign_exponent(mask, kv[k]) & 2) == 0) negbits |= 1u << k;
+            const bool f = (i < n) && ((antibits >> k) & 1u);
+            if (f && (ex[k] & 2) == 0) negbits |= 1u << k;
             ballots[k] = __ballot(f);
             if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(ballots[k]);
         }
@@ -196,7 +224,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             agg += row;
         }
         if (wave == 0) {
-            const uint64_t excl = lb_exclusive(bf.states, tile, L.epoch, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -293,7 +321,17 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
     __shared__ Key<W> s_kb, s_kl;
     __shared__ int s_edge_s[MAX_LEVELS], s_edge_e[MAX_LEVELS];
     __shared__ int s_wagg[NWAVES];
+    __shared__ int s_lvbit[MAX_LEVELS], s_lvsgn[MAX_LEVELS];
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    // the levels go to shared memory with constant indices: indexing the kernel
+    // argument by a loop variable would copy it to scratch memory
+#pragma unroll
+    for (int v = 0; v < MAX_LEVELS; ++v)
+        if (t == v) {
+            s_lvbit[v] = lv.bit[v];
+            s_lvsgn[v] = lv.sgn[v];
+        }
+    __syncthreads();
 
     auto key_at = [&](uint32_t x) -> Key<W> {
         return load_key_from<W>(keys, stride, tail_in[x] & SRC_MASK, from_limb);
@@ -313,15 +351,21 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         for (int i = 0; i < K2_ITEMS; ++i) key[i] = load_key_from<W>(keys, stride, src[i] & SRC_MASK, from_limb);
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) nvalid += (j0 + i <= last) ? 1 : 0;
-        if (nvalid > 0) s_last[t] = key[nvalid - 1];
+        // the last valid item, selected with constant indices (a variable index
+        // into the register array would put it in scratch memory)
+        Key<W> klast = key[0];
+#pragma unroll
+        for (int i = 1; i < K2_ITEMS; ++i)
+            if (i < nvalid) klast = key[i];
+        if (nvalid > 0) s_last[t] = klast;
         // tile edges: the block around `base` and the block around `last`, per level
         if (t == 0) s_kb = key[0];
-        if (nvalid > 0 && j0 + nvalid - 1 == last) s_kl = key[nvalid - 1];
+        if (nvalid > 0 && j0 + nvalid - 1 == last) s_kl = klast;
         __syncthreads();
         const Key<W> kb = s_kb, kl = s_kl;
         for (int task = wave; task < 2 * lv.n; task += NWAVES) {
             const int v = task >> 1;
-            const int bit = lv.bit[v];
+            const int bit = s_lvbit[v];
             if ((task & 1) == 0) {
                 uint32_t s = 0;
                 if (base > 0) {
@@ -352,7 +396,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         for (int i = 0; i < K2_ITEMS; ++i) dest[i] = (int64_t)(j0 + i) + const_off;
 
         for (int v = 0; v < lv.n; ++v) {
-            const int bit = lv.bit[v];
+            const int bit = s_lvbit[v];
             bool h[K2_ITEMS];
 #pragma unroll
             for (int i = 0; i < K2_ITEMS; ++i) {
@@ -392,7 +436,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             for (int w = wave + 1; w < NWAVES; ++w) suf = min(suf, s_wagg[w]);
             __syncthreads();
             const int es = s_edge_s[v], ee = s_edge_e[v];
-            const int sg = lv.sgn[v];
+            const int sg = s_lvsgn[v];
 #pragma unroll
             for (int i = 0; i < K2_ITEMS; ++i) {
                 const int s = sl[i] >= 0 ? sl[i] : (pre >= 0 ? pre : es);
@@ -424,9 +468,6 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
 }
 
 // ============================================================ k_merge (one coefficient column)
-
-constexpr int K3_ITEMS = 4;
-constexpr int K3_TILE = WG * K3_ITEMS;
 
 struct RotParams {
     double cosv, sinv;
@@ -492,8 +533,85 @@ __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, si
     }
 }
 
+// Tile of the one-column merge: 768 diagonals for 256-bit keys, 1024 otherwise,
+// so that the tile's keys and values fit shared memory five times per CU.
+template <int W>
+struct K3Cfg {
+    static constexpr int ITEMS = W == 4 ? 3 : 4;
+    static constexpr int TILE = WG * ITEMS;
+};
+
+// Merge-path split of every tile boundary of this gate's merge, one wave per
+// boundary: splits[b] = head entries among the first min(b * TILE, n + na).
+template <int W, int TILE>
+__global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
+    if (ctl->overflow) return;
+    const int p = L.seq & 1;
+    const uint32_t na = ctl->st[p].na;
+    if (na == 0) return;
+    const uint32_t n = ctl->st[p].n;
+    const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
+    const uint32_t ntot = n + na;
+    const uint32_t nb = (ntot + TILE - 1) / TILE + 1;
+    for (uint32_t b = blockIdx.x * NWAVES + (threadIdx.x >> 6); b < nb; b += gridDim.x * NWAVES) {
+        const uint32_t d = min(b * (uint32_t)TILE, ntot);
+        const uint32_t s = merge_split<W>(keys, bf.cap, n, bf.tkeys, na, d);
+        if (lane_id() == 0) bf.splits[b] = s;
+    }
+}
+
+// Loads a tile into shared memory with its merge values: slot 0 = head entry
+// i0-1, slots 1..ns = head entries i0..i1-1, slots ns+1..ns+1+nt = tail entries
+// j0..j1. A head value is the coefficient times cos when the term anticommutes,
+// a tail value the source coefficient times sin, negated for a negative sign
+// (src/Propagation/specializations.jl:40); the merge then only adds head + tail.
+template <int W, int TILE>
+__device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, const double* __restrict__ coeffs,
+                                            size_t stride, const uint32_t* __restrict__ tail,
+                                            const uint64_t* __restrict__ tkeys, uint32_t na, uint32_t i0,
+                                            uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
+                                            double sinv, uint64_t (*s_key)[TILE + 2], double* s_val) {
+    const int t = threadIdx.x;
+    constexpr int R = (TILE + 1 + WG - 1) / WG;
+    const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int x = t + r * WG;
+        size_t gi = (size_t)i0 + x;
+        gi = gi > 0 ? gi - 1 : 0;
+        gi = gi < hmax ? gi : hmax;
+        Key<W> k;
+#pragma unroll
+        for (int l = 0; l < W; ++l) k.l[l] = keys[l * stride + gi];
+        const double c = coeffs[gi];
+        if (x < ns + 1 && (x > 0 || i0 > 0)) {
+#pragma unroll
+            for (int l = 0; l < W; ++l) s_key[l][x] = k.l[l];
+            s_val[x] = anticommutes<W>(mask, k) ? c * cosv : c;
+        }
+    }
+    const uint32_t tmax = na - 1;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int y = t + r * WG;
+        const uint32_t gj = min(j0 + (uint32_t)y, tmax);
+        Key<W> k;
+#pragma unroll
+        for (int l = 0; l < W; ++l) k.l[l] = tkeys[l * stride + gj];
+        const uint32_t src = tail[gj];
+        double c = coeffs[src & SRC_MASK] * sinv;
+        if (src & SRC_NEG) c = -c;
+        if (y < nt + 1 && j0 + (uint32_t)y < na) {
+#pragma unroll
+            for (int l = 0; l < W; ++l) s_key[l][ns + 1 + y] = k.l[l];
+            s_val[ns + 1 + y] = c;
+        }
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp) {
+    constexpr int ITEMS = K3Cfg<W>::ITEMS, TILE = K3Cfg<W>::TILE;
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
@@ -503,29 +621,22 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     const double* __restrict__ coeffs = bf.coeffs[cur];
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
     double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
-    const uint32_t* __restrict__ tail = bf.tail[1];
     const size_t stride = bf.cap;
     const uint32_t ntot = n + na;
-    const uint32_t ntiles = (ntot + K3_TILE - 1) / K3_TILE;
-    __shared__ uint64_t s_key[W][K3_TILE + 2];
-    __shared__ uint32_t s_src[K3_TILE + 1];
-    __shared__ uint32_t s_split[2];
+    const uint32_t ntiles = (ntot + TILE - 1) / TILE;
+    __shared__ uint64_t s_key[W][TILE + 2];
+    __shared__ double s_val[TILE + 2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     TileSource ts;
     while (true) {
-        const uint32_t tile = ts.take(&ctl->counter[L.slot], &s_tile);
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
-        const uint32_t d0 = tile * (uint32_t)K3_TILE;
-        const uint32_t d1 = min(d0 + (uint32_t)K3_TILE, ntot);
-        if (wave < 2) {
-            const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
-            if (lane == 0) s_split[wave] = s;
-        }
-        __syncthreads();
-        const uint32_t i0 = s_split[0], i1 = s_split[1];
+        const uint32_t d0 = tile * (uint32_t)TILE;
+        const uint32_t d1 = min(d0 + (uint32_t)TILE, ntot);
+        const uint32_t i0 = bf.splits[tile], i1 = bf.splits[tile + 1];
         const uint32_t j0 = d0 - i0, j1 = d1 - i1;
         int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
         if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
@@ -534,7 +645,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             ns = 0;
             nt = 0;
         }
-        merge_load<W, K3_TILE>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src);
+        merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
+                             rp.sinv, s_key, s_val);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -548,10 +660,10 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             for (int l = 0; l < W; ++l) k.l[l] = s_key[l][ns + 1 + y];
             return k;
         };
-        const int dl = t * K3_ITEMS;
+        const int dl = t * ITEMS;
         const int dtot = ns + nt;
-        Key<W> okey[K3_ITEMS];
-        double oc[K3_ITEMS];
+        int slot[ITEMS];
+        double oc[ITEMS];
         uint32_t keep = 0;
         if (dl < dtot) {
             int lo = dl > nt ? dl - nt : 0, hi = min(dl, ns);
@@ -561,66 +673,40 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                 else hi = mid;
             }
             int x = lo, y = dl - lo;
-            const int x0 = x, y0 = y;
-            // every coefficient the steps below can read, loaded up front so the
-            // loads overlap instead of following the merge decisions one by one
-            double hcv[K3_ITEMS], tcv[K3_ITEMS + 1];
-            const int hlast = ns > 0 ? ns - 1 : 0;
-            const int tlast = (j0 + (uint32_t)nt < na) ? nt : (nt > 0 ? nt - 1 : 0);
 #pragma unroll
-            for (int k = 0; k < K3_ITEMS; ++k) hcv[k] = coeffs[i0 + min(x0 + k, hlast)];
-#pragma unroll
-            for (int k = 0; k <= K3_ITEMS; ++k) {
-                // tail slot y holds an entry iff y < nt, or y == nt and the entry after
-                // the tile exists; an empty slot reads coefficient 0 (never used)
-                const int yk = min(y0 + k, tlast);
-                const bool filled = yk < nt || j0 + (uint32_t)yk < na;
-                tcv[k] = coeffs[filled ? (s_src[yk] & SRC_MASK) : 0u];
-            }
-#pragma unroll
-            for (int k = 0; k < K3_ITEMS; ++k) {
-                if (x + y >= dtot) break;
-                bool take_s;
-                if (x >= ns) take_s = false;
-                else if (y >= nt) take_s = true;
-                else take_s = key_le<W>(S(x), T(y));
+            for (int k = 0; k < ITEMS; ++k) {
+                slot[k] = 0;
+                oc[k] = 0.0;
+                if (x + y >= dtot) continue;
+                const bool hs = x < ns, ht = y < nt;
+                const Key<W> ks = S(hs ? x : 0), kt = T(y);
+                const bool take_s = hs && (!ht || key_le<W>(ks, kt));
+                bool live;
+                Key<W> ko;
                 if (take_s) {
-                    const Key<W> ks = S(x);
-                    double c = pick(hcv, x - x0);
-                    if (anticommutes<W>(mask, ks)) c = c * rp.cosv;
-                    if (j0 + y < na && key_eq<W>(T(y), ks)) {
-                        double tc = pick(tcv, y - y0) * rp.sinv;
-                        if (s_src[y] & SRC_NEG) tc = -tc;
-                        c = c + tc;
-                    }
-                    okey[k] = ks;
+                    double c = s_val[1 + x];
+                    if (j0 + y < na && key_eq<W>(kt, ks)) c = c + s_val[ns + 1 + y];
+                    slot[k] = 1 + x;
                     oc[k] = c;
-                    keep |= 1u << k;
+                    ko = ks;
+                    live = true;
                     ++x;
                 } else {
-                    const Key<W> kt = T(y);
-                    const bool dup = (i0 + x > 0) && key_eq<W>(S(x - 1), kt);
-                    if (!dup) {
-                        double tc = pick(tcv, y - y0) * rp.sinv;
-                        if (s_src[y] & SRC_NEG) tc = -tc;
-                        okey[k] = kt;
-                        oc[k] = tc;
-                        keep |= 1u << k;
-                    }
+                    live = !((i0 + x > 0) && key_eq<W>(S(x - 1), kt));
+                    slot[k] = ns + 1 + y;
+                    oc[k] = s_val[ns + 1 + y];
+                    ko = kt;
                     ++y;
                 }
-                if ((keep >> k) & 1u) {
-                    const bool drop = (rp.max_weight >= 0 && weight<W>(okey[k]) > rp.max_weight) ||
-                                      (fabs(oc[k]) < rp.min_abs);
-                    if (drop) keep &= ~(1u << k);
-                }
+                if (live && !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs))
+                    keep |= 1u << k;
             }
         }
         uint32_t agg;
         const uint32_t mine = (uint32_t)__popc(keep);
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (wave == 0) {
-            const uint64_t excl = lb_exclusive(bf.states, tile, L.epoch, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -633,9 +719,10 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         __syncthreads();
         uint64_t pos = s_excl + pre;
 #pragma unroll
-        for (int k = 0; k < K3_ITEMS; ++k) {
+        for (int k = 0; k < ITEMS; ++k) {
             if ((keep >> k) & 1u) {
-                store_key<W>(out_keys, stride, pos, okey[k]);
+#pragma unroll
+                for (int l = 0; l < W; ++l) out_keys[l * stride + pos] = s_key[l][slot[k]];
                 out_coeffs[pos] = oc[k];
                 ++pos;
             }
@@ -680,7 +767,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     TileSource ts;
     while (true) {
-        const uint32_t tile = ts.take(&ctl->counter[L.slot], &s_tile);
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
         const uint32_t d0 = tile * (uint32_t)TB;
         const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
@@ -784,7 +871,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         uint32_t agg;
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (wave == 0) {
-            const uint64_t excl = lb_exclusive(bf.states, tile, L.epoch, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -810,14 +897,20 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
 
 // ============================================================ Pauli noise
 
-constexpr int KN_ITEMS = 8;
-constexpr int KN_TILE = WG * KN_ITEMS;
+// entries per thread of the one-column noise pass: fewer for 256-bit keys so the
+// loaded keys stay in registers
+template <int W>
+struct KNCfg {
+    static constexpr int ITEMS = W == 4 ? 4 : 8;
+    static constexpr int TILE = WG * ITEMS;
+};
 
 // c *= factor for the damped codes on `qubit`, then truncate every term and
 // compact in order (src/Base/Primitives/mapandtruncate.jl:135-178)
 template <int W>
 __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int qubit, uint32_t damp_codes,
                                                double factor, int max_weight, double min_abs) {
+    constexpr int ITEMS = KNCfg<W>::ITEMS, TILE = KNCfg<W>::TILE;
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
@@ -834,29 +927,28 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
     double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
     const size_t stride = bf.cap;
-    const uint32_t ntiles = (n + KN_TILE - 1) / KN_TILE;
-    const int ql = qubit >> 5, qs = 2 * (qubit & 31);
+    const uint32_t ntiles = (n + TILE - 1) / TILE;
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_cnt[KN_ITEMS][NWAVES];
+    __shared__ uint32_t s_cnt[ITEMS][NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
     while (true) {
-        const uint32_t tile = ts.take(&ctl->counter[L.slot], &s_tile);
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
-        const uint32_t base = tile * (uint32_t)KN_TILE;
-        uint64_t ballots[KN_ITEMS];
-        double cv[KN_ITEMS];
-        Key<W> kv[KN_ITEMS];
+        const uint32_t base = tile * (uint32_t)TILE;
+        uint64_t ballots[ITEMS];
+        double cv[ITEMS];
+        Key<W> kv[ITEMS];
 #pragma unroll
-        for (int k = 0; k < KN_ITEMS; ++k) {
+        for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = min(base + k * WG + t, n - 1);
             kv[k] = load_key<W>(keys, stride, i);
             cv[k] = coeffs[i];
         }
 #pragma unroll
-        for (int k = 0; k < KN_ITEMS; ++k) {
+        for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = base + k * WG + t;
-            const uint32_t pc = (uint32_t)((kv[k].l[ql] >> qs) & 3ull);
+            const uint32_t pc = code_at<W>(kv[k], qubit);
             if ((damp_codes >> pc) & 1u) cv[k] = cv[k] * factor;
             const bool keep =
                 (i < n) && !((max_weight >= 0 && weight<W>(kv[k]) > max_weight) || fabs(cv[k]) < min_abs);
@@ -864,9 +956,9 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
             if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(ballots[k]);
         }
         __syncthreads();
-        uint32_t agg = 0, before[KN_ITEMS];
+        uint32_t agg = 0, before[ITEMS];
 #pragma unroll
-        for (int k = 0; k < KN_ITEMS; ++k) {
+        for (int k = 0; k < ITEMS; ++k) {
             uint32_t rb = 0, row = 0;
 #pragma unroll
             for (int w = 0; w < NWAVES; ++w) {
@@ -877,7 +969,7 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
             agg += row;
         }
         if (wave == 0) {
-            const uint64_t excl = lb_exclusive(bf.states, tile, L.epoch, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -894,7 +986,7 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         __syncthreads();
         const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-        for (int k = 0; k < KN_ITEMS; ++k) {
+        for (int k = 0; k < ITEMS; ++k) {
             if ((ballots[k] >> lane) & 1ull) {
                 const uint64_t pos = s_excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);
                 store_key<W>(out_keys, stride, pos, kv[k]);
@@ -927,7 +1019,6 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
     double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
     const size_t stride = bf.cap;
     const uint32_t ntiles = (n + KNB_ROWS - 1) / KNB_ROWS;
-    const int ql = qubit >> 5, qs = 2 * (qubit & 31);
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_keep[KNB_ROWS];
     __shared__ uint32_t s_pos[KNB_ROWS];
@@ -935,7 +1026,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
     __shared__ uint64_t s_excl;
     TileSource ts;
     while (true) {
-        const uint32_t tile = ts.take(&ctl->counter[L.slot], &s_tile);
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)KNB_ROWS;
         for (int r = wave; r < KNB_ROWS; r += NWAVES) {
@@ -943,7 +1034,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
             uint32_t keep = 0;
             if (i < n) {
                 const Key<W> kk = load_key<W>(keys, stride, i);
-                const bool damped = (damp_codes >> ((kk.l[ql] >> qs) & 3ull)) & 1u;
+                const bool damped = (damp_codes >> code_at<W>(kk, qubit)) & 1u;
                 double m = 0.0;
                 for (int b = lane; b < B; b += 64) {
                     double c = coeffs[(size_t)i * B + b];
@@ -961,7 +1052,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (t < KNB_ROWS) s_pos[t] = pre;
         if (wave == 0) {
-            const uint64_t excl = lb_exclusive(bf.states, tile, L.epoch, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -982,7 +1073,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
             const uint64_t pos = s_excl + s_pos[r];
             const Key<W> kk = load_key<W>(keys, stride, i);
             if (lane == 0) store_key<W>(out_keys, stride, pos, kk);
-            const bool damped = (damp_codes >> ((kk.l[ql] >> qs) & 3ull)) & 1u;
+            const bool damped = (damp_codes >> code_at<W>(kk, qubit)) & 1u;
             for (int b = lane; b < B; b += 64) {
                 double c = coeffs[(size_t)i * B + b];
                 if (damped) c = c * factor[b];

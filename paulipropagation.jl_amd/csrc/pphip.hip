@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/pphip.h"
@@ -51,6 +52,12 @@ struct pp_handle {
     Bufs bufs{};
     size_t lb_cap = 0;
     unsigned gmax = 2048;          // persistent grid size
+    int cus = 256;                 // compute units
+    std::unordered_map<const void*, unsigned> resident;  // co-resident workgroups per kernel
+    uint32_t spin_limit = 1u << 16;  // bound of one look-back wait with static tiles
+    bool force_dynamic = false;      // take tiles from the atomic counter always
+    uint32_t safe_until = 0;         // gates before this sequence number use dynamic tiles
+    int64_t fallbacks = 0;           // gates run again with dynamic tiles
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr;          // pinned mirror
     uint32_t seq = 0;              // next gate's sequence number; the sum's state is st[seq & 1]
@@ -90,8 +97,10 @@ void free_terms(pp_handle* h) {
     }
     if (h->bufs.states) hipFree(h->bufs.states);
     if (h->bufs.tkeys) hipFree(h->bufs.tkeys);
+    if (h->bufs.splits) hipFree(h->bufs.splits);
     h->bufs.states = nullptr;
     h->bufs.tkeys = nullptr;
+    h->bufs.splits = nullptr;
 }
 
 int tile_rows(int B) {
@@ -104,7 +113,7 @@ int tile_rows(int B) {
 
 // look-back words for the largest compaction launch over up to cap entries
 size_t tiles_needed(size_t cap, int B) {
-    const size_t tile = B > 1 ? (size_t)std::min(KNB_ROWS, tile_rows(B)) : (size_t)K3_TILE;
+    const size_t tile = B > 1 ? (size_t)std::min(KNB_ROWS, tile_rows(B)) : (size_t)K3Cfg<4>::TILE;
     return cap / tile + 64;
 }
 
@@ -152,6 +161,7 @@ int grow(pp_handle* h, size_t newcap) {
     uint32_t* nt[2] = {nullptr, nullptr};
     uint64_t* ns = nullptr;
     uint64_t* ntk = nullptr;
+    uint32_t* nsp = nullptr;
     const size_t nlb = tiles_needed(newcap, B);
     bool ok = true;
     for (int b = 0; b < 2 && ok; ++b) {
@@ -161,6 +171,7 @@ int grow(pp_handle* h, size_t newcap) {
     }
     ok = ok && hipMalloc(&ns, nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&nsp, nlb * sizeof(uint32_t)) == hipSuccess;
     if (!ok) {
         for (int b = 0; b < 2; ++b) {
             if (nk[b]) hipFree(nk[b]);
@@ -169,6 +180,7 @@ int grow(pp_handle* h, size_t newcap) {
         }
         if (ns) hipFree(ns);
         if (ntk) hipFree(ntk);
+        if (nsp) hipFree(nsp);
         (void)hipGetLastError();
         return fail(PP_ERR_NOMEM, "device allocation of " + std::to_string(newcap) + " terms failed");
     }
@@ -189,6 +201,7 @@ int grow(pp_handle* h, size_t newcap) {
     }
     h->bufs.states = ns;
     h->bufs.tkeys = ntk;
+    h->bufs.splits = nsp;
     h->bufs.cap = (uint32_t)newcap;
     h->lb_cap = nlb;
     h->cap = newcap;
@@ -211,8 +224,24 @@ Launch lb_launch(pp_handle* h, uint32_t* glog) {
     L.seq = h->seq;
     L.epoch = next_epoch(h);
     L.slot = (h->lb_count++) & 1u;
+    L.dynamic = (h->force_dynamic || h->seq < h->safe_until) ? 1u : 0u;
+    L.spin_limit = h->spin_limit;
     L.glog = glog;
     return L;
+}
+
+// Workgroups of `fn` the whole chip holds at once (occupancy x compute units).
+unsigned resident_wgs(pp_handle* h, const void* fn, size_t shm) {
+    auto it = h->resident.find(fn);
+    if (it != h->resident.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, WG, shm) != hipSuccess || nb < 1) {
+        (void)hipGetLastError();
+        nb = 1;
+    }
+    const unsigned r = (unsigned)nb * (unsigned)h->cus;
+    h->resident[fn] = r;
+    return r;
 }
 
 // Grid of a persistent kernel: enough workgroups for `expect` entries at the
@@ -222,6 +251,12 @@ Launch lb_launch(pp_handle* h, uint32_t* glog) {
 unsigned grid_for(pp_handle* h, size_t expect, size_t per_tile) {
     const size_t tiles = (expect + per_tile - 1) / per_tile;
     return (unsigned)std::max<size_t>(1, std::min<size_t>(tiles, h->gmax));
+}
+
+// Grid of a look-back kernel: as grid_for, and never more workgroups than are
+// resident together, which static tiles need.
+unsigned lb_grid(pp_handle* h, const void* fn, size_t shm, size_t expect, size_t per_tile) {
+    return std::min(grid_for(h, expect, per_tile), resident_wgs(h, fn, shm));
 }
 
 int ensure_run(pp_handle* h, size_t ngates) {
@@ -331,8 +366,8 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode >= 1 && ev;
     if (all) hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL(k_branch<W>, dim3(grid_for(h, h->n_est, K1_TILE)), dim3(WG), 0, h->stream, h->bufs,
-                       h->d_ctl, lb_launch(h, glog), mask);
+    hipLaunchKernelGGL(k_branch<W>, dim3(lb_grid(h, (const void*)k_branch<W>, 0, h->n_est, K1_TILE)), dim3(WG), 0,
+                       h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask);
     if (all) hipEventRecord(ev[1], h->stream);
     Launch L2{};
     L2.seq = h->seq;
@@ -344,17 +379,23 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     if (mrg) hipEventRecord(ev[4], h->stream);
     if (h->B == 1) {
         RotParams rp{c1, s1, max_weight, min_abs};
-        hipLaunchKernelGGL(k_merge1<W>, dim3(grid_for(h, h->n_est + h->n_est / 2, K3_TILE)), dim3(WG), 0,
+        constexpr int TILE = K3Cfg<W>::TILE;
+        const size_t expect = h->n_est + h->n_est / 2;
+        Launch LP{};
+        LP.seq = h->seq;
+        hipLaunchKernelGGL((k_partition<W, TILE>), dim3(grid_for(h, expect, (size_t)TILE * NWAVES)), dim3(WG), 0,
+                           h->stream, h->bufs, h->d_ctl, LP);
+        hipLaunchKernelGGL(k_merge1<W>, dim3(lb_grid(h, (const void*)k_merge1<W>, 0, expect, TILE)), dim3(WG), 0,
                            h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp);
     } else {
         const int B = h->B;
         const int TB = tile_rows(B);
         const size_t shm = (size_t)TB * B * sizeof(double);
-        const unsigned g = grid_for(h, h->n_est + h->n_est / 2, TB);
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
-    hipLaunchKernelGGL((k_mergeB<W, TBV>), dim3(g), dim3(WG), shm, h->stream, h->bufs, h->d_ctl, L3, mask, B, \
-                       d_trig, max_weight, min_abs)
+    hipLaunchKernelGGL((k_mergeB<W, TBV>),                                                                    \
+                       dim3(lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, h->n_est + h->n_est / 2, TBV)),    \
+                       dim3(WG), shm, h->stream, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -374,11 +415,13 @@ void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const do
     const bool mrg = h->prof_mode >= 1 && ev;
     if (mrg) hipEventRecord(ev[4], h->stream);
     if (h->B == 1) {
-        hipLaunchKernelGGL(k_noise1<W>, dim3(grid_for(h, h->n_est, KN_TILE)), dim3(WG), 0, h->stream, h->bufs,
-                           h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
+        hipLaunchKernelGGL(k_noise1<W>, dim3(lb_grid(h, (const void*)k_noise1<W>, 0, h->n_est, KNCfg<W>::TILE)), dim3(WG),
+                           0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight,
+                           min_abs);
     } else {
-        hipLaunchKernelGGL(k_noiseB<W>, dim3(grid_for(h, h->n_est, KNB_ROWS)), dim3(WG), 0, h->stream, h->bufs,
-                           h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
+        hipLaunchKernelGGL(k_noiseB<W>, dim3(lb_grid(h, (const void*)k_noiseB<W>, 0, h->n_est, KNB_ROWS)),
+                           dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes,
+                           d_factor, max_weight, min_abs);
     }
     if (mrg) hipEventRecord(ev[5], h->stream);
     h->seq += 1;
@@ -431,10 +474,18 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, 
         if (h->h_ctl->overflow) {
             const uint32_t fseq = h->h_ctl->overflow_seq;
             const size_t gi = g + (size_t)(fseq - seq0);
-            const size_t need = h->h_ctl->overflow_need;
-            if (need > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
-            h->seq = fseq;  // the state before the failed gate is still in st[fseq & 1]
-            if (int rc = grow(h, std::max(need + need / 4, h->cap + h->cap / 2))) return rc;
+            h->seq = fseq;  // the state before the stopped gate is still in st[fseq & 1]
+            if (h->h_ctl->overflow == 1) {
+                const size_t need = h->h_ctl->overflow_need;
+                if (need > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
+                if (int rc = grow(h, std::max(need + need / 4, h->cap + h->cap / 2))) return rc;
+            } else {
+                // a look-back wait ran out: run that gate again with dynamic tiles
+                const GateState st = h->h_ctl->st[fseq & 1];
+                if (int rc = reset_ctl(h, st.n, st.cur)) return rc;
+                h->safe_until = fseq + 1;
+                h->fallbacks += 1;
+            }
             g = gi;
             continue;
         }
@@ -574,6 +625,7 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     h->gmax = (unsigned)std::max(64, 4 * cus);
+    h->cus = std::max(1, cus);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(PP_ERR_HIP, "hipStreamCreate failed"));
     if (hipMalloc(&h->d_ctl, sizeof(Ctl)) != hipSuccess ||
@@ -770,7 +822,7 @@ int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (enable_timing >= 0) h->prof_mode = std::min(enable_timing, 2);
     if (reset) {
-        h->gates = h->term_apps = 0;
+        h->gates = h->term_apps = h->fallbacks = 0;
         for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
             h->launches[k] = 0;
             h->ms[k] = 0.0;
@@ -780,9 +832,25 @@ int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     return PP_OK;
 }
 
+int pp_tune(pp_handle* h, int key, int64_t value) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    switch (key) {
+    case PP_TUNE_SPIN_LIMIT:
+        if (value < 1 || value > 0x7fffffff) return fail(PP_ERR_ARG, "spin limit must be in 1..2^31-1");
+        h->spin_limit = (uint32_t)value;
+        return PP_OK;
+    case PP_TUNE_DYNAMIC_TILES:
+        h->force_dynamic = value != 0;
+        return PP_OK;
+    default:
+        return fail(PP_ERR_ARG, "unknown tuning key");
+    }
+}
+
 int pp_profile_read(const pp_handle* h, pp_profile* out) {
     if (!h || !out) return fail(PP_ERR_ARG, "NULL argument");
     out->gates = h->gates;
+    out->fallbacks = h->fallbacks;
     out->term_applications = h->term_apps;
     for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
         out->launches[k] = h->launches[k];

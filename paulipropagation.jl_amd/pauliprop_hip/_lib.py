@@ -30,7 +30,7 @@ KERNEL_KINDS = ("branch", "tail_permute", "merge", "noise", "overlap")
 
 
 class pp_profile(ctypes.Structure):
-    _fields_ = [("gates", ctypes.c_int64), ("term_applications", ctypes.c_int64),
+    _fields_ = [("gates", ctypes.c_int64), ("term_applications", ctypes.c_int64), ("fallbacks", ctypes.c_int64),
                 ("launches", ctypes.c_int64 * 5), ("milliseconds", ctypes.c_double * 5),
                 ("algorithmic_bytes", ctypes.c_double * 5)]
 
@@ -62,7 +62,10 @@ SIGNATURES = [
     ("pp_synchronize", ctypes.c_int, [_P]),
     ("pp_profile_control", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     ("pp_profile_read", ctypes.c_int, [_P, ctypes.POINTER(pp_profile)]),
+    ("pp_tune", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
 ]
+
+PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES = 1, 2
 
 
 def lib():

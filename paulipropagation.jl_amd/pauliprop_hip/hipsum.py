@@ -139,12 +139,21 @@ class HIPPauliSum:
         """timing: 1 records HIP events around every kernel, 0 stops, -1 leaves it."""
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
+    def tune(self, spin_limit: int = None, dynamic_tiles: bool = None) -> None:
+        """Bound of a look-back wait (polls), and whether tiles are always claimed atomically."""
+        L = _lib.lib()
+        if spin_limit is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SPIN_LIMIT, int(spin_limit)))
+        if dynamic_tiles is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_DYNAMIC_TILES, int(bool(dynamic_tiles))))
+
     def profile(self) -> dict:
         p = _lib.pp_profile()
         _lib.check(_lib.lib().pp_profile_read(self._h, ctypes.byref(p)))
         return {
             "gates": p.gates,
             "term_applications": p.term_applications,
+            "fallbacks": p.fallbacks,
             "kernels": {k: {"launches": p.launches[i], "ms": p.milliseconds[i], "bytes": p.algorithmic_bytes[i]}
                         for i, k in enumerate(_lib.KERNEL_KINDS)},
         }

@@ -104,3 +104,27 @@ def test_empty_sum_and_errors():
     keys, c = terms_to_arrays([5, 5], 40, [1.0, 2.0])
     with pytest.raises(ValueError):
         dev.upload(keys, c)
+
+
+@pytest.mark.parametrize("mode", ["tiny_spin_limit", "dynamic_tiles"])
+def test_tile_claim_modes_are_bit_identical(mode):
+    # static tiles with a one-poll wait force look-back waits to run out, so gates
+    # are stopped and run again with atomic tile claims; both must match the oracle
+    print(f"tile mode {mode}", flush=True)
+    nq = 64
+    dev, ref = _pair(nq, 200000, seed=11)
+    if mode == "tiny_spin_limit":
+        dev.tune(spin_limit=1)
+    else:
+        dev.tune(dynamic_tiles=True)
+    W = key_words(nq)
+    for codes in ({20: 1}, {20: 3, 21: 3}, {5: 2, 60: 1}, {64: 3}):
+        m = mask_of(nq, codes)
+        dev.apply_rotation(m, 0.7, max_weight=10, min_abs_coeff=1e-4)
+        ref.rotation(int_to_limbs(m, W), math.cos(0.7), math.sin(0.7), max_weight=10, min_abs=1e-4)
+        assert_bitwise(*dev.download(), *ref.get(), f"{mode} {codes}")
+    dev.apply_pauli_noise(7, 0b1110, 0.2, max_weight=10, min_abs_coeff=1e-3)
+    ref.pauli_noise(6, 0b1110, 0.8, max_weight=10, min_abs=1e-3)
+    assert_bitwise(*dev.download(), *ref.get(), f"{mode} noise")
+    if mode == "tiny_spin_limit":
+        print("fallbacks", dev.profile()["fallbacks"], flush=True)
