@@ -269,76 +269,93 @@ __device__ __forceinline__ uint32_t lb_tile_id(uint32_t* counter, uint32_t* s_sl
     return *s_slot;
 }
 
-// predecessor words each lane reads per poll of the look-back
-#ifndef PP_LB_WORDS
-#define PP_LB_WORDS 1
-#endif
+// Status and value of a look-back word read by this launch: words of older
+// launches carry another epoch and read as "not ready" (status 0).
+__device__ __forceinline__ uint32_t lb_status(uint64_t w, uint32_t ep32) {
+    return (((w >> 40) & LB_EPOCH_MASK) == ep32) ? (uint32_t)(w >> 62) : 0u;
+}
 
-// Called by all 64 lanes of one wave. Publishes `agg` for `tile` and returns
-// the exclusive prefix of all earlier tiles. Each poll reads 64 * PP_LB_WORDS
-// predecessors (nearest first) and stops at the nearest inclusive prefix.
+// One poll of a window of up to 64 predecessor words, nearest first: lane l
+// reads word `base - l` when l < count. Finds the nearest inclusive prefix in
+// the window; returns false while a word up to it (or up to the window's end,
+// when it holds none) is not ready. On success *sum is the window's
+// contribution to the exclusive prefix and *found tells whether it ended at
+// an inclusive prefix.
+__device__ __forceinline__ bool lb_window(const uint64_t* words, int64_t base, int count, uint32_t ep32,
+                                          uint64_t* sum, bool* found) {
+    const int lane = lane_id();
+    const bool valid = lane < count;
+    const uint64_t w = lb_load(words + (valid && base - lane >= 0 ? base - lane : 0));
+    const uint32_t st = valid ? lb_status(w, ep32) : 0u;
+    const uint64_t pm = __ballot(st == 2u);
+    const int P = pm ? (__ffsll((unsigned long long)pm) - 1) : count;
+    if (__ballot(lane < P && st == 0u) != 0) return false;
+    *sum = wave_sum_u64(lane <= P && valid ? (w & LB_VAL_MASK) : 0ull);
+    *found = P < count;
+    return true;
+}
+
+// Two-level decoupled look-back. Called by all 64 lanes of one wave; publishes
+// `agg` for `tile` and returns the exclusive prefix of all earlier tiles.
+//
+// Tiles form groups of 64. A tile first reads its own group's earlier words;
+// when none of them holds an inclusive prefix it adds the group-level words
+// (one per earlier group: the group's aggregate, or its inclusive prefix),
+// 64 groups per poll. The last tile of a group publishes the group's word.
+// A prefix therefore resolves in a few polls however many tiles run at once,
+// where a flat look-back walks back through every running tile, 64 per poll.
 //
 // spin_limit bounds the polls of one wait (0 = unbounded). When it runs out,
 // *aborted is set and the partial sum read so far is returned: it never
 // exceeds the true prefix, so positions built from it stay in range.
-__device__ uint64_t lb_exclusive(uint64_t* states, uint32_t tile, uint32_t epoch, uint64_t agg,
+__device__ uint64_t lb_exclusive(uint64_t* states, uint64_t* gstates, uint32_t tile, uint32_t epoch, uint64_t agg,
                                  uint32_t spin_limit, bool* aborted) {
-    constexpr int NW = PP_LB_WORDS;
-    uint32_t spins = 0;
     const int lane = lane_id();
     const uint32_t ep32 = epoch & LB_EPOCH_MASK;
     const uint64_t ep = (uint64_t)ep32 << 40;
+    const uint32_t g = tile >> 6;
+    const int r = (int)(tile & 63u);
+    const bool last_of_group = r == 63;
     if (tile == 0) {
         if (lane == 0) lb_store(states, (2ull << 62) | ep | agg);
         return 0;
     }
     if (lane == 0) lb_store(states + tile, (1ull << 62) | ep | agg);
+    uint32_t spins = 0;
     uint64_t excl = 0;
-    int64_t base = (int64_t)tile - 1;
-    while (true) {
-        uint64_t val[NW];
-        uint32_t st[NW];
-        int P, lim;
+    bool found = false;
+    // this group's earlier tiles (tile 0 of the first group is always inclusive)
+    while (!lb_window(states, (int64_t)tile - 1, r, ep32, &excl, &found)) {
+        if (spin_limit && ++spins > spin_limit) {
+            *aborted = true;
+            return 0;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (!found) {
+        if (last_of_group && lane == 0) lb_store(gstates + g, (1ull << 62) | ep | (excl + agg));
+        int64_t gb = (int64_t)g - 1;
         while (true) {
-            int first_pref = NW;
-            // all loads first (clamped, unconditional) so that they are in flight together
-            uint64_t w[NW];
-#pragma unroll
-            for (int j = 0; j < NW; ++j) {
-                const int64_t idx = base - (NW * lane + j);
-                w[j] = lb_load(states + (idx >= 0 ? idx : 0));
+            uint64_t part = 0;
+            bool gfound = false;
+            const int cnt = gb + 1 < 64 ? (int)(gb + 1) : 64;
+            if (lb_window(gstates, gb, cnt, ep32, &part, &gfound)) {
+                excl += part;
+                if (gfound || gb < 64) break;  // group 0's word is inclusive once ready
+                gb -= 64;
+                continue;
             }
-#pragma unroll
-            for (int j = 0; j < NW; ++j) {
-                const int64_t idx = base - (NW * lane + j);
-                // before tile 0: never needed, tile 0 holds a prefix
-                const uint32_t sj = idx < 0 ? 2u : ((((w[j] >> 40) & LB_EPOCH_MASK) == ep32) ? (uint32_t)(w[j] >> 62) : 0u);
-                st[j] = sj;
-                val[j] = idx < 0 ? 0ull : (w[j] & LB_VAL_MASK);
-                if (sj == 2 && first_pref == NW) first_pref = j;
-            }
-            const uint64_t pm = __ballot(first_pref < NW);
-            P = pm ? (__ffsll((unsigned long long)pm) - 1) : 64;
-            const int jp = __shfl(first_pref, P < 64 ? P : 0, 64);
-            lim = (lane < P) ? NW : (lane == P ? jp + 1 : 0);
-            bool ready = true;
-#pragma unroll
-            for (int j = 0; j < NW; ++j) ready = ready && (j >= lim || st[j] != 0u);
-            if (__ballot(!ready) == 0) break;
             if (spin_limit && ++spins > spin_limit) {
                 *aborted = true;
                 return excl;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        uint64_t mine = 0;
-#pragma unroll
-        for (int j = 0; j < NW; ++j) mine += (j < lim) ? val[j] : 0ull;
-        excl += wave_sum_u64(mine);
-        if (P < 64) break;
-        base -= 64 * NW;
     }
-    if (lane == 0) lb_store(states + tile, (2ull << 62) | ep | (excl + agg));
+    if (lane == 0) {
+        lb_store(states + tile, (2ull << 62) | ep | (excl + agg));
+        if (last_of_group) lb_store(gstates + g, (2ull << 62) | ep | (excl + agg));
+    }
     return excl;
 }
 

@@ -41,7 +41,10 @@ struct Bufs {
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
     uint64_t* states;    // look-back words, one per tile
+    uint64_t* gstates;   // look-back words, one per group of 64 tiles
     uint32_t* splits;    // merge-path split of every merge tile boundary
+    uint64_t* ssamp;     // W words per sample: sum entries 0, 64, 128, ...
+    uint64_t* tsamp;     // W words per sample: permuted tail entries 63, 127, ...
     uint32_t cap;
 };
 
@@ -97,10 +100,11 @@ struct TileSource {
 
 // Exclusive prefix of this tile through the look-back; a wait that runs out
 // stops every later kernel and records the gate for the host to run again.
-__device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, uint64_t* states, uint32_t tile,
+__device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
                                               uint64_t agg) {
     bool aborted = false;
-    const uint64_t excl = lb_exclusive(states, tile, L.epoch, agg, L.dynamic ? 0u : L.spin_limit, &aborted);
+    const uint64_t excl =
+        lb_exclusive(bf.states, bf.gstates, tile, L.epoch, agg, L.dynamic ? 0u : L.spin_limit, &aborted);
     if (aborted && lane_id() == 0) {
         ctl->overflow_seq = L.seq;
         __threadfence();
@@ -224,7 +228,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             agg += row;
         }
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -304,6 +308,56 @@ __device__ __forceinline__ uint32_t wave_first_true(uint32_t lo, uint32_t hi, F 
     return lo;
 }
 
+// First x in [lo, hi) with fn(x) true, for fn false-then-true on [lo, hi]
+// and fn(hi - 1) known true. Probes backward from hi - 1 with strides that
+// grow 64-fold per round, so an edge d entries away costs about log64(d) + 1
+// rounds of probes near hi rather than a search over all of [lo, hi).
+template <typename F>
+__device__ __forceinline__ uint32_t wave_first_true_back(uint32_t lo, uint32_t hi, F fn) {
+    const int lane = lane_id();
+    int64_t known = (int64_t)hi - 1;  // fn(known) is true
+    int64_t stride = 1;
+    while (true) {
+        const int64_t x = known - stride * (lane + 1);
+        const bool inr = x >= (int64_t)lo;
+        const bool pc = fn(inr ? (uint32_t)x : lo);
+        const uint64_t m = __ballot(!(inr && pc));
+        if (m) {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            const int64_t xf = known - stride * (f + 1);  // false, or below lo
+            const int64_t xt = known - stride * f;        // true
+            const uint32_t a = (uint32_t)(xf + 1 > (int64_t)lo ? xf + 1 : (int64_t)lo);
+            return wave_first_true(a, (uint32_t)xt, fn);
+        }
+        known -= stride * 64;
+        stride *= 64;
+    }
+}
+
+// First x in [lo, hi) with fn(x) true, or hi, for fn false-then-true and
+// fn(lo) known false: the forward counterpart of wave_first_true_back.
+template <typename F>
+__device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi, F fn) {
+    const int lane = lane_id();
+    int64_t known = lo;  // fn(known) is false
+    int64_t stride = 1;
+    while (true) {
+        const int64_t x = known + stride * (lane + 1);
+        const bool inr = x < (int64_t)hi;
+        const bool pc = fn(inr ? (uint32_t)x : hi - 1);
+        const uint64_t m = __ballot(!inr || pc);
+        if (m) {
+            const int f = __ffsll((unsigned long long)m) - 1;
+            const int64_t xf = known + stride * f;        // false
+            const int64_t xt = known + stride * (f + 1);  // true, or at/after hi
+            const uint32_t b = (uint32_t)(xt < (int64_t)hi ? xt : (int64_t)hi);
+            return wave_first_true((uint32_t)xf + 1, b, fn);
+        }
+        known += stride * 64;
+        stride *= 64;
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask) {
     if (ctl->overflow) return;
@@ -371,7 +425,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 if (base > 0) {
                     const Key<W> kp = key_at(base - 1);
                     if (!eq_from<W>(kp, kb, bit)) s = base;
-                    else s = wave_first_true(0u, base, [&](uint32_t x) { return eq_from<W>(key_at(x), kb, bit); });
+                    else s = wave_first_true_back(0u, base, [&](uint32_t x) { return eq_from<W>(key_at(x), kb, bit); });
                 }
                 if (lane == 0) s_edge_s[v] = (int)s;
             } else {
@@ -379,7 +433,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 if (last + 1 < na) {
                     const Key<W> kn = key_at(last + 1);
                     if (!eq_from<W>(kn, kl, bit)) e = last + 1;
-                    else e = wave_first_true(last + 1, na, [&](uint32_t x) { return !eq_from<W>(key_at(x), kl, bit); });
+                    else e = wave_first_true_fwd(last + 1, na, [&](uint32_t x) { return !eq_from<W>(key_at(x), kl, bit); });
                 }
                 if (lane == 0) s_edge_e[v] = (int)e;
             }
@@ -458,12 +512,23 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                     tail_out[dest[i]] = src[i];
 #pragma unroll
                     for (int l = 0; l < W; ++l) bf.tkeys[l * stride + dest[i]] = key[i].l[l] ^ mask.l[l];
+                    if ((dest[i] & 63) == 63) {
+#pragma unroll
+                        for (int l = 0; l < W; ++l) bf.tsamp[(dest[i] >> 6) * W + l] = key[i].l[l] ^ mask.l[l];
+                    }
                 } else {
                     atomicOr(&ctl->err, 1u);
                 }
             }
         }
         __syncthreads();
+    }
+    // every 64th entry of the sum, for the merge partition
+    const uint32_t n = ctl->st[p].n;
+    for (uint32_t k = blockIdx.x * WG + t; 64ull * k < n; k += gridDim.x * WG) {
+        const Key<W> kk = load_key<W>(keys, stride, 64ull * k);
+#pragma unroll
+        for (int l = 0; l < W; ++l) bf.ssamp[(size_t)k * W + l] = kk.l[l];
     }
 }
 
@@ -543,20 +608,57 @@ struct K3Cfg {
 
 // Merge-path split of every tile boundary of this gate's merge, one wave per
 // boundary: splits[b] = head entries among the first min(b * TILE, n + na).
+//
+// The split at diagonal d is the first x in [max(0, d - na), min(d, n)) with
+// sum[x] > tail[d - 1 - x], or min(d, n). For d a multiple of 64 and x a
+// multiple of 64, d - 1 - x is 63 modulo 64, so the sampled entries alone
+// (every 64th sum entry, tail entries 63 modulo 64) locate the split to
+// within 64 positions; one more probe of 64 consecutive entries finds it.
+// The samples are small and shared by all boundaries, so the search reads
+// little beyond them.
 template <int W, int TILE>
 __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
+    static_assert(TILE % 64 == 0, "tile boundaries must fall on sampled diagonals");
     if (ctl->overflow) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
     if (na == 0) return;
     const uint32_t n = ctl->st[p].n;
     const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
+    const uint64_t* __restrict__ ss = bf.ssamp;
+    const uint64_t* __restrict__ ts = bf.tsamp;
+    const size_t stride = bf.cap;
     const uint32_t ntot = n + na;
     const uint32_t nb = (ntot + TILE - 1) / TILE + 1;
     for (uint32_t b = blockIdx.x * NWAVES + (threadIdx.x >> 6); b < nb; b += gridDim.x * NWAVES) {
-        const uint32_t d = min(b * (uint32_t)TILE, ntot);
-        const uint32_t s = merge_split<W>(keys, bf.cap, n, bf.tkeys, na, d);
-        if (lane_id() == 0) bf.splits[b] = s;
+        uint32_t split;
+        if (b == 0) {
+            split = 0;
+        } else if ((uint64_t)b * TILE >= ntot) {
+            split = n;
+        } else {
+            const uint32_t d = b * (uint32_t)TILE;
+            const uint32_t xlo = d > na ? d - na : 0u;
+            const uint32_t xhi = min(d, n);
+            // sampled points 64a in [xlo, xhi): tail partner d - 1 - 64a = 64c + 63
+            const uint32_t aL = (xlo + 63) / 64, aR = (xhi + 63) / 64;
+            const uint32_t cbase = d / 64 - 1;
+            const uint32_t aT = wave_first_true(aL, aR, [&](uint32_t a) {
+                Key<W> x, y;
+#pragma unroll
+                for (int l = 0; l < W; ++l) {
+                    x.l[l] = ss[(size_t)a * W + l];
+                    y.l[l] = ts[(size_t)(cbase - a) * W + l];
+                }
+                return !key_le<W>(x, y);
+            });
+            const uint32_t lo = aT > aL ? 64 * (aT - 1) + 1 : xlo;
+            const uint32_t hi = aT < aR ? 64 * aT : xhi;
+            split = wave_first_true(lo, hi, [&](uint32_t x) {
+                return !key_le<W>(load_key<W>(keys, stride, x), load_key<W>(bf.tkeys, stride, d - 1 - x));
+            });
+        }
+        if (lane_id() == 0) bf.splits[b] = split;
     }
 }
 
@@ -706,7 +808,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         const uint32_t mine = (uint32_t)__popc(keep);
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -871,7 +973,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         uint32_t agg;
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -969,7 +1071,7 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
             agg += row;
         }
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
@@ -1052,7 +1154,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (t < KNB_ROWS) s_pos[t] = pre;
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf.states, tile, agg);
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {

@@ -4,7 +4,12 @@
 //   keys[2]    W limb planes of cap uint64 each (limb l of term i at l*cap + i)
 //   coeffs[2]  cap x B doubles, row-major
 //   tail[2]    cap uint32 entries: index of an anticommuting term, sign in bit 31
+//   tkeys      W limb planes of cap words: permuted tail keys (term ^ mask)
 //   states     one look-back word per tile of the largest compaction launch
+//   gstates    one look-back word per group of 64 such tiles
+//   splits     merge-path split of every merge tile boundary
+//   ssamp      every 64th sum entry (W words each), written by the permutation
+//   tsamp      permuted tail entries 63 modulo 64 (W words each)
 //   ctl        the live size and buffer of the sum, double-buffered by gate
 //              parity, plus the overflow record (pp_kernels.h)
 // The host enqueues gates without reading sizes back; it synchronizes once
@@ -98,6 +103,12 @@ void free_terms(pp_handle* h) {
     if (h->bufs.states) hipFree(h->bufs.states);
     if (h->bufs.tkeys) hipFree(h->bufs.tkeys);
     if (h->bufs.splits) hipFree(h->bufs.splits);
+    if (h->bufs.gstates) hipFree(h->bufs.gstates);
+    if (h->bufs.ssamp) hipFree(h->bufs.ssamp);
+    if (h->bufs.tsamp) hipFree(h->bufs.tsamp);
+    h->bufs.gstates = nullptr;
+    h->bufs.ssamp = nullptr;
+    h->bufs.tsamp = nullptr;
     h->bufs.states = nullptr;
     h->bufs.tkeys = nullptr;
     h->bufs.splits = nullptr;
@@ -162,7 +173,12 @@ int grow(pp_handle* h, size_t newcap) {
     uint64_t* ns = nullptr;
     uint64_t* ntk = nullptr;
     uint32_t* nsp = nullptr;
+    uint64_t* ngs = nullptr;
+    uint64_t* nss = nullptr;
+    uint64_t* nts = nullptr;
+    const size_t nsamp = (newcap / 64 + 2) * W;
     const size_t nlb = tiles_needed(newcap, B);
+    const size_t ngw = nlb / 64 + 2;
     bool ok = true;
     for (int b = 0; b < 2 && ok; ++b) {
         ok = ok && hipMalloc(&nk[b], newcap * W * sizeof(uint64_t)) == hipSuccess;
@@ -172,6 +188,9 @@ int grow(pp_handle* h, size_t newcap) {
     ok = ok && hipMalloc(&ns, nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nsp, nlb * sizeof(uint32_t)) == hipSuccess;
+    ok = ok && hipMalloc(&ngs, ngw * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&nss, nsamp * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&nts, nsamp * sizeof(uint64_t)) == hipSuccess;
     if (!ok) {
         for (int b = 0; b < 2; ++b) {
             if (nk[b]) hipFree(nk[b]);
@@ -181,10 +200,14 @@ int grow(pp_handle* h, size_t newcap) {
         if (ns) hipFree(ns);
         if (ntk) hipFree(ntk);
         if (nsp) hipFree(nsp);
+        if (ngs) hipFree(ngs);
+        if (nss) hipFree(nss);
+        if (nts) hipFree(nts);
         (void)hipGetLastError();
         return fail(PP_ERR_NOMEM, "device allocation of " + std::to_string(newcap) + " terms failed");
     }
     HIP_TRY(hipMemsetAsync(ns, 0, nlb * sizeof(uint64_t), h->stream));
+    HIP_TRY(hipMemsetAsync(ngs, 0, ngw * sizeof(uint64_t), h->stream));
     if (n > 0) {
         for (int l = 0; l < W; ++l)
             HIP_TRY(hipMemcpyAsync(nk[0] + (size_t)l * newcap, h->bufs.keys[cur] + (size_t)l * h->cap,
@@ -202,6 +225,9 @@ int grow(pp_handle* h, size_t newcap) {
     h->bufs.states = ns;
     h->bufs.tkeys = ntk;
     h->bufs.splits = nsp;
+    h->bufs.gstates = ngs;
+    h->bufs.ssamp = nss;
+    h->bufs.tsamp = nts;
     h->bufs.cap = (uint32_t)newcap;
     h->lb_cap = nlb;
     h->cap = newcap;
@@ -214,6 +240,7 @@ uint32_t next_epoch(pp_handle* h) {
     if (h->epoch == 0) {
         // wrapped: words of old launches could match again, so clear them
         hipMemsetAsync(h->bufs.states, 0, h->lb_cap * sizeof(uint64_t), h->stream);
+        hipMemsetAsync(h->bufs.gstates, 0, (h->lb_cap / 64 + 2) * sizeof(uint64_t), h->stream);
         h->epoch = 1;
     }
     return h->epoch;
