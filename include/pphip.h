@@ -169,8 +169,11 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * kernels' occupancy. Every look-back wait is bounded by PP_TUNE_SPIN_LIMIT
  * polls; a gate whose wait runs out is run again with tiles claimed through an
  * atomic counter, which needs no residency (counted in pp_profile.fallbacks).
- * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always. */
-enum pp_tune_key { PP_TUNE_SPIN_LIMIT = 1, PP_TUNE_DYNAMIC_TILES = 2 };
+ * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always.
+ * PP_TUNE_CHAIN_TAILS != 0 (the default) lets the merge of a rotation in
+ * pp_run_circuit list the terms the next rotation touches, so that rotation
+ * skips its own pass over the sum; 0 turns this off (results are identical). */
+enum pp_tune_key { PP_TUNE_SPIN_LIMIT = 1, PP_TUNE_DYNAMIC_TILES = 2, PP_TUNE_CHAIN_TAILS = 3 };
 
 /* Set a tuning value. No counterpart in the reference. */
 int pp_tune(pp_handle *h, int key, int64_t value);

@@ -275,28 +275,49 @@ __device__ __forceinline__ uint32_t lb_status(uint64_t w, uint32_t ep32) {
     return (((w >> 40) & LB_EPOCH_MASK) == ep32) ? (uint32_t)(w >> 62) : 0u;
 }
 
-// One poll of a window of up to 64 predecessor words, nearest first: lane l
-// reads word `base - l` when l < count. Finds the nearest inclusive prefix in
-// the window; returns false while a word up to it (or up to the window's end,
-// when it holds none) is not ready. On success *sum is the window's
-// contribution to the exclusive prefix and *found tells whether it ended at
-// an inclusive prefix.
-__device__ __forceinline__ bool lb_window(const uint64_t* words, int64_t base, int count, uint32_t ep32,
-                                          uint64_t* sum, bool* found) {
+// One poll of a window of up to 64 predecessor words, nearest first, for NV
+// independent prefix chains whose words lie `cstride` words apart: lane l reads
+// word `base - l` of every chain when l < count. For each chain still active
+// it finds the nearest inclusive prefix in the window; the poll fails while a
+// word up to it (or up to the window's end, when it holds none) is not ready.
+// On success sum[c] is the window's contribution to chain c's exclusive prefix
+// and found[c] tells whether that chain ended at an inclusive prefix.
+template <int NV>
+__device__ __forceinline__ bool lb_window(const uint64_t* words, size_t cstride, int64_t base, int count,
+                                          uint32_t ep32, const bool (&active)[NV], uint64_t (&sum)[NV],
+                                          bool (&found)[NV]) {
     const int lane = lane_id();
     const bool valid = lane < count;
-    const uint64_t w = lb_load(words + (valid && base - lane >= 0 ? base - lane : 0));
-    const uint32_t st = valid ? lb_status(w, ep32) : 0u;
-    const uint64_t pm = __ballot(st == 2u);
-    const int P = pm ? (__ffsll((unsigned long long)pm) - 1) : count;
-    if (__ballot(lane < P && st == 0u) != 0) return false;
-    *sum = wave_sum_u64(lane <= P && valid ? (w & LB_VAL_MASK) : 0ull);
-    *found = P < count;
+    const size_t idx = valid && base - lane >= 0 ? (size_t)(base - lane) : 0;
+    uint64_t w[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) w[c] = lb_load(words + c * cstride + idx);
+    uint64_t s[NV];
+    bool f[NV];
+    bool ok = true;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        const uint32_t st = valid ? lb_status(w[c], ep32) : 0u;
+        const uint64_t pm = __ballot(st == 2u);
+        const int P = pm ? (__ffsll((unsigned long long)pm) - 1) : count;
+        ok = ok && (!active[c] || __ballot(lane < P && st == 0u) == 0);
+        s[c] = wave_sum_u64(lane <= P && valid ? (w[c] & LB_VAL_MASK) : 0ull);
+        f[c] = P < count;
+    }
+    if (!ok) return false;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        sum[c] = active[c] ? s[c] : 0ull;
+        found[c] = active[c] ? f[c] : true;
+    }
     return true;
 }
 
-// Two-level decoupled look-back. Called by all 64 lanes of one wave; publishes
-// `agg` for `tile` and returns the exclusive prefix of all earlier tiles.
+// Two-level decoupled look-back over NV prefix chains at once (the chains
+// share every poll). Called by all 64 lanes of one wave; publishes agg[c] for
+// `tile` and returns in excl[c] the exclusive prefix of all earlier tiles.
+// Tile words of chain c start at states + c * tstride, group words at
+// gstates + c * gstride.
 //
 // Tiles form groups of 64. A tile first reads its own group's earlier words;
 // when none of them holds an inclusive prefix it adds the group-level words
@@ -306,57 +327,81 @@ __device__ __forceinline__ bool lb_window(const uint64_t* words, int64_t base, i
 // where a flat look-back walks back through every running tile, 64 per poll.
 //
 // spin_limit bounds the polls of one wait (0 = unbounded). When it runs out,
-// *aborted is set and the partial sum read so far is returned: it never
-// exceeds the true prefix, so positions built from it stay in range.
-__device__ uint64_t lb_exclusive(uint64_t* states, uint64_t* gstates, uint32_t tile, uint32_t epoch, uint64_t agg,
-                                 uint32_t spin_limit, bool* aborted) {
+// *aborted is set and the partial sums read so far are returned: they never
+// exceed the true prefixes, so positions built from them stay in range.
+template <int NV>
+__device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates, size_t gstride, uint32_t tile,
+                             uint32_t epoch, const uint64_t (&agg)[NV], uint32_t spin_limit, bool* aborted,
+                             uint64_t (&excl)[NV]) {
     const int lane = lane_id();
     const uint32_t ep32 = epoch & LB_EPOCH_MASK;
     const uint64_t ep = (uint64_t)ep32 << 40;
     const uint32_t g = tile >> 6;
     const int r = (int)(tile & 63u);
     const bool last_of_group = r == 63;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) excl[c] = 0;
     if (tile == 0) {
-        if (lane == 0) lb_store(states, (2ull << 62) | ep | agg);
-        return 0;
+        if (lane == 0)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) lb_store(states + c * tstride, (2ull << 62) | ep | agg[c]);
+        return;
     }
-    if (lane == 0) lb_store(states + tile, (1ull << 62) | ep | agg);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) lb_store(states + c * tstride + tile, (1ull << 62) | ep | agg[c]);
     uint32_t spins = 0;
-    uint64_t excl = 0;
-    bool found = false;
+    bool active[NV], found[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) active[c] = true;
     // this group's earlier tiles (tile 0 of the first group is always inclusive)
-    while (!lb_window(states, (int64_t)tile - 1, r, ep32, &excl, &found)) {
+    while (!lb_window<NV>(states, tstride, (int64_t)tile - 1, r, ep32, active, excl, found)) {
         if (spin_limit && ++spins > spin_limit) {
             *aborted = true;
-            return 0;
+#pragma unroll
+            for (int c = 0; c < NV; ++c) excl[c] = 0;
+            return;
         }
         __builtin_amdgcn_s_sleep(1);
     }
-    if (!found) {
-        if (last_of_group && lane == 0) lb_store(gstates + g, (1ull << 62) | ep | (excl + agg));
-        int64_t gb = (int64_t)g - 1;
-        while (true) {
-            uint64_t part = 0;
-            bool gfound = false;
-            const int cnt = gb + 1 < 64 ? (int)(gb + 1) : 64;
-            if (lb_window(gstates, gb, cnt, ep32, &part, &gfound)) {
-                excl += part;
-                if (gfound || gb < 64) break;  // group 0's word is inclusive once ready
-                gb -= 64;
-                continue;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        active[c] = !found[c];
+        any = any || active[c];
+        if (active[c] && last_of_group && lane == 0)
+            lb_store(gstates + c * gstride + g, (1ull << 62) | ep | (excl[c] + agg[c]));
+    }
+    int64_t gb = (int64_t)g - 1;
+    while (any) {
+        uint64_t part[NV];
+        bool gfound[NV];
+        const int cnt = gb + 1 < 64 ? (int)(gb + 1) : 64;
+        if (lb_window<NV>(gstates, gstride, gb, cnt, ep32, active, part, gfound)) {
+            any = false;
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+                excl[c] += part[c];
+                // group 0's word is inclusive once ready
+                active[c] = active[c] && !gfound[c] && gb >= 64;
+                any = any || active[c];
             }
-            if (spin_limit && ++spins > spin_limit) {
-                *aborted = true;
-                return excl;
-            }
-            __builtin_amdgcn_s_sleep(1);
+            gb -= 64;
+            continue;
         }
+        if (spin_limit && ++spins > spin_limit) {
+            *aborted = true;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
     if (lane == 0) {
-        lb_store(states + tile, (2ull << 62) | ep | (excl + agg));
-        if (last_of_group) lb_store(gstates + g, (2ull << 62) | ep | (excl + agg));
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            lb_store(states + c * tstride + tile, (2ull << 62) | ep | (excl[c] + agg[c]));
+            if (last_of_group) lb_store(gstates + c * gstride + g, (2ull << 62) | ep | (excl[c] + agg[c]));
+        }
     }
-    return excl;
 }
 
 }  // namespace pp

@@ -40,12 +40,14 @@ struct Bufs {
     double* coeffs[2];   // cap x B
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
-    uint64_t* states;    // look-back words, one per tile
-    uint64_t* gstates;   // look-back words, one per group of 64 tiles
+    uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile
+    uint64_t* gstates;   // look-back words: two chains of lb_groups words, one per 64 tiles
     uint32_t* splits;    // merge-path split of every merge tile boundary
     uint64_t* ssamp;     // W words per sample: sum entries 0, 64, 128, ...
     uint64_t* tsamp;     // W words per sample: permuted tail entries 63, 127, ...
     uint32_t cap;
+    uint32_t lb_tiles;   // words per chain in states
+    uint32_t lb_groups;  // words per chain in gstates
 };
 
 struct GateState {
@@ -62,7 +64,13 @@ struct Ctl {
     uint32_t overflow_need;  // terms it needed
     uint32_t err;            // internal consistency bits
     uint32_t counter[2];     // tile counters; a look-back launch uses counter[slot], clears the other
-    uint32_t pad[2];
+    // A merge may also list the touched terms of the gate after it (its output
+    // tested against that gate's mask): tail[0] and st[].na then hold gate
+    // tail_seq's touched terms when tail_ready is set and the gate's mask
+    // equals tail_mask.
+    uint32_t tail_ready;
+    uint32_t tail_seq;
+    uint64_t tail_mask[4];
 };
 
 struct Launch {
@@ -100,17 +108,25 @@ struct TileSource {
 
 // Exclusive prefix of this tile through the look-back; a wait that runs out
 // stops every later kernel and records the gate for the host to run again.
-__device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
-                                              uint64_t agg) {
+template <int NV>
+__device__ __forceinline__ void lb_prefix_n(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
+                                            const uint64_t (&agg)[NV], uint64_t (&excl)[NV]) {
     bool aborted = false;
-    const uint64_t excl =
-        lb_exclusive(bf.states, bf.gstates, tile, L.epoch, agg, L.dynamic ? 0u : L.spin_limit, &aborted);
+    lb_exclusive<NV>(bf.states, bf.lb_tiles, bf.gstates, bf.lb_groups, tile, L.epoch, agg,
+                     L.dynamic ? 0u : L.spin_limit, &aborted, excl);
     if (aborted && lane_id() == 0) {
         ctl->overflow_seq = L.seq;
         __threadfence();
         atomicMax(&ctl->overflow, 2u);
     }
-    return excl;
+}
+
+__device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
+                                              uint64_t agg) {
+    const uint64_t a[1] = {agg};
+    uint64_t e[1];
+    lb_prefix_n<1>(ctl, L, bf, tile, a, e);
+    return e[0];
 }
 
 // a[i] for a small register array and a runtime index, without spilling it
@@ -157,6 +173,27 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, ui
 constexpr int K1_ITEMS = 16;
 constexpr int K1_TILE = WG * K1_ITEMS;
 
+// End of the branch step, once the touched count na is known: check room for
+// the na created terms, or hand the sum on unchanged when nothing is touched.
+__device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const Bufs& bf, int p, uint32_t n,
+                                              uint32_t cur, uint32_t na) {
+    ctl->st[p].na = na;
+    if (L.glog) {
+        L.glog[0] = n;
+        L.glog[1] = na;
+        L.glog[2] = n;
+    }
+    if ((uint64_t)n + na > (uint64_t)bf.cap) {
+        ctl->overflow_seq = L.seq;
+        const uint64_t need = (uint64_t)n + na;
+        ctl->overflow_need = need > 0xffffffffull ? 0xffffffffu : (uint32_t)need;
+        __threadfence();
+        ctl->overflow = 1u;
+    } else {
+        set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes
+    }
+}
+
 template <int W>
 __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
     if (!lb_start(ctl, L)) return;
@@ -169,6 +206,14 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             set_state(ctl, p ^ 1, 0u, cur);
             if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
         }
+        return;
+    }
+    // the touched terms may already be listed by the previous gate's merge
+    bool listed = ctl->tail_ready && ctl->tail_seq == L.seq;
+#pragma unroll
+    for (int l = 0; l < W; ++l) listed = listed && ctl->tail_mask[l] == mask.l[l];
+    if (listed) {
+        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, ctl->st[p].na);
         return;
     }
     const uint64_t* __restrict__ keys = bf.keys[cur];
@@ -231,24 +276,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
-                if (tile == ntiles - 1) {
-                    const uint32_t na = (uint32_t)(excl + agg);
-                    ctl->st[p].na = na;
-                    if (L.glog) {
-                        L.glog[0] = n;
-                        L.glog[1] = na;
-                        L.glog[2] = n;
-                    }
-                    if ((uint64_t)n + na > (uint64_t)bf.cap) {
-                        ctl->overflow_seq = L.seq;
-                        const uint64_t need = (uint64_t)n + na;
-                        ctl->overflow_need = need > 0xffffffffull ? 0xffffffffu : (uint32_t)need;
-                        __threadfence();
-                        ctl->overflow = 1u;
-                    } else {
-                        set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes
-                    }
-                }
+                if (tile == ntiles - 1) branch_finish(ctl, L, bf, p, n, cur, (uint32_t)(excl + agg));
             }
         }
         __syncthreads();
@@ -711,9 +739,16 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
     }
 }
 
-template <int W>
-__global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp) {
+// With EMIT, the kept output terms are also tested against `nmask`, the mask
+// of the next gate (a rotation), and the touched ones are listed in tail[0] in
+// output order, as k_branch of that gate would list them: the next k_branch
+// then only checks room. The list's offsets come from a second look-back
+// chain that shares every poll with the output offsets.
+template <int W, bool EMIT>
+__global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
+                                               Key<W> nmask) {
     constexpr int ITEMS = K3Cfg<W>::ITEMS, TILE = K3Cfg<W>::TILE;
+    static_assert(TILE < 65536, "kept and touched counts of a tile are packed in 16 bits each");
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
@@ -730,8 +765,9 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ double s_val[TILE + 2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
-    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_excl, s_excl_t;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    uint32_t* __restrict__ next_tail = bf.tail[0];
     TileSource ts;
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
@@ -766,7 +802,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         const int dtot = ns + nt;
         int slot[ITEMS];
         double oc[ITEMS];
-        uint32_t keep = 0;
+        uint32_t keep = 0, touch = 0, neg = 0;
         if (dl < dtot) {
             int lo = dl > nt ? dl - nt : 0, hi = min(dl, ns);
             while (lo < hi) {
@@ -800,32 +836,63 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                     ko = kt;
                     ++y;
                 }
-                if (live && !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs))
+                if (live && !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs)) {
                     keep |= 1u << k;
+                    if (EMIT && anticommutes<W>(nmask, ko)) {
+                        touch |= 1u << k;
+                        if ((sign_exponent<W>(nmask, ko) & 2) == 0) neg |= 1u << k;
+                    }
+                }
             }
         }
+        // kept count in the low half, touched count in the high half
         uint32_t agg;
-        const uint32_t mine = (uint32_t)__popc(keep);
+        const uint32_t mine = (uint32_t)__popc(keep) | ((uint32_t)__popc(touch) << 16);
         const uint32_t pre = block_excl_sum(mine, s_w, agg);
         if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
-                s_excl = excl;
-                if (tile == ntiles - 1) {
-                    const uint32_t nout = (uint32_t)(excl + agg);
-                    set_state(ctl, p ^ 1, nout, cur ^ 1);
-                    if (L.glog) L.glog[2] = nout;
+            if (EMIT) {
+                const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
+                uint64_t e2[2];
+                lb_prefix_n<2>(ctl, L, bf, tile, a2, e2);
+                if (lane == 0) {
+                    s_excl = e2[0];
+                    s_excl_t = e2[1];
+                    if (tile == ntiles - 1) {
+                        const uint32_t nout = (uint32_t)(e2[0] + a2[0]);
+                        set_state(ctl, p ^ 1, nout, cur ^ 1);
+                        ctl->st[p ^ 1].na = (uint32_t)(e2[1] + a2[1]);
+#pragma unroll
+                        for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                        ctl->tail_seq = L.seq + 1;
+                        ctl->tail_ready = 1u;
+                        if (L.glog) L.glog[2] = nout;
+                    }
+                }
+            } else {
+                const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg & 0xffffu);
+                if (lane == 0) {
+                    s_excl = excl;
+                    if (tile == ntiles - 1) {
+                        const uint32_t nout = (uint32_t)(excl + (agg & 0xffffu));
+                        set_state(ctl, p ^ 1, nout, cur ^ 1);
+                        if (L.glog) L.glog[2] = nout;
+                    }
                 }
             }
         }
         __syncthreads();
-        uint64_t pos = s_excl + pre;
+        uint64_t pos = s_excl + (pre & 0xffffu);
+        uint64_t tpos = EMIT ? s_excl_t + (pre >> 16) : 0;
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
             if ((keep >> k) & 1u) {
 #pragma unroll
                 for (int l = 0; l < W; ++l) out_keys[l * stride + pos] = s_key[l][slot[k]];
                 out_coeffs[pos] = oc[k];
+                if (EMIT && ((touch >> k) & 1u)) {
+                    next_tail[tpos] = (uint32_t)pos | (((neg >> k) & 1u) ? SRC_NEG : 0u);
+                    ++tpos;
+                }
                 ++pos;
             }
         }

@@ -61,6 +61,7 @@ struct pp_handle {
     std::unordered_map<const void*, unsigned> resident;  // co-resident workgroups per kernel
     uint32_t spin_limit = 1u << 16;  // bound of one look-back wait with static tiles
     bool force_dynamic = false;      // take tiles from the atomic counter always
+    bool chain_tails = true;         // merges list the next rotation's touched terms
     uint32_t safe_until = 0;         // gates before this sequence number use dynamic tiles
     int64_t fallbacks = 0;           // gates run again with dynamic tiles
     Ctl* d_ctl = nullptr;
@@ -179,16 +180,17 @@ int grow(pp_handle* h, size_t newcap) {
     const size_t nsamp = (newcap / 64 + 2) * W;
     const size_t nlb = tiles_needed(newcap, B);
     const size_t ngw = nlb / 64 + 2;
+    // two look-back chains (merge offsets and the next gate's touched list)
     bool ok = true;
     for (int b = 0; b < 2 && ok; ++b) {
         ok = ok && hipMalloc(&nk[b], newcap * W * sizeof(uint64_t)) == hipSuccess;
         ok = ok && hipMalloc(&nc[b], newcap * B * sizeof(double)) == hipSuccess;
         ok = ok && hipMalloc(&nt[b], newcap * sizeof(uint32_t)) == hipSuccess;
     }
-    ok = ok && hipMalloc(&ns, nlb * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&ns, 2 * nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nsp, nlb * sizeof(uint32_t)) == hipSuccess;
-    ok = ok && hipMalloc(&ngs, ngw * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&ngs, 2 * ngw * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nss, nsamp * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nts, nsamp * sizeof(uint64_t)) == hipSuccess;
     if (!ok) {
@@ -206,8 +208,8 @@ int grow(pp_handle* h, size_t newcap) {
         (void)hipGetLastError();
         return fail(PP_ERR_NOMEM, "device allocation of " + std::to_string(newcap) + " terms failed");
     }
-    HIP_TRY(hipMemsetAsync(ns, 0, nlb * sizeof(uint64_t), h->stream));
-    HIP_TRY(hipMemsetAsync(ngs, 0, ngw * sizeof(uint64_t), h->stream));
+    HIP_TRY(hipMemsetAsync(ns, 0, 2 * nlb * sizeof(uint64_t), h->stream));
+    HIP_TRY(hipMemsetAsync(ngs, 0, 2 * ngw * sizeof(uint64_t), h->stream));
     if (n > 0) {
         for (int l = 0; l < W; ++l)
             HIP_TRY(hipMemcpyAsync(nk[0] + (size_t)l * newcap, h->bufs.keys[cur] + (size_t)l * h->cap,
@@ -226,6 +228,8 @@ int grow(pp_handle* h, size_t newcap) {
     h->bufs.tkeys = ntk;
     h->bufs.splits = nsp;
     h->bufs.gstates = ngs;
+    h->bufs.lb_tiles = (uint32_t)nlb;
+    h->bufs.lb_groups = (uint32_t)ngw;
     h->bufs.ssamp = nss;
     h->bufs.tsamp = nts;
     h->bufs.cap = (uint32_t)newcap;
@@ -239,8 +243,8 @@ uint32_t next_epoch(pp_handle* h) {
     h->epoch = (h->epoch + 1) & LB_EPOCH_MASK;
     if (h->epoch == 0) {
         // wrapped: words of old launches could match again, so clear them
-        hipMemsetAsync(h->bufs.states, 0, h->lb_cap * sizeof(uint64_t), h->stream);
-        hipMemsetAsync(h->bufs.gstates, 0, (h->lb_cap / 64 + 2) * sizeof(uint64_t), h->stream);
+        hipMemsetAsync(h->bufs.states, 0, 2 * (size_t)h->bufs.lb_tiles * sizeof(uint64_t), h->stream);
+        hipMemsetAsync(h->bufs.gstates, 0, 2 * (size_t)h->bufs.lb_groups * sizeof(uint64_t), h->stream);
         h->epoch = 1;
     }
     return h->epoch;
@@ -384,9 +388,11 @@ int check_mask(const pp_handle* h, const uint64_t* mask) {
 // ---------------------------------------------------------------- one gate, enqueued
 
 // Rotation: cos/sin by value for B = 1, or at d_trig (B cos then B sin) for B > 1.
+// next_mask: the mask of the rotation enqueued right after this one, if any;
+// the merge then lists that gate's touched terms (B = 1).
 template <int W>
 void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
-                  double min_abs, uint32_t* glog, hipEvent_t* ev) {
+                  double min_abs, uint32_t* glog, hipEvent_t* ev, const uint64_t* next_mask) {
     const Key<W> mask = to_key<W>(mask_in);
     Levels lv;
     mask_levels(mask_in, W, &lv);
@@ -412,8 +418,15 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         LP.seq = h->seq;
         hipLaunchKernelGGL((k_partition<W, TILE>), dim3(grid_for(h, expect, (size_t)TILE * NWAVES)), dim3(WG), 0,
                            h->stream, h->bufs, h->d_ctl, LP);
-        hipLaunchKernelGGL(k_merge1<W>, dim3(lb_grid(h, (const void*)k_merge1<W>, 0, expect, TILE)), dim3(WG), 0,
-                           h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp);
+        if (next_mask) {
+            hipLaunchKernelGGL((k_merge1<W, true>), dim3(lb_grid(h, (const void*)k_merge1<W, true>, 0, expect, TILE)),
+                               dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp,
+                               to_key<W>(next_mask));
+        } else {
+            hipLaunchKernelGGL((k_merge1<W, false>),
+                               dim3(lb_grid(h, (const void*)k_merge1<W, false>, 0, expect, TILE)), dim3(WG), 0,
+                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+        }
     } else {
         const int B = h->B;
         const int TB = tile_rows(B);
@@ -463,15 +476,16 @@ struct HostGate {
     double a, b;           // B = 1: cos, sin (rotation) or factor (noise)
 };
 
-int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs) {
+int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs,
+            const uint64_t* next_mask) {
     uint32_t* glog = h->d_log + 3 * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
     if (g.kind == PP_GATE_ROTATION) {
         switch (h->W) {
-        case 1: enq_rotation<1>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
-        case 2: enq_rotation<2>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
-        default: enq_rotation<4>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        case 1: enq_rotation<1>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev, next_mask); break;
+        case 2: enq_rotation<2>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev, next_mask); break;
+        default: enq_rotation<4>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev, next_mask); break;
         }
     } else {
         switch (h->W) {
@@ -492,8 +506,10 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, 
     while (g < ng) {
         const size_t end = std::min(ng, g + RUN_CHUNK);
         const uint32_t seq0 = h->seq;
-        for (size_t i = g; i < end; ++i)
-            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs)) return rc;
+        for (size_t i = g; i < end; ++i) {
+            const bool chain = h->B == 1 && h->chain_tails && i + 1 < ng && gs[i + 1].kind == PP_GATE_ROTATION;
+            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[i + 1].mask : nullptr)) return rc;
+        }
         if (int rc = sync_ctl(h)) return rc;
         if (h->h_ctl->err)
             return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
@@ -868,6 +884,9 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         return PP_OK;
     case PP_TUNE_DYNAMIC_TILES:
         h->force_dynamic = value != 0;
+        return PP_OK;
+    case PP_TUNE_CHAIN_TAILS:
+        h->chain_tails = value != 0;
         return PP_OK;
     default:
         return fail(PP_ERR_ARG, "unknown tuning key");

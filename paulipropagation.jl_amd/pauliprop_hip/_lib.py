@@ -65,7 +65,7 @@ SIGNATURES = [
     ("pp_tune", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
 ]
 
-PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES = 1, 2
+PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS = 1, 2, 3
 
 
 def lib():

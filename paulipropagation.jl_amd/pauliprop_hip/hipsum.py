@@ -139,13 +139,16 @@ class HIPPauliSum:
         """timing: 1 records HIP events around every kernel, 0 stops, -1 leaves it."""
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
-    def tune(self, spin_limit: int = None, dynamic_tiles: bool = None) -> None:
-        """Bound of a look-back wait (polls), and whether tiles are always claimed atomically."""
+    def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None) -> None:
+        """Bound of a look-back wait (polls); whether tiles are always claimed
+        atomically; whether a merge lists the next rotation's touched terms."""
         L = _lib.lib()
         if spin_limit is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SPIN_LIMIT, int(spin_limit)))
         if dynamic_tiles is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_DYNAMIC_TILES, int(bool(dynamic_tiles))))
+        if chain_tails is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_CHAIN_TAILS, int(bool(chain_tails))))
 
     def profile(self) -> dict:
         p = _lib.pp_profile()
