@@ -126,6 +126,22 @@ int pp_run_circuit(pp_handle *h, const pp_gate *gates, size_t ngates, const doub
                    const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
                    double min_abs_coeff, int64_t *counts);
 
+/* Loss and gradient by the adjoint sweep: loss_b = overlapwithzero of column
+ * b after the Heisenberg-ordered gate list (as pp_run_circuit), and
+ * grad[p * B + b] = d loss_b / d theta[p * B + b]. The forward run records,
+ * per gate, where every input term went and the touched terms' coefficients;
+ * the sweep then applies the transpose of each gate's 1 / cos / +-sin map to
+ * the coefficient adjoints, last gate first, with every truncation held as it
+ * happened (noise gates are linear maps without parameters). checkpoint_gates
+ * = 0 records the whole run; k > 0 records k gates at a time, each segment run
+ * again from a snapshot of its input sum (one extra forward run), so records
+ * need room for k gates only. On return the handle holds the propagated sum.
+ * Replaces: rewindgradient(circuit, psum, params, overlapwithzero; kwargs...)
+ *           (src/gradient.jl:26-82), which rejects noise; here noise is allowed. */
+int pp_adjoint(pp_handle *h, const pp_gate *gates, size_t ngates, const double *theta, size_t nparams,
+               const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
+               double min_abs_coeff, size_t checkpoint_gates, double *loss, double *grad);
+
 /* overlapwithzero for every column: sum of the coefficients of terms with no
  * X or Y, accumulated in double-double and rounded once.
  * Replaces: overlapwithzero(::VectorPauliSum) (src/stateoverlap.jl:12-35). */

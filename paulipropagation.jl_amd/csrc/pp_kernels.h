@@ -48,20 +48,39 @@ struct Bufs {
     uint32_t cap;
     uint32_t lb_tiles;   // words per chain in states
     uint32_t lb_groups;  // words per chain in gstates
+    // Recording for the adjoint sweep (all null when not recording):
+    uint32_t* tpj;       // permuted tail entry -> its index in parent order
+    uint32_t* rec;       // per-gate record words, from the gate's roff
+    double* rcoef;       // per-gate coefficient rows of the touched terms, from the gate's coff
+    uint32_t rec_words;  // capacity of rec
+    uint32_t rec_rows;   // capacity of rcoef, in rows of B
 };
+
+// A gate's records. Rotation touching na > 0 of n terms, at rec + roff:
+//   [0, n)          term i: REC_FLAG if it anticommutes, | its output position
+//   [n, n + na)     touched term j (parent order): its index | SRC_NEG sign bit
+//   [n + na, +na)   touched term j: output position of its created term
+// and its coefficient row at rcoef + coff * B. Noise on n terms, at rec + roff:
+//   [0, n)          term i: REC_FLAG if damped, | its output position.
+// A position is REC_NONE when that output was truncated or never existed.
+constexpr uint32_t REC_FLAG = 0x80000000u;
+constexpr uint32_t REC_NONE = 0x7fffffffu;
 
 struct GateState {
     uint32_t n;     // live terms entering the gate
     uint32_t cur;   // buffer holding them
     uint32_t na;    // touched (anticommuting) terms, written by k_branch
-    uint32_t pad;
+    uint32_t roff;  // where the gate's records start (recording only)
+    uint32_t coff;  // where its coefficient rows start
+    uint32_t pad[3];
 };
 
 struct Ctl {
     GateState st[2];
     uint32_t overflow;       // sticky: a gate needed more room than cap
     uint32_t overflow_seq;   // that gate's sequence number
-    uint32_t overflow_need;  // terms it needed
+    uint32_t overflow_need;  // terms it needed (overflow 1), or record words (overflow 3)
+    uint32_t overflow_rows;  // record rows it needed (overflow 3)
     uint32_t err;            // internal consistency bits
     uint32_t counter[2];     // tile counters; a look-back launch uses counter[slot], clears the other
     // A merge may also list the touched terms of the gate after it (its output
@@ -80,8 +99,10 @@ struct Launch {
     uint32_t dynamic;     // 1: tiles from the atomic counter; 0: tile = blockIdx.x + k * gridDim.x
     uint32_t spin_limit;  // bound of one look-back wait with static tiles
     uint32_t pad;
-    uint32_t* glog;       // n_in, n_touched, n_out of this gate, or null
+    uint32_t* glog;       // n_in, n_touched, n_out, roff, coff of this gate, or null
 };
+
+constexpr int LOG_WORDS = 5;
 
 // Tile ids of a look-back kernel. Static tiles (tile = blockIdx.x + k * gridDim.x)
 // need no atomics; they rely on every workgroup of the grid being resident at
@@ -152,6 +173,26 @@ __device__ __forceinline__ void set_state(Ctl* ctl, int p, uint32_t n, uint32_t 
     ctl->st[p].na = 0u;
 }
 
+// The records of a gate do not fit: stop, and let the host grow them and run
+// the gate again.
+__device__ __forceinline__ void record_overflow(Ctl* ctl, const Launch& L, uint64_t words, uint64_t rows) {
+    ctl->overflow_seq = L.seq;
+    ctl->overflow_need = words > 0xffffffffull ? 0xffffffffu : (uint32_t)words;
+    ctl->overflow_rows = rows > 0xffffffffull ? 0xffffffffu : (uint32_t)rows;
+    __threadfence();
+    atomicMax(&ctl->overflow, 3u);
+}
+
+// The next gate's records start where this gate's start (it records nothing).
+__device__ __forceinline__ void carry_records(Ctl* ctl, const Launch& L, int p) {
+    ctl->st[p ^ 1].roff = ctl->st[p].roff;
+    ctl->st[p ^ 1].coff = ctl->st[p].coff;
+    if (L.glog) {
+        L.glog[3] = ctl->st[p].roff;
+        L.glog[4] = ctl->st[p].coff;
+    }
+}
+
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, uint32_t& total) {
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     const uint32_t incl = wave_incl_sum_u32(v);
@@ -178,10 +219,16 @@ constexpr int K1_TILE = WG * K1_ITEMS;
 __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const Bufs& bf, int p, uint32_t n,
                                               uint32_t cur, uint32_t na) {
     ctl->st[p].na = na;
+    const uint32_t roff = ctl->st[p].roff, coff = ctl->st[p].coff;
+    // a rotation touching nothing is the identity: no records
+    const uint64_t words = (bf.rec && na) ? (uint64_t)n + 2ull * na : 0ull;
+    const uint64_t rows = (bf.rec && na) ? (uint64_t)na : 0ull;
     if (L.glog) {
         L.glog[0] = n;
         L.glog[1] = na;
         L.glog[2] = n;
+        L.glog[3] = roff;
+        L.glog[4] = coff;
     }
     if ((uint64_t)n + na > (uint64_t)bf.cap) {
         ctl->overflow_seq = L.seq;
@@ -189,8 +236,12 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
         ctl->overflow_need = need > 0xffffffffull ? 0xffffffffu : (uint32_t)need;
         __threadfence();
         ctl->overflow = 1u;
+    } else if (roff + words > bf.rec_words || coff + rows > bf.rec_rows) {
+        record_overflow(ctl, L, roff + words, coff + rows);
     } else {
         set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes
+        ctl->st[p ^ 1].roff = (uint32_t)(roff + words);
+        ctl->st[p ^ 1].coff = (uint32_t)(coff + rows);
     }
 }
 
@@ -205,6 +256,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             ctl->st[p].na = 0u;
             set_state(ctl, p ^ 1, 0u, cur);
             if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
+            carry_records(ctl, L, p);
         }
         return;
     }
@@ -538,6 +590,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 // a rank outside [0, na) would be a bug; it is reported, never written
                 if (dest[i] >= 0 && dest[i] < (int64_t)na) {
                     tail_out[dest[i]] = src[i];
+                    if (bf.tpj) bf.tpj[dest[i]] = j0 + i;
 #pragma unroll
                     for (int l = 0; l < W; ++l) bf.tkeys[l * stride + dest[i]] = key[i].l[l] ^ mask.l[l];
                     if ((dest[i] & 63) == 63) {
@@ -557,6 +610,30 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         const Key<W> kk = load_key<W>(keys, stride, 64ull * k);
 #pragma unroll
         for (int l = 0; l < W; ++l) bf.ssamp[(size_t)k * W + l] = kk.l[l];
+    }
+}
+
+// ============================================================ k_record_tails
+//
+// Recording only: copies the touched terms of this gate (parent order, with
+// their signs) and their coefficient rows into the gate's records, for the
+// adjoint sweep. One thread per (term, column), grid-strided.
+__global__ __launch_bounds__(WG) void k_record_tails(Bufs bf, Ctl* ctl, Launch L, int B) {
+    if (ctl->overflow) return;
+    const int p = L.seq & 1;
+    const uint32_t na = ctl->st[p].na;
+    if (na == 0) return;
+    const uint32_t n = ctl->st[p].n, roff = ctl->st[p].roff, coff = ctl->st[p].coff;
+    const double* __restrict__ coeffs = bf.coeffs[ctl->st[p].cur];
+    const uint32_t* __restrict__ tail = bf.tail[0];
+    uint32_t* __restrict__ rsrc = bf.rec + (size_t)roff + n;
+    double* __restrict__ rc = bf.rcoef + (size_t)coff * B;
+    const uint64_t total = (uint64_t)na * B;
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < total; e += (uint64_t)gridDim.x * WG) {
+        const uint32_t j = (uint32_t)(e / B), b = (uint32_t)(e % B);
+        const uint32_t src = tail[j];
+        if (b == 0) rsrc[j] = src;
+        rc[e] = coeffs[(size_t)(src & SRC_MASK) * B + b];
     }
 }
 
@@ -588,13 +665,14 @@ template <int W, int TILE>
 __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, size_t stride,
                                            const uint32_t* __restrict__ tail, const uint64_t* __restrict__ tkeys,
                                            uint32_t na, uint32_t i0, uint32_t j0, int ns, int nt,
-                                           uint64_t (*s_key)[TILE + 2], uint32_t* s_src) {
+                                           uint64_t (*s_key)[TILE + 2], uint32_t* s_src,
+                                           const uint32_t* __restrict__ tpj, uint32_t* s_pj) {
     const int t = threadIdx.x;
     constexpr int R = (TILE + 1 + WG - 1) / WG;
     // head slots x = 0 .. ns (global index i0 + x - 1) and tail slots y = 0 .. nt
     // (global index j0 + y); indices are clamped so every load is issued unconditionally
     uint64_t hv[R][W], tv[R][W];
-    uint32_t sv[R];
+    uint32_t sv[R], pv[R];
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
     const uint32_t tmax = na > 0 ? na - 1 : 0;
 #pragma unroll
@@ -610,6 +688,7 @@ __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, si
             tv[r][l] = tkeys[l * stride + gj];
         }
         sv[r] = tail[gj];
+        if (tpj) pv[r] = tpj[gj];
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -620,6 +699,7 @@ __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, si
         }
         if (x < nt + 1 && j0 + (uint32_t)x < na) {
             s_src[x] = sv[r];
+            if (tpj) s_pj[x] = pv[r];
 #pragma unroll
             for (int l = 0; l < W; ++l) s_key[l][ns + 1 + x] = tv[r][l];
         }
@@ -700,7 +780,8 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
                                             size_t stride, const uint32_t* __restrict__ tail,
                                             const uint64_t* __restrict__ tkeys, uint32_t na, uint32_t i0,
                                             uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
-                                            double sinv, uint64_t (*s_key)[TILE + 2], double* s_val) {
+                                            double sinv, uint64_t (*s_key)[TILE + 2], double* s_val,
+                                            const uint32_t* __restrict__ tpj, uint32_t* s_pj) {
     const int t = threadIdx.x;
     constexpr int R = (TILE + 1 + WG - 1) / WG;
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
@@ -729,12 +810,14 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
 #pragma unroll
         for (int l = 0; l < W; ++l) k.l[l] = tkeys[l * stride + gj];
         const uint32_t src = tail[gj];
+        const uint32_t pj = tpj ? tpj[gj] : 0u;
         double c = coeffs[src & SRC_MASK] * sinv;
         if (src & SRC_NEG) c = -c;
         if (y < nt + 1 && j0 + (uint32_t)y < na) {
 #pragma unroll
             for (int l = 0; l < W; ++l) s_key[l][ns + 1 + y] = k.l[l];
             s_val[ns + 1 + y] = c;
+            if (tpj) s_pj[y] = pj;
         }
     }
 }
@@ -744,7 +827,10 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
 // output order, as k_branch of that gate would list them: the next k_branch
 // then only checks room. The list's offsets come from a second look-back
 // chain that shares every poll with the output offsets.
-template <int W, bool EMIT>
+//
+// With REC, every head term and every touched term's created term records its
+// output position (REC_NONE when truncated, merged away or absent).
+template <int W, bool EMIT, bool REC>
 __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
                                                Key<W> nmask) {
     constexpr int ITEMS = K3Cfg<W>::ITEMS, TILE = K3Cfg<W>::TILE;
@@ -763,9 +849,12 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     const uint32_t ntiles = (ntot + TILE - 1) / TILE;
     __shared__ uint64_t s_key[W][TILE + 2];
     __shared__ double s_val[TILE + 2];
+    __shared__ uint32_t s_pj[REC ? TILE + 2 : 1];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl, s_excl_t;
+    uint32_t* __restrict__ rhead = REC ? bf.rec + ctl->st[p].roff : nullptr;
+    uint32_t* __restrict__ rcreated = REC ? bf.rec + ctl->st[p].roff + n + na : nullptr;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     uint32_t* __restrict__ next_tail = bf.tail[0];
     TileSource ts;
@@ -784,7 +873,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             nt = 0;
         }
         merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
-                             rp.sinv, s_key, s_val);
+                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -802,6 +891,9 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         const int dtot = ns + nt;
         int slot[ITEMS];
         double oc[ITEMS];
+        uint32_t who[ITEMS];  // REC: head x + 1 (low half), consumed tail y + 1 (high half)
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) who[k] = 0;
         uint32_t keep = 0, touch = 0, neg = 0;
         if (dl < dtot) {
             int lo = dl > nt ? dl - nt : 0, hi = min(dl, ns);
@@ -823,17 +915,20 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                 Key<W> ko;
                 if (take_s) {
                     double c = s_val[1 + x];
-                    if (j0 + y < na && key_eq<W>(kt, ks)) c = c + s_val[ns + 1 + y];
+                    const bool merged = j0 + y < na && key_eq<W>(kt, ks);
+                    if (merged) c = c + s_val[ns + 1 + y];
                     slot[k] = 1 + x;
                     oc[k] = c;
                     ko = ks;
                     live = true;
+                    if (REC) who[k] = (uint32_t)(x + 1) | (merged ? (uint32_t)(y + 1) << 16 : 0u);
                     ++x;
                 } else {
                     live = !((i0 + x > 0) && key_eq<W>(S(x - 1), kt));
                     slot[k] = ns + 1 + y;
                     oc[k] = s_val[ns + 1 + y];
                     ko = kt;
+                    if (REC && live) who[k] = (uint32_t)(y + 1) << 16;
                     ++y;
                 }
                 if (live && !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs)) {
@@ -885,6 +980,17 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         uint64_t tpos = EMIT ? s_excl_t + (pre >> 16) : 0;
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
+            if (REC && who[k]) {
+                const uint32_t at = ((keep >> k) & 1u) ? (uint32_t)pos : REC_NONE;
+                const uint32_t hx = who[k] & 0xffffu, ty = who[k] >> 16;
+                if (hx) {
+                    Key<W> hk;
+#pragma unroll
+                    for (int l = 0; l < W; ++l) hk.l[l] = s_key[l][hx];
+                    rhead[i0 + hx - 1] = (anticommutes<W>(mask, hk) ? REC_FLAG : 0u) | at;
+                }
+                if (ty) rcreated[s_pj[ty - 1]] = at;
+            }
             if ((keep >> k) & 1u) {
 #pragma unroll
                 for (int l = 0; l < W; ++l) out_keys[l * stride + pos] = s_key[l][slot[k]];
@@ -929,6 +1035,9 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ int s_head[TB];          // head index, -(index+2) when scaled by cos, -1 none, INT_MIN dead
     __shared__ uint32_t s_tl[TB];       // tail src or ~0u; reused for the output offset
     __shared__ uint32_t s_keep[TB];
+    __shared__ uint32_t s_pj[TB + 1];    // recording: parent-order index of tail slot y
+    __shared__ uint32_t s_rh[TB];        // recording: head index | REC_FLAG if touched, or ~0u
+    __shared__ uint32_t s_rt[TB];        // recording: parent-order index of the consumed tail, or ~0u
     __shared__ uint32_t s_split[2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
@@ -953,7 +1062,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             ns = 0;
             nt = 0;
         }
-        merge_load<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src);
+        merge_load<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src, bf.tpj, s_pj);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -984,20 +1093,31 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             int head = -1;
             uint32_t tl = ~0u;
             bool live = false;
+            uint32_t rh = ~0u, rt = ~0u;
             if (take_s) {
                 okey = S(x);
                 head = (int)(i0 + x);
-                if (j0 + y < na && key_eq<W>(T(y), okey)) tl = s_src[y];
+                rh = (uint32_t)head;
+                if (j0 + y < na && key_eq<W>(T(y), okey)) {
+                    tl = s_src[y];
+                    rt = s_pj[y];
+                }
                 live = true;
-                if (anticommutes<W>(mask, okey)) head = -(head + 2);
+                if (anticommutes<W>(mask, okey)) {
+                    head = -(head + 2);
+                    rh |= REC_FLAG;
+                }
             } else {
                 okey = T(y);
                 const bool dup = (i0 + x > 0) && key_eq<W>(S(x - 1), okey);
                 if (!dup) {
                     tl = s_src[y];
+                    rt = s_pj[y];
                     live = true;
                 }
             }
+            s_rh[t] = rh;
+            s_rt[t] = rt;
             if (live && max_weight >= 0 && weight<W>(okey) > max_weight) live = false;
             s_head[t] = live ? head : INT_MIN;
             s_tl[t] = live ? tl : ~0u;
@@ -1005,6 +1125,8 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         } else if (t < TB) {
             s_keep[t] = 0u;
             s_head[t] = INT_MIN;
+            s_rh[t] = ~0u;
+            s_rt[t] = ~0u;
         }
         __syncthreads();
         for (int e = wave; e < TB; e += NWAVES) {
@@ -1055,6 +1177,12 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             store_key<W>(out_keys, stride, s_excl + pre, okey);
             s_tl[t] = pre;
         }
+        if (bf.rec && t < TB) {
+            const uint32_t at = mine ? (uint32_t)(s_excl + pre) : REC_NONE;
+            const uint32_t roff = ctl->st[p].roff;
+            if (s_rh[t] != ~0u) bf.rec[(size_t)roff + (s_rh[t] & SRC_MASK)] = (s_rh[t] & REC_FLAG) | at;
+            if (s_rt[t] != ~0u) bf.rec[(size_t)roff + n + na + s_rt[t]] = at;
+        }
         __syncthreads();
         for (int e = wave; e < TB; e += NWAVES) {
             if (!s_keep[e]) continue;
@@ -1088,7 +1216,13 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         if (blockIdx.x == 0 && t == 0) {
             set_state(ctl, p ^ 1, 0u, cur);
             if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
+            carry_records(ctl, L, p);
         }
+        return;
+    }
+    const uint32_t roff = ctl->st[p].roff;
+    if (bf.rec && (uint64_t)roff + n > bf.rec_words) {
+        if (blockIdx.x == 0 && t == 0) record_overflow(ctl, L, (uint64_t)roff + n, ctl->st[p].coff);
         return;
     }
     const uint64_t* __restrict__ keys = bf.keys[cur];
@@ -1106,6 +1240,7 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)TILE;
         uint64_t ballots[ITEMS];
+        uint32_t damped = 0;
         double cv[ITEMS];
         Key<W> kv[ITEMS];
 #pragma unroll
@@ -1118,7 +1253,10 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         for (int k = 0; k < ITEMS; ++k) {
             const uint32_t i = base + k * WG + t;
             const uint32_t pc = code_at<W>(kv[k], qubit);
-            if ((damp_codes >> pc) & 1u) cv[k] = cv[k] * factor;
+            if ((damp_codes >> pc) & 1u) {
+                cv[k] = cv[k] * factor;
+                damped |= 1u << k;
+            }
             const bool keep =
                 (i < n) && !((max_weight >= 0 && weight<W>(kv[k]) > max_weight) || fabs(cv[k]) < min_abs);
             ballots[k] = __ballot(keep);
@@ -1144,10 +1282,14 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(excl + agg);
                     set_state(ctl, p ^ 1, nout, cur ^ 1);
+                    ctl->st[p ^ 1].roff = roff + (bf.rec ? n : 0u);
+                    ctl->st[p ^ 1].coff = ctl->st[p].coff;
                     if (L.glog) {
                         L.glog[0] = n;
                         L.glog[1] = n;
                         L.glog[2] = nout;
+                        L.glog[3] = roff;
+                        L.glog[4] = ctl->st[p].coff;
                     }
                 }
             }
@@ -1156,11 +1298,15 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
-            if ((ballots[k] >> lane) & 1ull) {
-                const uint64_t pos = s_excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);
+            const bool kept = (ballots[k] >> lane) & 1ull;
+            const uint64_t pos = s_excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);
+            if (kept) {
                 store_key<W>(out_keys, stride, pos, kv[k]);
                 out_coeffs[pos] = cv[k];
             }
+            const uint32_t i = base + k * WG + t;
+            if (bf.rec && i < n)
+                bf.rec[(size_t)roff + i] = (((damped >> k) & 1u) ? REC_FLAG : 0u) | (kept ? (uint32_t)pos : REC_NONE);
         }
     }
 }
@@ -1179,7 +1325,13 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         if (blockIdx.x == 0 && t == 0) {
             set_state(ctl, p ^ 1, 0u, cur);
             if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
+            carry_records(ctl, L, p);
         }
+        return;
+    }
+    const uint32_t roff = ctl->st[p].roff;
+    if (bf.rec && (uint64_t)roff + n > bf.rec_words) {
+        if (blockIdx.x == 0 && t == 0) record_overflow(ctl, L, (uint64_t)roff + n, ctl->st[p].coff);
         return;
     }
     const uint64_t* __restrict__ keys = bf.keys[cur];
@@ -1227,18 +1379,27 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(excl + agg);
                     set_state(ctl, p ^ 1, nout, cur ^ 1);
+                    ctl->st[p ^ 1].roff = roff + (bf.rec ? n : 0u);
+                    ctl->st[p ^ 1].coff = ctl->st[p].coff;
                     if (L.glog) {
                         L.glog[0] = n;
                         L.glog[1] = n;
                         L.glog[2] = nout;
+                        L.glog[3] = roff;
+                        L.glog[4] = ctl->st[p].coff;
                     }
                 }
             }
         }
         __syncthreads();
         for (int r = wave; r < KNB_ROWS; r += NWAVES) {
-            if (!s_keep[r]) continue;
             const uint32_t i = base + r;
+            if (bf.rec && i < n && lane == 0) {
+                const Key<W> kr = load_key<W>(keys, stride, i);
+                const bool dmp = (damp_codes >> code_at<W>(kr, qubit)) & 1u;
+                bf.rec[(size_t)roff + i] = (dmp ? REC_FLAG : 0u) | (s_keep[r] ? (uint32_t)(s_excl + s_pos[r]) : REC_NONE);
+            }
+            if (!s_keep[r]) continue;
             const uint64_t pos = s_excl + s_pos[r];
             const Key<W> kk = load_key<W>(keys, stride, i);
             if (lane == 0) store_key<W>(out_keys, stride, pos, kk);
@@ -1334,6 +1495,105 @@ __global__ __launch_bounds__(WG) void k_overlap_final(const double* __restrict__
             out[b] = r.hi + r.lo;
         }
         __syncthreads();
+    }
+}
+
+// ============================================================ adjoint sweep
+//
+// The loss is overlapwithzero of the propagated sum. Its derivative with
+// respect to every coefficient (the adjoint) runs backwards through the gates
+// along the records of the forward run, with every truncation held as it
+// happened. A rotation maps input term i to outputs ph (itself, times cos
+// when it anticommutes) and pt (its created term, times sign * sin), so
+//   a_in[i] = a[ph] cos + a[pt] sign sin   (a[ph] alone when it commutes)
+//   dL/dtheta = sum over touched i of c_i (a[pt] sign cos - a[ph] sin),
+// the derivative of the branch rule of src/Propagation/specializations.jl:34-44.
+// Threads form rows of BP = 2^lgBP threads, thread b of a row handling
+// column b < B; coefficient rows are B doubles.
+
+// Adjoint of the final sum: 1 for strings without X or Y, else 0
+// (src/stateoverlap.jl:12-35).
+template <int W>
+__global__ __launch_bounds__(WG) void k_adj_seed(const uint64_t* __restrict__ keys, size_t stride, uint32_t n, int B,
+                                                 double* __restrict__ a) {
+    const uint64_t total = (uint64_t)n * B;
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < total; e += (uint64_t)gridDim.x * WG) {
+        const uint32_t i = (uint32_t)(e / B);
+        a[e] = contains_xy<W>(load_key<W>(keys, stride, i)) ? 0.0 : 1.0;
+    }
+}
+
+// Sum of a block's per-thread values of each column, in a fixed order.
+__device__ __forceinline__ void adj_block_columns(double g, int lgBP, int B, double* __restrict__ out) {
+    __shared__ double s_g[WG];
+    const int t = threadIdx.x, BP = 1 << lgBP, r = t >> lgBP, b = t & (BP - 1);
+    s_g[t] = g;
+    __syncthreads();
+    for (int k = (WG >> lgBP) / 2; k >= 1; k >>= 1) {
+        if (r < k) s_g[t] += s_g[t + k * BP];
+        __syncthreads();
+    }
+    if (r == 0 && b < B) out[b] = s_g[t];
+}
+
+// One rotation backwards. rec points at the gate's records, rcoef at its
+// coefficient rows, trig at cos_b then sin_b. Items [0, n) are the gate's input
+// terms (those that commute are done here), items [n, n + na) its touched terms.
+__global__ __launch_bounds__(WG) void k_adj_rotation(const double* __restrict__ a_out, double* __restrict__ a_in,
+                                                     const uint32_t* __restrict__ rec, uint32_t n, uint32_t na,
+                                                     const double* __restrict__ rcoef, int B, int lgBP,
+                                                     const double* __restrict__ trig, double* __restrict__ gpart) {
+    const int t = threadIdx.x, BP = 1 << lgBP, rows = WG >> lgBP, r = t >> lgBP, b = t & (BP - 1);
+    const bool col = b < B;
+    const double cb = col ? trig[b] : 0.0, sb = col ? trig[B + b] : 0.0;
+    const uint32_t* __restrict__ rsrc = rec + n;
+    const uint32_t* __restrict__ rpt = rec + n + na;
+    const uint32_t items = n + na;
+    double g = 0.0;
+    if (col) {
+        for (uint32_t it = blockIdx.x * rows + r; it < items; it += gridDim.x * rows) {
+            if (it < n) {
+                const uint32_t ph = rec[it];
+                if (ph & REC_FLAG) continue;  // touched: done with its created term
+                const uint32_t at = ph & SRC_MASK;
+                a_in[(size_t)it * B + b] = at != REC_NONE ? a_out[(size_t)at * B + b] : 0.0;
+            } else {
+                const uint32_t j = it - n;
+                const uint32_t src = rsrc[j], i = src & SRC_MASK;
+                const uint32_t hp = rec[i] & SRC_MASK, tp = rpt[j];
+                const double ah = hp != REC_NONE ? a_out[(size_t)hp * B + b] : 0.0;
+                double at = tp != REC_NONE ? a_out[(size_t)tp * B + b] : 0.0;
+                if (src & SRC_NEG) at = -at;
+                a_in[(size_t)i * B + b] = ah * cb + at * sb;
+                const double c = rcoef[(size_t)j * B + b];
+                g += c * (at * cb) - c * (ah * sb);
+            }
+        }
+    }
+    adj_block_columns(g, lgBP, B, gpart + (size_t)blockIdx.x * B);
+}
+
+// Pauli noise backwards: a_in[i] = a[pos] times the factor when i was damped.
+__global__ __launch_bounds__(WG) void k_adj_noise(const double* __restrict__ a_out, double* __restrict__ a_in,
+                                                  const uint32_t* __restrict__ rec, uint32_t n, int B,
+                                                  const double* __restrict__ factor) {
+    const uint64_t total = (uint64_t)n * B;
+    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < total; e += (uint64_t)gridDim.x * WG) {
+        const uint32_t i = (uint32_t)(e / B), b = (uint32_t)(e % B);
+        const uint32_t w = rec[i], at = w & SRC_MASK;
+        double v = at != REC_NONE ? a_out[(size_t)at * B + b] : 0.0;
+        if (w & REC_FLAG) v = v * factor[b];
+        a_in[e] = v;
+    }
+}
+
+// The gate's gradient per column: the blocks' partial sums in block order.
+__global__ __launch_bounds__(WG) void k_adj_final(const double* __restrict__ gpart, int nblocks, int B,
+                                                  double* __restrict__ out) {
+    for (int b = threadIdx.x; b < B; b += WG) {
+        double s = 0.0;
+        for (int k = 0; k < nblocks; ++k) s += gpart[(size_t)k * B + b];
+        out[b] = s;
     }
 }
 

@@ -71,12 +71,18 @@ struct pp_handle {
     uint32_t lb_count = 0;         // look-back launches so far (counter slot parity)
     uint32_t epoch = 0;
     // per-run device arrays
-    uint32_t* d_log = nullptr;     // 3 words per gate: n_in, n_touched, n_out
+    uint32_t* d_log = nullptr;     // LOG_WORDS per gate: n_in, n_touched, n_out, roff, coff
     uint32_t* h_log = nullptr;
     size_t log_cap = 0;            // gates
     double* d_trig = nullptr;      // per gate: B cos, B sin (rotation) or B factors (noise)
     double* h_trig = nullptr;
     size_t trig_cap = 0;           // doubles
+    // recording for the adjoint sweep (allocated on first use)
+    bool recording = false;
+    uint32_t* d_tpj = nullptr;
+    uint32_t* d_rec = nullptr;
+    double* d_rcoef = nullptr;
+    size_t tpj_cap = 0, rec_words = 0, rec_rows = 0;
     // overlap
     double* d_partial = nullptr;
     size_t partial_cap = 0;
@@ -151,12 +157,25 @@ int live(pp_handle* h, uint32_t* n, uint32_t* cur) {
     return PP_OK;
 }
 
-// Set the sum's state to (n, cur) and clear the overflow record and counters.
-int reset_ctl(pp_handle* h, uint32_t n, uint32_t cur) {
+// Set the sum's state (its size, buffer and record offsets) and clear the
+// overflow record and counters.
+int reset_ctl(pp_handle* h, const GateState& st) {
     std::memset(h->h_ctl, 0, sizeof(Ctl));
-    h->h_ctl->st[h->seq & 1] = GateState{n, cur, 0, 0};
-    h->n_est = n;
+    GateState& s = h->h_ctl->st[h->seq & 1];
+    s = GateState{};
+    s.n = st.n;
+    s.cur = st.cur;
+    s.roff = st.roff;
+    s.coff = st.coff;
+    h->n_est = st.n;
     return put_ctl(h);
+}
+
+int reset_ctl(pp_handle* h, uint32_t n, uint32_t cur) {
+    GateState st{};
+    st.n = n;
+    st.cur = cur;
+    return reset_ctl(h, st);
 }
 
 // (Re)allocate every term buffer for `newcap` terms, keeping the live sum.
@@ -165,8 +184,10 @@ int grow(pp_handle* h, size_t newcap) {
     if (newcap > MAX_TERMS + 1) newcap = MAX_TERMS + 1;
     const int W = h->W, B = h->B;
     uint32_t n = 0, cur = 0;
+    GateState keep{};
     if (h->cap) {
         if (int rc = live(h, &n, &cur)) return rc;
+        keep = h->h_ctl->st[h->seq & 1];
     }
     uint64_t* nk[2] = {nullptr, nullptr};
     double* nc[2] = {nullptr, nullptr};
@@ -236,7 +257,17 @@ int grow(pp_handle* h, size_t newcap) {
     h->lb_cap = nlb;
     h->cap = newcap;
     h->epoch = 0;
-    return reset_ctl(h, n, 0);
+    if (h->recording) {
+        if (h->d_tpj) hipFree(h->d_tpj);
+        h->d_tpj = nullptr;
+        h->tpj_cap = 0;
+        HIP_TRY(hipMalloc(&h->d_tpj, newcap * sizeof(uint32_t)));
+        h->tpj_cap = newcap;
+        h->bufs.tpj = h->d_tpj;
+    }
+    keep.n = n;
+    keep.cur = 0;
+    return reset_ctl(h, keep);
 }
 
 uint32_t next_epoch(pp_handle* h) {
@@ -297,8 +328,8 @@ int ensure_run(pp_handle* h, size_t ngates) {
         h->d_log = nullptr;
         h->h_log = nullptr;
         const size_t c = std::max<size_t>(ngates, 1024);
-        HIP_TRY(hipMalloc(&h->d_log, 3 * c * sizeof(uint32_t)));
-        HIP_TRY(hipHostMalloc(&h->h_log, 3 * c * sizeof(uint32_t), hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&h->d_log, LOG_WORDS * c * sizeof(uint32_t)));
+        HIP_TRY(hipHostMalloc(&h->h_log, LOG_WORDS * c * sizeof(uint32_t), hipHostMallocDefault));
         h->log_cap = c;
     }
     const size_t need = ngates * 2 * (size_t)h->B;
@@ -405,6 +436,9 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
+    if (h->recording)
+        hipLaunchKernelGGL(k_record_tails, dim3(grid_for(h, h->n_est * (size_t)h->B / 2, WG * 4)), dim3(WG), 0,
+                           h->stream, h->bufs, h->d_ctl, L2, h->B);
     if (all) hipEventRecord(ev[2], h->stream);
     hipLaunchKernelGGL(k_xorperm<W>, dim3(grid_for(h, h->n_est / 2, K2_TILE)), dim3(WG), 0, h->stream, h->bufs,
                        h->d_ctl, L2, lv, mask);
@@ -418,13 +452,17 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         LP.seq = h->seq;
         hipLaunchKernelGGL((k_partition<W, TILE>), dim3(grid_for(h, expect, (size_t)TILE * NWAVES)), dim3(WG), 0,
                            h->stream, h->bufs, h->d_ctl, LP);
-        if (next_mask) {
-            hipLaunchKernelGGL((k_merge1<W, true>), dim3(lb_grid(h, (const void*)k_merge1<W, true>, 0, expect, TILE)),
-                               dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp,
-                               to_key<W>(next_mask));
+        if (h->recording) {
+            hipLaunchKernelGGL((k_merge1<W, false, true>),
+                               dim3(lb_grid(h, (const void*)k_merge1<W, false, true>, 0, expect, TILE)), dim3(WG), 0,
+                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+        } else if (next_mask) {
+            hipLaunchKernelGGL((k_merge1<W, true, false>),
+                               dim3(lb_grid(h, (const void*)k_merge1<W, true, false>, 0, expect, TILE)), dim3(WG), 0,
+                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, to_key<W>(next_mask));
         } else {
-            hipLaunchKernelGGL((k_merge1<W, false>),
-                               dim3(lb_grid(h, (const void*)k_merge1<W, false>, 0, expect, TILE)), dim3(WG), 0,
+            hipLaunchKernelGGL((k_merge1<W, false, false>),
+                               dim3(lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE)), dim3(WG), 0,
                                h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
         }
     } else {
@@ -478,7 +516,7 @@ struct HostGate {
 
 int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs,
             const uint64_t* next_mask) {
-    uint32_t* glog = h->d_log + 3 * slot;
+    uint32_t* glog = h->d_log + LOG_WORDS * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
     if (g.kind == PP_GATE_ROTATION) {
@@ -500,14 +538,55 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
 
 // Run resolved gates (trig already staged at d_trig for B > 1), resuming
 // after an overflow; then account the gate log into the profile.
-int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, double min_abs, int64_t* counts) {
-    const size_t ng = gs.size();
-    size_t g = 0;
+// Grow the records to hold `words` words and `rows` coefficient rows, keeping
+// the first `keep_words` and `keep_rows` already written.
+int grow_records(pp_handle* h, size_t words, size_t rows, size_t keep_words, size_t keep_rows) {
+    const size_t B = (size_t)h->B;
+    if (words > 0xffffffffull || rows > 0xffffffffull)
+        return fail(PP_ERR_CAPACITY, "adjoint records exceed 2^32 words; use checkpoint segments");
+    if (words > h->rec_words) {
+        uint32_t* nr = nullptr;
+        HIP_TRY(hipMalloc(&nr, words * sizeof(uint32_t)));
+        if (keep_words)
+            HIP_TRY(hipMemcpyAsync(nr, h->d_rec, keep_words * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (h->d_rec) hipFree(h->d_rec);
+        h->d_rec = nr;
+        h->rec_words = words;
+    }
+    if (rows > h->rec_rows) {
+        double* nc = nullptr;
+        HIP_TRY(hipMalloc(&nc, std::max<size_t>(rows, 1) * B * sizeof(double)));
+        if (keep_rows)
+            HIP_TRY(hipMemcpyAsync(nc, h->d_rcoef, keep_rows * B * sizeof(double), hipMemcpyDeviceToDevice,
+                                   h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (h->d_rcoef) hipFree(h->d_rcoef);
+        h->d_rcoef = nc;
+        h->rec_rows = rows;
+    }
+    if (h->recording) {
+        h->bufs.rec = h->d_rec;
+        h->bufs.rcoef = h->d_rcoef;
+        h->bufs.rec_words = (uint32_t)h->rec_words;
+        h->bufs.rec_rows = (uint32_t)h->rec_rows;
+    }
+    return PP_OK;
+}
+
+// Run resolved gates first .. last-1 (trig staged at d_trig for B > 1, slots
+// are gate indices), resuming after an overflow; then account the gate log
+// into the profile. counts[i - first] receives gate i's input size.
+int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, size_t last, int max_weight,
+                 double min_abs, int64_t* counts) {
+    const size_t ng = last;
+    size_t g = first;
     while (g < ng) {
         const size_t end = std::min(ng, g + RUN_CHUNK);
         const uint32_t seq0 = h->seq;
         for (size_t i = g; i < end; ++i) {
-            const bool chain = h->B == 1 && h->chain_tails && i + 1 < ng && gs[i + 1].kind == PP_GATE_ROTATION;
+            const bool chain = h->B == 1 && h->chain_tails && !h->recording && i + 1 < ng &&
+                               gs[i + 1].kind == PP_GATE_ROTATION;
             if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[i + 1].mask : nullptr)) return rc;
         }
         if (int rc = sync_ctl(h)) return rc;
@@ -522,10 +601,19 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, 
                 const size_t need = h->h_ctl->overflow_need;
                 if (need > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
                 if (int rc = grow(h, std::max(need + need / 4, h->cap + h->cap / 2))) return rc;
+            } else if (h->h_ctl->overflow == 3) {
+                // the gate's records did not fit: grow them and run it again
+                const GateState st = h->h_ctl->st[fseq & 1];
+                const size_t words = std::max<size_t>((size_t)h->h_ctl->overflow_need * 3 / 2, 2 * h->rec_words);
+                const size_t rows = std::max<size_t>((size_t)h->h_ctl->overflow_rows * 3 / 2, 2 * h->rec_rows);
+                if (int rc = grow_records(h, std::min<size_t>(words, 0xffffffffull),
+                                          std::min<size_t>(rows, 0xffffffffull), st.roff, st.coff))
+                    return rc;
+                if (int rc = reset_ctl(h, st)) return rc;
             } else {
                 // a look-back wait ran out: run that gate again with dynamic tiles
                 const GateState st = h->h_ctl->st[fseq & 1];
-                if (int rc = reset_ctl(h, st.n, st.cur)) return rc;
+                if (int rc = reset_ctl(h, st)) return rc;
                 h->safe_until = fseq + 1;
                 h->fallbacks += 1;
             }
@@ -534,13 +622,15 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, int max_weight, 
         }
         g = end;
     }
-    if (ng == 0) return PP_OK;
-    HIP_TRY(hipMemcpyAsync(h->h_log, h->d_log, 3 * ng * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    if (first >= last) return PP_OK;
+    HIP_TRY(hipMemcpyAsync(h->h_log + LOG_WORDS * first, h->d_log + LOG_WORDS * first,
+                           LOG_WORDS * (last - first) * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     const double row = 8.0 * h->W + 8.0 * h->B;
-    for (size_t i = 0; i < ng; ++i) {
-        const uint32_t nin = h->h_log[3 * i], na = h->h_log[3 * i + 1], nout = h->h_log[3 * i + 2];
-        if (counts) counts[i] = nin;
+    for (size_t i = first; i < last; ++i) {
+        const uint32_t nin = h->h_log[LOG_WORDS * i], na = h->h_log[LOG_WORDS * i + 1],
+                       nout = h->h_log[LOG_WORDS * i + 2];
+        if (counts) counts[i - first] = nin;
         h->gates += 1;
         h->term_apps += nin;
         hipEvent_t* ev = h->prof_mode ? &h->ev[6 * i] : nullptr;
@@ -624,13 +714,301 @@ int single_gate(pp_handle* h, const HostGate& g, const double* hv1, const double
         HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, 2 * h->B * sizeof(double), hipMemcpyHostToDevice, h->stream));
     }
     std::vector<HostGate> gs(1, g);
-    if (int rc = run_resolved(h, gs, max_weight, min_abs, nullptr)) return rc;
+    if (int rc = run_resolved(h, gs, 0, 1, max_weight, min_abs, nullptr)) return rc;
     if (st) {
         st->n_in = h->h_log[0];
         st->n_touched = h->h_log[1];
         st->n_out = h->h_log[2];
     }
     return PP_OK;
+}
+
+// Validate and resolve gates for a run: cos/sin or noise factors, staged at
+// d_trig for B > 1. With atrig, every gate's B cos then B sin (rotation) or B
+// factors (noise) also go to atrig (2B doubles per gate).
+int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
+                  std::vector<HostGate>& gs, std::vector<double>* atrig) {
+    if (ngates > 0 && !gates) return fail(PP_ERR_ARG, "gates is NULL");
+    const int B = h->B;
+    gs.assign(ngates, HostGate{});
+    if (atrig) atrig->assign(ngates * 2 * (size_t)B, 0.0);
+    if (int rc = ensure_run(h, ngates)) return rc;
+    for (size_t g = 0; g < ngates; ++g) {
+        const pp_gate& gt = gates[g];
+        if (gt.param >= 0 && ((size_t)gt.param >= nparams || !theta))
+            return fail(PP_ERR_ARG, "gate " + std::to_string(g) + " reads a parameter beyond nparams");
+        auto val = [&](int b) { return gt.param < 0 ? gt.value : theta[(size_t)gt.param * B + b]; };
+        HostGate& hg = gs[g];
+        hg.kind = gt.kind;
+        if (gt.kind == PP_GATE_ROTATION) {
+            if (int rc = check_mask(h, gt.mask)) return rc;
+            hg.mask = gt.mask;
+            // cos and sin as two separate libm calls, as the reference evaluates them
+            // (src/Propagation/specializations.jl:30)
+            hg.a = std::cos(val(0));
+            hg.b = std::sin(val(0));
+            for (int b = 0; b < B; ++b) {
+                const double c = std::cos(val(b)), sn = std::sin(val(b));
+                if (B > 1) {
+                    h->h_trig[g * 2 * B + b] = c;
+                    h->h_trig[g * 2 * B + B + b] = sn;
+                }
+                if (atrig) {
+                    (*atrig)[g * 2 * B + b] = c;
+                    (*atrig)[g * 2 * B + B + b] = sn;
+                }
+            }
+        } else if (gt.kind == PP_GATE_PAULI_NOISE) {
+            if (gt.qubit < 0 || gt.qubit >= h->nqubits) return fail(PP_ERR_ARG, "noise qubit out of range");
+            if (gt.damp_codes & ~0xeu) return fail(PP_ERR_ARG, "damp_codes may only hold X, Y, Z (bits 1..3)");
+            hg.qubit = gt.qubit;
+            hg.damp_codes = gt.damp_codes;
+            for (int b = 0; b < B; ++b) {
+                const double lam = val(b);
+                if (!(lam >= 0.0 && lam <= 1.0)) return fail(PP_ERR_ARG, "noise strength must be in [0, 1]");
+                if (B > 1) h->h_trig[g * 2 * B + b] = 1 - lam;
+                if (atrig) (*atrig)[g * 2 * B + b] = 1 - lam;
+            }
+            hg.a = 1 - val(0);
+        } else {
+            return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
+        }
+    }
+    if (B > 1 && ngates > 0)
+        HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, ngates * 2 * B * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+    return PP_OK;
+}
+
+// Switch recording on (allocating its buffers) or off.
+int set_recording(pp_handle* h, bool on) {
+    h->recording = on;
+    if (!on) {
+        h->bufs.tpj = nullptr;
+        h->bufs.rec = nullptr;
+        h->bufs.rcoef = nullptr;
+        h->bufs.rec_words = h->bufs.rec_rows = 0;
+        return PP_OK;
+    }
+    if (h->tpj_cap < h->cap) {
+        if (h->d_tpj) hipFree(h->d_tpj);
+        h->d_tpj = nullptr;
+        h->tpj_cap = 0;
+        HIP_TRY(hipMalloc(&h->d_tpj, h->cap * sizeof(uint32_t)));
+        h->tpj_cap = h->cap;
+    }
+    if (!h->d_rec || !h->d_rcoef)
+        if (int rc = grow_records(h, std::max<size_t>(h->rec_words, std::max<size_t>(1 << 20, 4 * h->cap)),
+                                  std::max<size_t>(h->rec_rows, std::max<size_t>(1 << 16, h->cap)), 0, 0))
+            return rc;
+    h->bufs.tpj = h->d_tpj;
+    h->bufs.rec = h->d_rec;
+    h->bufs.rcoef = h->d_rcoef;
+    h->bufs.rec_words = (uint32_t)h->rec_words;
+    h->bufs.rec_rows = (uint32_t)h->rec_rows;
+    return PP_OK;
+}
+
+// A device copy of the live sum (keys and coefficients).
+struct Snapshot {
+    uint64_t* keys = nullptr;
+    double* coeffs = nullptr;
+    uint32_t n = 0;
+};
+
+void free_snapshot(Snapshot& sn) {
+    if (sn.keys) hipFree(sn.keys);
+    if (sn.coeffs) hipFree(sn.coeffs);
+    sn = Snapshot{};
+}
+
+int take_snapshot(pp_handle* h, Snapshot& sn) {
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    const size_t W = h->W, B = h->B, m = std::max<uint32_t>(n, 1);
+    HIP_TRY(hipMalloc(&sn.keys, m * W * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&sn.coeffs, m * B * sizeof(double)));
+    sn.n = n;
+    for (size_t l = 0; l < W && n; ++l)
+        HIP_TRY(hipMemcpyAsync(sn.keys + l * n, h->bufs.keys[cur] + l * h->cap, n * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, h->stream));
+    if (n)
+        HIP_TRY(hipMemcpyAsync(sn.coeffs, h->bufs.coeffs[cur], (size_t)n * B * sizeof(double),
+                               hipMemcpyDeviceToDevice, h->stream));
+    return PP_OK;
+}
+
+int restore_snapshot(pp_handle* h, const Snapshot& sn) {
+    if (sn.n > h->cap)
+        if (int rc = grow(h, sn.n)) return rc;
+    const size_t W = h->W, B = h->B, n = sn.n;
+    for (size_t l = 0; l < W && n; ++l)
+        HIP_TRY(hipMemcpyAsync(h->bufs.keys[0] + l * h->cap, sn.keys + l * n, n * sizeof(uint64_t),
+                               hipMemcpyDeviceToDevice, h->stream));
+    if (n)
+        HIP_TRY(hipMemcpyAsync(h->bufs.coeffs[0], sn.coeffs, n * B * sizeof(double), hipMemcpyDeviceToDevice,
+                               h->stream));
+    return reset_ctl(h, (uint32_t)n, 0);
+}
+
+template <int W>
+int seed_adjoint(pp_handle* h, double* a) {
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    if (n == 0) return PP_OK;
+    hipLaunchKernelGGL(k_adj_seed<W>, dim3(grid_for(h, (size_t)n * h->B, WG * 4)), dim3(WG), 0, h->stream,
+                       h->bufs.keys[cur], (size_t)h->cap, n, h->B, a);
+    HIP_TRY(hipGetLastError());
+    return PP_OK;
+}
+
+// Loss = overlapwithzero of the propagated sum, and its gradient, by a forward
+// run that records every gate (per segment of checkpoint_gates gates, from a
+// snapshot of the segment's input sum) and a reverse sweep over the records.
+int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
+                 int max_weight, double min_abs, size_t checkpoint_gates, double* loss, double* grad) {
+    const int B = h->B;
+    std::vector<HostGate> gs;
+    std::vector<double> atrig;
+    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, gs, &atrig)) return rc;
+    const size_t seg = checkpoint_gates ? std::min(checkpoint_gates, std::max<size_t>(ngates, 1))
+                                        : std::max<size_t>(ngates, 1);
+    const size_t nseg = ngates ? (ngates + seg - 1) / seg : 1;
+    // device arrays of the sweep
+    double* d_atrig = nullptr;
+    double* d_a[2] = {nullptr, nullptr};
+    double* d_gpart = nullptr;
+    double* d_dgrad = nullptr;
+    std::vector<Snapshot> snaps;
+    Snapshot final_sum;
+    int rc = PP_OK;
+    auto cleanup = [&]() {
+        if (d_atrig) hipFree(d_atrig);
+        if (d_a[0]) hipFree(d_a[0]);
+        if (d_a[1]) hipFree(d_a[1]);
+        if (d_gpart) hipFree(d_gpart);
+        if (d_dgrad) hipFree(d_dgrad);
+        for (auto& sn : snaps) free_snapshot(sn);
+        free_snapshot(final_sum);
+    };
+    auto run_segment = [&](size_t k) -> int {
+        return run_resolved(h, gs, k * seg, std::min(ngates, (k + 1) * seg), max_weight, min_abs, nullptr);
+    };
+    const int lgBP = B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1));
+    const unsigned gblocks = h->gmax;
+    do {
+        if (ngates) {
+            if (hipMalloc(&d_atrig, atrig.size() * sizeof(double)) != hipSuccess ||
+                hipMalloc(&d_dgrad, ngates * (size_t)B * sizeof(double)) != hipSuccess ||
+                hipMalloc(&d_gpart, (size_t)gblocks * B * sizeof(double)) != hipSuccess) {
+                rc = fail(PP_ERR_NOMEM, "adjoint: device allocation failed");
+                break;
+            }
+            if (hipMemcpyAsync(d_atrig, atrig.data(), atrig.size() * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream) != hipSuccess ||
+                hipMemsetAsync(d_dgrad, 0, ngates * (size_t)B * sizeof(double), h->stream) != hipSuccess) {
+                rc = fail(PP_ERR_HIP, "adjoint: staging failed");
+                break;
+            }
+        }
+        // pass 1 (segments only): snapshots of every segment's input sum
+        if (nseg > 1) {
+            snaps.resize(nseg);
+            for (size_t k = 0; k < nseg && !rc; ++k) {
+                if ((rc = take_snapshot(h, snaps[k]))) break;
+                rc = run_segment(k);
+            }
+            if (rc) break;
+            if ((rc = take_snapshot(h, final_sum))) break;
+        }
+        size_t acap = 0;
+        int acur = 0;  // d_a[acur] holds the adjoint of the current end of the sweep
+        for (size_t kk = nseg; kk-- > 0;) {
+            if (nseg > 1) {
+                if ((rc = restore_snapshot(h, snaps[kk]))) break;
+            } else {
+                uint32_t n = 0, cur = 0;
+                if ((rc = live(h, &n, &cur))) break;
+                if ((rc = reset_ctl(h, n, cur))) break;
+            }
+            if ((rc = set_recording(h, true))) break;
+            rc = run_segment(kk);
+            set_recording(h, false);
+            if (rc) break;
+            if (kk == nseg - 1) {
+                // the sweep starts at the final sum: loss and seed
+                acap = h->cap;
+                if (hipMalloc(&d_a[0], acap * B * sizeof(double)) != hipSuccess ||
+                    hipMalloc(&d_a[1], acap * B * sizeof(double)) != hipSuccess) {
+                    rc = fail(PP_ERR_NOMEM, "adjoint: device allocation failed");
+                    break;
+                }
+                switch (h->W) {
+                case 1: rc = overlap_impl<1>(h, loss); break;
+                case 2: rc = overlap_impl<2>(h, loss); break;
+                default: rc = overlap_impl<4>(h, loss); break;
+                }
+                if (rc) break;
+                switch (h->W) {
+                case 1: rc = seed_adjoint<1>(h, d_a[0]); break;
+                case 2: rc = seed_adjoint<2>(h, d_a[0]); break;
+                default: rc = seed_adjoint<4>(h, d_a[0]); break;
+                }
+                if (rc) break;
+            }
+            // the sweep through this segment, last gate first
+            const size_t g0 = kk * seg, g1 = std::min(ngates, (kk + 1) * seg);
+            for (size_t g = g1; g-- > g0;) {
+                const uint32_t* lg = h->h_log + LOG_WORDS * g;
+                const uint32_t nin = lg[0], na = lg[1], nout = lg[2], roff = lg[3], coff = lg[4];
+                if (std::max(nin, nout) > acap) {
+                    rc = fail(PP_ERR_STATE, "adjoint: a gate's sum outgrew the adjoint buffers");
+                    break;
+                }
+                const double* tr = d_atrig + g * 2 * (size_t)B;
+                if (gs[g].kind == PP_GATE_ROTATION) {
+                    if (na == 0 || nin == 0) continue;  // identity
+                    const unsigned rows = WG >> lgBP;
+                    const unsigned nb = (unsigned)std::min<size_t>(gblocks, ((size_t)nin + na + rows - 1) / rows);
+                    hipLaunchKernelGGL(k_adj_rotation, dim3(nb), dim3(WG), 0, h->stream, d_a[acur], d_a[acur ^ 1],
+                                       h->d_rec + roff, nin, na, h->d_rcoef + (size_t)coff * B, B, lgBP, tr,
+                                       d_gpart);
+                    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(WG), 0, h->stream, d_gpart, (int)nb, B,
+                                       d_dgrad + g * (size_t)B);
+                } else {
+                    if (nin == 0) continue;
+                    hipLaunchKernelGGL(k_adj_noise, dim3(grid_for(h, (size_t)nin * B, WG * 4)), dim3(WG), 0,
+                                       h->stream, d_a[acur], d_a[acur ^ 1], h->d_rec + roff, nin, B, tr);
+                }
+                if (hipGetLastError() != hipSuccess) {
+                    rc = fail(PP_ERR_HIP, "adjoint: kernel launch failed");
+                    break;
+                }
+                acur ^= 1;
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+        // per-gate gradients, summed into parameters in gate order
+        std::vector<double> dg(ngates * (size_t)B);
+        if (ngates && hipMemcpyAsync(dg.data(), d_dgrad, dg.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                     h->stream) != hipSuccess) {
+            rc = fail(PP_ERR_HIP, "adjoint: download failed");
+            break;
+        }
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail(PP_ERR_HIP, "adjoint: synchronization failed");
+            break;
+        }
+        for (size_t q = 0; q < nparams * (size_t)B; ++q) grad[q] = 0.0;
+        for (size_t g = 0; g < ngates; ++g)
+            if (gs[g].kind == PP_GATE_ROTATION && gates[g].param >= 0)
+                for (int b = 0; b < B; ++b) grad[(size_t)gates[g].param * B + b] += dg[g * B + b];
+        // leave the propagated sum in the handle
+        if (nseg > 1) rc = restore_snapshot(h, final_sum);
+    } while (false);
+    cleanup();
+    return rc;
 }
 
 }  // namespace
@@ -798,52 +1176,29 @@ int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const doub
                    const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight,
                    double min_abs_coeff, int64_t* counts) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
-    if (ngates > 0 && !gates) return fail(PP_ERR_ARG, "gates is NULL");
     (void)lut_changes;
     (void)lut_signs;
     (void)nluts;
-    const int B = h->B;
-    // validate and resolve every gate before anything runs
-    std::vector<HostGate> gs(ngates);
-    if (int rc = ensure_run(h, ngates)) return rc;
-    for (size_t g = 0; g < ngates; ++g) {
-        const pp_gate& gt = gates[g];
-        if (gt.param >= 0 && ((size_t)gt.param >= nparams || !theta))
-            return fail(PP_ERR_ARG, "gate " + std::to_string(g) + " reads a parameter beyond nparams");
-        auto val = [&](int b) { return gt.param < 0 ? gt.value : theta[(size_t)gt.param * B + b]; };
-        HostGate& hg = gs[g];
-        hg.kind = gt.kind;
-        if (gt.kind == PP_GATE_ROTATION) {
-            if (int rc = check_mask(h, gt.mask)) return rc;
-            hg.mask = gt.mask;
-            // cos and sin as two separate libm calls, as the reference evaluates them
-            // (src/Propagation/specializations.jl:30)
-            hg.a = std::cos(val(0));
-            hg.b = std::sin(val(0));
-            if (B > 1)
-                for (int b = 0; b < B; ++b) {
-                    h->h_trig[g * 2 * B + b] = std::cos(val(b));
-                    h->h_trig[g * 2 * B + B + b] = std::sin(val(b));
-                }
-        } else if (gt.kind == PP_GATE_PAULI_NOISE) {
-            if (gt.qubit < 0 || gt.qubit >= h->nqubits) return fail(PP_ERR_ARG, "noise qubit out of range");
-            if (gt.damp_codes & ~0xeu) return fail(PP_ERR_ARG, "damp_codes may only hold X, Y, Z (bits 1..3)");
-            hg.qubit = gt.qubit;
-            hg.damp_codes = gt.damp_codes;
-            for (int b = 0; b < B; ++b) {
-                const double lam = val(b);
-                if (!(lam >= 0.0 && lam <= 1.0)) return fail(PP_ERR_ARG, "noise strength must be in [0, 1]");
-                if (B > 1) h->h_trig[g * 2 * B + b] = 1 - lam;
-            }
-            hg.a = 1 - val(0);
-        } else {
-            return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
-        }
-    }
-    if (B > 1 && ngates > 0)
-        HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, ngates * 2 * B * sizeof(double), hipMemcpyHostToDevice,
-                               h->stream));
-    return run_resolved(h, gs, max_weight, min_abs_coeff, counts);
+    std::vector<HostGate> gs;
+    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, gs, nullptr)) return rc;
+    return run_resolved(h, gs, 0, ngates, max_weight, min_abs_coeff, counts);
+}
+
+int pp_adjoint(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
+               const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight,
+               double min_abs_coeff, size_t checkpoint_gates, double* loss, double* grad) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    if (!loss || (nparams > 0 && !grad)) return fail(PP_ERR_ARG, "loss or grad is NULL");
+    (void)lut_changes;
+    (void)lut_signs;
+    (void)nluts;
+    const bool chain = h->chain_tails;
+    h->chain_tails = false;
+    const int rc = adjoint_impl(h, gates, ngates, theta, nparams, max_weight, min_abs_coeff, checkpoint_gates, loss,
+                                grad);
+    h->chain_tails = chain;
+    set_recording(h, false);
+    return rc;
 }
 
 int pp_overlap_zero(pp_handle* h, double* out_b) {

@@ -11,6 +11,7 @@ from .circuits import (bricklayertopology, staircasetopology, rectangletopology,
                        ibmeagletopology, rxlayer, rylayer, rzlayer, rxxlayer, ryylayer, rzzlayer,
                        depolarizinglayer, tfitrottercircuit, tiltedtfitrottercircuit, heisenbergtrottercircuit,
                        hardwareefficientcircuit, efficientsu2circuit)
-from .hipsum import HIPPauliSum, propagate, overlapwithzero, run_heisenberg
+from .hipsum import (HIPPauliSum, PackedCircuit, propagate, overlapwithzero, run_heisenberg, adjointgradient,
+                     rewindgradient)
 
 __version__ = "0.1.0"

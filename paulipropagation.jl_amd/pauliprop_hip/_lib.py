@@ -58,6 +58,8 @@ SIGNATURES = [
                                             ctypes.POINTER(pp_gate_stats)]),
     ("pp_run_circuit", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
                                       ctypes.c_double, _I64P]),
+    ("pp_adjoint", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
+                                  ctypes.c_double, _SZ, _DP, _DP]),
     ("pp_overlap_zero", ctypes.c_int, [_P, _DP]),
     ("pp_synchronize", ctypes.c_int, [_P]),
     ("pp_profile_control", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),

@@ -193,6 +193,22 @@ class PackedCircuit:
         return self.counts[first:last]
 
 
+    def adjoint(self, psum: HIPPauliSum, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0):
+        """(loss[B], grad[P x B]) of overlapwithzero after the whole circuit, by
+        the device adjoint sweep; grad rows are in Heisenberg parameter order.
+        psum holds the observable and is left holding the propagated sum."""
+        B = psum.batch
+        if self.theta.shape[1] != B:
+            raise ValueError(f"the circuit has {self.theta.shape[1]} parameter columns, the sum {B}")
+        loss = np.zeros(B)
+        grad = np.zeros((max(self.nparams, 1), B))
+        _lib.check(_lib.lib().pp_adjoint(
+            psum._h, self.gates, self.ngates, _dp(self.theta), self.nparams, _u64p(self.lut_c), _dp(self.lut_s),
+            self.nluts, _max_weight_arg(max_weight), float(min_abs_coeff), int(checkpoint_gates), _dp(loss),
+            _dp(grad)))
+        return loss, grad[:self.nparams]
+
+
 def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, min_abs_coeff=1e-10,
                    counts: bool = False):
     """Run an already Heisenberg-ordered circuit; hthetas is P x B (or P) in that order."""
@@ -236,6 +252,38 @@ def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-
     hth = np.array(hth) if P else None
     cnt = run_heisenberg(psum, hcirc, hth, max_weight=max_weight, min_abs_coeff=min_abs_coeff, counts=counts)
     return (psum, cnt) if counts else psum
+
+
+def adjointgradient(circuit, psum, thetas, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0):
+    """(expectation, gradient) of overlapwithzero(propagate(circuit, psum, thetas))
+    on the GPU by the adjoint sweep, with truncations held fixed as the forward
+    run made them. thetas is P (or P x B) in Schrodinger order and the gradient
+    comes back in the same shape and order. Noise gates are allowed.
+
+    The reference's counterpart is rewindgradient(circuit, psum, params,
+    overlapwithzero; kwargs...) (src/gradient.jl:26-82)."""
+    P = countparameters(circuit)
+    th = np.asarray(thetas, dtype=np.float64)
+    vec = th.ndim == 1
+    th = th.reshape(P, -1)
+    B = th.shape[1]
+    if not isinstance(psum, HIPPauliSum):
+        psum = HIPPauliSum.from_paulisum(psum, batch=B)
+    pc = PackedCircuit(circuit, th, psum.nqubits, batch=B)
+    loss, grad = pc.adjoint(psum, max_weight=max_weight, min_abs_coeff=min_abs_coeff,
+                            checkpoint_gates=checkpoint_gates)
+    grad = grad[::-1].copy()  # Heisenberg parameter rows run in reverse Schrodinger order
+    if vec:
+        return float(loss[0]), grad[:, 0].copy()
+    return loss, grad
+
+
+def rewindgradient(circuit, psum, thetas, overlapfunc=None, max_weight=math.inf, min_abs_coeff=1e-10):
+    """rewindgradient(circuit, psum, params, overlapwithzero; kwargs...) -> (expec, grad)
+    (src/gradient.jl:26-30), for the loss overlapwithzero."""
+    if overlapfunc is not None and overlapfunc is not overlapwithzero:
+        raise NotImplementedError("the device gradient supports overlapwithzero only")
+    return adjointgradient(circuit, psum, thetas, max_weight=max_weight, min_abs_coeff=min_abs_coeff)
 
 
 def overlapwithzero(psum):

@@ -1,0 +1,128 @@
+"""GPU parity of the adjoint sweep (pp_adjoint): against the oracle's reverse
+sweep, against central finite differences of the GPU forward run
+(test/test_gradient.jl:9-31 of the reference: eps 1e-6, atol 1e-7), and for
+checkpoint segments, batches and the 48-qubit ansatz workload."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle_adjoint
+import pauliprop_hip as pp
+from pauliprop_hip import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _ansatz(n, layers, noise):
+    circ = []
+    for _ in range(layers):
+        pp.rylayer(circ, n)
+        pp.rzlayer(circ, n)
+        pp.rzzlayer(circ, pp.staircasetopology(n))
+        if noise:
+            pp.depolarizinglayer(circ, n, 0.02)
+    obs = pp.PauliSum(n)
+    for i in range(1, n):
+        obs.add(["Z", "Z"], [i, i + 1])
+    return circ, obs
+
+
+def _gpu_loss(circ, obs, th, **kw):
+    return pp.propagate(circ, obs, th, **kw).overlapwithzero()
+
+
+@pytest.mark.parametrize("noise", [False, True])
+def test_adjoint_matches_oracle_and_central_differences(noise):
+    print(f"adjoint vs oracle and FD, noise={noise}", flush=True)
+    n = 8
+    circ, obs = _ansatz(n, 3, noise)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(3 + noise).normal(0, 0.3, size=P)
+    loss, grad = pp.adjointgradient(circ, obs, th, max_weight=5, min_abs_coeff=0.0)
+    rloss, rgrad = oracle_adjoint(circ, n, th, obs.terms, max_weight=5, min_abs_coeff=0.0)
+    assert loss == pytest.approx(rloss[0], rel=1e-12, abs=1e-14)
+    scale = max(1.0, float(np.abs(rgrad).max()))
+    np.testing.assert_allclose(grad, rgrad[:, 0], rtol=0, atol=1e-12 * scale)
+    # weight truncation does not depend on theta, so central differences see the same masks
+    eps = 1e-6
+    for k in range(0, P, 5):
+        tp, tm = th.copy(), th.copy()
+        tp[k] += eps
+        tm[k] -= eps
+        fd = (_gpu_loss(circ, obs, tp, max_weight=5, min_abs_coeff=0.0) -
+              _gpu_loss(circ, obs, tm, max_weight=5, min_abs_coeff=0.0)) / (2 * eps)
+        assert abs(fd - grad[k]) < 1e-7, (k, fd, grad[k])
+
+
+def test_adjoint_with_coefficient_truncation_matches_oracle():
+    # truncation by |c| is held as the forward run made it, in both implementations
+    print("adjoint with min_abs_coeff", flush=True)
+    n = 10
+    circ, obs = _ansatz(n, 4, True)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(8).normal(0, 0.4, size=P)
+    loss, grad = pp.adjointgradient(circ, obs, th, max_weight=6, min_abs_coeff=1e-4)
+    rloss, rgrad = oracle_adjoint(circ, n, th, obs.terms, max_weight=6, min_abs_coeff=1e-4)
+    assert loss == pytest.approx(rloss[0], rel=1e-12, abs=1e-14)
+    scale = max(1.0, float(np.abs(rgrad).max()))
+    np.testing.assert_allclose(grad, rgrad[:, 0], rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("segment", [1, 7, 40])
+def test_checkpoint_segments_give_identical_gradients(segment):
+    # each segment is run again from a snapshot of its input sum; the rerun is
+    # bit-identical, so the gradient is too, and the handle ends on the final sum
+    print(f"checkpoint segment {segment}", flush=True)
+    n = 9
+    circ, obs = _ansatz(n, 3, True)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(21).normal(0, 0.3, size=P)
+    pc = pp.PackedCircuit(circ, th, n)
+    whole = pp.HIPPauliSum.from_paulisum(obs)
+    l0, g0 = pc.adjoint(whole, max_weight=6, min_abs_coeff=1e-6)
+    seg = pp.HIPPauliSum.from_paulisum(obs)
+    l1, g1 = pc.adjoint(seg, max_weight=6, min_abs_coeff=1e-6, checkpoint_gates=segment)
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(g0, g1)
+    k0, c0 = whole.download()
+    k1, c1 = seg.download()
+    assert np.array_equal(k0, k1) and np.array_equal(c0.view(np.uint64), c1.view(np.uint64))
+
+
+def test_batched_adjoint_matches_oracle():
+    print("batched adjoint", flush=True)
+    n = 8
+    circ, obs = _ansatz(n, 3, True)
+    P = pp.countparameters(circ)
+    B = 5
+    th = np.random.default_rng(4).normal(0, 0.3, size=(P, B))
+    loss, grad = pp.adjointgradient(circ, obs, th, max_weight=5, min_abs_coeff=1e-5)
+    rloss, rgrad = oracle_adjoint(circ, n, th, obs.terms, max_weight=5, min_abs_coeff=1e-5, batch=B)
+    np.testing.assert_allclose(loss, rloss, rtol=1e-12, atol=1e-14)
+    scale = max(1.0, float(np.abs(rgrad).max()))
+    np.testing.assert_allclose(grad, rgrad, rtol=0, atol=1e-12 * scale)
+
+
+def test_ansatz_48_qubits_matches_oracle():
+    # the 48-qubit, 12-layer ansatz workload: about 1700 parameters, 67k peak terms;
+    # the records outgrow their first allocation, so gates are run again after growth
+    print("48-qubit ansatz adjoint", flush=True)
+    wl = workloads.config4()
+    loss, grad = pp.adjointgradient(wl["circuit"], wl["obs"], wl["thetas"], max_weight=wl["max_weight"],
+                                    min_abs_coeff=wl["min_abs_coeff"])
+    rloss, rgrad = oracle_adjoint(wl["circuit"], wl["nqubits"], wl["thetas"], wl["obs"].terms,
+                                  max_weight=wl["max_weight"], min_abs_coeff=wl["min_abs_coeff"])
+    assert loss == pytest.approx(rloss[0], rel=1e-12)
+    scale = float(np.abs(rgrad).max())
+    np.testing.assert_allclose(grad, rgrad[:, 0], rtol=0, atol=1e-11 * scale)
+
+
+def test_adjoint_rejects_bad_arguments():
+    n = 6
+    circ, obs = _ansatz(n, 1, False)
+    P = pp.countparameters(circ)
+    with pytest.raises(ValueError):
+        pp.adjointgradient(circ, obs, np.zeros(P + 1))
+    with pytest.raises(NotImplementedError):
+        pp.rewindgradient(circ, obs, np.zeros(P), overlapfunc=lambda s: 0.0)
