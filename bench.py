@@ -58,7 +58,21 @@ def main() -> int:
     ap.add_argument("--config", default="config2", choices=["config1", "config2", "config3"])
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (scales the HBM counters)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = 32 << 20
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": n * 8}))
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is designed but not built",
@@ -112,8 +126,10 @@ def main() -> int:
     avg_ms = kst["ms"] / max(kst["launches"], 1)
     avg_bytes = kst["bytes"] / max(kst["launches"], 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(args.config, kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"k_{kname}",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": f"k_{kname}",
                 "avg_launch_us": round(avg_ms * 1e3, 2), "avg_algorithmic_bytes": round(avg_bytes),
                 "launches_per_step": kst["launches"]}
     kernels = {k: {"launches": v["launches"], "ms_timed_step": round(v["ms"], 2),
@@ -123,6 +139,10 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pp, pc, psum, keys0, coeffs0, counts, wl, args.cpu_seconds)
+
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_detail(pp, PackedCircuit)
 
     L = 127 + 144 if args.config == "config2" else None
     out = {
@@ -145,10 +165,104 @@ def main() -> int:
         "detail": {"term_applications_per_step": apps // args.steps, "peak_terms": int(counts.max()),
                    "final_terms": final_terms, "overlapwithzero": overlap if B == 1 else list(overlap)[:4],
                    "gates_touching_terms": kernels.get("merge", {}).get("launches"), "kernels": kernels,
-                   "layer_gates": L},
+                   "layer_gates": L, "adjoint": adj},
     }
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(config, kname):
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each
+    scaled by what the counter reads for a plain copy of known size), or None
+    when no summary exists for it."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"k_{kname}1_{config}_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.]+)", open(path).read())
+    if not m:
+        return None, None
+    return round(float(m.group(1))), os.path.relpath(path, ROOT)
+
+
+def adjoint_detail(pp, PackedCircuit):
+    """Gradients/s of the adjoint sweep against plain forward finite differences
+    (2P forward runs per gradient, h = 1e-6) on the same GPU: config 4 (48
+    qubits, 1716 parameters, B = 1) and config 5 (the same ansatz with
+    depolarizing noise, B = 32 gradients in one batched sweep; its FD baseline
+    is 32 separate B = 1 gradients). Forward runs are timed and multiplied out;
+    the agreement is measured on a few parameters."""
+    import numpy as np
+    from pauliprop_hip import workloads
+    out = {}
+    for name in ("config4", "config5"):
+        wl = workloads.CONFIGS[name]()
+        nq, mw, mac = wl["nqubits"], wl["max_weight"], wl["min_abs_coeff"]
+        th = np.asarray(wl["thetas"], dtype=np.float64)
+        B = 1 if th.ndim == 1 else th.shape[1]
+        P = pp.countparameters(wl["circuit"])
+        pc = PackedCircuit(wl["circuit"], th, nq, batch=B)
+        k0, c0 = wl["obs"].to_arrays(pc.W, B)
+        s = pp.HIPPauliSum(nq, batch=B, capacity=1 << 18)
+
+        def adjoint():
+            s.upload(k0, c0)
+            return pc.adjoint(s, max_weight=mw, min_abs_coeff=mac)
+
+        adjoint()
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            loss, grad = adjoint()
+        t_adj = (time.perf_counter() - t) / reps
+        # plain forward of one column (the FD baseline's unit of work)
+        th1 = th if B == 1 else th[:, 0]
+        pf = PackedCircuit(wl["circuit"], th1, nq)
+        s1 = pp.HIPPauliSum(nq, capacity=1 << 18)
+        k1, c1 = wl["obs"].to_arrays(pf.W, 1)
+
+        def forward(pcx):
+            s1.upload(k1, c1)
+            pcx.run(s1, max_weight=mw, min_abs_coeff=mac)
+            return s1.overlapwithzero()
+
+        forward(pf)
+        t = time.perf_counter()
+        for _ in range(reps):
+            forward(pf)
+        t_fwd = (time.perf_counter() - t) / reps
+        # agreement with central differences on a few parameters of column 0,
+        # through forward runs of the same batch (so the batch-max truncation
+        # sees the same columns)
+        def loss0(thx):
+            s.upload(k0, c0)
+            PackedCircuit(wl["circuit"], thx, nq, batch=B).run(s, max_weight=mw, min_abs_coeff=mac)
+            ov = s.overlapwithzero()
+            return float(ov if B == 1 else ov[0])
+
+        h = 1e-6
+        diffs = []
+        for k in np.linspace(0, P - 1, 4).astype(int):
+            tp, tm = th.copy(), th.copy()
+            if B == 1:
+                tp[k] += h
+                tm[k] -= h
+            else:
+                tp[k, 0] += h
+                tm[k, 0] -= h
+            fd = (loss0(tp) - loss0(tm)) / (2 * h)
+            # Heisenberg parameter rows run in reverse Schrodinger order
+            diffs.append(abs(fd - float(grad[P - 1 - k, 0])))
+        adj_rate = B / t_adj
+        fd_rate = 1.0 / (2 * P * t_fwd)
+        out[name] = {"params": P, "batch": B, "adjoint_s": round(t_adj, 4), "forward_s": round(t_fwd, 4),
+                     "grads_per_s": round(adj_rate, 2), "fd_grads_per_s": round(fd_rate, 4),
+                     "adjoint_over_fd": round(adj_rate / fd_rate, 1), "loss0": float(loss[0]),
+                     "max_abs_diff_vs_fd": float(max(diffs)), "fd_params_checked": len(diffs)}
+        log(f"adjoint {name}: {t_adj * 1e3:.1f} ms per sweep of {B} gradient(s), forward {t_fwd * 1e3:.1f} ms, "
+            f"x{adj_rate / fd_rate:.0f} over FD, max |adj - fd| {max(diffs):.2e}")
+    return out
 
 
 def workload_text(name, wl) -> str:

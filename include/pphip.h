@@ -153,7 +153,7 @@ int pp_synchronize(pp_handle *h);
 /* Kernel kinds of the per-handle profile. */
 enum pp_kernel_kind {
     PP_KERNEL_BRANCH = 0,        /* anticommuting terms, compacted in order */
-    PP_KERNEL_TAIL_PERMUTE = 1,  /* tail into ascending order of term ^ mask */
+    PP_KERNEL_TAIL_PERMUTE = 1,  /* tail into ascending order of term ^ mask, and the merge partition */
     PP_KERNEL_MERGE = 2,         /* merge with truncation */
     PP_KERNEL_NOISE = 3,         /* Pauli noise with truncation */
     PP_KERNEL_OVERLAP = 4,       /* overlapwithzero partial sums */

@@ -16,6 +16,7 @@
 // per chunk of gates to catch an overflow, grows the buffers and resumes at
 // the gate that overflowed.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -419,8 +420,20 @@ int check_mask(const pp_handle* h, const uint64_t* mask) {
 // ---------------------------------------------------------------- one gate, enqueued
 
 // Rotation: cos/sin by value for B = 1, or at d_trig (B cos then B sin) for B > 1.
+// Launch on the handle's stream; with events, the dispatch itself records
+// them at its start and end (no marker packets between kernels).
+template <typename F, typename... Args>
+void launch(pp_handle* h, hipEvent_t e0, hipEvent_t e1, F kernel, unsigned grid, size_t shm, Args... args) {
+    if (e0 || e1)
+        hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(WG), (uint32_t)shm, h->stream, e0, e1, 0u, args...);
+    else
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(WG), shm, h->stream, args...);
+}
+
 // next_mask: the mask of the rotation enqueued right after this one, if any;
 // the merge then lists that gate's touched terms (B = 1).
+// Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
+// kernel timed, mode 2); merge [4, 5] (mode 1 and 2).
 template <int W>
 void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
                   double min_abs, uint32_t* glog, hipEvent_t* ev, const uint64_t* next_mask) {
@@ -429,41 +442,38 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     mask_levels(mask_in, W, &lv);
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode >= 1 && ev;
-    if (all) hipEventRecord(ev[0], h->stream);
-    hipLaunchKernelGGL(k_branch<W>, dim3(lb_grid(h, (const void*)k_branch<W>, 0, h->n_est, K1_TILE)), dim3(WG), 0,
-                       h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask);
-    if (all) hipEventRecord(ev[1], h->stream);
+    auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
+    launch(h, E(0, all), E(1, all), k_branch<W>, lb_grid(h, (const void*)k_branch<W>, 0, h->n_est, K1_TILE), 0,
+           h->bufs, h->d_ctl, lb_launch(h, glog), mask);
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
     if (h->recording)
-        hipLaunchKernelGGL(k_record_tails, dim3(grid_for(h, h->n_est * (size_t)h->B / 2, WG * 4)), dim3(WG), 0,
-                           h->stream, h->bufs, h->d_ctl, L2, h->B);
-    if (all) hipEventRecord(ev[2], h->stream);
-    hipLaunchKernelGGL(k_xorperm<W>, dim3(grid_for(h, h->n_est / 2, K2_TILE)), dim3(WG), 0, h->stream, h->bufs,
-                       h->d_ctl, L2, lv, mask);
-    if (all) hipEventRecord(ev[3], h->stream);
-    if (mrg) hipEventRecord(ev[4], h->stream);
-    if (h->B == 1) {
+        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->n_est * (size_t)h->B / 2, WG * 4), 0, h->bufs,
+               h->d_ctl, L2, h->B);
+    const bool part = h->B == 1;
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->n_est / 2, K2_TILE), 0, h->bufs,
+           h->d_ctl, L2, lv, mask);
+    const size_t expect = h->n_est + h->n_est / 2;
+    if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
         constexpr int TILE = K3Cfg<W>::TILE;
-        const size_t expect = h->n_est + h->n_est / 2;
         Launch LP{};
         LP.seq = h->seq;
-        hipLaunchKernelGGL((k_partition<W, TILE>), dim3(grid_for(h, expect, (size_t)TILE * NWAVES)), dim3(WG), 0,
-                           h->stream, h->bufs, h->d_ctl, LP);
+        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
+               h->d_ctl, LP);
         if (h->recording) {
-            hipLaunchKernelGGL((k_merge1<W, false, true>),
-                               dim3(lb_grid(h, (const void*)k_merge1<W, false, true>, 0, expect, TILE)), dim3(WG), 0,
-                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, true>,
+                   lb_grid(h, (const void*)k_merge1<W, false, true>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+                   lb_launch(h, glog), mask, rp, mask);
         } else if (next_mask) {
-            hipLaunchKernelGGL((k_merge1<W, true, false>),
-                               dim3(lb_grid(h, (const void*)k_merge1<W, true, false>, 0, expect, TILE)), dim3(WG), 0,
-                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, to_key<W>(next_mask));
+            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, true, false>,
+                   lb_grid(h, (const void*)k_merge1<W, true, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+                   lb_launch(h, glog), mask, rp, to_key<W>(next_mask));
         } else {
-            hipLaunchKernelGGL((k_merge1<W, false, false>),
-                               dim3(lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE)), dim3(WG), 0,
-                               h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
+                   lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+                   lb_launch(h, glog), mask, rp, mask);
         }
     } else {
         const int B = h->B;
@@ -471,9 +481,8 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
-    hipLaunchKernelGGL((k_mergeB<W, TBV>),                                                                    \
-                       dim3(lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, h->n_est + h->n_est / 2, TBV)),    \
-                       dim3(WG), shm, h->stream, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs)
+    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -483,7 +492,6 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         }
 #undef LAUNCH_MB
     }
-    if (mrg) hipEventRecord(ev[5], h->stream);
     h->seq += 1;
 }
 
@@ -491,17 +499,14 @@ template <int W>
 void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const double* d_factor, int max_weight,
                double min_abs, uint32_t* glog, hipEvent_t* ev) {
     const bool mrg = h->prof_mode >= 1 && ev;
-    if (mrg) hipEventRecord(ev[4], h->stream);
+    hipEvent_t e0 = mrg ? ev[4] : nullptr, e1 = mrg ? ev[5] : nullptr;
     if (h->B == 1) {
-        hipLaunchKernelGGL(k_noise1<W>, dim3(lb_grid(h, (const void*)k_noise1<W>, 0, h->n_est, KNCfg<W>::TILE)), dim3(WG),
-                           0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight,
-                           min_abs);
+        launch(h, e0, e1, k_noise1<W>, lb_grid(h, (const void*)k_noise1<W>, 0, h->n_est, KNCfg<W>::TILE), 0, h->bufs,
+               h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
     } else {
-        hipLaunchKernelGGL(k_noiseB<W>, dim3(lb_grid(h, (const void*)k_noiseB<W>, 0, h->n_est, KNB_ROWS)),
-                           dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes,
-                           d_factor, max_weight, min_abs);
+        launch(h, e0, e1, k_noiseB<W>, lb_grid(h, (const void*)k_noiseB<W>, 0, h->n_est, KNB_ROWS), 0, h->bufs,
+               h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
     }
-    if (mrg) hipEventRecord(ev[5], h->stream);
     h->seq += 1;
 }
 
@@ -642,18 +647,20 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         if (gs[i].kind == PP_GATE_ROTATION) {
             int mask_limbs = 0;
             for (int l = 0; l < h->W; ++l) mask_limbs += gs[i].mask[l] != 0;
+            // every rotation launches all its kernels (those of a gate that touches
+            // nothing exit at once), so launches count dispatches as a profiler does
             h->launches[PP_KERNEL_BRANCH] += 1;
             h->alg_bytes[PP_KERNEL_BRANCH] += 8.0 * mask_limbs * nin + 4.0 * na;
             if (h->prof_mode == 2) h->ms[PP_KERNEL_BRANCH] += span(0);
+            h->launches[PP_KERNEL_TAIL_PERMUTE] += 1;
+            h->launches[PP_KERNEL_MERGE] += 1;
+            if (h->prof_mode == 2) h->ms[PP_KERNEL_TAIL_PERMUTE] += span(2);
+            if (h->prof_mode >= 1) h->ms[PP_KERNEL_MERGE] += span(4);
             if (na > 0) {
                 Levels lv;
                 mask_levels(gs[i].mask, h->W, &lv);
-                h->launches[PP_KERNEL_TAIL_PERMUTE] += 1;
                 h->alg_bytes[PP_KERNEL_TAIL_PERMUTE] += na * (8.0 + 8.0 * (h->W - lv.from_limb));
-                h->launches[PP_KERNEL_MERGE] += 1;
                 h->alg_bytes[PP_KERNEL_MERGE] += row * ((double)nin + (double)nout);
-                if (h->prof_mode == 2) h->ms[PP_KERNEL_TAIL_PERMUTE] += span(2);
-                if (h->prof_mode >= 1) h->ms[PP_KERNEL_MERGE] += span(4);
             }
         } else {
             h->launches[PP_KERNEL_NOISE] += 1;
