@@ -13,8 +13,9 @@ of the K timed steps.
 
 roofline: the kernel with the most device time, its algorithmic bytes per
 launch ((K + 8B)(n_in + n_out) for a merge: one read and one write of the live
-set) over its average launch time, from HIP events recorded on the library's
-stream around every launch inside the timed region.
+set) over its average launch time, from start/stop events that every merge
+dispatch of the last timed step records on the library's stream; traffic is
+the HBM bytes per launch from the committed rocprof counter summary.
 
 cpu_baseline: the CPU oracle (a single-threaded C restatement of the
 reference's VectorPauliSum path) timed on the host on the heaviest layer of the
@@ -55,7 +56,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="config2", choices=["config1", "config2", "config3"])
+    ap.add_argument("--workload", default="kicked_ising_heavyhex_127q",
+                    choices=["tfim_brick_32q", "kicked_ising_heavyhex_127q", "heisenberg_square_64q_b64"])
     ap.add_argument("--cpu-seconds", type=float, default=25.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
@@ -84,7 +86,7 @@ def main() -> int:
     from pauliprop_hip import workloads
     from pauliprop_hip.hipsum import PackedCircuit
 
-    wl = workloads.CONFIGS[args.config]()
+    wl = workloads.WORKLOADS[args.workload]()
     nq, B = wl["nqubits"], wl.get("batch", 1)
     mw, mac = wl["max_weight"], wl["min_abs_coeff"]
     pc = PackedCircuit(wl["circuit"], wl["thetas"], nq, batch=B)
@@ -126,7 +128,7 @@ def main() -> int:
     avg_ms = kst["ms"] / max(kst["launches"], 1)
     avg_bytes = kst["bytes"] / max(kst["launches"], 1)
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic, traffic_src = measured_traffic(args.config, kname)
+    traffic, traffic_src = measured_traffic(args.workload, kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": f"k_{kname}",
@@ -144,7 +146,7 @@ def main() -> int:
     if not args.no_adjoint:
         adj = adjoint_detail(pp, PackedCircuit)
 
-    L = 127 + 144 if args.config == "config2" else None
+    L = 127 + 144 if args.workload == "kicked_ising_heavyhex_127q" else None
     out = {
         "metric": "Pauli-term gate applications/s + % HBM peak; adjoint grads/s vs plain FD",
         "value": round(value, 1),
@@ -158,7 +160,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload_text(args.config, wl), "nqubits": nq, "key_bits": 64 * pc.W, "batch": B,
+        "config": {"workload": workload_text(args.workload, wl), "nqubits": nq, "key_bits": 64 * pc.W, "batch": B,
                    "gates": pc.ngates, "max_weight": mw, "min_abs_coeff": mac},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -171,13 +173,13 @@ def main() -> int:
     return 0
 
 
-def measured_traffic(config, kname):
+def measured_traffic(workload, kname):
     """HBM bytes per launch of the dominant kernel from the committed rocprof
     counter summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each
     scaled by what the counter reads for a plain copy of known size), or None
     when no summary exists for it."""
     import re
-    path = os.path.join(ROOT, "profiles", f"k_{kname}1_{config}_traffic.md")
+    path = os.path.join(ROOT, "profiles", f"k_{kname}1_{workload}_traffic.md")
     if not os.path.exists(path):
         return None, None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.]+)", open(path).read())
@@ -188,16 +190,16 @@ def measured_traffic(config, kname):
 
 def adjoint_detail(pp, PackedCircuit):
     """Gradients/s of the adjoint sweep against plain forward finite differences
-    (2P forward runs per gradient, h = 1e-6) on the same GPU: config 4 (48
-    qubits, 1716 parameters, B = 1) and config 5 (the same ansatz with
-    depolarizing noise, B = 32 gradients in one batched sweep; its FD baseline
-    is 32 separate B = 1 gradients). Forward runs are timed and multiplied out;
+    (2P forward runs per gradient, h = 1e-6) on the same GPU: the 48-qubit
+    line ansatz (1716 parameters, B = 1) and the same ansatz with depolarizing
+    noise (B = 32 gradients in one batched sweep; its FD baseline is 32
+    separate B = 1 gradients). Forward runs are timed and multiplied out;
     the agreement is measured on a few parameters."""
     import numpy as np
     from pauliprop_hip import workloads
     out = {}
-    for name in ("config4", "config5"):
-        wl = workloads.CONFIGS[name]()
+    for name in ("ansatz_line_48q", "ansatz_line_noisy_48q_b32"):
+        wl = workloads.WORKLOADS[name]()
         nq, mw, mac = wl["nqubits"], wl["max_weight"], wl["min_abs_coeff"]
         th = np.asarray(wl["thetas"], dtype=np.float64)
         B = 1 if th.ndim == 1 else th.shape[1]
@@ -267,10 +269,10 @@ def adjoint_detail(pp, PackedCircuit):
 
 def workload_text(name, wl) -> str:
     return {
-        "config1": "config1: 32q periodic-brick TFIM, 32 layers, theta 0.1, Z16, UInt64 keys",
-        "config2": "config2: 127q heavy-hex kicked Ising, 20 x (RX(U[0,pi/2]) on 127 qubits + RZZ(-pi/2) on 144 "
+        "tfim_brick_32q": "TFIM on a 32-qubit periodic brick, 32 layers, theta 0.1, Z16, UInt64 keys",
+        "kicked_ising_heavyhex_127q": "kicked Ising on the 127-qubit heavy-hex graph, 20 x (RX(U[0,pi/2]) on 127 qubits + RZZ(-pi/2) on 144 "
                    "edges), Z63, UInt256 keys",
-        "config3": "config3: 64q 8x8 periodic Heisenberg, 10 steps, B=64, sum ZZ, UInt128 keys",
+        "heisenberg_square_64q_b64": "Heisenberg on the 8x8 periodic square lattice (64 qubits), 10 steps, B=64, sum ZZ, UInt128 keys",
     }[name]
 
 
@@ -281,7 +283,7 @@ def cpu_baseline(pp, pc, psum, keys0, coeffs0, counts, wl, budget_s):
     import ctypes
     import oracle as orc
 
-    L = 127 + 144 if wl["name"] == "config2" else max(1, pc.ngates // 20)
+    L = 127 + 144 if wl["name"] == "kicked_ising_heavyhex_127q" else max(1, pc.ngates // 20)
     nl = pc.ngates // L
     loads = [int(counts[i * L:(i + 1) * L].sum()) for i in range(nl)]
     layer = int(np.argmax(loads))

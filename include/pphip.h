@@ -5,7 +5,7 @@
  * The reference has no FFI for this path: it dispatches in Julia on the
  * propagation cache type. Each entry point below replaces one Julia method
  * of that dispatch (paths relative to the reference checkout); a Julia host
- * package binds them with `ccall` (the binding is shown in INTEGRATION.md).
+ * package binds them with `ccall`.
  *
  * Conventions
  *   - A Pauli string on n qubits is W = 1, 2 or 4 little-endian uint64 limbs

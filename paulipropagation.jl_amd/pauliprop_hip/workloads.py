@@ -1,22 +1,27 @@
-"""The benchmark workloads: circuits, parameters and observables of the five
-configurations this backend is measured on, with a portable seeded generator
-so that every host and device run reads identical parameter bits.
+"""The benchmark workloads: circuits, parameters and observables this backend
+is measured on, with a portable seeded generator so that every host and
+device run reads identical parameter bits.
 
-  config1  32 qubits, periodic brick TFIM, 32 layers, all angles 0.1, Z16,
-           max_weight 6, min_abs_coeff 1e-4 (the reference README example)
-  config2  127 qubits, heavy-hex (IBM Eagle) kicked Ising: 20 layers of
-           RX(theta_h) on every qubit then RZZ(-pi/2) on every edge,
-           theta_h ~ U[0, pi/2] per gate (seed 127001), Z63,
-           max_weight 12, min_abs_coeff 1e-5
-  config3  64 qubits, 8x8 periodic lattice Heisenberg Trotter steps
-           (RXX, RYY, RZZ on every edge, then RZ on every qubit), 10 steps,
-           B = 64 columns theta ~ U[0, 0.2] (seed 6401), observable
-           sum_edges Z_i Z_j, max_weight 8, min_abs_coeff 1e-6
-  config4  48 qubits, 12 layers of RY, RZ on every qubit then RZZ on the
-           open line, theta ~ N(0, 0.3^2) (seed 4801), sum_i Z_i Z_{i+1},
-           max_weight 7, min_abs_coeff 1e-5
-  config5  config 4 plus DepolarizingNoise(q, 0.01) on every qubit after
-           each layer, B = 32 columns (seed 4802)
+  tfim_brick_32q             32 qubits, periodic brick TFIM, 32 layers, all
+                             angles 0.1, Z16, max_weight 6, min_abs_coeff 1e-4
+                             (the reference README example)
+  kicked_ising_heavyhex_127q 127 qubits, heavy-hex (IBM Eagle) kicked Ising:
+                             20 layers of RX(theta_h) on every qubit then
+                             RZZ(-pi/2) on every edge, theta_h ~ U[0, pi/2] per
+                             gate (seed 127001), Z63, max_weight 12,
+                             min_abs_coeff 1e-5
+  heisenberg_square_64q_b64  64 qubits, 8x8 periodic lattice Heisenberg
+                             Trotter steps (RXX, RYY, RZZ on every edge, then
+                             RZ on every qubit), 10 steps, B = 64 columns
+                             theta ~ U[0, 0.2] (seed 6401), observable
+                             sum_edges Z_i Z_j, max_weight 8, min_abs_coeff 1e-6
+  ansatz_line_48q            48 qubits, 12 layers of RY, RZ on every qubit then
+                             RZZ on the open line, theta ~ N(0, 0.3^2) (seed
+                             4801), sum_i Z_i Z_{i+1}, max_weight 7,
+                             min_abs_coeff 1e-5
+  ansatz_line_noisy_48q_b32  the same ansatz plus DepolarizingNoise(q, 0.01) on
+                             every qubit after each layer, B = 32 columns
+                             (seed 4802)
 """
 from __future__ import annotations
 
@@ -68,15 +73,15 @@ class SplitMix64:
         return out
 
 
-def config1():
+def tfim_brick_32q():
     n = 32
     circ = C.tfitrottercircuit(n, 32, topology=C.bricklayertopology(n, periodic=True))
     th = np.full(countparameters(circ), 0.1)
     obs = PauliSum.from_string(PauliString(n, "Z", 16))
-    return dict(name="config1", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=6, min_abs_coeff=1e-4)
+    return dict(name="tfim_brick_32q", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=6, min_abs_coeff=1e-4)
 
 
-def config2(nlayers: int = 20):
+def kicked_ising_heavyhex_127q(nlayers: int = 20):
     n = 127
     topo = C.heavyhextopology()
     circ = []
@@ -89,11 +94,11 @@ def config2(nlayers: int = 20):
         th.extend(rng.uniforms(n, 0.0, math.pi / 2))
         th.extend([-math.pi / 2] * len(topo))
     obs = PauliSum.from_string(PauliString(n, "Z", 63))
-    return dict(name="config2", nqubits=n, circuit=circ, thetas=np.array(th), obs=obs, max_weight=12,
+    return dict(name="kicked_ising_heavyhex_127q", nqubits=n, circuit=circ, thetas=np.array(th), obs=obs, max_weight=12,
                 min_abs_coeff=1e-5)
 
 
-def config3(nsteps: int = 10, B: int = 64):
+def heisenberg_square_64q_b64(nsteps: int = 10, B: int = 64):
     n = 64
     topo = C.rectangletopology(8, 8, periodic=True)
     circ = []
@@ -107,7 +112,7 @@ def config3(nsteps: int = 10, B: int = 64):
     obs = PauliSum(n)
     for a, b in topo:
         obs.add(["Z", "Z"], [a, b], 1.0)
-    return dict(name="config3", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=8, min_abs_coeff=1e-6,
+    return dict(name="heisenberg_square_64q_b64", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=8, min_abs_coeff=1e-6,
                 batch=B)
 
 
@@ -126,21 +131,22 @@ def _hea(n=48, nlayers=12, noise=False):
     return circ, obs
 
 
-def config4():
+def ansatz_line_48q():
     n = 48
     circ, obs = _hea(n, 12, noise=False)
     P = countparameters(circ)
     th = SplitMix64(4801).normals(P, 0.3)
-    return dict(name="config4", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=7, min_abs_coeff=1e-5)
+    return dict(name="ansatz_line_48q", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=7, min_abs_coeff=1e-5)
 
 
-def config5(B: int = 32):
+def ansatz_line_noisy_48q_b32(B: int = 32):
     n = 48
     circ, obs = _hea(n, 12, noise=True)
     P = countparameters(circ)
     th = SplitMix64(4802).normals(P * B, 0.3).reshape(P, B)
-    return dict(name="config5", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=7, min_abs_coeff=1e-5,
+    return dict(name="ansatz_line_noisy_48q_b32", nqubits=n, circuit=circ, thetas=th, obs=obs, max_weight=7, min_abs_coeff=1e-5,
                 batch=B)
 
 
-CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config4": config4, "config5": config5}
+WORKLOADS = {f.__name__: f for f in (tfim_brick_32q, kicked_ising_heavyhex_127q, heisenberg_square_64q_b64,
+                                     ansatz_line_48q, ansatz_line_noisy_48q_b32)}

@@ -108,7 +108,7 @@ def test_ansatz_48_qubits_matches_oracle():
     # the 48-qubit, 12-layer ansatz workload: about 1700 parameters, 67k peak terms;
     # the records outgrow their first allocation, so gates are run again after growth
     print("48-qubit ansatz adjoint", flush=True)
-    wl = workloads.config4()
+    wl = workloads.ansatz_line_48q()
     loss, grad = pp.adjointgradient(wl["circuit"], wl["obs"], wl["thetas"], max_weight=wl["max_weight"],
                                     min_abs_coeff=wl["min_abs_coeff"])
     rloss, rgrad = oracle_adjoint(wl["circuit"], wl["nqubits"], wl["thetas"], wl["obs"].terms,

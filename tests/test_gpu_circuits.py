@@ -12,7 +12,7 @@ import pauliprop_hip as pp
 pytestmark = pytest.mark.gpu
 
 
-def _config1(nlayers=32):
+def _tfim_brick_32q(nlayers=32):
     n = 32
     circ = pp.tfitrottercircuit(n, nlayers, topology=pp.bricklayertopology(n, periodic=True))
     th = np.full(pp.countparameters(circ), 0.1)
@@ -20,23 +20,23 @@ def _config1(nlayers=32):
     return n, circ, th, obs
 
 
-def test_config1_readme_value():
+def test_tfim_brick_32q_readme_value():
     # README.md:114 of the reference: overlapwithzero -> 0.154596728241...
-    n, circ, th, obs = _config1()
+    n, circ, th, obs = _tfim_brick_32q()
     dev = pp.propagate(circ, obs, th, max_weight=6, min_abs_coeff=1e-4)
     ref, _ = oracle_propagate(circ, n, th, obs.terms, max_weight=6, min_abs_coeff=1e-4)
-    assert_bitwise(*dev.download(), *ref.get(), "config 1")
+    assert_bitwise(*dev.download(), *ref.get(), "32-qubit TFIM")
     ov = dev.overlapwithzero()
     assert abs(ov - 0.154596728241) < 1e-12
     assert rel_close(ov, float(ref.overlap_zero()[0]))
 
 
-def test_config1_weight_only_counts():
-    n, circ, th, obs = _config1()
+def test_tfim_brick_32q_weight_only_counts():
+    n, circ, th, obs = _tfim_brick_32q()
     dev, counts = pp.propagate(circ, obs, th, max_weight=6, min_abs_coeff=0.0, counts=True)
     ref, rcounts = oracle_propagate(circ, n, th, obs.terms, max_weight=6, min_abs_coeff=0.0)
     assert np.array_equal(counts, rcounts)
-    assert_bitwise(*dev.download(), *ref.get(), "config 1 weight-only")
+    assert_bitwise(*dev.download(), *ref.get(), "32-qubit TFIM weight-only")
     assert rel_close(dev.overlapwithzero(), fsum_overlap(*ref.get())[0])
 
 

@@ -45,7 +45,7 @@ def _eccentricity(edges, src):
     return max(dist.values())
 
 
-def test_config2_observable_qubit_is_a_central_degree3_node():
+def test_heavyhex_observable_qubit_is_a_central_degree3_node():
     edges = pp.ibmeagletopology
     deg = {}
     for a, b in edges:

@@ -14,7 +14,7 @@ import pauliprop_hip as pp
 from pauliprop_hip.paulis import int_to_limbs, symboltoint
 
 
-def test_config1_readme_value():
+def test_tfim_brick_32q_readme_value():
     # README.md:49-114 of the reference: 32 qubits, periodic brick TFIM, 32 layers, dt 0.1,
     # max_weight 6, min_abs_coeff 1e-4 -> overlapwithzero 0.154596728241...
     n = 32
