@@ -116,6 +116,15 @@ int pp_apply_rotation(pp_handle *h, const uint64_t *mask, const double *cos_b, c
 int pp_apply_pauli_noise(pp_handle *h, int qubit, uint32_t damp_codes, const double *factor_b,
                          int max_weight, double min_abs_coeff, pp_gate_stats *stats);
 
+/* AmplitudeDampingNoise step on `qubit` (gamma_b: B values, equal across
+ * columns): X and Y scale by sqrt(1 - gamma); Z keeps 1 - gamma and branches
+ * into I with gamma; then every output is truncated (no truncation when no
+ * term has X, Y or Z there).
+ * Replaces: applymergetruncate!(::AmplitudeDampingNoise, cache, gamma; ...)
+ *           (src/Propagation/specializations.jl:250-285). */
+int pp_apply_amplitude_damping(pp_handle *h, int qubit, const double *gamma_b, int max_weight,
+                               double min_abs_coeff, pp_gate_stats *stats);
+
 /* Run a Heisenberg-ordered gate list (the reference's _propagate! loop over
  * applymergetruncate!, src/Base/propagate.jl:44-71). theta is nparams x B
  * row-major. Clifford tables are nluts slots of 16 entries: lut_changes

@@ -72,7 +72,8 @@ struct GateState {
     uint32_t na;    // touched (anticommuting) terms, written by k_branch
     uint32_t roff;  // where the gate's records start (recording only)
     uint32_t coff;  // where its coefficient rows start
-    uint32_t pad[3];
+    uint32_t run;   // amplitude damping: some term has X, Y or Z on the qubit (merge runs even when na = 0)
+    uint32_t pad[2];
 };
 
 struct Ctl {
@@ -171,7 +172,12 @@ __device__ __forceinline__ void set_state(Ctl* ctl, int p, uint32_t n, uint32_t 
     ctl->st[p].n = n;
     ctl->st[p].cur = cur;
     ctl->st[p].na = 0u;
+    ctl->st[p].run = 0u;
 }
+
+// Whether the merge of the gate in state p has work: created terms, or (for
+// amplitude damping) terms whose coefficients change.
+__device__ __forceinline__ bool merge_runs(const Ctl* ctl, int p) { return ctl->st[p].na || ctl->st[p].run; }
 
 // The records of a gate do not fit: stop, and let the host grow them and run
 // the gate again.
@@ -239,13 +245,17 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
     } else if (roff + words > bf.rec_words || coff + rows > bf.rec_rows) {
         record_overflow(ctl, L, roff + words, coff + rows);
     } else {
-        set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes
+        set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes (or is damped)
         ctl->st[p ^ 1].roff = (uint32_t)(roff + words);
         ctl->st[p ^ 1].coff = (uint32_t)(coff + rows);
     }
 }
 
-template <int W>
+// DAMP: amplitude damping on the qubit whose Z the mask holds (the rule of
+// src/Propagation/specializations.jl:258-276): the listed terms are those with
+// Z there (they branch into I with gamma * c), and any term with X, Y or Z
+// there sets st[p].run, since its coefficient changes.
+template <int W, bool DAMP>
 __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
@@ -261,7 +271,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
         return;
     }
     // the touched terms may already be listed by the previous gate's merge
-    bool listed = ctl->tail_ready && ctl->tail_seq == L.seq;
+    bool listed = !DAMP && ctl->tail_ready && ctl->tail_seq == L.seq;
 #pragma unroll
     for (int l = 0; l < W; ++l) listed = listed && ctl->tail_mask[l] == mask.l[l];
     if (listed) {
@@ -296,6 +306,17 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
 #pragma unroll
             for (int k = 0; k < K1_ITEMS; ++k) v[k] = keys[l * stride + min(base + k * WG + t, n - 1)];
             const uint64_t m = mask.l[l], sm = swap_pairs(m);
+            if (DAMP) {
+                uint32_t any = 0;
+#pragma unroll
+                for (int k = 0; k < K1_ITEMS; ++k) {
+                    const bool in = base + k * WG + t < n;
+                    antibits |= (uint32_t)(in && (v[k] & m) == m) << k;  // Z on the qubit
+                    any |= (uint32_t)(in && (v[k] & m) != 0);
+                }
+                if (__ballot(any != 0) && lane == 0) atomicOr(&ctl->st[p].run, 1u);
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < K1_ITEMS; ++k) {
                 antibits ^= (uint32_t)(__popcll(sm & v[k]) & 1) << k;
@@ -306,7 +327,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
         for (int k = 0; k < K1_ITEMS; ++k) {
             const uint32_t i = base + k * WG + t;
             const bool f = (i < n) && ((antibits >> k) & 1u);
-            if (f && (ex[k] & 2) == 0) negbits |= 1u << k;
+            if (!DAMP && f && (ex[k] & 2) == 0) negbits |= 1u << k;
             ballots[k] = __ballot(f);
             if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(ballots[k]);
         }
@@ -639,10 +660,16 @@ __global__ __launch_bounds__(WG) void k_record_tails(Bufs bf, Ctl* ctl, Launch L
 
 // ============================================================ k_merge (one coefficient column)
 
+// Values of one branching gate. Rotation: heads that anticommute times cosv,
+// created terms times sign * sinv. Amplitude damping on qubit damp_q >= 0:
+// heads with Z there times cosv = 1 - gamma, with X or Y times fxy =
+// sqrt(1 - gamma); created terms (Z -> I) times sinv = gamma.
 struct RotParams {
     double cosv, sinv;
     int max_weight;   // < 0: no weight truncation
     double min_abs;
+    int damp_q = -1;
+    double fxy = 1.0;
 };
 
 // number of head entries among the first d merged entries; heads go first on ties
@@ -730,7 +757,7 @@ __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
     if (ctl->overflow) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
-    if (na == 0) return;
+    if (!merge_runs(ctl, p)) return;
     const uint32_t n = ctl->st[p].n;
     const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
     const uint64_t* __restrict__ ss = bf.ssamp;
@@ -781,7 +808,8 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
                                             const uint64_t* __restrict__ tkeys, uint32_t na, uint32_t i0,
                                             uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
                                             double sinv, uint64_t (*s_key)[TILE + 2], double* s_val,
-                                            const uint32_t* __restrict__ tpj, uint32_t* s_pj) {
+                                            const uint32_t* __restrict__ tpj, uint32_t* s_pj, int damp_q,
+                                            double fxy) {
     const int t = threadIdx.x;
     constexpr int R = (TILE + 1 + WG - 1) / WG;
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
@@ -798,9 +826,17 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
         if (x < ns + 1 && (x > 0 || i0 > 0)) {
 #pragma unroll
             for (int l = 0; l < W; ++l) s_key[l][x] = k.l[l];
-            s_val[x] = anticommutes<W>(mask, k) ? c * cosv : c;
+            double v;
+            if (damp_q >= 0) {
+                const uint32_t code = code_at<W>(k, damp_q);
+                v = code == 3u ? c * cosv : (code ? c * fxy : c);
+            } else {
+                v = anticommutes<W>(mask, k) ? c * cosv : c;
+            }
+            s_val[x] = v;
         }
     }
+    if (na == 0) return;  // amplitude damping with no Z on the qubit: nothing created
     const uint32_t tmax = na - 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -838,7 +874,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
-    if (na == 0) return;
+    if (!merge_runs(ctl, p)) return;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const double* __restrict__ coeffs = bf.coeffs[cur];
@@ -873,7 +909,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             nt = 0;
         }
         merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
-                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj);
+                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -1013,11 +1049,12 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
 
 template <int W, int TB>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
-                                               const double* __restrict__ trig, int max_weight, double min_abs) {
+                                               const double* __restrict__ trig, int max_weight, double min_abs,
+                                               int damp_q, double fxy) {
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t na = ctl->st[p].na;
-    if (na == 0) return;
+    if (!merge_runs(ctl, p)) return;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const double* __restrict__ coeffs = bf.coeffs[cur];
@@ -1038,6 +1075,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint32_t s_pj[TB + 1];    // recording: parent-order index of tail slot y
     __shared__ uint32_t s_rh[TB];        // recording: head index | REC_FLAG if touched, or ~0u
     __shared__ uint32_t s_rt[TB];        // recording: parent-order index of the consumed tail, or ~0u
+    __shared__ uint8_t s_hxy[TB];        // amplitude damping: head has X or Y on the qubit
     __shared__ uint32_t s_split[2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
@@ -1094,6 +1132,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             uint32_t tl = ~0u;
             bool live = false;
             uint32_t rh = ~0u, rt = ~0u;
+            bool xy = false;
             if (take_s) {
                 okey = S(x);
                 head = (int)(i0 + x);
@@ -1103,7 +1142,11 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     rt = s_pj[y];
                 }
                 live = true;
-                if (anticommutes<W>(mask, okey)) {
+                if (damp_q >= 0) {
+                    const uint32_t code = code_at<W>(okey, damp_q);
+                    if (code == 3u) head = -(head + 2);
+                    xy = code == 1u || code == 2u;
+                } else if (anticommutes<W>(mask, okey)) {
                     head = -(head + 2);
                     rh |= REC_FLAG;
                 }
@@ -1118,6 +1161,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             }
             s_rh[t] = rh;
             s_rt[t] = rt;
+            s_hxy[t] = xy ? 1 : 0;
             if (live && max_weight >= 0 && weight<W>(okey) > max_weight) live = false;
             s_head[t] = live ? head : INT_MIN;
             s_tl[t] = live ? tl : ~0u;
@@ -1140,6 +1184,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                 if (head != -1) {
                     if (head >= 0) {
                         c = coeffs[(size_t)head * B + b];
+                        if (s_hxy[e]) c = c * fxy;
                     } else {
                         const int hi = -head - 2;
                         c = coeffs[(size_t)hi * B + b] * cosb[b];

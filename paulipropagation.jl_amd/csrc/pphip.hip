@@ -443,8 +443,9 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode >= 1 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
-    launch(h, E(0, all), E(1, all), k_branch<W>, lb_grid(h, (const void*)k_branch<W>, 0, h->n_est, K1_TILE), 0,
-           h->bufs, h->d_ctl, lb_launch(h, glog), mask);
+    launch(h, E(0, all), E(1, all), k_branch<W, false>,
+           lb_grid(h, (const void*)k_branch<W, false>, 0, h->n_est, K1_TILE), 0, h->bufs, h->d_ctl,
+           lb_launch(h, glog), mask);
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
@@ -482,7 +483,61 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0)
+        switch (TB) {
+        case 256: LAUNCH_MB(256); break;
+        case 128: LAUNCH_MB(128); break;
+        case 64: LAUNCH_MB(64); break;
+        case 32: LAUNCH_MB(32); break;
+        default: LAUNCH_MB(16); break;
+        }
+#undef LAUNCH_MB
+    }
+    h->seq += 1;
+}
+
+// Amplitude damping (gamma) on `qubit`: the rotation's kernels with the
+// damping rule (src/Propagation/specializations.jl:258-276); d_trig holds
+// 1 - gamma_b then gamma_b for B > 1 (all columns share fxy = sqrt(1 - gamma)).
+template <int W>
+void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double* d_trig, int max_weight,
+                 double min_abs, uint32_t* glog, hipEvent_t* ev) {
+    uint64_t m[4] = {0, 0, 0, 0};
+    m[qubit >> 5] = 3ull << (2 * (qubit & 31));
+    const Key<W> mask = to_key<W>(m);
+    Levels lv;
+    mask_levels(m, W, &lv);
+    const bool all = h->prof_mode == 2 && ev;
+    const bool mrg = h->prof_mode >= 1 && ev;
+    auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
+    launch(h, E(0, all), E(1, all), k_branch<W, true>,
+           lb_grid(h, (const void*)k_branch<W, true>, 0, h->n_est, K1_TILE), 0, h->bufs, h->d_ctl,
+           lb_launch(h, glog), mask);
+    Launch L2{};
+    L2.seq = h->seq;
+    L2.glog = glog;
+    const bool part = h->B == 1;
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->n_est / 2, K2_TILE), 0, h->bufs,
+           h->d_ctl, L2, lv, mask);
+    const size_t expect = h->n_est + h->n_est / 2;
+    if (part) {
+        RotParams rp{1 - gamma, gamma, max_weight, min_abs, qubit, fxy};
+        constexpr int TILE = K3Cfg<W>::TILE;
+        Launch LP{};
+        LP.seq = h->seq;
+        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
+               h->d_ctl, LP);
+        launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
+               lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+               lb_launch(h, glog), mask, rp, mask);
+    } else {
+        const int B = h->B;
+        const int TB = tile_rows(B);
+        const size_t shm = (size_t)TB * B * sizeof(double);
+        const Launch L3 = lb_launch(h, glog);
+#define LAUNCH_MB(TBV)                                                                                         \
+    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -512,11 +567,11 @@ void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const do
 
 // A gate of a run, resolved on the host: what enq_* needs.
 struct HostGate {
-    int kind;              // PP_GATE_ROTATION or PP_GATE_PAULI_NOISE
+    int kind;              // PP_GATE_ROTATION, PP_GATE_PAULI_NOISE or PP_GATE_AMP_DAMPING
     const uint64_t* mask;
     int qubit;
     uint32_t damp_codes;
-    double a, b;           // B = 1: cos, sin (rotation) or factor (noise)
+    double a, b;           // B = 1: cos, sin (rotation), factor (noise); gamma, sqrt(1 - gamma) (damping)
 };
 
 int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs,
@@ -524,7 +579,13 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
     uint32_t* glog = h->d_log + LOG_WORDS * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
-    if (g.kind == PP_GATE_ROTATION) {
+    if (g.kind == PP_GATE_AMP_DAMPING) {
+        switch (h->W) {
+        case 1: enq_damping<1>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        case 2: enq_damping<2>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        default: enq_damping<4>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        }
+    } else if (g.kind == PP_GATE_ROTATION) {
         switch (h->W) {
         case 1: enq_rotation<1>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev, next_mask); break;
         case 2: enq_rotation<2>(h, g.mask, g.a, g.b, dt, max_weight, min_abs, glog, ev, next_mask); break;
@@ -644,9 +705,14 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
             if (ev && hipEventElapsedTime(&ms, ev[a], ev[a + 1]) == hipSuccess) return ms;
             return 0.0;
         };
-        if (gs[i].kind == PP_GATE_ROTATION) {
+        if (gs[i].kind == PP_GATE_ROTATION || gs[i].kind == PP_GATE_AMP_DAMPING) {
+            uint64_t m[4] = {0, 0, 0, 0};
+            if (gs[i].kind == PP_GATE_ROTATION)
+                for (int l = 0; l < h->W; ++l) m[l] = gs[i].mask[l];
+            else
+                m[gs[i].qubit >> 5] = 3ull << (2 * (gs[i].qubit & 31));
             int mask_limbs = 0;
-            for (int l = 0; l < h->W; ++l) mask_limbs += gs[i].mask[l] != 0;
+            for (int l = 0; l < h->W; ++l) mask_limbs += m[l] != 0;
             // every rotation launches all its kernels (those of a gate that touches
             // nothing exit at once), so launches count dispatches as a profiler does
             h->launches[PP_KERNEL_BRANCH] += 1;
@@ -658,7 +724,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
             if (h->prof_mode >= 1) h->ms[PP_KERNEL_MERGE] += span(4);
             if (na > 0) {
                 Levels lv;
-                mask_levels(gs[i].mask, h->W, &lv);
+                mask_levels(m, h->W, &lv);
                 h->alg_bytes[PP_KERNEL_TAIL_PERMUTE] += na * (8.0 + 8.0 * (h->W - lv.from_limb));
                 h->alg_bytes[PP_KERNEL_MERGE] += row * ((double)nin + (double)nout);
             }
@@ -777,6 +843,23 @@ int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const doubl
                 if (atrig) (*atrig)[g * 2 * B + b] = 1 - lam;
             }
             hg.a = 1 - val(0);
+        } else if (gt.kind == PP_GATE_AMP_DAMPING) {
+            if (gt.qubit < 0 || gt.qubit >= h->nqubits) return fail(PP_ERR_ARG, "damping qubit out of range");
+            const double g0 = val(0);
+            for (int b = 0; b < B; ++b) {
+                const double gam = val(b);
+                if (!(gam >= 0.0 && gam <= 1.0)) return fail(PP_ERR_ARG, "noise strength must be in [0, 1]");
+                if (gam != g0)
+                    return fail(PP_ERR_UNSUPPORTED, "amplitude damping with a different gamma per batch column");
+                if (B > 1) {
+                    h->h_trig[g * 2 * B + b] = 1 - gam;
+                    h->h_trig[g * 2 * B + B + b] = gam;
+                }
+            }
+            hg.qubit = gt.qubit;
+            hg.a = g0;
+            hg.b = std::sqrt(1 - g0);
+            if (atrig) return fail(PP_ERR_UNSUPPORTED, "the adjoint sweep does not take amplitude damping yet");
         } else {
             return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
         }
@@ -1177,6 +1260,24 @@ int pp_apply_pauli_noise(pp_handle* h, int qubit, uint32_t damp_codes, const dou
     if (damp_codes & ~0xeu) return fail(PP_ERR_ARG, "damp_codes may only hold X, Y, Z (bits 1..3)");
     HostGate g{PP_GATE_PAULI_NOISE, nullptr, qubit, damp_codes, factor_b[0], 0.0};
     return single_gate(h, g, factor_b, nullptr, max_weight, min_abs_coeff, stats);
+}
+
+int pp_apply_amplitude_damping(pp_handle* h, int qubit, const double* gamma_b, int max_weight, double min_abs_coeff,
+                               pp_gate_stats* stats) {
+    if (!h || !gamma_b) return fail(PP_ERR_ARG, "NULL argument");
+    if (qubit < 0 || qubit >= h->nqubits) return fail(PP_ERR_ARG, "damping qubit out of range");
+    for (int b = 0; b < h->B; ++b) {
+        if (!(gamma_b[b] >= 0.0 && gamma_b[b] <= 1.0)) return fail(PP_ERR_ARG, "noise strength must be in [0, 1]");
+        if (gamma_b[b] != gamma_b[0])
+            return fail(PP_ERR_UNSUPPORTED, "amplitude damping with a different gamma per batch column");
+    }
+    std::vector<double> keep(h->B), created(h->B);
+    for (int b = 0; b < h->B; ++b) {
+        keep[b] = 1 - gamma_b[b];
+        created[b] = gamma_b[b];
+    }
+    HostGate g{PP_GATE_AMP_DAMPING, nullptr, qubit, 0, gamma_b[0], std::sqrt(1 - gamma_b[0])};
+    return single_gate(h, g, keep.data(), created.data(), max_weight, min_abs_coeff, stats);
 }
 
 int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,

@@ -127,6 +127,15 @@ class HIPPauliSum:
                                                    ctypes.byref(st)))
         return st
 
+    def apply_amplitude_damping(self, qubit: int, gamma, max_weight=math.inf, min_abs_coeff=1e-10):
+        """AmplitudeDampingNoise(qubit) with strength gamma (1-based qubit)."""
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(gamma, dtype=np.float64), (self.batch,)))
+        st = _lib.pp_gate_stats()
+        _lib.check(_lib.lib().pp_apply_amplitude_damping(self._h, int(qubit) - 1, _dp(g),
+                                                         _max_weight_arg(max_weight), float(min_abs_coeff),
+                                                         ctypes.byref(st)))
+        return st
+
     def overlapwithzero(self):
         out = np.zeros(self.batch, dtype=np.float64)
         _lib.check(_lib.lib().pp_overlap_zero(self._h, _dp(out)))

@@ -145,3 +145,24 @@ def test_listed_touched_terms_match_own_pass(mode):
     assert_bitwise(*results[0], *results[1], f"chained vs own pass ({mode})")
     ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=8, min_abs_coeff=1e-6)
     assert_bitwise(*results[0], *ref.get(), f"chained vs oracle ({mode})")
+
+
+@pytest.mark.parametrize("B", [1, 3])
+def test_amplitude_damping_circuit(B):
+    print(f"amplitude damping circuit B={B}", flush=True)
+    nq = 14
+    circ = []
+    for _ in range(3):
+        pp.rxlayer(circ, nq)
+        pp.rzzlayer(circ, pp.bricklayertopology(nq))
+        for q in range(1, nq + 1, 3):
+            circ.append(pp.FrozenGate(pp.AmplitudeDampingNoise(q), 0.05))
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(12).uniform(-1, 1, size=(P, B))
+    obs = pp.PauliSum(nq)
+    for q in (3, 8):
+        obs.add(["Z"], [q])
+    dev = pp.propagate(circ, obs, th if B > 1 else th[:, 0], max_weight=6, min_abs_coeff=1e-7)
+    ref, _ = oracle_propagate(circ, nq, th if B > 1 else th[:, 0], obs.terms, max_weight=6, min_abs_coeff=1e-7,
+                              batch=B)
+    assert_bitwise(*dev.download(), *ref.get(), f"amplitude damping circuit B={B}")

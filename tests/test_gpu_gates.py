@@ -128,3 +128,40 @@ def test_tile_claim_modes_are_bit_identical(mode):
     assert_bitwise(*dev.download(), *ref.get(), f"{mode} noise")
     if mode == "tiny_spin_limit":
         print("fallbacks", dev.profile()["fallbacks"], flush=True)
+
+
+@pytest.mark.parametrize("nq,qubit", [(32, 1), (32, 17), (64, 33), (127, 64), (127, 127)])
+@pytest.mark.parametrize("n", [1, 5, 3000, 120000])
+@pytest.mark.parametrize("min_abs", [0.0, 1e-2])
+def test_amplitude_damping(nq, qubit, n, min_abs):
+    # Z branches into I (gamma c) and keeps (1 - gamma) c; X, Y scale by sqrt(1 - gamma)
+    # (src/Propagation/specializations.jl:258-276)
+    print(f"damping nq={nq} qubit={qubit} n={n} min_abs={min_abs}", flush=True)
+    dev, ref = _pair(nq, n, seed=3 * n + qubit)
+    for gamma in (0.1, 0.37):
+        dev.apply_amplitude_damping(qubit, gamma, max_weight=10, min_abs_coeff=min_abs)
+        ref.amplitude_damping(qubit - 1, gamma, max_weight=10, min_abs=min_abs)
+        assert_bitwise(*dev.download(), *ref.get(), f"gamma {gamma}")
+
+
+def test_amplitude_damping_without_z_and_without_support():
+    # only X/Y on the qubit: coefficients change and everything is truncated, nothing
+    # is created; no support at all: the sum is left as it is (no truncation)
+    nq = 40
+    keys, c = terms_to_arrays([1 << 2 * 4, 2 << 2 * 4, (1 << 2 * 4) | (3 << 2 * 7)], nq, [0.5, 1e-3, 0.2])
+    for qubit in (5, 20):
+        dev = HIPPauliSum(nq)
+        dev.upload(keys, c)
+        ref = OracleSum(key_words(nq), 1).set(keys, c)
+        dev.apply_amplitude_damping(qubit, 0.5, min_abs_coeff=1e-2)
+        ref.amplitude_damping(qubit - 1, 0.5, min_abs=1e-2)
+        assert_bitwise(*dev.download(), *ref.get(), f"qubit {qubit}")
+    assert len(dev.download()[0]) == 3  # qubit 20 carries no Pauli: untouched, nothing truncated
+
+
+def test_amplitude_damping_arguments():
+    dev = HIPPauliSum(40)
+    with pytest.raises(ValueError):
+        dev.apply_amplitude_damping(41, 0.1)
+    with pytest.raises(ValueError):
+        dev.apply_amplitude_damping(3, 1.5)
