@@ -125,6 +125,16 @@ int pp_apply_pauli_noise(pp_handle *h, int qubit, uint32_t damp_codes, const dou
 int pp_apply_amplitude_damping(pp_handle *h, int qubit, const double *gamma_b, int max_weight,
                                double min_abs_coeff, pp_gate_stats *stats);
 
+/* CliffordGate step on nq = 1 or 2 qubits: a term whose Pauli codes there form
+ * idx = code(qubits[0]) | code(qubits[1]) << 2 becomes term ^ changes[idx]
+ * (16 entries of W limbs) with its coefficient times signs[idx]; terms are
+ * truncated only when max_weight >= 0 (then by weight and min_abs_coeff); the
+ * sum is sorted again on the device.
+ * Replaces: applymergetruncate!(::CliffordGate, cache; ...)
+ *           (src/Propagation/specializations.jl:144-192, cliffordgates.jl:176-182). */
+int pp_apply_clifford(pp_handle *h, int nq, const int *qubits, const uint64_t *changes, const double *signs,
+                      int max_weight, double min_abs_coeff, pp_gate_stats *stats);
+
 /* Run a Heisenberg-ordered gate list (the reference's _propagate! loop over
  * applymergetruncate!, src/Base/propagate.jl:44-71). theta is nparams x B
  * row-major. Clifford tables are nluts slots of 16 entries: lut_changes

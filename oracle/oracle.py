@@ -114,6 +114,21 @@ class OracleSum:
         lib().ppo_get(self._s, u64p(keys), dp(coeffs))
         return keys[:n], coeffs[:n]
 
+    def get_sorted(self):
+        """get(), rows in ascending key order (a Clifford leaves the oracle's
+        sum unsorted until the next branching gate, as the reference does)."""
+        keys, coeffs = self.get()
+        if len(keys) > 1:
+            order = np.lexsort(tuple(keys[:, l] for l in range(keys.shape[1])))
+            keys, coeffs = keys[order], coeffs[order]
+        return keys, coeffs
+
+    def clifford(self, qubits0, changes, signs, max_weight=math.inf, min_abs=0.0):
+        q = (ctypes.c_int * 2)(*(list(qubits0) + [0] * (2 - len(qubits0))))
+        c = np.ascontiguousarray(changes, dtype=np.uint64)
+        sg = np.ascontiguousarray(signs, dtype=np.float64)
+        lib().ppo_clifford(self._s, len(qubits0), q, u64p(c), dp(sg), _mw(max_weight), float(min_abs))
+
     def rotation(self, mask_limbs, cosv, sinv, max_weight=math.inf, min_abs=0.0):
         m = np.ascontiguousarray(mask_limbs, dtype=np.uint64)
         c = np.ascontiguousarray(np.broadcast_to(cosv, (self.B,)), dtype=np.float64)

@@ -28,6 +28,7 @@
 
 #include "../../include/pphip.h"
 #include "pp_kernels.h"
+#include "pp_sort.h"
 
 using namespace pp;
 
@@ -84,6 +85,14 @@ struct pp_handle {
     uint32_t* d_rec = nullptr;
     double* d_rcoef = nullptr;
     size_t tpj_cap = 0, rec_words = 0, rec_rows = 0;
+    // Clifford: radix-sort buffers (sized for cap) and the gate tables of a run
+    uint64_t* d_rs_k[2] = {nullptr, nullptr};
+    uint32_t* d_rs_v[2] = {nullptr, nullptr};
+    uint32_t* d_rs_hist = nullptr;
+    size_t rs_cap = 0, rs_tiles = 0;
+    uint64_t* d_lut_c = nullptr;
+    double* d_lut_s = nullptr;
+    size_t lut_cap = 0;            // tables
     // overlap
     double* d_partial = nullptr;
     size_t partial_cap = 0;
@@ -567,19 +576,117 @@ void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const do
 
 // A gate of a run, resolved on the host: what enq_* needs.
 struct HostGate {
-    int kind;              // PP_GATE_ROTATION, PP_GATE_PAULI_NOISE or PP_GATE_AMP_DAMPING
+    int kind;              // enum pp_gate_kind
     const uint64_t* mask;
     int qubit;
     uint32_t damp_codes;
     double a, b;           // B = 1: cos, sin (rotation), factor (noise); gamma, sqrt(1 - gamma) (damping)
+    int nq = 0, q1 = 0, lut = 0;  // Clifford: qubits (qubit, q1) and table slot
 };
+
+// Radix-sort buffers for `cap` terms.
+int ensure_sort(pp_handle* h) {
+    if (h->rs_cap >= h->cap && h->d_rs_hist) return PP_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int b = 0; b < 2; ++b) {
+        if (h->d_rs_k[b]) hipFree(h->d_rs_k[b]);
+        if (h->d_rs_v[b]) hipFree(h->d_rs_v[b]);
+        h->d_rs_k[b] = nullptr;
+        h->d_rs_v[b] = nullptr;
+    }
+    if (h->d_rs_hist) hipFree(h->d_rs_hist);
+    h->d_rs_hist = nullptr;
+    h->rs_cap = 0;
+    const size_t cap = std::max<size_t>(h->cap, 1);
+    const size_t tiles = cap / RS_TILE + 2;
+    for (int b = 0; b < 2; ++b) {
+        HIP_TRY(hipMalloc(&h->d_rs_k[b], cap * sizeof(uint64_t)));
+        HIP_TRY(hipMalloc(&h->d_rs_v[b], cap * sizeof(uint32_t)));
+    }
+    HIP_TRY(hipMalloc(&h->d_rs_hist, RS_BINS * tiles * sizeof(uint32_t)));
+    h->rs_cap = cap;
+    h->rs_tiles = tiles;
+    return PP_OK;
+}
+
+// One stable LSD sort of (rs_k[c], rs_v[c]) over key bits [0, nbits); returns
+// the buffer index holding the result. n is read on the device at n_ptr.
+int enq_radix_sort(pp_handle* h, const uint32_t* n_ptr, int c, int nbits) {
+    const unsigned grid = grid_for(h, h->n_est, RS_TILE);
+    for (int shift = 0; shift < nbits; shift += 8) {
+        const int bits = std::min(8, nbits - shift);
+        hipLaunchKernelGGL(k_rs_upsweep, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_k[c], shift,
+                           bits, h->d_rs_hist, (uint32_t)h->rs_tiles);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_hist,
+                           (uint32_t)h->rs_tiles);
+        hipLaunchKernelGGL(k_rs_downsweep, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_k[c],
+                           h->d_rs_v[c], h->d_rs_k[c ^ 1], h->d_rs_v[c ^ 1], shift, bits, h->d_rs_hist,
+                           (uint32_t)h->rs_tiles);
+        c ^= 1;
+    }
+    return c;
+}
+
+// Clifford on up to two qubits with table slot `lut` (uploaded to d_lut_*):
+// map + truncate + compact, then re-sort inside segments (see pp_sort.h).
+template <int W>
+void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_abs, uint32_t* glog, hipEvent_t* ev) {
+    const bool mrg = h->prof_mode >= 1 && ev;
+    const int q0 = g.qubit, q1 = g.nq == 2 ? g.q1 : g.qubit;
+    const int qmin = std::min(q0, q1), qmax = std::max(q0, q1);
+    CliffArgs a{};
+    a.change = h->d_lut_c + (size_t)g.lut * 16 * W;
+    a.sign = h->d_lut_s + (size_t)g.lut * 16;
+    a.nq = g.nq;
+    a.q0 = q0;
+    a.q1 = q1;
+    a.max_weight = max_weight;
+    a.min_abs = min_abs;
+    a.wlo = 2 * qmin;
+    a.wbits = 2 * (qmax - qmin) + 2;
+    a.composite = a.wbits <= 32 ? 1 : 0;
+    const int p = h->seq & 1;
+    const uint32_t* n_ptr = &h->d_ctl->st[p ^ 1].n;
+    launch(h, mrg ? ev[4] : nullptr, mrg ? ev[5] : nullptr, k_clifford_map<W>,
+           lb_grid(h, (const void*)k_clifford_map<W>, 0, h->n_est, KC_TILE), 0, h->bufs, h->d_ctl,
+           lb_launch(h, glog), a, h->B, h->d_rs_k[0], h->d_rs_v[0]);
+    int c = 0;
+    if (a.composite) {
+        int segbits = 1;
+        while (segbits < 31 && (1ull << segbits) <= h->cap) ++segbits;
+        c = enq_radix_sort(h, n_ptr, 0, a.wbits + segbits);
+    } else {
+        // wide window: the new key's limbs from the window's lowest, least significant first
+        const unsigned grid = grid_for(h, h->n_est, WG * 4);
+        bool first = true;
+        for (int l = a.wlo / 64; l < W; ++l) {
+            const int lo = l == a.wlo / 64 ? a.wlo % 64 : 0;
+            hipLaunchKernelGGL(k_rs_limb_keys<W>, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, p, h->bufs, l, lo,
+                               first ? nullptr : h->d_rs_v[c], h->d_rs_k[c ^ 1], h->d_rs_v[c ^ 1]);
+            c ^= 1;
+            first = false;
+            c = enq_radix_sort(h, n_ptr, c, 64 - lo);
+        }
+    }
+    hipLaunchKernelGGL(k_clifford_gather<W>, dim3(grid_for(h, h->n_est, WG * 4)), dim3(WG), 0, h->stream, h->bufs,
+                       h->d_ctl, p, h->d_rs_v[c], h->B);
+    h->seq += 1;
+}
+
 
 int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs,
             const uint64_t* next_mask) {
     uint32_t* glog = h->d_log + LOG_WORDS * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
-    if (g.kind == PP_GATE_AMP_DAMPING) {
+    if (g.kind == PP_GATE_CLIFFORD) {
+        if (int rc = ensure_sort(h)) return rc;
+        switch (h->W) {
+        case 1: enq_clifford<1>(h, g, max_weight, min_abs, glog, ev); break;
+        case 2: enq_clifford<2>(h, g, max_weight, min_abs, glog, ev); break;
+        default: enq_clifford<4>(h, g, max_weight, min_abs, glog, ev); break;
+        }
+    } else if (g.kind == PP_GATE_AMP_DAMPING) {
         switch (h->W) {
         case 1: enq_damping<1>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
         case 2: enq_damping<2>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
@@ -781,7 +888,7 @@ int key_cmp(const uint64_t* a, const uint64_t* b, int W) {
 int single_gate(pp_handle* h, const HostGate& g, const double* hv1, const double* hv2, int max_weight,
                 double min_abs, pp_gate_stats* st) {
     if (int rc = ensure_run(h, 1)) return rc;
-    if (h->B > 1) {
+    if (h->B > 1 && hv1) {  // per-column values (a Clifford has none)
         std::memcpy(h->h_trig, hv1, h->B * sizeof(double));
         if (hv2) std::memcpy(h->h_trig + h->B, hv2, h->B * sizeof(double));
         HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, 2 * h->B * sizeof(double), hipMemcpyHostToDevice, h->stream));
@@ -800,8 +907,10 @@ int single_gate(pp_handle* h, const HostGate& g, const double* hv1, const double
 // d_trig for B > 1. With atrig, every gate's B cos then B sin (rotation) or B
 // factors (noise) also go to atrig (2B doubles per gate).
 int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
-                  std::vector<HostGate>& gs, std::vector<double>* atrig) {
+                  const uint64_t* lut_changes, const double* lut_signs, size_t nluts, std::vector<HostGate>& gs,
+                  std::vector<double>* atrig) {
     if (ngates > 0 && !gates) return fail(PP_ERR_ARG, "gates is NULL");
+    if (nluts > 0 && (!lut_changes || !lut_signs)) return fail(PP_ERR_ARG, "Clifford tables are NULL");
     const int B = h->B;
     gs.assign(ngates, HostGate{});
     if (atrig) atrig->assign(ngates * 2 * (size_t)B, 0.0);
@@ -860,6 +969,18 @@ int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const doubl
             hg.a = g0;
             hg.b = std::sqrt(1 - g0);
             if (atrig) return fail(PP_ERR_UNSUPPORTED, "the adjoint sweep does not take amplitude damping yet");
+        } else if (gt.kind == PP_GATE_CLIFFORD) {
+            if (gt.nq < 1 || gt.nq > 2) return fail(PP_ERR_ARG, "a Clifford gate acts on 1 or 2 qubits");
+            for (int a = 0; a < gt.nq; ++a)
+                if (gt.qubits[a] < 0 || gt.qubits[a] >= h->nqubits)
+                    return fail(PP_ERR_ARG, "Clifford qubit out of range");
+            if (gt.nq == 2 && gt.qubits[0] == gt.qubits[1]) return fail(PP_ERR_ARG, "Clifford qubits must differ");
+            if (gt.lut < 0 || (size_t)gt.lut >= nluts) return fail(PP_ERR_ARG, "Clifford table slot out of range");
+            hg.qubit = gt.qubits[0];
+            hg.q1 = gt.nq == 2 ? gt.qubits[1] : gt.qubits[0];
+            hg.nq = gt.nq;
+            hg.lut = gt.lut;
+            if (atrig) return fail(PP_ERR_UNSUPPORTED, "the adjoint sweep does not take Clifford gates yet");
         } else {
             return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
         }
@@ -867,6 +988,25 @@ int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const doubl
     if (B > 1 && ngates > 0)
         HIP_TRY(hipMemcpyAsync(h->d_trig, h->h_trig, ngates * 2 * B * sizeof(double), hipMemcpyHostToDevice,
                                h->stream));
+    if (nluts > 0) {
+        if (nluts > h->lut_cap) {
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            if (h->d_lut_c) hipFree(h->d_lut_c);
+            if (h->d_lut_s) hipFree(h->d_lut_s);
+            h->d_lut_c = nullptr;
+            h->d_lut_s = nullptr;
+            h->lut_cap = 0;
+            HIP_TRY(hipMalloc(&h->d_lut_c, nluts * 16 * h->W * sizeof(uint64_t)));
+            HIP_TRY(hipMalloc(&h->d_lut_s, nluts * 16 * sizeof(double)));
+            h->lut_cap = nluts;
+        }
+        // the tables of a run go to the device before any of its gates
+        HIP_TRY(hipMemcpyAsync(h->d_lut_c, lut_changes, nluts * 16 * h->W * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->d_lut_s, lut_signs, nluts * 16 * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
     return PP_OK;
 }
 
@@ -956,11 +1096,13 @@ int seed_adjoint(pp_handle* h, double* a) {
 // run that records every gate (per segment of checkpoint_gates gates, from a
 // snapshot of the segment's input sum) and a reverse sweep over the records.
 int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
-                 int max_weight, double min_abs, size_t checkpoint_gates, double* loss, double* grad) {
+                 const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight, double min_abs,
+                 size_t checkpoint_gates, double* loss, double* grad) {
     const int B = h->B;
     std::vector<HostGate> gs;
     std::vector<double> atrig;
-    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, gs, &atrig)) return rc;
+    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, gs, &atrig))
+        return rc;
     const size_t seg = checkpoint_gates ? std::min(checkpoint_gates, std::max<size_t>(ngates, 1))
                                         : std::max<size_t>(ngates, 1);
     const size_t nseg = ngates ? (ngates + seg - 1) / seg : 1;
@@ -1163,6 +1305,16 @@ int pp_destroy(pp_handle* h) {
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
     if (h->d_out) hipFree(h->d_out);
+    if (h->d_tpj) hipFree(h->d_tpj);
+    if (h->d_rec) hipFree(h->d_rec);
+    if (h->d_rcoef) hipFree(h->d_rcoef);
+    for (int b = 0; b < 2; ++b) {
+        if (h->d_rs_k[b]) hipFree(h->d_rs_k[b]);
+        if (h->d_rs_v[b]) hipFree(h->d_rs_v[b]);
+    }
+    if (h->d_rs_hist) hipFree(h->d_rs_hist);
+    if (h->d_lut_c) hipFree(h->d_lut_c);
+    if (h->d_lut_s) hipFree(h->d_lut_s);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
     return PP_OK;
@@ -1280,15 +1432,27 @@ int pp_apply_amplitude_damping(pp_handle* h, int qubit, const double* gamma_b, i
     return single_gate(h, g, keep.data(), created.data(), max_weight, min_abs_coeff, stats);
 }
 
+int pp_apply_clifford(pp_handle* h, int nq, const int* qubits, const uint64_t* changes, const double* signs,
+                      int max_weight, double min_abs_coeff, pp_gate_stats* stats) {
+    if (!h || !qubits || !changes || !signs) return fail(PP_ERR_ARG, "NULL argument");
+    pp_gate gt{};
+    gt.kind = PP_GATE_CLIFFORD;
+    gt.param = -1;
+    gt.nq = nq;
+    for (int a = 0; a < nq && a < 2; ++a) gt.qubits[a] = qubits[a];
+    gt.lut = 0;
+    std::vector<HostGate> gs;
+    if (int rc = resolve_gates(h, &gt, 1, nullptr, 0, changes, signs, 1, gs, nullptr)) return rc;
+    return single_gate(h, gs[0], nullptr, nullptr, max_weight, min_abs_coeff, stats);
+}
+
 int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
                    const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight,
                    double min_abs_coeff, int64_t* counts) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
-    (void)lut_changes;
-    (void)lut_signs;
-    (void)nluts;
     std::vector<HostGate> gs;
-    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, gs, nullptr)) return rc;
+    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, gs, nullptr))
+        return rc;
     return run_resolved(h, gs, 0, ngates, max_weight, min_abs_coeff, counts);
 }
 
@@ -1297,13 +1461,10 @@ int pp_adjoint(pp_handle* h, const pp_gate* gates, size_t ngates, const double* 
                double min_abs_coeff, size_t checkpoint_gates, double* loss, double* grad) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (!loss || (nparams > 0 && !grad)) return fail(PP_ERR_ARG, "loss or grad is NULL");
-    (void)lut_changes;
-    (void)lut_signs;
-    (void)nluts;
     const bool chain = h->chain_tails;
     h->chain_tails = false;
-    const int rc = adjoint_impl(h, gates, ngates, theta, nparams, max_weight, min_abs_coeff, checkpoint_gates, loss,
-                                grad);
+    const int rc = adjoint_impl(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
+                                min_abs_coeff, checkpoint_gates, loss, grad);
     h->chain_tails = chain;
     set_recording(h, false);
     return rc;

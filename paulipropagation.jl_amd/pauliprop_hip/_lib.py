@@ -58,6 +58,8 @@ SIGNATURES = [
                                             ctypes.POINTER(pp_gate_stats)]),
     ("pp_apply_amplitude_damping", ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_double,
                                                    ctypes.POINTER(pp_gate_stats)]),
+    ("pp_apply_clifford", ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _U64P, _DP, ctypes.c_int,
+                                          ctypes.c_double, ctypes.POINTER(pp_gate_stats)]),
     ("pp_run_circuit", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
                                       ctypes.c_double, _I64P]),
     ("pp_adjoint", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,

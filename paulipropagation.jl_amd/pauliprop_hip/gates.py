@@ -200,6 +200,29 @@ def _qubit0(nqubits: int, q: int) -> int:
     return q - 1
 
 
+def clifford_table(symbol: str, qubits0, W: int):
+    """The 16-entry table of a Clifford on 0-based qubits: per gathered index
+    idx = code(q0) | code(q1) << 2, the XOR change of the whole key (W limbs,
+    limb 0 first) and the sign (entries past 4^nq are identities)."""
+    table = CLIFFORD_MAP[str(symbol).upper()]
+    changes: list[int] = []
+    signs: list[float] = []
+    for idx in range(16):
+        if idx < len(table):
+            new, sign = table[idx]
+            change = 0
+            for a, q in enumerate(qubits0):
+                old_p = (idx >> (2 * a)) & 3
+                new_p = (new >> (2 * a)) & 3
+                change |= (old_p ^ new_p) << (2 * q)
+            changes.extend((change >> (64 * l)) & ((1 << 64) - 1) for l in range(W))
+            signs.append(float(sign))
+        else:
+            changes.extend([0] * W)
+            signs.append(1.0)
+    return changes, signs
+
+
 def pack_circuit(circuit, nqubits: int, W: int, param_offset=None):
     """Pack a Heisenberg-ordered gate list into pp_gate descriptors.
 
@@ -246,20 +269,9 @@ def pack_circuit(circuit, nqubits: int, W: int, param_offset=None):
             slot_key = (inner.symbol, tuple(qs))
             if slot_key not in lut_slots:
                 lut_slots[slot_key] = len(lut_slots)
-                table = CLIFFORD_MAP[inner.symbol]
-                for idx in range(16):
-                    if idx < len(table):
-                        new, sign = table[idx]
-                        change = 0
-                        for a, q in enumerate(qs):
-                            old_p = (idx >> (2 * a)) & 3
-                            new_p = (new >> (2 * a)) & 3
-                            change |= (old_p ^ new_p) << (2 * q)
-                        lut_changes.extend((change >> (64 * l)) & ((1 << 64) - 1) for l in range(W))
-                        lut_signs.append(float(sign))
-                    else:
-                        lut_changes.extend([0] * W)
-                        lut_signs.append(1.0)
+                ch, sg = clifford_table(inner.symbol, qs, W)
+                lut_changes.extend(ch)
+                lut_signs.extend(sg)
             d.lut = lut_slots[slot_key]
         else:
             raise NotImplementedError(f"gate {gate!r} is not supported by this backend")

@@ -136,6 +136,22 @@ class HIPPauliSum:
                                                          ctypes.byref(st)))
         return st
 
+    def apply_clifford(self, symbol: str, qinds, max_weight=math.inf, min_abs_coeff=1e-10):
+        """CliffordGate(symbol, qinds) (1-based qubits); the sum is sorted again on the device."""
+        from .gates import CliffordGate, clifford_table
+        g = CliffordGate(symbol, qinds)
+        qs = [q - 1 for q in g.qinds]
+        if any(not 0 <= q < self.nqubits for q in qs):
+            raise ValueError(f"Clifford qubits {g.qinds} out of range 1..{self.nqubits}")
+        ch, sg = clifford_table(g.symbol, qs, self.W)
+        c = np.ascontiguousarray(ch, dtype=np.uint64)
+        s = np.ascontiguousarray(sg, dtype=np.float64)
+        q = (ctypes.c_int * 2)(*(qs + [0] * (2 - len(qs))))
+        st = _lib.pp_gate_stats()
+        _lib.check(_lib.lib().pp_apply_clifford(self._h, len(qs), q, _u64p(c), _dp(s), _max_weight_arg(max_weight),
+                                                float(min_abs_coeff), ctypes.byref(st)))
+        return st
+
     def overlapwithzero(self):
         out = np.zeros(self.batch, dtype=np.float64)
         _lib.check(_lib.lib().pp_overlap_zero(self._h, _dp(out)))

@@ -166,3 +166,30 @@ def test_amplitude_damping_circuit(B):
     ref, _ = oracle_propagate(circ, nq, th if B > 1 else th[:, 0], obs.terms, max_weight=6, min_abs_coeff=1e-7,
                               batch=B)
     assert_bitwise(*dev.download(), *ref.get(), f"amplitude damping circuit B={B}")
+
+
+@pytest.mark.parametrize("B", [1, 2])
+def test_clifford_circuit(B):
+    # rotations, CNOT/H/S layers and noise in one gate list (pp_run_circuit with tables)
+    print(f"Clifford circuit B={B}", flush=True)
+    nq = 20
+    circ = []
+    for layer in range(3):
+        pp.rylayer(circ, nq)
+        for q in range(1, nq, 2):
+            circ.append(pp.CliffordGate("CNOT", [q, q + 1]))
+        circ.append(pp.CliffordGate("H", [layer + 1]))
+        circ.append(pp.CliffordGate("S", [nq - layer]))
+        pp.rzlayer(circ, nq)
+        pp.depolarizinglayer(circ, nq, 0.01)
+    circ.append(pp.CliffordGate("CZ", [2, 19]))  # the last gate leaves the oracle unsorted
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(31).uniform(-1, 1, size=(P, B))
+    obs = pp.PauliSum(nq)
+    for q in (4, 11, 17):
+        obs.add(["Z"], [q])
+    for mw in (math.inf, 6):
+        dev = pp.propagate(circ, obs, th if B > 1 else th[:, 0], max_weight=mw, min_abs_coeff=1e-6)
+        ref, _ = oracle_propagate(circ, nq, th if B > 1 else th[:, 0], obs.terms, max_weight=mw,
+                                  min_abs_coeff=1e-6, batch=B)
+        assert_bitwise(*dev.download(), *ref.get_sorted(), f"Clifford circuit B={B} max_weight={mw}")

@@ -165,3 +165,37 @@ def test_amplitude_damping_arguments():
         dev.apply_amplitude_damping(41, 0.1)
     with pytest.raises(ValueError):
         dev.apply_amplitude_damping(3, 1.5)
+
+
+@pytest.mark.parametrize("nq,symbol,qinds", [
+    (32, "H", [1]), (32, "S", [17]), (32, "SX", [32]), (64, "H", [33]), (127, "SY", [127]),
+    (32, "CNOT", [3, 4]), (32, "CZ", [20, 9]), (64, "SWAP", [31, 34]), (127, "CNOT", [64, 63]),
+    (64, "CNOT", [1, 40]), (127, "ZZPIHALF", [2, 120]),  # windows wider than 32 bits: limb rounds
+])
+@pytest.mark.parametrize("n", [1, 5, 3000, 120000])
+@pytest.mark.parametrize("max_weight", [math.inf, 9])
+def test_clifford(nq, symbol, qinds, n, max_weight):
+    # map every term, truncate only under a weight limit, and sort again on the device;
+    # the oracle leaves its sum unsorted (src/Base/Primitives/map.jl:63), so it is compared sorted
+    from pauliprop_hip.gates import clifford_table
+    print(f"clifford {symbol}{qinds} nq={nq} n={n} max_weight={max_weight}", flush=True)
+    dev, ref = _pair(nq, n, seed=n + 7 * nq)
+    ch, sg = clifford_table(symbol, [q - 1 for q in qinds], key_words(nq))
+    for rep in range(2):
+        dev.apply_clifford(symbol, qinds, max_weight=max_weight, min_abs_coeff=1e-3)
+        ref.clifford([q - 1 for q in qinds], ch, sg, max_weight=max_weight, min_abs=1e-3)
+        assert_bitwise(*dev.download(), *ref.get_sorted(), f"{symbol} application {rep}")
+
+
+def test_clifford_batched_and_arguments():
+    from pauliprop_hip.gates import clifford_table
+    nq = 64
+    dev, ref = _pair(nq, 50000, seed=5, batch=4)
+    ch, sg = clifford_table("CNOT", [9, 10], key_words(nq))
+    dev.apply_clifford("CNOT", [10, 11])
+    ref.clifford([9, 10], ch, sg)
+    assert_bitwise(*dev.download(), *ref.get_sorted(), "batched CNOT")
+    with pytest.raises(ValueError):
+        dev.apply_clifford("CNOT", [3, 3])
+    with pytest.raises(ValueError):
+        dev.apply_clifford("H", [65])
