@@ -95,6 +95,15 @@ int pp_info(pp_handle *h, int *nqubits, int *key_words, int *batch, size_t *n, s
  *           (src/PauliDataTypes/vectorpaulisum.jl:21-49). */
 int pp_upload(pp_handle *h, const uint64_t *keys, const double *coeffs, size_t n);
 
+/* Replace the sum with n terms in any order, equal strings allowed: sorted on
+ * the device (stable LSD radix sort, limb by limb), every run of equal strings
+ * summed in input order, then truncated.
+ * Replaces: mergeandtruncate!(cache) on an unsorted vector, sortterms! +
+ *           _deduplicateandtruncatecpu! (src/Base/Merge/mergeandtruncate.jl:87-120,
+ *           src/Base/Merge/deduplicate.jl:14-104). */
+int pp_load_merge(pp_handle *h, const uint64_t *keys, const double *coeffs, size_t n, int max_weight,
+                  double min_abs_coeff);
+
 /* Copy the live terms, ascending by key, into caller memory with room for
  * `cap` terms; *n receives the term count (also when cap is too small, in
  * which case nothing is copied and PP_ERR_ARG is returned).

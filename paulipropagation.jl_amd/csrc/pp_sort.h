@@ -127,16 +127,17 @@ __global__ __launch_bounds__(WG) void k_rs_downsweep(const Ctl* ctl, const uint3
     }
 }
 
-// Values 0 .. n-1 and, from the sum's limb `limb` of the term at perm[i] (or
-// of term i when perm is null), bits [lo, 64) as the key: one round of the
-// wide-window sort.
-template <int W>
-__global__ __launch_bounds__(WG) void k_rs_limb_keys(const Ctl* ctl, int p, const Bufs bf, int limb, int lo,
+// One round of a limb-by-limb sort: key = bits [lo, 64) of limb `limb` of term
+// perm[i] (term i when perm is null), value = that term's index. The terms are
+// the Clifford's mapped sum (the other buffer of the gate's input, n terms) or,
+// with `plain`, buffer `plain_buf` with n at n_ptr.
+__global__ __launch_bounds__(WG) void k_rs_limb_keys(const Ctl* ctl, int p, const Bufs bf, int plain, int plain_buf,
+                                                     const uint32_t* n_ptr, int limb, int lo,
                                                      const uint32_t* __restrict__ perm, uint64_t* __restrict__ kout,
                                                      uint32_t* __restrict__ vout) {
     if (ctl->overflow) return;
-    const uint32_t n = ctl->st[p ^ 1].n;
-    const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur ^ 1];
+    const uint32_t n = *n_ptr;
+    const uint64_t* __restrict__ keys = plain ? bf.keys[plain_buf] : bf.keys[ctl->st[p].cur ^ 1];
     for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
         const uint32_t src = perm ? perm[i] : i;
         kout[i] = keys[(size_t)limb * bf.cap + src] >> lo;
@@ -288,6 +289,94 @@ __global__ __launch_bounds__(WG) void k_clifford_gather(Bufs bf, Ctl* ctl, int p
         for (int b = 0; b < B; ++b) dcoef[(size_t)i * B + b] = scoef[(size_t)s * B + b];
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->st[p ^ 1].cur = dst;
+}
+
+// ============================================================ k_reduce_runs
+//
+// Terms of buffer 0 (n at n_ptr, any order, equal strings allowed), in the
+// stable order perm: every run of equal strings becomes one term whose
+// coefficients are the run's summed left to right (input order), truncated,
+// and compacted in ascending order into buffer 1; the count goes to st[p ^ 1]
+// (mergeandtruncate! on an unsorted vector, src/Base/Merge/mergeandtruncate.jl:104-120).
+constexpr int KR_ITEMS = 4;
+constexpr int KR_TILE = WG * KR_ITEMS;
+
+template <int W>
+__global__ __launch_bounds__(WG) void k_reduce_runs(Bufs bf, Ctl* ctl, Launch L, const uint32_t* __restrict__ perm,
+                                                    int B, int max_weight, double min_abs) {
+    if (!lb_start(ctl, L)) return;
+    const int p = L.seq & 1;
+    const uint32_t n = ctl->st[p].n;
+    const uint64_t* __restrict__ keys = bf.keys[0];
+    const double* __restrict__ coeffs = bf.coeffs[0];
+    uint64_t* __restrict__ out_keys = bf.keys[1];
+    double* __restrict__ out_coeffs = bf.coeffs[1];
+    const size_t stride = bf.cap;
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    if (n == 0) {
+        if (blockIdx.x == 0 && t == 0) set_state(ctl, p ^ 1, 0u, 1u);
+        return;
+    }
+    const uint32_t ntiles = (n + KR_TILE - 1) / KR_TILE;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_w[NWAVES];
+    __shared__ uint64_t s_excl;
+    TileSource ts;
+    while (true) {
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
+        if (tile >= ntiles) break;
+        const uint32_t i0 = tile * (uint32_t)KR_TILE + t * KR_ITEMS;
+        uint32_t keep = 0;
+        Key<W> prev = load_key<W>(keys, stride, perm[i0 > 0 ? min(i0 - 1, n - 1) : 0]);
+        Key<W> kk[KR_ITEMS];
+#pragma unroll
+        for (int k = 0; k < KR_ITEMS; ++k) {
+            const uint32_t i = i0 + k;
+            kk[k] = load_key<W>(keys, stride, perm[min(i, n - 1)]);
+            const bool head = i < n && (i == 0 || !key_eq<W>(kk[k], prev));
+            prev = kk[k];
+            if (!head) continue;
+            // the run's sum, left to right, and its largest magnitude over the batch
+            double m = 0.0;
+            for (int b = 0; b < B; ++b) {
+                double c = coeffs[(size_t)perm[i] * B + b];
+                for (uint32_t j = i + 1; j < n; ++j) {
+                    const uint32_t pj = perm[j];
+                    if (!key_eq<W>(load_key<W>(keys, stride, pj), kk[k])) break;
+                    c = c + coeffs[(size_t)pj * B + b];
+                }
+                m = fmax(m, fabs(c));
+            }
+            if (!((max_weight >= 0 && weight<W>(kk[k]) > max_weight) || m < min_abs)) keep |= 1u << k;
+        }
+        uint32_t agg;
+        const uint32_t pre = block_excl_sum((uint32_t)__popc(keep), s_w, agg);
+        if (wave == 0) {
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
+            if (lane == 0) {
+                s_excl = excl;
+                if (tile == ntiles - 1) set_state(ctl, p ^ 1, (uint32_t)(excl + agg), 1u);
+            }
+        }
+        __syncthreads();
+        uint64_t pos = s_excl + pre;
+#pragma unroll
+        for (int k = 0; k < KR_ITEMS; ++k) {
+            if (!((keep >> k) & 1u)) continue;
+            const uint32_t i = i0 + k;
+            store_key<W>(out_keys, stride, pos, kk[k]);
+            for (int b = 0; b < B; ++b) {
+                double c = coeffs[(size_t)perm[i] * B + b];
+                for (uint32_t j = i + 1; j < n; ++j) {
+                    const uint32_t pj = perm[j];
+                    if (!key_eq<W>(load_key<W>(keys, stride, pj), kk[k])) break;
+                    c = c + coeffs[(size_t)pj * B + b];
+                }
+                out_coeffs[pos * B + b] = c;
+            }
+            ++pos;
+        }
+    }
 }
 
 }  // namespace pp

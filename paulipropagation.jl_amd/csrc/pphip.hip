@@ -661,8 +661,8 @@ void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_ab
         bool first = true;
         for (int l = a.wlo / 64; l < W; ++l) {
             const int lo = l == a.wlo / 64 ? a.wlo % 64 : 0;
-            hipLaunchKernelGGL(k_rs_limb_keys<W>, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, p, h->bufs, l, lo,
-                               first ? nullptr : h->d_rs_v[c], h->d_rs_k[c ^ 1], h->d_rs_v[c ^ 1]);
+            hipLaunchKernelGGL(k_rs_limb_keys, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, p, h->bufs, 0, 0, n_ptr,
+                               l, lo, first ? nullptr : h->d_rs_v[c], h->d_rs_k[c ^ 1], h->d_rs_v[c ^ 1]);
             c ^= 1;
             first = false;
             c = enq_radix_sort(h, n_ptr, c, 64 - lo);
@@ -1243,6 +1243,30 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     return rc;
 }
 
+// Terms in any order with repeats: upload as given, sort on the device (stable,
+// limb by limb), sum every run of equal strings in input order, truncate.
+template <int W>
+int load_merge_impl(pp_handle* h, int max_weight, double min_abs) {
+    if (int rc = ensure_sort(h)) return rc;
+    const int p = h->seq & 1;
+    const uint32_t* n_ptr = &h->d_ctl->st[p].n;
+    const unsigned grid = grid_for(h, h->n_est, WG * 4);
+    int c = 0;
+    for (int l = 0; l < W; ++l) {
+        hipLaunchKernelGGL(k_rs_limb_keys, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, p, h->bufs, 1, 0, n_ptr, l,
+                           0, l == 0 ? nullptr : h->d_rs_v[c], h->d_rs_k[c ^ 1], h->d_rs_v[c ^ 1]);
+        c ^= 1;
+        c = enq_radix_sort(h, n_ptr, c, 64);
+    }
+    hipLaunchKernelGGL(k_reduce_runs<W>, dim3(lb_grid(h, (const void*)k_reduce_runs<W>, 0, h->n_est, KR_TILE)),
+                       dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, nullptr), h->d_rs_v[c], h->B,
+                       max_weight, min_abs);
+    HIP_TRY(hipGetLastError());
+    if (int rc = sync_ctl(h)) return rc;
+    if (h->h_ctl->overflow) return fail(PP_ERR_STATE, "load_merge: a look-back wait ran out");
+    return reset_ctl(h, h->h_ctl->st[p ^ 1].n, 1);
+}
+
 }  // namespace
 
 // ============================================================ C ABI
@@ -1371,6 +1395,39 @@ int pp_upload(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n
                                h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return reset_ctl(h, (uint32_t)n, 0);
+}
+
+int pp_load_merge(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n, int max_weight,
+                  double min_abs_coeff) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    if (n > 0 && (!keys || !coeffs)) return fail(PP_ERR_ARG, "keys or coeffs is NULL");
+    if (n > MAX_TERMS) return fail(PP_ERR_CAPACITY, "term count exceeds 2^31 - 1");
+    const int W = h->W, B = h->B, qbits = 2 * h->nqubits;
+    for (size_t i = 0; i < n; ++i)
+        for (int l = 0; l < W; ++l) {
+            const int lo = l * 64;
+            if (qbits >= lo + 64) continue;
+            const uint64_t allowed = qbits <= lo ? 0ull : ((1ull << (qbits - lo)) - 1);
+            if (keys[i * W + l] & ~allowed) return fail(PP_ERR_ARG, "a term acts on a qubit beyond nqubits");
+        }
+    if (int rc = reset_ctl(h, 0, 0)) return rc;
+    if (n > h->cap)
+        if (int rc = grow(h, n)) return rc;
+    if (n == 0) return PP_OK;
+    std::vector<uint64_t> planes((size_t)W * n);
+    for (size_t i = 0; i < n; ++i)
+        for (int l = 0; l < W; ++l) planes[(size_t)l * n + i] = keys[i * W + l];
+    for (int l = 0; l < W; ++l)
+        HIP_TRY(hipMemcpyAsync(h->bufs.keys[0] + (size_t)l * h->cap, &planes[(size_t)l * n], n * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipMemcpyAsync(h->bufs.coeffs[0], coeffs, n * B * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (int rc = reset_ctl(h, (uint32_t)n, 0)) return rc;
+    switch (W) {
+    case 1: return load_merge_impl<1>(h, max_weight, min_abs_coeff);
+    case 2: return load_merge_impl<2>(h, max_weight, min_abs_coeff);
+    default: return load_merge_impl<4>(h, max_weight, min_abs_coeff);
+    }
 }
 
 int pp_download(pp_handle* h, uint64_t* keys, double* coeffs, size_t cap, size_t* n) {

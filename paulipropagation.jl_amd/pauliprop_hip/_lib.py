@@ -52,6 +52,7 @@ SIGNATURES = [
                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     ("pp_upload", ctypes.c_int, [_P, _U64P, _DP, _SZ]),
     ("pp_download", ctypes.c_int, [_P, _U64P, _DP, _SZ, ctypes.POINTER(_SZ)]),
+    ("pp_load_merge", ctypes.c_int, [_P, _U64P, _DP, _SZ, ctypes.c_int, ctypes.c_double]),
     ("pp_apply_rotation", ctypes.c_int, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_double,
                                          ctypes.POINTER(pp_gate_stats)]),
     ("pp_apply_pauli_noise", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, _DP, ctypes.c_int, ctypes.c_double,

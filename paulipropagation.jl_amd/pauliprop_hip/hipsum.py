@@ -71,6 +71,16 @@ class HIPPauliSum:
             raise ValueError("keys and coeffs have different lengths")
         _lib.check(_lib.lib().pp_upload(self._h, _u64p(keys), _dp(coeffs), keys.shape[0]))
 
+    def load_merge(self, keys: np.ndarray, coeffs: np.ndarray, max_weight=math.inf, min_abs_coeff=0.0) -> None:
+        """Replace the sum with terms in any order, equal strings summed in input
+        order, then truncated -- all on the device."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.batch)
+        if keys.shape[0] != coeffs.shape[0]:
+            raise ValueError("keys and coeffs have different lengths")
+        _lib.check(_lib.lib().pp_load_merge(self._h, _u64p(keys), _dp(coeffs), keys.shape[0],
+                                            _max_weight_arg(max_weight), float(min_abs_coeff)))
+
     def download(self):
         """(keys n x W uint64 ascending, coeffs n x B float64)"""
         n = len(self)

@@ -199,3 +199,33 @@ def test_clifford_batched_and_arguments():
         dev.apply_clifford("CNOT", [3, 3])
     with pytest.raises(ValueError):
         dev.apply_clifford("H", [65])
+
+
+def test_load_merge_known_answer():
+    # test/test_mergeandtruncate.jl:8-20: terms [1,1,2,2,3], coefficients
+    # [.60, -.55, .06, .06, 1.0], min_abs_coeff 0.1 -> {2: 0.12, 3: 1.0}
+    dev = HIPPauliSum(4)
+    keys = np.array([[1], [1], [2], [2], [3]], dtype=np.uint64)
+    dev.load_merge(keys, np.array([0.60, -0.55, 0.06, 0.06, 1.0]), min_abs_coeff=0.1)
+    k, c = dev.download()
+    assert k[:, 0].tolist() == [2, 3]
+    assert c[:, 0].tolist() == [0.06 + 0.06, 1.0]
+
+
+@pytest.mark.parametrize("nq", [20, 64, 127])
+@pytest.mark.parametrize("n", [1, 7, 5000, 100000])
+@pytest.mark.parametrize("B", [1, 3])
+def test_load_merge_with_repeats(nq, n, B):
+    # strings drawn from a small pool so that runs of equal strings occur, in random order
+    print(f"load_merge nq={nq} n={n} B={B}", flush=True)
+    rng = np.random.default_rng(n + nq + B)
+    pool = random_terms(nq, max(1, n // 4), rng, 0.3)
+    W = key_words(nq)
+    picks = rng.integers(0, len(pool), size=n)
+    keys = np.array([int_to_limbs(pool[i], W) for i in picks], dtype=np.uint64).reshape(-1, W)
+    coeffs = rng.standard_normal((n, B))
+    for mw, mac in ((math.inf, 0.0), (8, 0.05)):
+        dev = HIPPauliSum(nq, batch=B)
+        dev.load_merge(keys, coeffs, max_weight=mw, min_abs_coeff=mac)
+        ref = OracleSum(W, B).load_merge(keys, coeffs, max_weight=mw, min_abs=mac)
+        assert_bitwise(*dev.download(), *ref.get(), f"max_weight {mw} min_abs {mac}")
