@@ -56,6 +56,13 @@ def lib():
             ("ppo_weight", ctypes.c_int, [_U64P, ctypes.c_int]),
             ("ppo_contains_xy", ctypes.c_int, [_U64P, ctypes.c_int]),
             ("ppo_xor_sort_tail", ctypes.c_int, [_U64P, _DP, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _U64P]),
+            ("ppo_multi_create", _P, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            ("ppo_multi_destroy", None, [_P]),
+            ("ppo_multi_set", ctypes.c_int, [_P, _U64P, _DP, ctypes.c_size_t]),
+            ("ppo_multi_size", ctypes.c_size_t, [_P]),
+            ("ppo_multi_get", None, [_P, _U64P, _DP]),
+            ("ppo_multi_run", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
         ]
         for name, res, args in sig:
             f = getattr(L, name)
@@ -184,6 +191,51 @@ def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math
     s.set(keys, coeffs)
     counts = s.run(gates, len(hc), hth, P, lc, ls, max_weight, min_abs_coeff)
     return s, counts
+
+
+class MultiOracleSum:
+    """The zone-parallel restatement of MultiPauliSum: 2^zbits zones, rotations
+    and Pauli noise on nthreads OpenMP threads (the all-cores CPU baseline)."""
+
+    def __init__(self, W: int, B: int = 1, zbits: int = 3):
+        self.W, self.B = W, B
+        self._s = lib().ppo_multi_create(W, B, zbits)
+        if not self._s:
+            raise ValueError("zbits must be in 0..8")
+
+    def __del__(self):
+        if getattr(self, "_s", None):
+            lib().ppo_multi_destroy(self._s)
+            self._s = None
+
+    def set(self, keys, coeffs):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, self.W)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, self.B)
+        if lib().ppo_multi_set(self._s, u64p(keys), dp(coeffs), keys.shape[0]) != 0:
+            raise ValueError("duplicate Pauli string")
+        return self
+
+    def __len__(self):
+        return lib().ppo_multi_size(self._s)
+
+    def get_sorted(self):
+        n = len(self)
+        keys = np.zeros((max(n, 1), self.W), dtype=np.uint64)
+        coeffs = np.zeros((max(n, 1), self.B), dtype=np.float64)
+        lib().ppo_multi_get(self._s, u64p(keys), dp(coeffs))
+        keys, coeffs = keys[:n], coeffs[:n]
+        if n > 1:
+            order = np.lexsort(tuple(keys[:, l] for l in range(self.W)))
+            keys, coeffs = keys[order], coeffs[order]
+        return keys, coeffs
+
+    def run(self, gates_ptr, ngates, theta, nparams, max_weight, min_abs, nthreads, counts=None):
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        cp = counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)) if counts is not None else None
+        rc = lib().ppo_multi_run(self._s, gates_ptr, ngates, dp(th), nparams, _mw(max_weight), float(min_abs),
+                                 int(nthreads), cp)
+        if rc != 0:
+            raise ValueError("the zone baseline runs rotations and Pauli noise only")
 
 
 def oracle_adjoint(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,

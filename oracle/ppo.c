@@ -685,3 +685,196 @@ int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const d
     free(sv);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* zones: the MultiPauliSum restatement (the all-cores CPU baseline)   */
+/* ------------------------------------------------------------------ */
+
+/* A term's zone: bit r is the parity of term & mask_r, with pseudo-random masks
+ * from SplitMix64's mixer, seed 0x9e3779b97f4a7c15, one draw per 64-bit word
+ * (src/Base/MultiSum/multisumstorage.jl:64-151). The map is linear over GF(2),
+ * so a rotation moves the created terms of zone z to zone z ^ zone(mask). */
+#define PPO_ZONE_SEED 0x9e3779b97f4a7c15ULL
+#define PPO_MAX_ZBITS 8
+
+struct ppo_multi {
+    int W, B, zbits, Z;
+    uint64_t masks[PPO_MAX_ZBITS][4];
+    ppo_sum **zones;
+};
+
+static uint64_t mix64(uint64_t x) {
+    uint64_t z = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+static int zone_of(const ppo_multi *m, const uint64_t *key) {
+    int z = 0;
+    for (int r = 0; r < m->zbits; ++r) {
+        uint64_t f = 0;
+        for (int l = 0; l < m->W; ++l) f ^= key[l] & m->masks[r][l];
+        z |= (__builtin_popcountll(f) & 1) << r;
+    }
+    return z;
+}
+
+ppo_multi *ppo_multi_create(int W, int B, int zbits) {
+    if (zbits < 0 || zbits > PPO_MAX_ZBITS) return NULL;
+    ppo_multi *m = xmalloc(sizeof(ppo_multi));
+    m->W = W;
+    m->B = B;
+    m->zbits = zbits;
+    m->Z = 1 << zbits;
+    uint64_t state = PPO_ZONE_SEED;
+    memset(m->masks, 0, sizeof(m->masks));
+    for (int r = 0; r < zbits; ++r)
+        for (int l = 0; l < W; ++l) {
+            state += PPO_ZONE_SEED;
+            m->masks[r][l] = mix64(state);
+        }
+    m->zones = xmalloc(m->Z * sizeof(ppo_sum *));
+    for (int z = 0; z < m->Z; ++z) m->zones[z] = ppo_create(W, B);
+    return m;
+}
+
+void ppo_multi_destroy(ppo_multi *m) {
+    if (!m) return;
+    for (int z = 0; z < m->Z; ++z) ppo_destroy(m->zones[z]);
+    free(m->zones);
+    free(m);
+}
+
+/* Distinct terms in any order: sorted once, then dealt to their zones in
+ * order, so every zone is sorted. */
+int ppo_multi_set(ppo_multi *m, const uint64_t *keys, const double *coeffs, size_t n) {
+    ppo_sum *all = ppo_create(m->W, m->B);
+    int rc = ppo_set(all, keys, coeffs, n);
+    if (rc) {
+        ppo_destroy(all);
+        return rc;
+    }
+    int W = m->W, B = m->B;
+    for (int z = 0; z < m->Z; ++z) m->zones[z]->n = 0;
+    for (size_t i = 0; i < all->n; ++i) {
+        ppo_sum *zs = m->zones[zone_of(m, all->keys + i * W)];
+        reserve(zs, zs->n + 1);
+        memcpy(zs->keys + zs->n * W, all->keys + i * W, W * sizeof(uint64_t));
+        memcpy(zs->coeffs + zs->n * B, all->coeffs + i * B, B * sizeof(double));
+        zs->n += 1;
+        zs->sorted = 1;
+    }
+    ppo_destroy(all);
+    return 0;
+}
+
+size_t ppo_multi_size(const ppo_multi *m) {
+    size_t n = 0;
+    for (int z = 0; z < m->Z; ++z) n += m->zones[z]->n;
+    return n;
+}
+
+/* All terms, zone after zone (each zone ascending). */
+void ppo_multi_get(const ppo_multi *m, uint64_t *keys, double *coeffs) {
+    size_t o = 0;
+    for (int z = 0; z < m->Z; ++z) {
+        const ppo_sum *zs = m->zones[z];
+        memcpy(keys + o * m->W, zs->keys, zs->n * m->W * sizeof(uint64_t));
+        memcpy(coeffs + o * m->B, zs->coeffs, zs->n * m->B * sizeof(double));
+        o += zs->n;
+    }
+}
+
+/* The branch half of a rotation on one zone: anticommuting terms keep c * cos
+ * in place and their created terms (parent order) go to a new tail. */
+static size_t branch_zone(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv, uint64_t **tko,
+                          double **tco) {
+    const int W = s->W, B = s->B;
+    size_t na = 0;
+    for (size_t i = 0; i < s->n; ++i) na += anticommutes_w(mask, s->keys + i * W, W);
+    uint64_t *tk = xmalloc((na ? na : 1) * W * sizeof(uint64_t));
+    double *tc = xmalloc((na ? na : 1) * B * sizeof(double));
+    size_t j = 0;
+    for (size_t i = 0; i < s->n && j < na; ++i) {
+        uint64_t *k = s->keys + i * W;
+        if (!anticommutes_w(mask, k, W)) continue;
+        int e = sign_exponent_w(mask, k, W);
+        double sign = (double)((e & 2) - 1);
+        for (int l = 0; l < W; ++l) tk[j * W + l] = k[l] ^ mask[l];
+        double *c = s->coeffs + i * B;
+        for (int b = 0; b < B; ++b) {
+            tc[j * B + b] = (c[b] * sv[b]) * sign;
+            c[b] = c[b] * cv[b];
+        }
+        ++j;
+    }
+    *tko = tk;
+    *tco = tc;
+    return na;
+}
+
+/* Run rotations and Pauli noise over the zones on nthreads threads: a rotation
+ * branches every zone in parallel, then every zone merges the tail of its
+ * partner zone (zone ^ zone(mask)), also in parallel. Other gate kinds are
+ * not part of this baseline (returns -1). counts receives n_in per gate. */
+int ppo_multi_run(ppo_multi *m, const ppo_gate *gates, size_t ngates, const double *theta, size_t nparams,
+                  int max_weight, double min_abs, int nthreads, int64_t *counts) {
+    const int W = m->W, B = m->B, Z = m->Z;
+    uint64_t **tk = xmalloc(Z * sizeof(uint64_t *));
+    double **tc = xmalloc(Z * sizeof(double *));
+    size_t *nt = xmalloc(Z * sizeof(size_t));
+    double *cv = xmalloc(B * sizeof(double)), *sv = xmalloc(B * sizeof(double)), *fv = xmalloc(B * sizeof(double));
+    int rc = 0;
+    for (size_t g = 0; g < ngates && !rc; ++g) {
+        const ppo_gate *gt = &gates[g];
+        if (gt->param >= 0 && (size_t)gt->param >= nparams) {
+            rc = -1;
+            break;
+        }
+        if (counts) counts[g] = (int64_t)ppo_multi_size(m);
+        if (gt->kind == PPO_ROTATION) {
+            for (int b = 0; b < B; ++b) {
+                const double th = gate_value(gt, theta, B, b);
+                cv[b] = cos(th);
+                sv[b] = sin(th);
+            }
+            const int zm = zone_of(m, gt->mask);
+            size_t total = 0;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total)
+            for (int z = 0; z < Z; ++z) {
+                nt[z] = branch_zone(m->zones[z], gt->mask, cv, sv, &tk[z], &tc[z]);
+                total += nt[z];
+            }
+            if (total > 0) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+                for (int z = 0; z < Z; ++z) {
+                    const int src = z ^ zm;
+                    ppo_xor_sort_tail(tk[src], tc[src], nt[src], W, B, gt->mask);
+                }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+                for (int z = 0; z < Z; ++z) {
+                    const int src = z ^ zm;
+                    merge_head_tail(m->zones[z], tk[src], tc[src], nt[src], max_weight, min_abs);
+                }
+            }
+            for (int z = 0; z < Z; ++z) {
+                free(tk[z]);
+                free(tc[z]);
+            }
+        } else if (gt->kind == PPO_PAULI_NOISE) {
+            for (int b = 0; b < B; ++b) fv[b] = 1.0 - gate_value(gt, theta, B, b);
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+            for (int z = 0; z < Z; ++z) ppo_pauli_noise(m->zones[z], gt->qubit, gt->damp_codes, fv, max_weight, min_abs);
+        } else {
+            rc = -1;
+        }
+    }
+    free(tk);
+    free(tc);
+    free(nt);
+    free(cv);
+    free(sv);
+    free(fv);
+    return rc;
+}
+

@@ -86,6 +86,18 @@ int ppo_contains_xy(const uint64_t *a, int W);
  * ascending run, by XOR block-reversal passes; returns the pass count. */
 int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, const uint64_t *mask);
 
+/* The zone-parallel restatement of MultiPauliSum (src/Base/MultiSum/): 2^zbits
+ * zones by a GF(2)-linear hash, rotations branched and merged zone by zone on
+ * nthreads OpenMP threads. It is the all-cores CPU baseline. */
+typedef struct ppo_multi ppo_multi;
+ppo_multi *ppo_multi_create(int W, int B, int zbits);
+void ppo_multi_destroy(ppo_multi *m);
+int ppo_multi_set(ppo_multi *m, const uint64_t *keys, const double *coeffs, size_t n);
+size_t ppo_multi_size(const ppo_multi *m);
+void ppo_multi_get(const ppo_multi *m, uint64_t *keys, double *coeffs);
+int ppo_multi_run(ppo_multi *m, const ppo_gate *gates, size_t ngates, const double *theta, size_t nparams,
+                  int max_weight, double min_abs, int nthreads, int64_t *counts);
+
 #ifdef __cplusplus
 }
 #endif

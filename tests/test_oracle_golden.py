@@ -116,3 +116,38 @@ def test_overlap_is_sum_over_IZ_strings():
     s = OracleSum(2, 1).set(keys, c)
     k, cc = s.get()
     assert s.overlap_zero()[0] == pytest.approx(fsum_overlap(k, cc)[0], abs=1e-12)
+
+
+@pytest.mark.parametrize("zbits,threads", [(0, 1), (2, 4), (3, 3)])
+def test_zone_parallel_baseline_equals_single_sum(zbits, threads):
+    # the MultiPauliSum restatement (zones by a GF(2)-linear hash, tails sent to
+    # zone ^ zone(mask)) gives the single sum's result bit for bit
+    import ctypes
+    from oracle import MultiOracleSum, OracleSum, lib
+    from pauliprop_hip.gates import pack_circuit, toheisenberg
+    nq = 24
+    circ = []
+    for _ in range(3):
+        pp.rxlayer(circ, nq)
+        pp.rzzlayer(circ, pp.bricklayertopology(nq, periodic=True))
+        pp.depolarizinglayer(circ, nq, 0.02)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(zbits).uniform(0, 1.2, size=P)
+    hc, hth = toheisenberg(circ, list(th))
+    gates, _, _, _ = pack_circuit(hc, nq, 1)
+    obs = {pp.symboltoint(nq, "Z", q): 1.0 for q in (5, 12, 20)}
+    keys = np.array([[k] for k in obs], dtype=np.uint64)
+    coeffs = np.ones((len(obs), 1))
+    for mw, mac in ((6, 0.0), (8, 1e-4)):
+        single = OracleSum(1, 1).set(keys, coeffs)
+        c1 = np.zeros(len(hc), dtype=np.int64)
+        lib().ppo_run(single._s, ctypes.cast(gates, ctypes.c_void_p), len(hc), np.ascontiguousarray(hth).ctypes.data_as(
+            ctypes.POINTER(ctypes.c_double)), P, None, None, -1 if mw == math.inf else mw, mac,
+            c1.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        multi = MultiOracleSum(1, 1, zbits).set(keys, coeffs)
+        c2 = np.zeros(len(hc), dtype=np.int64)
+        multi.run(ctypes.cast(gates, ctypes.c_void_p), len(hc), np.array(hth), P, mw, mac, threads, c2)
+        assert np.array_equal(c1, c2)
+        k1, v1 = single.get_sorted()
+        k2, v2 = multi.get_sorted()
+        assert np.array_equal(k1, k2) and np.array_equal(v1.view(np.uint64), v2.view(np.uint64))
