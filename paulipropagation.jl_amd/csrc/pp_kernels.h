@@ -40,6 +40,7 @@ struct Bufs {
     double* coeffs[2];   // cap x B
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
+    double* tcoef;       // B = 1: the permuted tail's source coefficients (null for B > 1)
     uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile
     uint64_t* gstates;   // look-back words: two chains of lb_groups words, one per 64 tiles
     uint32_t* splits;    // merge-path split of every merge tile boundary
@@ -91,7 +92,48 @@ struct Ctl {
     uint32_t tail_ready;
     uint32_t tail_seq;
     uint64_t tail_mask[4];
+    uint32_t pad2[2];
 };
+static_assert(sizeof(Ctl) % 16 == 0, "the control block is read in 16-byte pieces");
+
+// The control block as a kernel sees it when it starts, read in one round trip
+// (all pieces requested before any is used). The sizes, buffers and lists in
+// it change only between launches; the counters, err and the overflow record
+// are read again where they matter.
+struct CtlView {
+    uint32_t overflow;
+    GateState s;  // st[p] of the launching gate's parity p
+    uint32_t tail_ready, tail_seq;
+    uint64_t tail_mask[4];
+};
+
+__device__ __forceinline__ CtlView load_ctl(const Ctl* ctl, int p) {
+    constexpr int NQ = (int)(sizeof(Ctl) / 16);
+    const uint4* src = reinterpret_cast<const uint4*>(ctl);
+    uint4 q[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) q[i] = src[i];
+    Ctl c;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) __builtin_memcpy(reinterpret_cast<char*>(&c) + 16 * i, &q[i], 16);
+    CtlView v;
+    v.overflow = c.overflow;
+    v.s.n = p ? c.st[1].n : c.st[0].n;
+    v.s.cur = p ? c.st[1].cur : c.st[0].cur;
+    v.s.na = p ? c.st[1].na : c.st[0].na;
+    v.s.roff = p ? c.st[1].roff : c.st[0].roff;
+    v.s.coff = p ? c.st[1].coff : c.st[0].coff;
+    v.s.run = p ? c.st[1].run : c.st[0].run;
+    v.tail_ready = c.tail_ready;
+    v.tail_seq = c.tail_seq;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) v.tail_mask[l] = c.tail_mask[l];
+    // all fields are needed here, so their loads go out together instead of
+    // being sunk past the first branch one round trip at a time
+    asm volatile("" ::"s"(v.overflow), "s"(v.s.n), "s"(v.s.cur), "s"(v.s.na), "s"(v.s.roff), "s"(v.s.coff),
+                 "s"(v.s.run), "s"(v.tail_ready), "s"(v.tail_seq));
+    return v;
+}
 
 struct Launch {
     uint32_t seq;         // gate sequence number (parity selects the state)
@@ -162,6 +204,12 @@ __device__ __forceinline__ double pick(const double (&a)[N], int i) {
 
 // common start of a look-back kernel: skip everything after an overflow;
 // clear the counter the next look-back launch will use
+__device__ __forceinline__ bool lb_start(Ctl* ctl, const Launch& L, const CtlView& c) {
+    if (c.overflow) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->counter[L.slot ^ 1u] = 0u;
+    return true;
+}
+
 __device__ __forceinline__ bool lb_start(Ctl* ctl, const Launch& L) {
     if (ctl->overflow) return false;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->counter[L.slot ^ 1u] = 0u;
@@ -223,9 +271,9 @@ constexpr int K1_TILE = WG * K1_ITEMS;
 // End of the branch step, once the touched count na is known: check room for
 // the na created terms, or hand the sum on unchanged when nothing is touched.
 __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const Bufs& bf, int p, uint32_t n,
-                                              uint32_t cur, uint32_t na) {
+                                              uint32_t cur, uint32_t na, const GateState& sp) {
     ctl->st[p].na = na;
-    const uint32_t roff = ctl->st[p].roff, coff = ctl->st[p].coff;
+    const uint32_t roff = sp.roff, coff = sp.coff;
     // a rotation touching nothing is the identity: no records
     const uint64_t words = (bf.rec && na) ? (uint64_t)n + 2ull * na : 0ull;
     const uint64_t rows = (bf.rec && na) ? (uint64_t)na : 0ull;
@@ -257,9 +305,14 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
 // there sets st[p].run, since its coefficient changes.
 template <int W, bool DAMP>
 __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
-    if (!lb_start(ctl, L)) return;
+    // every argument the early exits read, loaded with the control block
+#pragma unroll
+    for (int l = 0; l < W; ++l) asm volatile("" ::"s"(mask.l[l]));
+    asm volatile("" ::"s"(bf.cap), "s"(bf.rec_words), "s"(bf.rec_rows), "s"(bf.rec), "s"(L.glog));
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
-    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint32_t n = c.s.n, cur = c.s.cur;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     if (n == 0) {
         if (blockIdx.x == 0 && t == 0) {
@@ -271,11 +324,11 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
         return;
     }
     // the touched terms may already be listed by the previous gate's merge
-    bool listed = !DAMP && ctl->tail_ready && ctl->tail_seq == L.seq;
+    bool listed = !DAMP && c.tail_ready && c.tail_seq == L.seq;
 #pragma unroll
-    for (int l = 0; l < W; ++l) listed = listed && ctl->tail_mask[l] == mask.l[l];
+    for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
     if (listed) {
-        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, ctl->st[p].na);
+        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s);
         return;
     }
     const uint64_t* __restrict__ keys = bf.keys[cur];
@@ -349,7 +402,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
-                if (tile == ntiles - 1) branch_finish(ctl, L, bf, p, n, cur, (uint32_t)(excl + agg));
+                if (tile == ntiles - 1) branch_finish(ctl, L, bf, p, n, cur, (uint32_t)(excl + agg), c.s);
             }
         }
         __syncthreads();
@@ -461,11 +514,20 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
 
 template <int W>
 __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask) {
-    if (ctl->overflow) return;
+    // the first tile's entries (tile = blockIdx.x) go out with the control
+    // block; indices are clamped into the buffer and the values sanitized below
+    uint32_t psrc[K2_ITEMS];
+#pragma unroll
+    for (int i = 0; i < K2_ITEMS; ++i)
+        psrc[i] = bf.tail[0][min(blockIdx.x * (uint32_t)K2_TILE + threadIdx.x * K2_ITEMS + i, bf.cap - 1)];
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (c.overflow) return;
     const int p = L.seq & 1;
-    const uint32_t na = ctl->st[p].na;
+    const uint32_t na = c.s.na;
     if (na == 0) return;
-    const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
+    const uint64_t* __restrict__ keys = bf.keys[c.s.cur];
+    const double* __restrict__ coeffs = bf.coeffs[c.s.cur];
+    const uint32_t nterms = c.s.n;
     const size_t stride = bf.cap;
     const uint32_t* __restrict__ tail_in = bf.tail[0];
     uint32_t* __restrict__ tail_out = bf.tail[1];
@@ -500,8 +562,14 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         Key<W> key[K2_ITEMS];
         const uint32_t j0 = base + t * K2_ITEMS;
         int nvalid = 0;
+        if (tile == blockIdx.x) {
+            // prefetched; entries past the list are stale, so their indices are clamped to a term
 #pragma unroll
-        for (int i = 0; i < K2_ITEMS; ++i) src[i] = tail_in[min(j0 + i, last)];
+            for (int i = 0; i < K2_ITEMS; ++i) src[i] = j0 + i <= last ? psrc[i] : min(psrc[i] & SRC_MASK, nterms - 1);
+        } else {
+#pragma unroll
+            for (int i = 0; i < K2_ITEMS; ++i) src[i] = tail_in[min(j0 + i, last)];
+        }
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) key[i] = load_key_from<W>(keys, stride, src[i] & SRC_MASK, from_limb);
 #pragma unroll
@@ -599,18 +667,23 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 dest[i] += (int64_t)sg * ((int64_t)s + (int64_t)e);
             }
         }
-        // the limbs below from_limb were not needed for the ranks; load them now, all at once
+        // the limbs below from_limb were not needed for the ranks; load them now, all
+        // at once, with the source coefficients the merge will scale
+        double cf[K2_ITEMS];
 #pragma unroll
-        for (int i = 0; i < K2_ITEMS; ++i)
+        for (int i = 0; i < K2_ITEMS; ++i) {
 #pragma unroll
             for (int l = 0; l < W; ++l)
                 if (l < from_limb) key[i].l[l] = keys[l * stride + (src[i] & SRC_MASK)];
+            cf[i] = bf.tcoef ? coeffs[src[i] & SRC_MASK] : 0.0;
+        }
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) {
             if (i < nvalid) {
                 // a rank outside [0, na) would be a bug; it is reported, never written
                 if (dest[i] >= 0 && dest[i] < (int64_t)na) {
                     tail_out[dest[i]] = src[i];
+                    if (bf.tcoef) bf.tcoef[dest[i]] = cf[i];
                     if (bf.tpj) bf.tpj[dest[i]] = j0 + i;
 #pragma unroll
                     for (int l = 0; l < W; ++l) bf.tkeys[l * stride + dest[i]] = key[i].l[l] ^ mask.l[l];
@@ -626,7 +699,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         __syncthreads();
     }
     // every 64th entry of the sum, for the merge partition
-    const uint32_t n = ctl->st[p].n;
+    const uint32_t n = c.s.n;
     for (uint32_t k = blockIdx.x * WG + t; 64ull * k < n; k += gridDim.x * WG) {
         const Key<W> kk = load_key<W>(keys, stride, 64ull * k);
 #pragma unroll
@@ -754,12 +827,13 @@ struct K3Cfg {
 template <int W, int TILE>
 __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
     static_assert(TILE % 64 == 0, "tile boundaries must fall on sampled diagonals");
-    if (ctl->overflow) return;
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (c.overflow) return;
     const int p = L.seq & 1;
-    const uint32_t na = ctl->st[p].na;
-    if (!merge_runs(ctl, p)) return;
-    const uint32_t n = ctl->st[p].n;
-    const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
+    const uint32_t na = c.s.na;
+    if (!(c.s.na || c.s.run)) return;
+    const uint32_t n = c.s.n;
+    const uint64_t* __restrict__ keys = bf.keys[c.s.cur];
     const uint64_t* __restrict__ ss = bf.ssamp;
     const uint64_t* __restrict__ ts = bf.tsamp;
     const size_t stride = bf.cap;
@@ -809,7 +883,7 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
                                             uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
                                             double sinv, uint64_t (*s_key)[TILE + 2], double* s_val,
                                             const uint32_t* __restrict__ tpj, uint32_t* s_pj, int damp_q,
-                                            double fxy) {
+                                            double fxy, const double* __restrict__ tcoef) {
     const int t = threadIdx.x;
     constexpr int R = (TILE + 1 + WG - 1) / WG;
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
@@ -847,7 +921,7 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
         for (int l = 0; l < W; ++l) k.l[l] = tkeys[l * stride + gj];
         const uint32_t src = tail[gj];
         const uint32_t pj = tpj ? tpj[gj] : 0u;
-        double c = coeffs[src & SRC_MASK] * sinv;
+        double c = tcoef[gj] * sinv;
         if (src & SRC_NEG) c = -c;
         if (y < nt + 1 && j0 + (uint32_t)y < na) {
 #pragma unroll
@@ -871,11 +945,17 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                                                Key<W> nmask) {
     constexpr int ITEMS = K3Cfg<W>::ITEMS, TILE = K3Cfg<W>::TILE;
     static_assert(TILE < 65536, "kept and touched counts of a tile are packed in 16 bits each");
-    if (!lb_start(ctl, L)) return;
+    // the first static tile's splits go out with the control block (tile =
+    // blockIdx.x; the index is clamped into the array, and used only if the
+    // tile is that one)
+    const uint32_t pt = min((uint32_t)blockIdx.x, bf.lb_tiles - 2);
+    const uint32_t pre0 = bf.splits[pt], pre1 = bf.splits[pt + 1];
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
-    const uint32_t na = ctl->st[p].na;
-    if (!merge_runs(ctl, p)) return;
-    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint32_t na = c.s.na;
+    if (!(na || c.s.run)) return;
+    const uint32_t n = c.s.n, cur = c.s.cur;
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const double* __restrict__ coeffs = bf.coeffs[cur];
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
@@ -889,8 +969,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl, s_excl_t;
-    uint32_t* __restrict__ rhead = REC ? bf.rec + ctl->st[p].roff : nullptr;
-    uint32_t* __restrict__ rcreated = REC ? bf.rec + ctl->st[p].roff + n + na : nullptr;
+    uint32_t* __restrict__ rhead = REC ? bf.rec + c.s.roff : nullptr;
+    uint32_t* __restrict__ rcreated = REC ? bf.rec + c.s.roff + n + na : nullptr;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     uint32_t* __restrict__ next_tail = bf.tail[0];
     TileSource ts;
@@ -899,7 +979,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         if (tile >= ntiles) break;
         const uint32_t d0 = tile * (uint32_t)TILE;
         const uint32_t d1 = min(d0 + (uint32_t)TILE, ntot);
-        const uint32_t i0 = bf.splits[tile], i1 = bf.splits[tile + 1];
+        const bool first = tile == pt && !L.dynamic;
+        const uint32_t i0 = first ? pre0 : bf.splits[tile], i1 = first ? pre1 : bf.splits[tile + 1];
         const uint32_t j0 = d0 - i0, j1 = d1 - i1;
         int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
         if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
@@ -909,7 +990,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             nt = 0;
         }
         merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
-                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy);
+                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -1051,11 +1132,12 @@ template <int W, int TB>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
                                                int damp_q, double fxy) {
-    if (!lb_start(ctl, L)) return;
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
-    const uint32_t na = ctl->st[p].na;
-    if (!merge_runs(ctl, p)) return;
-    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint32_t na = c.s.na;
+    if (!(na || c.s.run)) return;
+    const uint32_t n = c.s.n, cur = c.s.cur;
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const double* __restrict__ coeffs = bf.coeffs[cur];
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
@@ -1224,7 +1306,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         }
         if (bf.rec && t < TB) {
             const uint32_t at = mine ? (uint32_t)(s_excl + pre) : REC_NONE;
-            const uint32_t roff = ctl->st[p].roff;
+            const uint32_t roff = c.s.roff;
             if (s_rh[t] != ~0u) bf.rec[(size_t)roff + (s_rh[t] & SRC_MASK)] = (s_rh[t] & REC_FLAG) | at;
             if (s_rt[t] != ~0u) bf.rec[(size_t)roff + n + na + s_rt[t]] = at;
         }
@@ -1253,9 +1335,10 @@ template <int W>
 __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int qubit, uint32_t damp_codes,
                                                double factor, int max_weight, double min_abs) {
     constexpr int ITEMS = KNCfg<W>::ITEMS, TILE = KNCfg<W>::TILE;
-    if (!lb_start(ctl, L)) return;
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
-    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint32_t n = c.s.n, cur = c.s.cur;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     if (n == 0) {
         if (blockIdx.x == 0 && t == 0) {
@@ -1265,9 +1348,9 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
         }
         return;
     }
-    const uint32_t roff = ctl->st[p].roff;
+    const uint32_t roff = c.s.roff;
     if (bf.rec && (uint64_t)roff + n > bf.rec_words) {
-        if (blockIdx.x == 0 && t == 0) record_overflow(ctl, L, (uint64_t)roff + n, ctl->st[p].coff);
+        if (blockIdx.x == 0 && t == 0) record_overflow(ctl, L, (uint64_t)roff + n, c.s.coff);
         return;
     }
     const uint64_t* __restrict__ keys = bf.keys[cur];

@@ -48,7 +48,7 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr size_t MAX_TERMS = 0x7fffffffu;  // indices carry a sign bit
-constexpr size_t RUN_CHUNK = 256;          // gates enqueued between overflow checks
+constexpr size_t RUN_CHUNK = 64;           // gates enqueued between overflow checks
 
 }  // namespace
 
@@ -69,7 +69,6 @@ struct pp_handle {
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr;          // pinned mirror
     uint32_t seq = 0;              // next gate's sequence number; the sum's state is st[seq & 1]
-    size_t n_est = 0;              // live terms at the last synchronization (sizes the grids)
     uint32_t lb_count = 0;         // look-back launches so far (counter slot parity)
     uint32_t epoch = 0;
     // per-run device arrays
@@ -119,6 +118,8 @@ void free_terms(pp_handle* h) {
     }
     if (h->bufs.states) hipFree(h->bufs.states);
     if (h->bufs.tkeys) hipFree(h->bufs.tkeys);
+    if (h->bufs.tcoef) hipFree(h->bufs.tcoef);
+    h->bufs.tcoef = nullptr;
     if (h->bufs.splits) hipFree(h->bufs.splits);
     if (h->bufs.gstates) hipFree(h->bufs.gstates);
     if (h->bufs.ssamp) hipFree(h->bufs.ssamp);
@@ -148,7 +149,6 @@ size_t tiles_needed(size_t cap, int B) {
 int sync_ctl(pp_handle* h) {
     HIP_TRY(hipMemcpyAsync(h->h_ctl, h->d_ctl, sizeof(Ctl), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    h->n_est = h->h_ctl->st[h->seq & 1].n;
     return PP_OK;
 }
 
@@ -177,7 +177,6 @@ int reset_ctl(pp_handle* h, const GateState& st) {
     s.cur = st.cur;
     s.roff = st.roff;
     s.coff = st.coff;
-    h->n_est = st.n;
     return put_ctl(h);
 }
 
@@ -204,6 +203,7 @@ int grow(pp_handle* h, size_t newcap) {
     uint32_t* nt[2] = {nullptr, nullptr};
     uint64_t* ns = nullptr;
     uint64_t* ntk = nullptr;
+    double* ntc = nullptr;
     uint32_t* nsp = nullptr;
     uint64_t* ngs = nullptr;
     uint64_t* nss = nullptr;
@@ -220,6 +220,7 @@ int grow(pp_handle* h, size_t newcap) {
     }
     ok = ok && hipMalloc(&ns, 2 * nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
+    if (B == 1) ok = ok && hipMalloc(&ntc, newcap * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&nsp, nlb * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&ngs, 2 * ngw * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nss, nsamp * sizeof(uint64_t)) == hipSuccess;
@@ -232,6 +233,7 @@ int grow(pp_handle* h, size_t newcap) {
         }
         if (ns) hipFree(ns);
         if (ntk) hipFree(ntk);
+        if (ntc) hipFree(ntc);
         if (nsp) hipFree(nsp);
         if (ngs) hipFree(ngs);
         if (nss) hipFree(nss);
@@ -257,6 +259,7 @@ int grow(pp_handle* h, size_t newcap) {
     }
     h->bufs.states = ns;
     h->bufs.tkeys = ntk;
+    h->bufs.tcoef = ntc;
     h->bufs.splits = nsp;
     h->bufs.gstates = ngs;
     h->bufs.lb_tiles = (uint32_t)nlb;
@@ -316,10 +319,12 @@ unsigned resident_wgs(pp_handle* h, const void* fn, size_t shm) {
     return r;
 }
 
-// Grid of a persistent kernel: enough workgroups for `expect` entries at the
-// size seen at the last synchronization, at most what the chip holds at once.
-// Too few only means more tiles per workgroup; every workgroup beyond the
-// work still costs a dispatch.
+// Grid of a persistent kernel: enough workgroups for `expect` entries, at most
+// what the chip holds at once. Gates run back to back with the live size on
+// the device only, so the forward pipeline passes the capacity: a grid sized
+// from an older, smaller count makes every workgroup walk several tiles in
+// series on the gates where the sum has grown, which costs far more than the
+// idle workgroups a full grid dispatches on the gates where it is small.
 unsigned grid_for(pp_handle* h, size_t expect, size_t per_tile) {
     const size_t tiles = (expect + per_tile - 1) / per_tile;
     return (unsigned)std::max<size_t>(1, std::min<size_t>(tiles, h->gmax));
@@ -453,18 +458,18 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool mrg = h->prof_mode >= 1 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
     launch(h, E(0, all), E(1, all), k_branch<W, false>,
-           lb_grid(h, (const void*)k_branch<W, false>, 0, h->n_est, K1_TILE), 0, h->bufs, h->d_ctl,
+           lb_grid(h, (const void*)k_branch<W, false>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
            lb_launch(h, glog), mask);
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
     if (h->recording)
-        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->n_est * (size_t)h->B / 2, WG * 4), 0, h->bufs,
+        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
                h->d_ctl, L2, h->B);
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->n_est / 2, K2_TILE), 0, h->bufs,
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->cap, K2_TILE), 0, h->bufs,
            h->d_ctl, L2, lv, mask);
-    const size_t expect = h->n_est + h->n_est / 2;
+    const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
         constexpr int TILE = K3Cfg<W>::TILE;
@@ -520,15 +525,15 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
     const bool mrg = h->prof_mode >= 1 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
     launch(h, E(0, all), E(1, all), k_branch<W, true>,
-           lb_grid(h, (const void*)k_branch<W, true>, 0, h->n_est, K1_TILE), 0, h->bufs, h->d_ctl,
+           lb_grid(h, (const void*)k_branch<W, true>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
            lb_launch(h, glog), mask);
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->n_est / 2, K2_TILE), 0, h->bufs,
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->cap, K2_TILE), 0, h->bufs,
            h->d_ctl, L2, lv, mask);
-    const size_t expect = h->n_est + h->n_est / 2;
+    const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{1 - gamma, gamma, max_weight, min_abs, qubit, fxy};
         constexpr int TILE = K3Cfg<W>::TILE;
@@ -565,10 +570,10 @@ void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const do
     const bool mrg = h->prof_mode >= 1 && ev;
     hipEvent_t e0 = mrg ? ev[4] : nullptr, e1 = mrg ? ev[5] : nullptr;
     if (h->B == 1) {
-        launch(h, e0, e1, k_noise1<W>, lb_grid(h, (const void*)k_noise1<W>, 0, h->n_est, KNCfg<W>::TILE), 0, h->bufs,
+        launch(h, e0, e1, k_noise1<W>, lb_grid(h, (const void*)k_noise1<W>, 0, h->cap, KNCfg<W>::TILE), 0, h->bufs,
                h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
     } else {
-        launch(h, e0, e1, k_noiseB<W>, lb_grid(h, (const void*)k_noiseB<W>, 0, h->n_est, KNB_ROWS), 0, h->bufs,
+        launch(h, e0, e1, k_noiseB<W>, lb_grid(h, (const void*)k_noiseB<W>, 0, h->cap, KNB_ROWS), 0, h->bufs,
                h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
     }
     h->seq += 1;
@@ -612,7 +617,7 @@ int ensure_sort(pp_handle* h) {
 // One stable LSD sort of (rs_k[c], rs_v[c]) over key bits [0, nbits); returns
 // the buffer index holding the result. n is read on the device at n_ptr.
 int enq_radix_sort(pp_handle* h, const uint32_t* n_ptr, int c, int nbits) {
-    const unsigned grid = grid_for(h, h->n_est, RS_TILE);
+    const unsigned grid = grid_for(h, h->cap, RS_TILE);
     for (int shift = 0; shift < nbits; shift += 8) {
         const int bits = std::min(8, nbits - shift);
         hipLaunchKernelGGL(k_rs_upsweep, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_k[c], shift,
@@ -648,7 +653,7 @@ void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_ab
     const int p = h->seq & 1;
     const uint32_t* n_ptr = &h->d_ctl->st[p ^ 1].n;
     launch(h, mrg ? ev[4] : nullptr, mrg ? ev[5] : nullptr, k_clifford_map<W>,
-           lb_grid(h, (const void*)k_clifford_map<W>, 0, h->n_est, KC_TILE), 0, h->bufs, h->d_ctl,
+           lb_grid(h, (const void*)k_clifford_map<W>, 0, h->cap, KC_TILE), 0, h->bufs, h->d_ctl,
            lb_launch(h, glog), a, h->B, h->d_rs_k[0], h->d_rs_v[0]);
     int c = 0;
     if (a.composite) {
@@ -657,7 +662,7 @@ void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_ab
         c = enq_radix_sort(h, n_ptr, 0, a.wbits + segbits);
     } else {
         // wide window: the new key's limbs from the window's lowest, least significant first
-        const unsigned grid = grid_for(h, h->n_est, WG * 4);
+        const unsigned grid = grid_for(h, h->cap, WG * 4);
         bool first = true;
         for (int l = a.wlo / 64; l < W; ++l) {
             const int lo = l == a.wlo / 64 ? a.wlo % 64 : 0;
@@ -668,7 +673,7 @@ void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_ab
             c = enq_radix_sort(h, n_ptr, c, 64 - lo);
         }
     }
-    hipLaunchKernelGGL(k_clifford_gather<W>, dim3(grid_for(h, h->n_est, WG * 4)), dim3(WG), 0, h->stream, h->bufs,
+    hipLaunchKernelGGL(k_clifford_gather<W>, dim3(grid_for(h, h->cap, WG * 4)), dim3(WG), 0, h->stream, h->bufs,
                        h->d_ctl, p, h->d_rs_v[c], h->B);
     h->seq += 1;
 }
@@ -1250,7 +1255,7 @@ int load_merge_impl(pp_handle* h, int max_weight, double min_abs) {
     if (int rc = ensure_sort(h)) return rc;
     const int p = h->seq & 1;
     const uint32_t* n_ptr = &h->d_ctl->st[p].n;
-    const unsigned grid = grid_for(h, h->n_est, WG * 4);
+    const unsigned grid = grid_for(h, h->cap, WG * 4);
     int c = 0;
     for (int l = 0; l < W; ++l) {
         hipLaunchKernelGGL(k_rs_limb_keys, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, p, h->bufs, 1, 0, n_ptr, l,
@@ -1258,7 +1263,7 @@ int load_merge_impl(pp_handle* h, int max_weight, double min_abs) {
         c ^= 1;
         c = enq_radix_sort(h, n_ptr, c, 64);
     }
-    hipLaunchKernelGGL(k_reduce_runs<W>, dim3(lb_grid(h, (const void*)k_reduce_runs<W>, 0, h->n_est, KR_TILE)),
+    hipLaunchKernelGGL(k_reduce_runs<W>, dim3(lb_grid(h, (const void*)k_reduce_runs<W>, 0, h->cap, KR_TILE)),
                        dim3(WG), 0, h->stream, h->bufs, h->d_ctl, lb_launch(h, nullptr), h->d_rs_v[c], h->B,
                        max_weight, min_abs);
     HIP_TRY(hipGetLastError());
