@@ -200,6 +200,7 @@ typedef struct pp_profile {
     int64_t launches[PP_KERNEL_KINDS];
     double milliseconds[PP_KERNEL_KINDS];
     double algorithmic_bytes[PP_KERNEL_KINDS];
+    int64_t identity_gates;      /* rotations left out as touching no term (see pp_tune) */
 } pp_profile;
 
 /* Event timing: 0 off, 1 around the merge and noise kernels, 2 around every
@@ -216,8 +217,17 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always.
  * PP_TUNE_CHAIN_TAILS != 0 (the default) lets the merge of a rotation in
  * pp_run_circuit list the terms the next rotation touches, so that rotation
- * skips its own pass over the sum; 0 turns this off (results are identical). */
-enum pp_tune_key { PP_TUNE_SPIN_LIMIT = 1, PP_TUNE_DYNAMIC_TILES = 2, PP_TUNE_CHAIN_TAILS = 3 };
+ * skips its own pass over the sum; 0 turns this off (results are identical).
+ * PP_TUNE_SKIP_IDENTITY != 0 (the default) lets pp_run_circuit leave out the
+ * rotations that touch no term, found from which Paulis occur on each qubit
+ * (read from the device every 64 gates and widened gate by gate); they count
+ * in pp_profile.identity_gates. 0 turns this off (results are identical). */
+enum pp_tune_key {
+    PP_TUNE_SPIN_LIMIT = 1,
+    PP_TUNE_DYNAMIC_TILES = 2,
+    PP_TUNE_CHAIN_TAILS = 3,
+    PP_TUNE_SKIP_IDENTITY = 4
+};
 
 /* Set a tuning value. No counterpart in the reference. */
 int pp_tune(pp_handle *h, int key, int64_t value);

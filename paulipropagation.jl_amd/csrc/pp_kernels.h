@@ -299,38 +299,15 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
     }
 }
 
-// DAMP: amplitude damping on the qubit whose Z the mask holds (the rule of
-// src/Propagation/specializations.jl:258-276): the listed terms are those with
-// Z there (they branch into I with gamma * c), and any term with X, Y or Z
-// there sets st[p].run, since its coefficient changes.
-template <int W, bool DAMP>
-__global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
-    // every argument the early exits read, loaded with the control block
-#pragma unroll
-    for (int l = 0; l < W; ++l) asm volatile("" ::"s"(mask.l[l]));
-    asm volatile("" ::"s"(bf.cap), "s"(bf.rec_words), "s"(bf.rec_rows), "s"(bf.rec), "s"(L.glog));
-    const CtlView c = load_ctl(ctl, L.seq & 1);
-    if (!lb_start(ctl, L, c)) return;
-    const int p = L.seq & 1;
-    const uint32_t n = c.s.n, cur = c.s.cur;
+// Compaction of the terms that anticommute with `mask` (DAMP: that hold Z
+// where the mask does) into tail[0], in parent order, with SRC_NEG where the
+// created term's coefficient is negated; tiles of K1_TILE terms through the
+// look-back. fin(count) runs once, on the last tile, after every earlier
+// tile's count is in.
+template <int W, bool DAMP, typename Fin>
+__device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Launch& L, const Key<W>& mask, int p,
+                                            uint32_t n, uint32_t cur, Fin fin) {
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
-    if (n == 0) {
-        if (blockIdx.x == 0 && t == 0) {
-            ctl->st[p].na = 0u;
-            set_state(ctl, p ^ 1, 0u, cur);
-            if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
-            carry_records(ctl, L, p);
-        }
-        return;
-    }
-    // the touched terms may already be listed by the previous gate's merge
-    bool listed = !DAMP && c.tail_ready && c.tail_seq == L.seq;
-#pragma unroll
-    for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
-    if (listed) {
-        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s);
-        return;
-    }
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const size_t stride = bf.cap;
     uint32_t* __restrict__ tail = bf.tail[0];
@@ -402,7 +379,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
                 s_excl = excl;
-                if (tile == ntiles - 1) branch_finish(ctl, L, bf, p, n, cur, (uint32_t)(excl + agg), c.s);
+                if (tile == ntiles - 1) fin((uint32_t)(excl + agg));
             }
         }
         __syncthreads();
@@ -417,6 +394,43 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
             }
         }
     }
+}
+
+// DAMP: amplitude damping on the qubit whose Z the mask holds (the rule of
+// src/Propagation/specializations.jl:258-276): the listed terms are those with
+// Z there (they branch into I with gamma * c), and any term with X, Y or Z
+// there sets st[p].run, since its coefficient changes.
+template <int W, bool DAMP>
+__global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
+    // every argument the early exits read, loaded with the control block
+#pragma unroll
+    for (int l = 0; l < W; ++l) asm volatile("" ::"s"(mask.l[l]));
+    asm volatile("" ::"s"(bf.cap), "s"(bf.rec_words), "s"(bf.rec_rows), "s"(bf.rec), "s"(L.glog));
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
+    const int p = L.seq & 1;
+    const uint32_t n = c.s.n, cur = c.s.cur;
+    const int t = threadIdx.x;
+    if (n == 0) {
+        if (blockIdx.x == 0 && t == 0) {
+            ctl->st[p].na = 0u;
+            set_state(ctl, p ^ 1, 0u, cur);
+            if (L.glog) L.glog[0] = L.glog[1] = L.glog[2] = 0u;
+            carry_records(ctl, L, p);
+        }
+        return;
+    }
+    // the touched terms may already be listed, by the previous gate's merge
+    // or (when that gate touched nothing) by its permutation kernel
+    bool listed = !DAMP && c.tail_ready && c.tail_seq == L.seq;
+#pragma unroll
+    for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
+    if (listed) {
+        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s);
+        return;
+    }
+    branch_scan<W, DAMP>(bf, ctl, L, mask, p, n, cur,
+                         [&](uint32_t na) { branch_finish(ctl, L, bf, p, n, cur, na, c.s); });
 }
 
 // ============================================================ k_xorperm
@@ -512,8 +526,20 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
     }
 }
 
-template <int W>
-__global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask) {
+// The permutation of the touched terms into ascending order of term ^ mask
+// (rank formula above), the tail keys, coefficients and partition samples.
+//
+// BRANCH: the gate's k_branch was not launched, because the touched terms are
+// listed already (by the previous gate's merge, or by this kernel of the
+// previous gate); this kernel then does the branch step's end (room check,
+// state of an identity gate), on a look-back launch. If the gate touches
+// nothing and the next gate is a rotation (has_next, its mask nmask), the
+// kernel lists that rotation's touched terms instead, so that its k_branch
+// is not needed either. check: report (err bit 4) a list that is not this
+// gate's.
+template <int W, bool BRANCH>
+__global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask, Key<W> nmask,
+                                                uint32_t has_next, uint32_t check) {
     // the first tile's entries (tile = blockIdx.x) go out with the control
     // block; indices are clamped into the buffer and the values sanitized below
     uint32_t psrc[K2_ITEMS];
@@ -524,6 +550,45 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
     if (c.overflow) return;
     const int p = L.seq & 1;
     const uint32_t na = c.s.na;
+    if (BRANCH) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctl->counter[L.slot ^ 1u] = 0u;
+            if (check) {
+                bool listed = c.tail_ready && c.tail_seq == L.seq;
+#pragma unroll
+                for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
+                if (!listed) atomicOr(&ctl->err, 4u);
+            }
+        }
+        const uint32_t n = c.s.n, cur = c.s.cur;
+        if (na == 0 && has_next) {
+            // the identity: the state passes on, with the next rotation's list
+            auto fin = [&](uint32_t nt) {
+                set_state(ctl, p ^ 1, n, cur);
+                ctl->st[p ^ 1].na = nt;
+                ctl->st[p ^ 1].roff = c.s.roff;
+                ctl->st[p ^ 1].coff = c.s.coff;
+#pragma unroll
+                for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                ctl->tail_seq = L.seq + 1;
+                ctl->tail_ready = 1u;
+                if (L.glog) {
+                    L.glog[0] = n;
+                    L.glog[1] = 0u;
+                    L.glog[2] = n;
+                    L.glog[3] = c.s.roff;
+                    L.glog[4] = c.s.coff;
+                }
+            };
+            if (n == 0) {
+                if (blockIdx.x == 0 && threadIdx.x == 0) fin(0u);
+            } else {
+                branch_scan<W, false>(bf, ctl, L, nmask, p, n, cur, fin);
+            }
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) branch_finish(ctl, L, bf, p, n, cur, na, c.s);
+    }
     if (na == 0) return;
     const uint64_t* __restrict__ keys = bf.keys[c.s.cur];
     const double* __restrict__ coeffs = bf.coeffs[c.s.cur];
@@ -1621,6 +1686,72 @@ __global__ __launch_bounds__(WG) void k_overlap_final(const double* __restrict__
             DD r{0.0, 0.0};
             for (int w = 0; w < NWAVES; ++w) r = dd_add(r, DD{s_hi[w], s_lo[w]});
             out[b] = r.hi + r.lo;
+        }
+        __syncthreads();
+    }
+}
+
+// ============================================================ Pauli presence
+//
+// Which Paulis occur on each qubit anywhere in the sum: three bit planes in
+// the key's layout (bit 2q of limb word l set when some term holds X, Y or Z
+// on qubit q). A rotation whose generator anticommutes, qubit by qubit, with
+// nothing present on its qubits touches no term, so the host can leave it
+// out (it is the identity, xortailmerge.jl:121-123). partial is gridDim x 3W
+// words; the sum's state is st[seq & 1] (nothing is read after an overflow).
+template <int W>
+__global__ __launch_bounds__(WG) void k_presence(Bufs bf, const Ctl* ctl, uint32_t seq,
+                                                 uint64_t* __restrict__ partial) {
+    __shared__ uint64_t s_w[NWAVES][3 * W];
+    const int p = seq & 1;
+    const uint32_t n = ctl->overflow ? 0u : ctl->st[p].n;
+    const uint64_t* __restrict__ keys = bf.keys[ctl->st[p].cur];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    uint64_t acc[3 * W];
+#pragma unroll
+    for (int j = 0; j < 3 * W; ++j) acc[j] = 0;
+    for (uint32_t i = blockIdx.x * WG + t; i < n; i += gridDim.x * WG) {
+#pragma unroll
+        for (int l = 0; l < W; ++l) {
+            const uint64_t k = keys[(size_t)l * bf.cap + i];
+            const uint64_t lo = k & ALT, hi = (k >> 1) & ALT;
+            acc[l] |= lo & ~hi;       // X
+            acc[W + l] |= hi & ~lo;   // Y
+            acc[2 * W + l] |= lo & hi;  // Z
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3 * W; ++j) {
+        uint64_t v = acc[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
+        if (lane == 0) s_w[wave][j] = v;
+    }
+    __syncthreads();
+    if (t < 3 * W) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) v |= s_w[w][t];
+        partial[(size_t)blockIdx.x * 3 * W + t] = v;
+    }
+}
+
+// out[j] = OR of partial[b * words + j] over nblocks blocks; one workgroup.
+__global__ __launch_bounds__(WG) void k_presence_final(const uint64_t* __restrict__ partial, int nblocks, int words,
+                                                       uint64_t* __restrict__ out) {
+    __shared__ uint64_t s_w[NWAVES];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    for (int j = 0; j < words; ++j) {
+        uint64_t v = 0;
+        for (int b = t; b < nblocks; b += WG) v |= partial[(size_t)b * words + j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
+        if (lane == 0) s_w[wave] = v;
+        __syncthreads();
+        if (t == 0) {
+            uint64_t r = 0;
+            for (int w = 0; w < NWAVES; ++w) r |= s_w[w];
+            out[j] = r;
         }
         __syncthreads();
     }

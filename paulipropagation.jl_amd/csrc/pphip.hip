@@ -49,6 +49,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr size_t MAX_TERMS = 0x7fffffffu;  // indices carry a sign bit
 constexpr size_t RUN_CHUNK = 64;           // gates enqueued between overflow checks
+constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4 limbs
 
 }  // namespace
 
@@ -64,6 +65,9 @@ struct pp_handle {
     uint32_t spin_limit = 1u << 16;  // bound of one look-back wait with static tiles
     bool force_dynamic = false;      // take tiles from the atomic counter always
     bool chain_tails = true;         // merges list the next rotation's touched terms
+    bool prev_lists = false;         // the last gate enqueued lists the next rotation's touched terms
+    bool skip_identity = true;       // leave out rotations the Pauli presence shows touch nothing
+    int64_t identity_gates = 0;      // rotations left out that way
     uint32_t safe_until = 0;         // gates before this sequence number use dynamic tiles
     int64_t fallbacks = 0;           // gates run again with dynamic tiles
     Ctl* d_ctl = nullptr;
@@ -96,6 +100,8 @@ struct pp_handle {
     double* d_partial = nullptr;
     size_t partial_cap = 0;
     double* d_out = nullptr;
+    uint64_t* d_pres_part = nullptr;  // presence partials, gmax x 12 words, then the 12 result words
+    uint64_t* h_pres = nullptr;       // pinned copy of the result
     // profile
     int prof_mode = 0;             // 0 off, 1 merge/noise kernels, 2 every kernel
     std::vector<hipEvent_t> ev;    // 6 per gate slot
@@ -171,6 +177,7 @@ int live(pp_handle* h, uint32_t* n, uint32_t* cur) {
 // overflow record and counters.
 int reset_ctl(pp_handle* h, const GateState& st) {
     std::memset(h->h_ctl, 0, sizeof(Ctl));
+    h->prev_lists = false;
     GateState& s = h->h_ctl->st[h->seq & 1];
     s = GateState{};
     s.n = st.n;
@@ -457,9 +464,17 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode >= 1 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
-    launch(h, E(0, all), E(1, all), k_branch<W, false>,
-           lb_grid(h, (const void*)k_branch<W, false>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
-           lb_launch(h, glog), mask);
+    // Chained rotations (B = 1, not recording): every gate lists the next
+    // one's touched terms (its merge, or its permutation kernel when it touches
+    // nothing), so after the first gate of a chain k_branch is not launched and
+    // the permutation kernel ends the branch step.
+    const bool chained = h->B == 1 && !h->recording && h->chain_tails && h->prof_mode != 2;
+    const bool listed = chained && h->prev_lists;
+    h->prev_lists = false;
+    if (!listed)
+        launch(h, E(0, all), E(1, all), k_branch<W, false>,
+               lb_grid(h, (const void*)k_branch<W, false>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
+               lb_launch(h, glog), mask);
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
@@ -467,8 +482,14 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
                h->d_ctl, L2, h->B);
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->cap, K2_TILE), 0, h->bufs,
-           h->d_ctl, L2, lv, mask);
+    if (chained)
+        launch(h, E(2, all), nullptr, k_xorperm<W, true>,
+               lb_grid(h, (const void*)k_xorperm<W, true>, 0, h->cap, K2_TILE), 0, h->bufs, h->d_ctl,
+               lb_launch(h, glog), lv, mask, next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u,
+               listed ? 1u : 0u);
+    else
+        launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
+               h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
@@ -507,6 +528,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         }
 #undef LAUNCH_MB
     }
+    h->prev_lists = chained && next_mask;
     h->seq += 1;
 }
 
@@ -531,8 +553,8 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
     L2.seq = h->seq;
     L2.glog = glog;
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W>, grid_for(h, h->cap, K2_TILE), 0, h->bufs,
-           h->d_ctl, L2, lv, mask);
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
+           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{1 - gamma, gamma, max_weight, min_abs, qubit, fxy};
@@ -684,6 +706,7 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
     uint32_t* glog = h->d_log + LOG_WORDS * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
+    if (g.kind != PP_GATE_ROTATION) h->prev_lists = false;
     if (g.kind == PP_GATE_CLIFFORD) {
         if (int rc = ensure_sort(h)) return rc;
         switch (h->W) {
@@ -752,28 +775,158 @@ int grow_records(pp_handle* h, size_t words, size_t rows, size_t keep_words, siz
     return PP_OK;
 }
 
+// Which Paulis may occur on each qubit: X, Y and Z bit planes in the key's
+// layout (bit 2q of limb q / 32), exact when read from the device and widened
+// on the host gate by gate, so that they stay a superset until the next read.
+struct Presence {
+    uint64_t x[4], y[4], z[4];
+};
+
+// The X, Y and Z positions of a mask's qubits.
+void mask_planes(const uint64_t* m, int l, uint64_t* mx, uint64_t* my, uint64_t* mz) {
+    const uint64_t lo = m[l] & 0x5555555555555555ull, hi = (m[l] >> 1) & 0x5555555555555555ull;
+    *mx = lo & ~hi;
+    *my = hi & ~lo;
+    *mz = lo & hi;
+}
+
+// Whether some term may anticommute with the mask: only if some qubit of the
+// mask may hold one of the two Paulis that anticommute with the mask's there.
+// (A term anticommutes when an odd number of its qubits do; none means none.)
+bool may_touch(const Presence& P, const uint64_t* m, int W) {
+    uint64_t any = 0;
+    for (int l = 0; l < W; ++l) {
+        uint64_t mx, my, mz;
+        mask_planes(m, l, &mx, &my, &mz);
+        any |= ((P.y[l] | P.z[l]) & mx) | ((P.x[l] | P.z[l]) & my) | ((P.x[l] | P.y[l]) & mz);
+    }
+    return any != 0;
+}
+
+// After a rotation that may touch terms: a created term holds t * m on every
+// qubit of the mask (t the term's Pauli there, I included), so m and every
+// product with a present Pauli may occur.
+void widen_rotation(Presence& P, const uint64_t* m, int W) {
+    for (int l = 0; l < W; ++l) {
+        uint64_t mx, my, mz;
+        mask_planes(m, l, &mx, &my, &mz);
+        const uint64_t x = P.x[l], y = P.y[l], z = P.z[l];
+        P.x[l] = x | mx | (z & my) | (y & mz);  // Z.Y ~ X, Y.Z ~ X
+        P.y[l] = y | my | (z & mx) | (x & mz);  // Z.X ~ Y, X.Z ~ Y
+        P.z[l] = z | mz | (y & mx) | (x & my);  // Y.X ~ Z, X.Y ~ Z
+    }
+}
+
+// After a Clifford: anything may occur on its qubits.
+void widen_qubit(Presence& P, int q) {
+    const uint64_t b = 1ull << (2 * (q & 31));
+    P.x[q >> 5] |= b;
+    P.y[q >> 5] |= b;
+    P.z[q >> 5] |= b;
+}
+
+// Enqueue the presence of the sum's current state into h_pres (valid after
+// the next stream synchronization).
+int enq_presence(pp_handle* h) {
+    const unsigned grid = grid_for(h, h->cap, (size_t)WG * 16);
+    switch (h->W) {
+    case 1:
+        hipLaunchKernelGGL(k_presence<1>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
+                           h->d_pres_part);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_presence<2>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
+                           h->d_pres_part);
+        break;
+    default:
+        hipLaunchKernelGGL(k_presence<4>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
+                           h->d_pres_part);
+        break;
+    }
+    uint64_t* res = h->d_pres_part + (size_t)h->gmax * PRES_WORDS;
+    hipLaunchKernelGGL(k_presence_final, dim3(1), dim3(WG), 0, h->stream, h->d_pres_part, (int)grid, 3 * h->W, res);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(h->h_pres, res, 3 * h->W * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+    return PP_OK;
+}
+
+Presence read_presence(const pp_handle* h) {
+    Presence P{};
+    for (int l = 0; l < h->W; ++l) {
+        P.x[l] = h->h_pres[l];
+        P.y[l] = h->h_pres[h->W + l];
+        P.z[l] = h->h_pres[2 * h->W + l];
+    }
+    return P;
+}
+
 // Run resolved gates first .. last-1 (trig staged at d_trig for B > 1, slots
 // are gate indices), resuming after an overflow; then account the gate log
 // into the profile. counts[i - first] receives gate i's input size.
+//
+// Rotations that touch no term are the identity (xortailmerge.jl:121-123:
+// nothing branches, nothing is truncated). Unless recording, the host reads
+// which Paulis occur on each qubit at every synchronization and leaves out the
+// rotations of the next chunk that this presence, widened gate by gate, shows
+// to touch nothing; their log entries are filled in afterwards.
 int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, size_t last, int max_weight,
                  double min_abs, int64_t* counts) {
     const size_t ng = last;
     size_t g = first;
+    const bool skip = h->skip_identity && !h->recording && last - first > 1;
+    std::vector<uint8_t> skipped(last - first, 0);
+    std::vector<size_t> order;  // gates of the chunk that are enqueued
+    order.reserve(RUN_CHUNK);
+    bool pres_ok = false;
+    if (skip && first < last) {
+        if (int rc = enq_presence(h)) return rc;
+        if (int rc = sync_ctl(h)) return rc;
+        pres_ok = h->h_ctl->overflow == 0;
+    }
     while (g < ng) {
         const size_t end = std::min(ng, g + RUN_CHUNK);
         const uint32_t seq0 = h->seq;
-        for (size_t i = g; i < end; ++i) {
-            const bool chain = h->B == 1 && h->chain_tails && !h->recording && i + 1 < ng &&
-                               gs[i + 1].kind == PP_GATE_ROTATION;
-            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[i + 1].mask : nullptr)) return rc;
+        order.clear();
+        if (pres_ok) {
+            Presence P = read_presence(h);
+            for (size_t i = g; i < end; ++i) {
+                const HostGate& q = gs[i];
+                if (q.kind == PP_GATE_ROTATION) {
+                    if (!may_touch(P, q.mask, h->W)) {
+                        skipped[i - first] = 1;
+                        continue;
+                    }
+                    widen_rotation(P, q.mask, h->W);
+                } else if (q.kind == PP_GATE_CLIFFORD) {
+                    widen_qubit(P, q.qubit);
+                    if (q.nq == 2) widen_qubit(P, q.q1);
+                }
+                order.push_back(i);
+            }
+        } else {
+            for (size_t i = g; i < end; ++i) order.push_back(i);
         }
+        // a chunk starts a new chain: the last gate of the previous one listed
+        // the terms of the next gate of the circuit, which may have been left out
+        h->prev_lists = false;
+        for (size_t k = 0; k < order.size(); ++k) {
+            const size_t i = order[k];
+            const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
+            const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng &&
+                               gs[nx].kind == PP_GATE_ROTATION;
+            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr)) return rc;
+        }
+        if (skip && end < ng)
+            if (int rc = enq_presence(h)) return rc;
         if (int rc = sync_ctl(h)) return rc;
         if (h->h_ctl->err)
             return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
                                           std::to_string(h->h_ctl->err) + ")");
+        pres_ok = skip && h->h_ctl->overflow == 0;
         if (h->h_ctl->overflow) {
             const uint32_t fseq = h->h_ctl->overflow_seq;
-            const size_t gi = g + (size_t)(fseq - seq0);
+            const size_t gi = order[fseq - seq0];
+            for (size_t i = gi; i < end; ++i) skipped[i - first] = 0;
             h->seq = fseq;  // the state before the stopped gate is still in st[fseq & 1]
             if (h->h_ctl->overflow == 1) {
                 const size_t need = h->h_ctl->overflow_need;
@@ -804,6 +957,20 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
     HIP_TRY(hipMemcpyAsync(h->h_log + LOG_WORDS * first, h->d_log + LOG_WORDS * first,
                            LOG_WORDS * (last - first) * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    // a gate left out passes the sum on: its size is the next run gate's input
+    // size, or the final size
+    uint32_t n_after = h->h_ctl->st[h->seq & 1].n;
+    for (size_t i = last; i-- > first;) {
+        uint32_t* lg = h->h_log + LOG_WORDS * i;
+        if (skipped[i - first]) {
+            lg[0] = n_after;
+            lg[1] = 0;
+            lg[2] = n_after;
+            lg[3] = lg[4] = 0;
+        } else {
+            n_after = lg[0];
+        }
+    }
     const double row = 8.0 * h->W + 8.0 * h->B;
     for (size_t i = first; i < last; ++i) {
         const uint32_t nin = h->h_log[LOG_WORDS * i], na = h->h_log[LOG_WORDS * i + 1],
@@ -811,6 +978,10 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         if (counts) counts[i - first] = nin;
         h->gates += 1;
         h->term_apps += nin;
+        if (skipped[i - first]) {
+            h->identity_gates += 1;
+            continue;
+        }
         hipEvent_t* ev = h->prof_mode ? &h->ev[6 * i] : nullptr;
         auto span = [&](int a) -> double {
             float ms = 0.f;
@@ -1312,7 +1483,9 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
         return bail(fail(PP_ERR_HIP, "hipStreamCreate failed"));
     if (hipMalloc(&h->d_ctl, sizeof(Ctl)) != hipSuccess ||
         hipHostMalloc(&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess)
+        hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->d_pres_part, ((size_t)h->gmax + 1) * PRES_WORDS * sizeof(uint64_t)) != hipSuccess ||
+        hipHostMalloc(&h->h_pres, PRES_WORDS * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
         return bail(fail(PP_ERR_NOMEM, "device allocation failed"));
     if (int rc = reset_ctl(h, 0, 0)) return bail(rc);
     if (int rc = grow(h, std::max<size_t>(capacity, 1024))) return bail(rc);
@@ -1334,6 +1507,8 @@ int pp_destroy(pp_handle* h) {
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
     if (h->d_out) hipFree(h->d_out);
+    if (h->d_pres_part) hipFree(h->d_pres_part);
+    if (h->h_pres) hipHostFree(h->h_pres);
     if (h->d_tpj) hipFree(h->d_tpj);
     if (h->d_rec) hipFree(h->d_rec);
     if (h->d_rcoef) hipFree(h->d_rcoef);
@@ -1551,7 +1726,7 @@ int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (enable_timing >= 0) h->prof_mode = std::min(enable_timing, 2);
     if (reset) {
-        h->gates = h->term_apps = h->fallbacks = 0;
+        h->gates = h->term_apps = h->fallbacks = h->identity_gates = 0;
         for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
             h->launches[k] = 0;
             h->ms[k] = 0.0;
@@ -1574,6 +1749,9 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
     case PP_TUNE_CHAIN_TAILS:
         h->chain_tails = value != 0;
         return PP_OK;
+    case PP_TUNE_SKIP_IDENTITY:
+        h->skip_identity = value != 0;
+        return PP_OK;
     default:
         return fail(PP_ERR_ARG, "unknown tuning key");
     }
@@ -1584,6 +1762,7 @@ int pp_profile_read(const pp_handle* h, pp_profile* out) {
     out->gates = h->gates;
     out->fallbacks = h->fallbacks;
     out->term_applications = h->term_apps;
+    out->identity_gates = h->identity_gates;
     for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
         out->launches[k] = h->launches[k];
         out->milliseconds[k] = h->ms[k];

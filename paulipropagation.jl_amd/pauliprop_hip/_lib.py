@@ -32,7 +32,7 @@ KERNEL_KINDS = ("branch", "tail_permute", "merge", "noise", "overlap")
 class pp_profile(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_int64), ("term_applications", ctypes.c_int64), ("fallbacks", ctypes.c_int64),
                 ("launches", ctypes.c_int64 * 5), ("milliseconds", ctypes.c_double * 5),
-                ("algorithmic_bytes", ctypes.c_double * 5)]
+                ("algorithmic_bytes", ctypes.c_double * 5), ("identity_gates", ctypes.c_int64)]
 
 
 _lib = None
@@ -72,7 +72,7 @@ SIGNATURES = [
     ("pp_tune", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int64]),
 ]
 
-PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS = 1, 2, 3
+PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS, PP_TUNE_SKIP_IDENTITY = 1, 2, 3, 4
 
 
 def lib():

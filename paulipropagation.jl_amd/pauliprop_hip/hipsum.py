@@ -174,10 +174,14 @@ class HIPPauliSum:
         """timing: 1 records HIP events around every kernel, 0 stops, -1 leaves it."""
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
-    def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None) -> None:
+    def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None,
+             skip_identity: bool = None) -> None:
         """Bound of a look-back wait (polls); whether tiles are always claimed
-        atomically; whether a merge lists the next rotation's touched terms."""
+        atomically; whether a merge lists the next rotation's touched terms;
+        whether circuits leave out the rotations that touch no term."""
         L = _lib.lib()
+        if skip_identity is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SKIP_IDENTITY, int(bool(skip_identity))))
         if spin_limit is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SPIN_LIMIT, int(spin_limit)))
         if dynamic_tiles is not None:
@@ -192,6 +196,7 @@ class HIPPauliSum:
             "gates": p.gates,
             "term_applications": p.term_applications,
             "fallbacks": p.fallbacks,
+            "identity_gates": p.identity_gates,
             "kernels": {k: {"launches": p.launches[i], "ms": p.milliseconds[i], "bytes": p.algorithmic_bytes[i]}
                         for i, k in enumerate(_lib.KERNEL_KINDS)},
         }

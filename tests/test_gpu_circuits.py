@@ -147,6 +147,49 @@ def test_listed_touched_terms_match_own_pass(mode):
     assert_bitwise(*results[0], *ref.get(), f"chained vs oracle ({mode})")
 
 
+@pytest.mark.parametrize("B", [1, 2])
+def test_identity_rotations_left_out(B):
+    # a light cone: the observable sits on two qubits, so most early rotations
+    # touch nothing and are left out (per-qubit Pauli presence read every 64
+    # gates, widened over rotations, noise and Cliffords in between); the sum,
+    # every gate's input size and the oracle's must agree bit for bit
+    print(f"identity rotations left out, B={B}", flush=True)
+    nq = 60
+    circ = []
+    topo = pp.bricklayertopology(nq)
+    for layer in range(6):
+        pp.rxlayer(circ, nq)
+        pp.rzzlayer(circ, topo)
+        if layer == 2:
+            pp.depolarizinglayer(circ, nq, 0.01)
+        if layer == 3:
+            for q in range(20, 40, 2):
+                circ.append(pp.CliffordGate("CNOT", [q, q + 1]))
+            circ.append(pp.CliffordGate("H", [31]))
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(17).uniform(0.2, 1.3, size=(P, B))
+    obs = pp.PauliSum(nq)
+    obs.add(["Z"], [30])
+    obs.add(["Z", "Z"], [29, 31], 0.5)
+    thb = th if B > 1 else th[:, 0]
+    runs = []
+    for skip in (True, False):
+        dev = pp.HIPPauliSum.from_paulisum(obs, batch=B)
+        dev.tune(skip_identity=skip)
+        dev.profile_control(reset=True)
+        _, cnt = pp.propagate(circ, dev, thb, max_weight=7, min_abs_coeff=1e-6, counts=True)
+        runs.append((dev.download(), np.array(cnt), dev.profile()["identity_gates"]))
+    (k1, c1), n1, skipped = runs[0]
+    (k0, c0), n0, none = runs[1]
+    print(f"left out {skipped} of {len(circ)} gates", flush=True)
+    assert skipped > len(circ) // 4 and none == 0
+    assert_bitwise(k1, c1, k0, c0, f"left out vs all gates, B={B}")
+    assert np.array_equal(n1, n0)
+    ref, rcnt = oracle_propagate(circ, nq, thb, obs.terms, max_weight=7, min_abs_coeff=1e-6, batch=B)
+    assert_bitwise(k1, c1, *ref.get_sorted(), f"left out vs oracle, B={B}")
+    assert np.array_equal(n1, np.asarray(rcnt))
+
+
 @pytest.mark.parametrize("B", [1, 3])
 def test_amplitude_damping_circuit(B):
     print(f"amplitude damping circuit B={B}", flush=True)
