@@ -41,6 +41,8 @@ struct Bufs {
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
     double* tcoef;       // B = 1: the permuted tail's source coefficients (null for B > 1)
+    uint64_t* tl_keys;   // W planes of cap words: keys of the terms listed in tail[0], in list order
+    double* tl_coef;     // B = 1: their coefficients (null for B > 1)
     uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile
     uint64_t* gstates;   // look-back words: two chains of lb_groups words, one per 64 tiles
     uint32_t* splits;    // merge-path split of every merge tile boundary
@@ -92,7 +94,10 @@ struct Ctl {
     uint32_t tail_ready;
     uint32_t tail_seq;
     uint64_t tail_mask[4];
-    uint32_t pad2[2];
+    // ssamp holds every 64th entry of the input of gate ssamp_for - 1 (0: of
+    // no gate's): a merge writes them for the next gate as it writes its output
+    uint32_t ssamp_for;
+    uint32_t pad2;
 };
 static_assert(sizeof(Ctl) % 16 == 0, "the control block is read in 16-byte pieces");
 
@@ -105,6 +110,7 @@ struct CtlView {
     GateState s;  // st[p] of the launching gate's parity p
     uint32_t tail_ready, tail_seq;
     uint64_t tail_mask[4];
+    uint32_t ssamp_for;
 };
 
 __device__ __forceinline__ CtlView load_ctl(const Ctl* ctl, int p) {
@@ -126,12 +132,13 @@ __device__ __forceinline__ CtlView load_ctl(const Ctl* ctl, int p) {
     v.s.run = p ? c.st[1].run : c.st[0].run;
     v.tail_ready = c.tail_ready;
     v.tail_seq = c.tail_seq;
+    v.ssamp_for = c.ssamp_for;
 #pragma unroll
     for (int l = 0; l < 4; ++l) v.tail_mask[l] = c.tail_mask[l];
     // all fields are needed here, so their loads go out together instead of
     // being sunk past the first branch one round trip at a time
     asm volatile("" ::"s"(v.overflow), "s"(v.s.n), "s"(v.s.cur), "s"(v.s.na), "s"(v.s.roff), "s"(v.s.coff),
-                 "s"(v.s.run), "s"(v.tail_ready), "s"(v.tail_seq));
+                 "s"(v.s.run), "s"(v.tail_ready), "s"(v.tail_seq), "s"(v.ssamp_for));
     return v;
 }
 
@@ -269,9 +276,10 @@ constexpr int K1_ITEMS = 16;
 constexpr int K1_TILE = WG * K1_ITEMS;
 
 // End of the branch step, once the touched count na is known: check room for
-// the na created terms, or hand the sum on unchanged when nothing is touched.
+// the na created terms, or hand the sum on unchanged when nothing is touched
+// (with its samples, when they are this gate's: ssamp_for as read at start).
 __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const Bufs& bf, int p, uint32_t n,
-                                              uint32_t cur, uint32_t na, const GateState& sp) {
+                                              uint32_t cur, uint32_t na, const GateState& sp, uint32_t ssamp_for) {
     ctl->st[p].na = na;
     const uint32_t roff = sp.roff, coff = sp.coff;
     // a rotation touching nothing is the identity: no records
@@ -296,14 +304,15 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
         set_state(ctl, p ^ 1, n, cur);  // final when nothing anticommutes (or is damped)
         ctl->st[p ^ 1].roff = (uint32_t)(roff + words);
         ctl->st[p ^ 1].coff = (uint32_t)(coff + rows);
+        if (na == 0 && ssamp_for == L.seq + 1u) ctl->ssamp_for = L.seq + 2u;
     }
 }
 
 // Compaction of the terms that anticommute with `mask` (DAMP: that hold Z
 // where the mask does) into tail[0], in parent order, with SRC_NEG where the
-// created term's coefficient is negated; tiles of K1_TILE terms through the
-// look-back. fin(count) runs once, on the last tile, after every earlier
-// tile's count is in.
+// created term's coefficient is negated, and the terms themselves into
+// tl_keys / tl_coef; tiles of K1_TILE terms through the look-back. fin(count)
+// runs once, on the last tile, after every earlier tile's count is in.
 template <int W, bool DAMP, typename Fin>
 __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Launch& L, const Key<W>& mask, int p,
                                             uint32_t n, uint32_t cur, Fin fin) {
@@ -391,6 +400,10 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
                 const uint32_t i = base + k * WG + t;
                 const uint32_t pos = (uint32_t)excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);
                 tail[pos] = i | (((negbits >> k) & 1u) ? SRC_NEG : 0u);
+                // the listed term itself, for the permutation
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + pos] = keys[l * stride + i];
+                if (bf.tl_coef) bf.tl_coef[pos] = bf.coeffs[cur][i];
             }
         }
     }
@@ -426,11 +439,11 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
 #pragma unroll
     for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
     if (listed) {
-        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s);
+        if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s, c.ssamp_for);
         return;
     }
     branch_scan<W, DAMP>(bf, ctl, L, mask, p, n, cur,
-                         [&](uint32_t na) { branch_finish(ctl, L, bf, p, n, cur, na, c.s); });
+                         [&](uint32_t na) { branch_finish(ctl, L, bf, p, n, cur, na, c.s, c.ssamp_for); });
 }
 
 // ============================================================ k_xorperm
@@ -543,9 +556,13 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
     // the first tile's entries (tile = blockIdx.x) go out with the control
     // block; indices are clamped into the buffer and the values sanitized below
     uint32_t psrc[K2_ITEMS];
+    Key<W> pkey[K2_ITEMS];
 #pragma unroll
-    for (int i = 0; i < K2_ITEMS; ++i)
-        psrc[i] = bf.tail[0][min(blockIdx.x * (uint32_t)K2_TILE + threadIdx.x * K2_ITEMS + i, bf.cap - 1)];
+    for (int i = 0; i < K2_ITEMS; ++i) {
+        const uint32_t x = min(blockIdx.x * (uint32_t)K2_TILE + threadIdx.x * K2_ITEMS + i, bf.cap - 1);
+        psrc[i] = bf.tail[0][x];
+        pkey[i] = load_key_from<W>(bf.tl_keys, bf.cap, x, lv.from_limb);
+    }
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (c.overflow) return;
     const int p = L.seq & 1;
@@ -572,6 +589,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
                 ctl->tail_seq = L.seq + 1;
                 ctl->tail_ready = 1u;
+                if (c.ssamp_for == L.seq + 1u) ctl->ssamp_for = L.seq + 2u;
                 if (L.glog) {
                     L.glog[0] = n;
                     L.glog[1] = 0u;
@@ -587,12 +605,10 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             }
             return;
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) branch_finish(ctl, L, bf, p, n, cur, na, c.s);
+        if (blockIdx.x == 0 && threadIdx.x == 0) branch_finish(ctl, L, bf, p, n, cur, na, c.s, c.ssamp_for);
     }
     if (na == 0) return;
     const uint64_t* __restrict__ keys = bf.keys[c.s.cur];
-    const double* __restrict__ coeffs = bf.coeffs[c.s.cur];
-    const uint32_t nterms = c.s.n;
     const size_t stride = bf.cap;
     const uint32_t* __restrict__ tail_in = bf.tail[0];
     uint32_t* __restrict__ tail_out = bf.tail[1];
@@ -615,9 +631,9 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         }
     __syncthreads();
 
-    auto key_at = [&](uint32_t x) -> Key<W> {
-        return load_key_from<W>(keys, stride, tail_in[x] & SRC_MASK, from_limb);
-    };
+    // list entry x's key (the list holds the terms themselves in list order)
+    const uint64_t* __restrict__ lkeys = bf.tl_keys;
+    auto key_at = [&](uint32_t x) -> Key<W> { return load_key_from<W>(lkeys, stride, x, from_limb); };
 
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const uint32_t base = tile * (uint32_t)K2_TILE;
@@ -628,15 +644,19 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         const uint32_t j0 = base + t * K2_ITEMS;
         int nvalid = 0;
         if (tile == blockIdx.x) {
-            // prefetched; entries past the list are stale, so their indices are clamped to a term
+            // prefetched (entries past the list are stale and never written out)
 #pragma unroll
-            for (int i = 0; i < K2_ITEMS; ++i) src[i] = j0 + i <= last ? psrc[i] : min(psrc[i] & SRC_MASK, nterms - 1);
+            for (int i = 0; i < K2_ITEMS; ++i) {
+                src[i] = psrc[i];
+                key[i] = pkey[i];
+            }
         } else {
 #pragma unroll
-            for (int i = 0; i < K2_ITEMS; ++i) src[i] = tail_in[min(j0 + i, last)];
+            for (int i = 0; i < K2_ITEMS; ++i) {
+                src[i] = tail_in[min(j0 + i, last)];
+                key[i] = key_at(min(j0 + i, last));
+            }
         }
-#pragma unroll
-        for (int i = 0; i < K2_ITEMS; ++i) key[i] = load_key_from<W>(keys, stride, src[i] & SRC_MASK, from_limb);
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) nvalid += (j0 + i <= last) ? 1 : 0;
         // the last valid item, selected with constant indices (a variable index
@@ -737,10 +757,11 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         double cf[K2_ITEMS];
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) {
+            const uint32_t x = min(j0 + i, last);
 #pragma unroll
             for (int l = 0; l < W; ++l)
-                if (l < from_limb) key[i].l[l] = keys[l * stride + (src[i] & SRC_MASK)];
-            cf[i] = bf.tcoef ? coeffs[src[i] & SRC_MASK] : 0.0;
+                if (l < from_limb) key[i].l[l] = lkeys[l * stride + x];
+            cf[i] = bf.tcoef ? bf.tl_coef[x] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) {
@@ -763,7 +784,9 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         }
         __syncthreads();
     }
-    // every 64th entry of the sum, for the merge partition
+    // every 64th entry of the sum, for the merge partition, unless the merge
+    // that wrote the sum left them
+    if (c.ssamp_for == L.seq + 1u) return;
     const uint32_t n = c.s.n;
     for (uint32_t k = blockIdx.x * WG + t; 64ull * k < n; k += gridDim.x * WG) {
         const Key<W> kk = load_key<W>(keys, stride, 64ull * k);
@@ -1031,6 +1054,9 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint64_t s_key[W][TILE + 2];
     __shared__ double s_val[TILE + 2];
     __shared__ uint32_t s_pj[REC ? TILE + 2 : 1];
+    __shared__ uint16_t s_slot[TILE];
+    __shared__ uint32_t s_tl[EMIT ? TILE : 1];
+    __shared__ uint16_t s_tlp[EMIT ? TILE : 1];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl, s_excl_t;
@@ -1142,6 +1168,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                         for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
                         ctl->tail_seq = L.seq + 1;
                         ctl->tail_ready = 1u;
+                        ctl->ssamp_for = L.seq + 2u;
                         if (L.glog) L.glog[2] = nout;
                     }
                 }
@@ -1152,18 +1179,24 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                     if (tile == ntiles - 1) {
                         const uint32_t nout = (uint32_t)(excl + (agg & 0xffffu));
                         set_state(ctl, p ^ 1, nout, cur ^ 1);
+                        ctl->ssamp_for = L.seq + 2u;
                         if (L.glog) L.glog[2] = nout;
                     }
                 }
             }
         }
         __syncthreads();
-        uint64_t pos = s_excl + (pre & 0xffffu);
-        uint64_t tpos = EMIT ? s_excl_t + (pre >> 16) : 0;
+        // The kept outputs go to LDS in output order (their slots, and their
+        // values over s_val, whose staged inputs every thread has read by now),
+        // then out in consecutive addresses: a thread's own outputs are ITEMS
+        // apart in the wave, which would split every store over 3x the lines.
+        const uint32_t obase = (uint32_t)s_excl, tbase = EMIT ? (uint32_t)s_excl_t : 0u;
+        uint32_t lp = pre & 0xffffu, lt = pre >> 16;
 #pragma unroll
         for (int k = 0; k < ITEMS; ++k) {
+            const uint32_t pos = obase + lp;
             if (REC && who[k]) {
-                const uint32_t at = ((keep >> k) & 1u) ? (uint32_t)pos : REC_NONE;
+                const uint32_t at = ((keep >> k) & 1u) ? pos : REC_NONE;
                 const uint32_t hx = who[k] & 0xffffu, ty = who[k] >> 16;
                 if (hx) {
                     Key<W> hk;
@@ -1174,14 +1207,40 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                 if (ty) rcreated[s_pj[ty - 1]] = at;
             }
             if ((keep >> k) & 1u) {
-#pragma unroll
-                for (int l = 0; l < W; ++l) out_keys[l * stride + pos] = s_key[l][slot[k]];
-                out_coeffs[pos] = oc[k];
+                s_slot[lp] = (uint16_t)slot[k];
+                s_val[lp] = oc[k];
                 if (EMIT && ((touch >> k) & 1u)) {
-                    next_tail[tpos] = (uint32_t)pos | (((neg >> k) & 1u) ? SRC_NEG : 0u);
-                    ++tpos;
+                    s_tl[lt] = pos | (((neg >> k) & 1u) ? SRC_NEG : 0u);
+                    s_tlp[lt] = (uint16_t)lp;
+                    ++lt;
                 }
-                ++pos;
+                ++lp;
+            }
+        }
+        __syncthreads();
+        // every 64th output is also the next gate's partition sample
+        const uint32_t nkept = agg & 0xffffu;
+        for (uint32_t j = t; j < nkept; j += WG) {
+            const int sl = s_slot[j];
+            const uint32_t o = obase + j;
+#pragma unroll
+            for (int l = 0; l < W; ++l) out_keys[l * stride + o] = s_key[l][sl];
+            out_coeffs[o] = s_val[j];
+            if ((o & 63u) == 0) {
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.ssamp[(size_t)(o >> 6) * W + l] = s_key[l][sl];
+            }
+        }
+        if (EMIT) {
+            // the next rotation's list: positions, and the terms themselves
+            const uint32_t ntouch = agg >> 16;
+            for (uint32_t j = t; j < ntouch; j += WG) {
+                const int lpj = s_tlp[j];
+                const int sl = s_slot[lpj];
+                next_tail[tbase + j] = s_tl[j];
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + tbase + j] = s_key[l][sl];
+                bf.tl_coef[tbase + j] = s_val[lpj];
             }
         }
     }

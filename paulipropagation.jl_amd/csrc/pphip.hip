@@ -5,10 +5,13 @@
 //   coeffs[2]  cap x B doubles, row-major
 //   tail[2]    cap uint32 entries: index of an anticommuting term, sign in bit 31
 //   tkeys      W limb planes of cap words: permuted tail keys (term ^ mask)
+//   tl_keys    W limb planes of cap words: the terms listed in tail[0], in list
+//              order (tl_coef: their coefficients, B = 1)
 //   states     one look-back word per tile of the largest compaction launch
 //   gstates    one look-back word per group of 64 such tiles
 //   splits     merge-path split of every merge tile boundary
-//   ssamp      every 64th sum entry (W words each), written by the permutation
+//   ssamp      every 64th sum entry (W words each), written by the merge that
+//              made the sum, else by the permutation
 //   tsamp      permuted tail entries 63 modulo 64 (W words each)
 //   ctl        the live size and buffer of the sum, double-buffered by gate
 //              parity, plus the overflow record (pp_kernels.h)
@@ -126,6 +129,10 @@ void free_terms(pp_handle* h) {
     if (h->bufs.tkeys) hipFree(h->bufs.tkeys);
     if (h->bufs.tcoef) hipFree(h->bufs.tcoef);
     h->bufs.tcoef = nullptr;
+    if (h->bufs.tl_keys) hipFree(h->bufs.tl_keys);
+    if (h->bufs.tl_coef) hipFree(h->bufs.tl_coef);
+    h->bufs.tl_keys = nullptr;
+    h->bufs.tl_coef = nullptr;
     if (h->bufs.splits) hipFree(h->bufs.splits);
     if (h->bufs.gstates) hipFree(h->bufs.gstates);
     if (h->bufs.ssamp) hipFree(h->bufs.ssamp);
@@ -211,6 +218,8 @@ int grow(pp_handle* h, size_t newcap) {
     uint64_t* ns = nullptr;
     uint64_t* ntk = nullptr;
     double* ntc = nullptr;
+    uint64_t* nlk = nullptr;
+    double* nlc = nullptr;
     uint32_t* nsp = nullptr;
     uint64_t* ngs = nullptr;
     uint64_t* nss = nullptr;
@@ -228,6 +237,8 @@ int grow(pp_handle* h, size_t newcap) {
     ok = ok && hipMalloc(&ns, 2 * nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
     if (B == 1) ok = ok && hipMalloc(&ntc, newcap * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&nlk, newcap * W * sizeof(uint64_t)) == hipSuccess;
+    if (B == 1) ok = ok && hipMalloc(&nlc, newcap * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&nsp, nlb * sizeof(uint32_t)) == hipSuccess;
     ok = ok && hipMalloc(&ngs, 2 * ngw * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&nss, nsamp * sizeof(uint64_t)) == hipSuccess;
@@ -241,6 +252,8 @@ int grow(pp_handle* h, size_t newcap) {
         if (ns) hipFree(ns);
         if (ntk) hipFree(ntk);
         if (ntc) hipFree(ntc);
+        if (nlk) hipFree(nlk);
+        if (nlc) hipFree(nlc);
         if (nsp) hipFree(nsp);
         if (ngs) hipFree(ngs);
         if (nss) hipFree(nss);
@@ -267,6 +280,8 @@ int grow(pp_handle* h, size_t newcap) {
     h->bufs.states = ns;
     h->bufs.tkeys = ntk;
     h->bufs.tcoef = ntc;
+    h->bufs.tl_keys = nlk;
+    h->bufs.tl_coef = nlc;
     h->bufs.splits = nsp;
     h->bufs.gstates = ngs;
     h->bufs.lb_tiles = (uint32_t)nlb;
