@@ -247,6 +247,8 @@ __device__ __forceinline__ DD wave_sum_dd(DD v) {
 // only the words themselves are communicated.
 
 constexpr uint64_t LB_VAL_MASK = (1ull << 40) - 1;
+constexpr uint32_t LB_GROUP_SHIFT = 6;  // tiles per group: 1 << LB_GROUP_SHIFT (at most 64)
+constexpr uint32_t LB_GROUP = 1u << LB_GROUP_SHIFT;
 constexpr uint32_t LB_EPOCH_MASK = (1u << 22) - 1;
 
 __device__ __forceinline__ uint64_t lb_load(const uint64_t* p) {
@@ -336,9 +338,9 @@ __device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates
     const int lane = lane_id();
     const uint32_t ep32 = epoch & LB_EPOCH_MASK;
     const uint64_t ep = (uint64_t)ep32 << 40;
-    const uint32_t g = tile >> 6;
-    const int r = (int)(tile & 63u);
-    const bool last_of_group = r == 63;
+    const uint32_t g = tile >> LB_GROUP_SHIFT;
+    const int r = (int)(tile & (LB_GROUP - 1u));
+    const bool last_of_group = r == (int)LB_GROUP - 1;
 #pragma unroll
     for (int c = 0; c < NV; ++c) excl[c] = 0;
     if (tile == 0) {

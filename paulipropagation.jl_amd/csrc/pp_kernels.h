@@ -1795,24 +1795,33 @@ __global__ __launch_bounds__(WG) void k_presence(Bufs bf, const Ctl* ctl, uint32
     }
 }
 
-// out[j] = OR of partial[b * words + j] over nblocks blocks; one workgroup.
-__global__ __launch_bounds__(WG) void k_presence_final(const uint64_t* __restrict__ partial, int nblocks, int words,
+// out[j] = OR of partial[b * 3W + j] over nblocks blocks; one workgroup, every
+// block's words requested in one pass.
+template <int W>
+__global__ __launch_bounds__(WG) void k_presence_final(const uint64_t* __restrict__ partial, int nblocks,
                                                        uint64_t* __restrict__ out) {
-    __shared__ uint64_t s_w[NWAVES];
+    __shared__ uint64_t s_w[NWAVES][3 * W];
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
-    for (int j = 0; j < words; ++j) {
-        uint64_t v = 0;
-        for (int b = t; b < nblocks; b += WG) v |= partial[(size_t)b * words + j];
+    uint64_t acc[3 * W];
+#pragma unroll
+    for (int j = 0; j < 3 * W; ++j) acc[j] = 0;
+    for (int b = t; b < nblocks; b += WG) {
+#pragma unroll
+        for (int j = 0; j < 3 * W; ++j) acc[j] |= partial[(size_t)b * 3 * W + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 3 * W; ++j) {
+        uint64_t v = acc[j];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) v |= __shfl_xor(v, o, 64);
-        if (lane == 0) s_w[wave] = v;
-        __syncthreads();
-        if (t == 0) {
-            uint64_t r = 0;
-            for (int w = 0; w < NWAVES; ++w) r |= s_w[w];
-            out[j] = r;
-        }
-        __syncthreads();
+        if (lane == 0) s_w[wave][j] = v;
+    }
+    __syncthreads();
+    if (t < 3 * W) {
+        uint64_t v = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) v |= s_w[w][t];
+        out[t] = v;
     }
 }
 

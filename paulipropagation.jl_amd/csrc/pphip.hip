@@ -226,7 +226,7 @@ int grow(pp_handle* h, size_t newcap) {
     uint64_t* nts = nullptr;
     const size_t nsamp = (newcap / 64 + 2) * W;
     const size_t nlb = tiles_needed(newcap, B);
-    const size_t ngw = nlb / 64 + 2;
+    const size_t ngw = nlb / LB_GROUP + 2;
     // two look-back chains (merge offsets and the next gate's touched list)
     bool ok = true;
     for (int b = 0; b < 2 && ok; ++b) {
@@ -844,22 +844,24 @@ void widen_qubit(Presence& P, int q) {
 // the next stream synchronization).
 int enq_presence(pp_handle* h) {
     const unsigned grid = grid_for(h, h->cap, (size_t)WG * 16);
+    uint64_t* res = h->d_pres_part + (size_t)h->gmax * PRES_WORDS;
     switch (h->W) {
     case 1:
         hipLaunchKernelGGL(k_presence<1>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
                            h->d_pres_part);
+        hipLaunchKernelGGL(k_presence_final<1>, dim3(1), dim3(WG), 0, h->stream, h->d_pres_part, (int)grid, res);
         break;
     case 2:
         hipLaunchKernelGGL(k_presence<2>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
                            h->d_pres_part);
+        hipLaunchKernelGGL(k_presence_final<2>, dim3(1), dim3(WG), 0, h->stream, h->d_pres_part, (int)grid, res);
         break;
     default:
         hipLaunchKernelGGL(k_presence<4>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
                            h->d_pres_part);
+        hipLaunchKernelGGL(k_presence_final<4>, dim3(1), dim3(WG), 0, h->stream, h->d_pres_part, (int)grid, res);
         break;
     }
-    uint64_t* res = h->d_pres_part + (size_t)h->gmax * PRES_WORDS;
-    hipLaunchKernelGGL(k_presence_final, dim3(1), dim3(WG), 0, h->stream, h->d_pres_part, (int)grid, 3 * h->W, res);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(h->h_pres, res, 3 * h->W * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
     return PP_OK;
