@@ -557,11 +557,13 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
     // block; indices are clamped into the buffer and the values sanitized below
     uint32_t psrc[K2_ITEMS];
     Key<W> pkey[K2_ITEMS];
+    double pcf[K2_ITEMS];
 #pragma unroll
     for (int i = 0; i < K2_ITEMS; ++i) {
         const uint32_t x = min(blockIdx.x * (uint32_t)K2_TILE + threadIdx.x * K2_ITEMS + i, bf.cap - 1);
         psrc[i] = bf.tail[0][x];
-        pkey[i] = load_key_from<W>(bf.tl_keys, bf.cap, x, lv.from_limb);
+        pkey[i] = load_key<W>(bf.tl_keys, bf.cap, x);
+        pcf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
     }
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (c.overflow) return;
@@ -641,6 +643,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         const uint32_t last = base + cnt - 1;
         uint32_t src[K2_ITEMS];
         Key<W> key[K2_ITEMS];
+        double cf[K2_ITEMS];  // the source coefficients the merge will scale
         const uint32_t j0 = base + t * K2_ITEMS;
         int nvalid = 0;
         if (tile == blockIdx.x) {
@@ -649,12 +652,15 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             for (int i = 0; i < K2_ITEMS; ++i) {
                 src[i] = psrc[i];
                 key[i] = pkey[i];
+                cf[i] = pcf[i];
             }
         } else {
 #pragma unroll
             for (int i = 0; i < K2_ITEMS; ++i) {
-                src[i] = tail_in[min(j0 + i, last)];
-                key[i] = key_at(min(j0 + i, last));
+                const uint32_t x = min(j0 + i, last);
+                src[i] = tail_in[x];
+                key[i] = load_key<W>(lkeys, stride, x);
+                cf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
             }
         }
 #pragma unroll
@@ -751,17 +757,6 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 const int e = el[i] < INF_POS ? el[i] : (suf < INF_POS ? suf : ee);
                 dest[i] += (int64_t)sg * ((int64_t)s + (int64_t)e);
             }
-        }
-        // the limbs below from_limb were not needed for the ranks; load them now, all
-        // at once, with the source coefficients the merge will scale
-        double cf[K2_ITEMS];
-#pragma unroll
-        for (int i = 0; i < K2_ITEMS; ++i) {
-            const uint32_t x = min(j0 + i, last);
-#pragma unroll
-            for (int l = 0; l < W; ++l)
-                if (l < from_limb) key[i].l[l] = lkeys[l * stride + x];
-            cf[i] = bf.tcoef ? bf.tl_coef[x] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) {
