@@ -9,7 +9,11 @@ SplitMix64 seed 127001; observable Z63; max_weight 12, min_abs_coeff 1e-5.
 A step is one full propagation of that circuit from the observable, which is
 uploaded before the timed region; the sum stays in HBM throughout. `value` is
 sum over gates of the live terms entering each gate, divided by the wall time
-of the K timed steps.
+of the K timed steps. Rotations that touch no term are the identity; the
+library finds most of them from per-qubit Pauli presence and launches nothing
+for them (they still count with their input size). detail reports how many,
+and the time of one step with every rotation launched, whose result is
+compared bit for bit.
 
 roofline: the kernel with the most device time, its algorithmic bytes per
 launch ((K + 8B)(n_in + n_out) for a merge: one read and one write of the live
@@ -138,6 +142,21 @@ def main() -> int:
                    "GBps": round(v["bytes"] / v["ms"] / 1e6, 1) if v["ms"] > 0 else None}
                for k, v in prof["kernels"].items() if v["launches"]}
 
+    # Transparency about the rotations left out as touching no term: the same
+    # step with every rotation launched, timed once after the timed region, and
+    # its result compared bit for bit.
+    identity = int(prof.get("identity_gates", 0))
+    psum.tune(skip_identity=False)
+    t = time.perf_counter()
+    step()
+    ms_all = (time.perf_counter() - t) * 1e3
+    k_all, c_all = psum.download()
+    psum.tune(skip_identity=True)
+    step()
+    k_one, c_one = psum.download()
+    same_all = bool(np.array_equal(k_all, k_one) and np.array_equal(c_all.view(np.uint64), c_one.view(np.uint64)))
+    log(f"every rotation launched: {ms_all:.1f} ms per step, result bit-identical: {same_all}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pp, pc, psum, keys0, coeffs0, counts, wl, args.cpu_seconds)
@@ -167,6 +186,9 @@ def main() -> int:
         "detail": {"term_applications_per_step": apps // args.steps, "peak_terms": int(counts.max()),
                    "final_terms": final_terms, "overlapwithzero": overlap if B == 1 else list(overlap)[:4],
                    "gates_touching_terms": kernels.get("merge", {}).get("launches"), "kernels": kernels,
+                   "rotations_left_out_as_identity": identity,
+                   "ms_per_step_every_rotation_launched": round(ms_all, 3),
+                   "every_rotation_launched_bit_identical": same_all,
                    "layer_gates": L, "adjoint": adj},
     }
     print(json.dumps(out))

@@ -175,6 +175,37 @@ int pp_adjoint(pp_handle *h, const pp_gate *gates, size_t ngates, const double *
  * Replaces: overlapwithzero(::VectorPauliSum) (src/stateoverlap.jl:12-35). */
 int pp_overlap_zero(pp_handle *h, double *out_b);
 
+/* Product states of pp_overlap. */
+enum pp_overlap_kind {
+    PP_OVERLAP_ZERO = 0,          /* |0...0>: overlapwithzero (src/stateoverlap.jl:35) */
+    PP_OVERLAP_PLUS = 1,          /* |+...+>: overlapwithplus (src/stateoverlap.jl:44) */
+    PP_OVERLAP_COMPUTATIONAL = 2  /* a computational basis state: overlapwithcomputational (src/stateoverlap.jl:59-80) */
+};
+
+/* Overlap of the sum with a product state, for every column: the sum over
+ * terms of c times the term's expectation in the state (0 or +-1), in
+ * double-double. PP_OVERLAP_COMPUTATIONAL reads `ones` (key_words limbs, the
+ * key layout): bit 2q set for every qubit q (0-based) in state |1>; terms with
+ * X or Y vanish and the others count (-1)^(number of Z on those qubits).
+ * `ones` may be NULL for the other kinds.
+ * Replaces: overlapwithplus / overlapwithcomputational (src/stateoverlap.jl:44,59-80)
+ * and overlapwithzero (:35) on a VectorPauliSum. */
+int pp_overlap(pp_handle *h, int kind, const uint64_t *ones, double *out_b);
+
+/* The B coefficients of one Pauli string (key_words limbs), zeros when it is
+ * absent. overlapwithmaxmixed is pp_get_coeff of the identity (key 0).
+ * Replaces: getcoeff(::VectorPauliSum, trm) (src/Base/termsum.jl:104-124) and
+ * overlapwithmaxmixed (src/stateoverlap.jl:87-93). */
+int pp_get_coeff(pp_handle *h, const uint64_t *key, double *out_b);
+
+/* scalarproduct for every column: the sum, over Pauli strings present in both
+ * sums, of the products of their coefficients (Tr[psum1 psum2] / 2^n).
+ * Both handles must have the same qubit count and batch. The terms of h1 are
+ * looked up in h2; the call waits for both handles' work.
+ * Replaces: scalarproduct(psum1, psum2) (src/stateoverlap.jl:120-135);
+ * overlapwithpaulisum(rho, psum) is 2^n times it (:102-104). */
+int pp_scalar_product(pp_handle *h1, pp_handle *h2, double *out_b);
+
 /* Wait for all work on the handle's stream. */
 int pp_synchronize(pp_handle *h);
 

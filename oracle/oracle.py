@@ -47,6 +47,9 @@ def lib():
             ("ppo_clifford", None, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _U64P, _DP, ctypes.c_int,
                                     ctypes.c_double]),
             ("ppo_overlap_zero", None, [_P, _DP]),
+            ("ppo_overlap_kind", None, [_P, ctypes.c_int, _U64P, _DP]),
+            ("ppo_get_coeff", None, [_P, _U64P, _DP]),
+            ("ppo_scalar_product", ctypes.c_int, [_P, _P, _DP]),
             ("ppo_run", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
                                        ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
             ("ppo_adjoint", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, ctypes.c_int,
@@ -152,6 +155,25 @@ class OracleSum:
     def overlap_zero(self):
         out = np.zeros(self.B, dtype=np.float64)
         lib().ppo_overlap_zero(self._s, dp(out))
+        return out
+
+    def overlap_kind(self, kind: int, ones=None):
+        """kind 0 |0..0>, 1 |+..+>, 2 computational with `ones` (W limbs, bit 2q per qubit in |1>)."""
+        o = np.ascontiguousarray(ones if ones is not None else np.zeros(self.W), dtype=np.uint64)
+        out = np.zeros(self.B, dtype=np.float64)
+        lib().ppo_overlap_kind(self._s, int(kind), u64p(o), dp(out))
+        return out
+
+    def get_coeff(self, key_limbs):
+        k = np.ascontiguousarray(key_limbs, dtype=np.uint64)
+        out = np.zeros(self.B, dtype=np.float64)
+        lib().ppo_get_coeff(self._s, u64p(k), dp(out))
+        return out
+
+    def scalar_product(self, other: "OracleSum"):
+        out = np.zeros(self.B, dtype=np.float64)
+        if lib().ppo_scalar_product(self._s, other._s, dp(out)) != 0:
+            raise ValueError("scalar product of sums with different key widths or batches")
         return out
 
     def run(self, gates, ngates, theta, nparams, lut_changes, lut_signs, max_weight, min_abs):

@@ -521,6 +521,62 @@ void ppo_overlap_zero(const ppo_sum *s, double *out) {
             for (int b = 0; b < B; ++b) out[b] += s->coeffs[i * B + b];
 }
 
+/* Pauli code of qubit q (0-based): I 0, X 1, Y 2, Z 3 (src/PauliAlgebra/bitoperations.jl). */
+static unsigned code_of(const uint64_t *key, int q) { return (unsigned)(key[q >> 5] >> (2 * (q & 31))) & 3u; }
+
+/* The expectation of one Pauli string in a product state, qubit by qubit
+ * (src/stateoverlap.jl:12-80): kind 0 |0..0> (X or Y anywhere -> 0), kind 1
+ * |+..+> (Y or Z anywhere -> 0, src/stateoverlap.jl:44), kind 2 the basis
+ * state with qubit q in |1> when bit 2q of `ones` is set (X or Y -> 0, else
+ * (-1)^(number of Z on those qubits), _calcsignwithones :70-80). */
+static int state_factor(const uint64_t *key, int W, int kind, const uint64_t *ones) {
+    int f = 1;
+    for (int q = 0; q < 32 * W; ++q) {
+        unsigned c = code_of(key, q);
+        if (kind == 1) {
+            if (c == 2u || c == 3u) return 0;
+        } else {
+            if (c == 1u || c == 2u) return 0;
+            if (kind == 2 && c == 3u && ((ones[q >> 5] >> (2 * (q & 31))) & 1u)) f = -f;
+        }
+    }
+    return f;
+}
+
+void ppo_overlap_kind(const ppo_sum *s, int kind, const uint64_t *ones, double *out) {
+    int W = s->W, B = s->B;
+    for (int b = 0; b < B; ++b) out[b] = 0.0;
+    for (size_t i = 0; i < s->n; ++i) {
+        int f = state_factor(s->keys + i * W, W, kind, ones);
+        if (f)
+            for (int b = 0; b < B; ++b) out[b] += f * s->coeffs[i * B + b];
+    }
+}
+
+/* getcoeff (src/Base/termsum.jl:104-124): every entry equal to `key` is
+ * added (a linear scan also covers an unsorted sum). */
+void ppo_get_coeff(const ppo_sum *s, const uint64_t *key, double *out) {
+    int W = s->W, B = s->B;
+    for (int b = 0; b < B; ++b) out[b] = 0.0;
+    for (size_t i = 0; i < s->n; ++i)
+        if (memcmp(s->keys + i * W, key, W * sizeof(uint64_t)) == 0)
+            for (int b = 0; b < B; ++b) out[b] += s->coeffs[i * B + b];
+}
+
+/* scalarproduct (src/stateoverlap.jl:120-135): loop over b, look every term up in a. */
+int ppo_scalar_product(const ppo_sum *a, const ppo_sum *b, double *out) {
+    if (a->W != b->W || a->B != b->B) return -1;
+    int W = a->W, B = a->B;
+    double *c = xmalloc(B * sizeof(double));
+    for (int k = 0; k < B; ++k) out[k] = 0.0;
+    for (size_t i = 0; i < b->n; ++i) {
+        ppo_get_coeff(a, b->keys + i * W, c);
+        for (int k = 0; k < B; ++k) out[k] += c[k] * b->coeffs[i * B + k];
+    }
+    free(c);
+    return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* gate lists                                                          */
 /* ------------------------------------------------------------------ */

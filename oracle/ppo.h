@@ -63,6 +63,13 @@ void ppo_clifford(ppo_sum *s, int nq, const int *qubits, const uint64_t *changes
                   const double *signs, int max_weight, double min_abs);
 /* sum of coefficients over terms without X or Y, in array order, per column */
 void ppo_overlap_zero(const ppo_sum *s, double *out);
+/* overlap with a product state, per column: kind 0 |0..0>, 1 |+..+>, 2 the
+ * computational basis state with ones where bit 2q of `ones` is set */
+void ppo_overlap_kind(const ppo_sum *s, int kind, const uint64_t *ones, double *out);
+/* the coefficients of `key` (zeros when absent) */
+void ppo_get_coeff(const ppo_sum *s, const uint64_t *key, double *out);
+/* sum over common strings of coefficient products; -1 on mismatched W or B */
+int ppo_scalar_product(const ppo_sum *a, const ppo_sum *b, double *out);
 
 /* Run a Heisenberg-ordered gate list. theta is P x B row-major. counts, if not
  * NULL, receives n_in for every gate (ngates entries). */

@@ -1660,11 +1660,41 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
     }
 }
 
-// ============================================================ overlapwithzero
+// ============================================================ readouts
+//
+// Overlap with a product state (src/stateoverlap.jl:12-80): every term counts
+// with its coefficient times its expectation in the state, 0 or +-1.
+//   RD_ZERO   |0...0>: terms holding X or Y vanish (containsXorY), others count +1
+//   RD_PLUS   |+...+>: terms holding Y or Z vanish (containsYorZ), others count +1
+//   RD_COMP   a computational basis state: terms holding X or Y vanish, the
+//             others count (-1)^(number of Z on the qubits in state 1); `ones`
+//             has bit 2q set for every such qubit q (the key layout)
+constexpr int RD_ZERO = 0, RD_PLUS = 1, RD_COMP = 2;
 
-// per-block double-double partial sums of the coefficients of terms with no X or Y
+struct Readout {
+    int kind;
+    int pad;
+    uint64_t ones[4];
+};
+
 template <int W>
-__global__ __launch_bounds__(WG) void k_overlap1(Bufs bf, const Ctl* ctl, uint32_t seq, double* __restrict__ partial) {
+__device__ __forceinline__ int readout_factor(const Readout& r, const Key<W>& k) {
+    uint64_t orth = 0;
+    int nz = 0;
+#pragma unroll
+    for (int l = 0; l < W; ++l) {
+        const uint64_t lo = k.l[l] & ALT, hi = (k.l[l] >> 1) & ALT;  // X 01, Y 10, Z 11
+        orth |= r.kind == RD_PLUS ? hi : (lo ^ hi);
+        nz += __popcll(lo & hi & r.ones[l]);
+    }
+    if (orth) return 0;
+    return (r.kind == RD_COMP && (nz & 1)) ? -1 : 1;
+}
+
+// per-block double-double partial sums of the terms' contributions
+template <int W>
+__global__ __launch_bounds__(WG) void k_overlap1(Bufs bf, const Ctl* ctl, uint32_t seq, Readout rd,
+                                                 double* __restrict__ partial) {
     __shared__ double s_hi[NWAVES], s_lo[NWAVES];
     const int p = seq & 1;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
@@ -1673,8 +1703,8 @@ __global__ __launch_bounds__(WG) void k_overlap1(Bufs bf, const Ctl* ctl, uint32
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     DD acc{0.0, 0.0};
     for (uint32_t i = blockIdx.x * WG + t; i < n; i += gridDim.x * WG) {
-        const Key<W> kk = load_key<W>(keys, bf.cap, i);
-        if (!contains_xy<W>(kk)) acc = dd_add_d(acc, coeffs[i]);
+        const int f = readout_factor<W>(rd, load_key<W>(keys, bf.cap, i));
+        if (f) acc = dd_add_d(acc, f > 0 ? coeffs[i] : -coeffs[i]);
     }
     acc = wave_sum_dd(acc);
     if (lane == 0) {
@@ -1692,7 +1722,7 @@ __global__ __launch_bounds__(WG) void k_overlap1(Bufs bf, const Ctl* ctl, uint32
 
 // B columns: a wave per row, lanes over columns; partial is gridDim x B x 2
 template <int W>
-__global__ __launch_bounds__(WG) void k_overlapB(Bufs bf, const Ctl* ctl, uint32_t seq, int B,
+__global__ __launch_bounds__(WG) void k_overlapB(Bufs bf, const Ctl* ctl, uint32_t seq, Readout rd, int B,
                                                  double* __restrict__ partial) {
     extern __shared__ double s_dd[];  // NWAVES x B x 2
     const int p = seq & 1;
@@ -1704,8 +1734,81 @@ __global__ __launch_bounds__(WG) void k_overlapB(Bufs bf, const Ctl* ctl, uint32
         DD acc{0.0, 0.0};
         const int b = b0 + lane;
         for (uint32_t i = blockIdx.x * NWAVES + wave; i < n; i += gridDim.x * NWAVES) {
-            const Key<W> kk = load_key<W>(keys, bf.cap, i);
-            if (!contains_xy<W>(kk) && b < B) acc = dd_add_d(acc, coeffs[(size_t)i * B + b]);
+            const int f = readout_factor<W>(rd, load_key<W>(keys, bf.cap, i));
+            if (f && b < B) {
+                const double c = coeffs[(size_t)i * B + b];
+                acc = dd_add_d(acc, f > 0 ? c : -c);
+            }
+        }
+        if (b < B) {
+            s_dd[(wave * B + b) * 2] = acc.hi;
+            s_dd[(wave * B + b) * 2 + 1] = acc.lo;
+        }
+    }
+    __syncthreads();
+    for (int b = t; b < B; b += WG) {
+        DD r{0.0, 0.0};
+        for (int w = 0; w < NWAVES; ++w) r = dd_add(r, DD{s_dd[(w * B + b) * 2], s_dd[(w * B + b) * 2 + 1]});
+        partial[((size_t)blockIdx.x * B + b) * 2] = r.hi;
+        partial[((size_t)blockIdx.x * B + b) * 2 + 1] = r.lo;
+    }
+}
+
+// First position in the sorted sum whose term is not below `key` (n if none).
+template <int W>
+__device__ __forceinline__ uint32_t sum_lower_bound(const uint64_t* __restrict__ keys, size_t stride, uint32_t n,
+                                                    const Key<W>& key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (key_le<W>(key, load_key<W>(keys, stride, mid))) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// getcoeff (src/Base/termsum.jl:104-124 on a sorted, duplicate-free sum): the
+// B coefficients of `key`, or zeros when it is absent. One workgroup; the
+// search runs 64-ary in one wave.
+template <int W>
+__global__ __launch_bounds__(WG) void k_getcoeff(Bufs bf, const Ctl* ctl, uint32_t seq, Key<W> key, int B,
+                                                 double* __restrict__ out) {
+    __shared__ uint32_t s_pos;
+    const int p = seq & 1;
+    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint64_t* __restrict__ keys = bf.keys[cur];
+    if (threadIdx.x < 64) {
+        const uint32_t x = wave_first_true(0u, n, [&](uint32_t i) { return key_le<W>(key, load_key<W>(keys, bf.cap, i)); });
+        if (threadIdx.x == 0) s_pos = (x < n && key_eq<W>(load_key<W>(keys, bf.cap, x), key)) ? x : 0xffffffffu;
+    }
+    __syncthreads();
+    const uint32_t x = s_pos;
+    for (int b = threadIdx.x; b < B; b += WG) out[b] = x == 0xffffffffu ? 0.0 : bf.coeffs[cur][(size_t)x * B + b];
+}
+
+// scalarproduct (src/stateoverlap.jl:120-135): for every term of sum A, its
+// coefficients times those of the same term in sum B (binary search), summed
+// per column in double-double; partial is gridDim x B x 2. A wave per term of
+// A, lanes over columns.
+template <int W>
+__global__ __launch_bounds__(WG) void k_scalar(Bufs ba, const Ctl* ca, uint32_t sa, Bufs bb, const Ctl* cb,
+                                               uint32_t sb, int B, double* __restrict__ partial) {
+    extern __shared__ double s_dd[];  // NWAVES x B x 2
+    const uint32_t na = ca->st[sa & 1].n, cura = ca->st[sa & 1].cur;
+    const uint32_t nb = cb->st[sb & 1].n, curb = cb->st[sb & 1].cur;
+    const uint64_t* __restrict__ ka = ba.keys[cura];
+    const uint64_t* __restrict__ kb = bb.keys[curb];
+    const double* __restrict__ cfa = ba.coeffs[cura];
+    const double* __restrict__ cfb = bb.coeffs[curb];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    for (int b0 = 0; b0 < B; b0 += 64) {
+        DD acc{0.0, 0.0};
+        const int b = b0 + lane;
+        for (uint32_t i = blockIdx.x * NWAVES + wave; i < na; i += gridDim.x * NWAVES) {
+            const Key<W> k = load_key<W>(ka, ba.cap, i);
+            const uint32_t j = sum_lower_bound<W>(kb, bb.cap, nb, k);
+            if (j < nb && key_eq<W>(load_key<W>(kb, bb.cap, j), k) && b < B)
+                acc = dd_add_d(acc, cfa[(size_t)i * B + b] * cfb[(size_t)j * B + b]);
         }
         if (b < B) {
             s_dd[(wave * B + b) * 2] = acc.hi;

@@ -1037,8 +1037,20 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
     return PP_OK;
 }
 
+int ensure_partial(pp_handle* h, size_t need) {
+    if (need <= h->partial_cap) return PP_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->d_partial) hipFree(h->d_partial);
+    h->d_partial = nullptr;
+    h->partial_cap = 0;
+    HIP_TRY(hipMalloc(&h->d_partial, need * sizeof(double)));
+    h->partial_cap = need;
+    return PP_OK;
+}
+
+// Double-double partials of `rd`'s readout, finished into B doubles at out.
 template <int W>
-int overlap_impl(pp_handle* h, double* out) {
+int overlap_impl(pp_handle* h, double* out, const Readout& rd = Readout{}) {
     const int B = h->B;
     uint32_t n = 0, cur = 0;
     if (int rc = live(h, &n, &cur)) return rc;
@@ -1048,18 +1060,13 @@ int overlap_impl(pp_handle* h, double* out) {
     }
     const unsigned per = B == 1 ? WG : NWAVES;
     const unsigned g = std::min<unsigned>(2048u, (n + per - 1) / per);
-    const size_t need = (size_t)g * B * 2;
-    if (need > h->partial_cap) {
-        if (h->d_partial) hipFree(h->d_partial);
-        h->d_partial = nullptr;
-        HIP_TRY(hipMalloc(&h->d_partial, need * sizeof(double)));
-        h->partial_cap = need;
-    }
+    if (int rc = ensure_partial(h, (size_t)g * B * 2)) return rc;
     if (B == 1) {
-        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, h->d_partial);
+        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, rd,
+                           h->d_partial);
     } else {
         hipLaunchKernelGGL(k_overlapB<W>, dim3(g), dim3(WG), (size_t)NWAVES * B * 2 * sizeof(double), h->stream,
-                           h->bufs, h->d_ctl, h->seq, B, h->d_partial);
+                           h->bufs, h->d_ctl, h->seq, rd, B, h->d_partial);
     }
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, h->d_partial, (int)g, B, h->d_out);
@@ -1068,6 +1075,40 @@ int overlap_impl(pp_handle* h, double* out) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->launches[PP_KERNEL_OVERLAP] += 1;
     h->alg_bytes[PP_KERNEL_OVERLAP] += (8.0 * h->W + 8.0 * B) * (double)n;
+    return PP_OK;
+}
+
+template <int W>
+int getcoeff_impl(pp_handle* h, const uint64_t* key, double* out) {
+    if (int rc = sync_ctl(h)) return rc;
+    hipLaunchKernelGGL(k_getcoeff<W>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, to_key<W>(key),
+                       h->B, h->d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, h->d_out, h->B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return PP_OK;
+}
+
+// Terms of a are looked up in b; a's stream runs the kernels after b's is idle.
+template <int W>
+int scalar_impl(pp_handle* a, pp_handle* b, double* out) {
+    const int B = a->B;
+    if (int rc = sync_ctl(b)) return rc;
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(a, &n, &cur)) return rc;
+    if (n == 0) {
+        for (int c = 0; c < B; ++c) out[c] = 0.0;
+        return PP_OK;
+    }
+    const unsigned g = std::min<unsigned>(2048u, (n + NWAVES - 1) / NWAVES);
+    if (int rc = ensure_partial(a, (size_t)g * B * 2)) return rc;
+    hipLaunchKernelGGL(k_scalar<W>, dim3(g), dim3(WG), (size_t)NWAVES * B * 2 * sizeof(double), a->stream, a->bufs,
+                       a->d_ctl, a->seq, b->bufs, b->d_ctl, b->seq, B, a->d_partial);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, a->stream, a->d_partial, (int)g, B, a->d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, a->d_out, B * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
     return PP_OK;
 }
 
@@ -1730,6 +1771,51 @@ int pp_overlap_zero(pp_handle* h, double* out_b) {
     case 1: return overlap_impl<1>(h, out_b);
     case 2: return overlap_impl<2>(h, out_b);
     default: return overlap_impl<4>(h, out_b);
+    }
+}
+
+int pp_overlap(pp_handle* h, int kind, const uint64_t* ones, double* out_b) {
+    if (!h || !out_b) return fail(PP_ERR_ARG, "NULL argument");
+    if (kind != PP_OVERLAP_ZERO && kind != PP_OVERLAP_PLUS && kind != PP_OVERLAP_COMPUTATIONAL)
+        return fail(PP_ERR_ARG, "unknown overlap kind");
+    Readout rd{};
+    rd.kind = kind == PP_OVERLAP_PLUS ? RD_PLUS : (kind == PP_OVERLAP_COMPUTATIONAL ? RD_COMP : RD_ZERO);
+    if (kind == PP_OVERLAP_COMPUTATIONAL) {
+        if (!ones) return fail(PP_ERR_ARG, "computational overlap needs the one-bit mask");
+        for (int l = 0; l < h->W; ++l) {
+            if (ones[l] & ~0x5555555555555555ull)
+                return fail(PP_ERR_ARG, "one-bit mask: only bit 2q of a qubit q may be set");
+            rd.ones[l] = ones[l];
+        }
+        const int nq = h->nqubits;
+        for (int q = nq; q < 32 * h->W; ++q)
+            if ((rd.ones[q >> 5] >> (2 * (q & 31))) & 1ull) return fail(PP_ERR_ARG, "one-bit index out of range");
+    }
+    switch (h->W) {
+    case 1: return overlap_impl<1>(h, out_b, rd);
+    case 2: return overlap_impl<2>(h, out_b, rd);
+    default: return overlap_impl<4>(h, out_b, rd);
+    }
+}
+
+int pp_get_coeff(pp_handle* h, const uint64_t* key, double* out_b) {
+    if (!h || !key || !out_b) return fail(PP_ERR_ARG, "NULL argument");
+    switch (h->W) {
+    case 1: return getcoeff_impl<1>(h, key, out_b);
+    case 2: return getcoeff_impl<2>(h, key, out_b);
+    default: return getcoeff_impl<4>(h, key, out_b);
+    }
+}
+
+int pp_scalar_product(pp_handle* h1, pp_handle* h2, double* out_b) {
+    if (!h1 || !h2 || !out_b) return fail(PP_ERR_ARG, "NULL argument");
+    if (h1->nqubits != h2->nqubits || h1->W != h2->W)
+        return fail(PP_ERR_ARG, "scalar product of sums on different numbers of qubits");
+    if (h1->B != h2->B) return fail(PP_ERR_ARG, "scalar product of sums with different batch sizes");
+    switch (h1->W) {
+    case 1: return scalar_impl<1>(h1, h2, out_b);
+    case 2: return scalar_impl<2>(h1, h2, out_b);
+    default: return scalar_impl<4>(h1, h2, out_b);
     }
 }
 

@@ -11,7 +11,8 @@ from .circuits import (bricklayertopology, staircasetopology, rectangletopology,
                        ibmeagletopology, rxlayer, rylayer, rzlayer, rxxlayer, ryylayer, rzzlayer,
                        depolarizinglayer, tfitrottercircuit, tiltedtfitrottercircuit, heisenbergtrottercircuit,
                        hardwareefficientcircuit, efficientsu2circuit)
-from .hipsum import (HIPPauliSum, PackedCircuit, propagate, overlapwithzero, run_heisenberg, adjointgradient,
-                     rewindgradient)
+from .hipsum import (HIPPauliSum, PackedCircuit, propagate, overlapwithzero, overlapwithplus,
+                     overlapwithcomputational, overlapwithmaxmixed, overlapwithpaulisum, getcoeff, scalarproduct,
+                     run_heisenberg, adjointgradient, rewindgradient)
 
 __version__ = "0.1.0"

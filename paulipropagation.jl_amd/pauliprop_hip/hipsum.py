@@ -16,7 +16,7 @@ import numpy as np
 
 from . import _lib
 from .gates import countparameters, pack_circuit, toheisenberg
-from .paulis import PauliString, PauliSum, key_words, limbs_to_int
+from .paulis import PauliString, PauliSum, int_to_limbs, key_words, limbs_to_int
 
 _U64P = ctypes.POINTER(ctypes.c_uint64)
 _DP = ctypes.POINTER(ctypes.c_double)
@@ -166,6 +166,56 @@ class HIPPauliSum:
         out = np.zeros(self.batch, dtype=np.float64)
         _lib.check(_lib.lib().pp_overlap_zero(self._h, _dp(out)))
         return float(out[0]) if self.batch == 1 else out
+
+    def _column(self, out):
+        return float(out[0]) if self.batch == 1 else out
+
+    def overlapwithplus(self):
+        """<+|psum|+>: the coefficients of strings with no Y or Z (src/stateoverlap.jl:44)."""
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_overlap(self._h, _lib.PP_OVERLAP_PLUS, None, _dp(out)))
+        return self._column(out)
+
+    def overlapwithcomputational(self, onebitinds):
+        """<x|psum|x> for the basis state with ones at the 1-based qubits
+        `onebitinds` (src/stateoverlap.jl:59-80)."""
+        ones = 0
+        for q in onebitinds:
+            q = int(q)
+            if not 1 <= q <= self.nqubits:
+                raise ValueError(f"qubit index {q} out of range 1..{self.nqubits}")
+            ones |= 1 << (2 * (q - 1))
+        m = np.ascontiguousarray(int_to_limbs(ones, self.W), dtype=np.uint64)
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_overlap(self._h, _lib.PP_OVERLAP_COMPUTATIONAL, _u64p(m), _dp(out)))
+        return self._column(out)
+
+    def getcoeff(self, pstr):
+        """The coefficient(s) of a Pauli string (a PauliString or its integer
+        term), zero when absent (src/Base/termsum.jl:104-124)."""
+        term = pstr.term if isinstance(pstr, PauliString) else int(pstr)
+        if term < 0 or term >> (2 * self.nqubits):
+            raise ValueError("Pauli string acts beyond the sum's qubits")
+        k = np.ascontiguousarray(int_to_limbs(term, self.W), dtype=np.uint64)
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_get_coeff(self._h, _u64p(k), _dp(out)))
+        return self._column(out)
+
+    def overlapwithmaxmixed(self):
+        """Tr[psum I / 2^n]: the identity's coefficient (src/stateoverlap.jl:87-93)."""
+        return self.getcoeff(0)
+
+    def scalarproduct(self, other):
+        """Sum over strings present in both of the coefficient products,
+        Tr[psum1 psum2] / 2^n (src/stateoverlap.jl:120-135)."""
+        if isinstance(other, PauliString):
+            c = self.getcoeff(other)
+            return c * other.coeff
+        if not isinstance(other, HIPPauliSum):
+            raise TypeError("scalarproduct of a HIPPauliSum needs a HIPPauliSum or a PauliString")
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_scalar_product(self._h, other._h, _dp(out)))
+        return self._column(out)
 
     def synchronize(self) -> None:
         _lib.check(_lib.lib().pp_synchronize(self._h))
@@ -334,8 +384,47 @@ def overlapwithzero(psum):
     raise TypeError("overlapwithzero runs on a HIPPauliSum; upload the sum first")
 
 
+def overlapwithplus(psum):
+    """<+|psum|+> (src/stateoverlap.jl:44)."""
+    return _needs_device(psum, "overlapwithplus").overlapwithplus()
+
+
+def overlapwithcomputational(psum, onebitinds):
+    """<x|psum|x> with ones at `onebitinds` (src/stateoverlap.jl:59-80)."""
+    return _needs_device(psum, "overlapwithcomputational").overlapwithcomputational(onebitinds)
+
+
+def overlapwithmaxmixed(psum):
+    """Tr[psum I / 2^n] (src/stateoverlap.jl:87-93)."""
+    return _needs_device(psum, "overlapwithmaxmixed").overlapwithmaxmixed()
+
+
+def getcoeff(psum, pstr):
+    """Coefficient of a Pauli string in the sum (src/Base/termsum.jl:104-124)."""
+    return _needs_device(psum, "getcoeff").getcoeff(pstr)
+
+
+def scalarproduct(a, b):
+    """Tr[a b] / 2^n for HIPPauliSums (or one PauliString) (src/stateoverlap.jl:120-144)."""
+    if isinstance(a, PauliString) and isinstance(b, HIPPauliSum):
+        a, b = b, a
+    return _needs_device(a, "scalarproduct").scalarproduct(b)
+
+
+def overlapwithpaulisum(rho, psum):
+    """Tr[rho psum] = 2^n scalarproduct(rho, psum) (src/stateoverlap.jl:102-104)."""
+    return scalarproduct(rho, psum) * (2.0 ** _needs_device(rho, "overlapwithpaulisum").nqubits)
+
+
+def _needs_device(psum, name):
+    if isinstance(psum, HIPPauliSum):
+        return psum
+    raise TypeError(f"{name} runs on a HIPPauliSum; upload the sum first")
+
+
 def key_to_int(row) -> int:
     return limbs_to_int(row)
 
 
-__all__ = ["HIPPauliSum", "propagate", "overlapwithzero", "run_heisenberg", "key_words"]
+__all__ = ["HIPPauliSum", "propagate", "overlapwithzero", "overlapwithplus", "overlapwithcomputational",
+           "overlapwithmaxmixed", "overlapwithpaulisum", "getcoeff", "scalarproduct", "run_heisenberg", "key_words"]
