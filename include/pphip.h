@@ -48,7 +48,8 @@ enum pp_gate_kind {
     PP_GATE_ROTATION = 0,    /* PauliRotation: mask = generator */
     PP_GATE_PAULI_NOISE = 1, /* PauliNoise family: damp_codes on qubit */
     PP_GATE_AMP_DAMPING = 2, /* AmplitudeDampingNoise on qubit */
-    PP_GATE_CLIFFORD = 3     /* CliffordGate: table `lut` on qubits[0..nq) */
+    PP_GATE_CLIFFORD = 3,    /* CliffordGate: table `lut` on qubits[0..nq) */
+    PP_GATE_IMAG_ROTATION = 4 /* ImaginaryPauliRotation exp(-tau/2 P): mask, parameter tau, flags bit 0 */
 };
 
 /* One gate of a Heisenberg-ordered gate list. Parametrized gates read
@@ -63,6 +64,7 @@ typedef struct pp_gate {
     int32_t nq;           /* Clifford: number of qubits (1 or 2) */
     int32_t qubits[2];    /* Clifford qubits, 0-based; qubits[0] is the table's low code */
     int32_t lut;          /* Clifford: table slot */
+    int32_t flags;        /* ImaginaryPauliRotation: bit 0 = normalize by the identity's coefficient */
 } pp_gate;
 
 typedef struct pp_gate_stats {
@@ -117,6 +119,17 @@ int pp_download(pp_handle *h, uint64_t *keys, double *coeffs, size_t cap, size_t
  *           (src/Propagation/specializations.jl:53, 365-388). */
 int pp_apply_rotation(pp_handle *h, const uint64_t *mask, const double *cos_b, const double *sin_b,
                       int max_weight, double min_abs_coeff, pp_gate_stats *stats);
+
+/* ImaginaryPauliRotation step (Schrodinger picture): terms that commute with
+ * the generator keep c cosh and branch into (c sinh) sign at term ^ mask; the
+ * sum is merged; with `normalize` every coefficient is divided by (multiplied
+ * by 1 / ) the identity's coefficient; every term is then truncated.
+ * cosh_b / sinh_b are B host values of cosh(tau), sinh(tau). n_touched counts
+ * the commuting terms.
+ * Replaces: applymergetruncate!(::ImaginaryPauliRotation, cache, tau;
+ *           normalize_coeffs) (src/Propagation/specializations.jl:86-133). */
+int pp_apply_imaginary_rotation(pp_handle *h, const uint64_t *mask, const double *cosh_b, const double *sinh_b,
+                                int normalize, int max_weight, double min_abs_coeff, pp_gate_stats *stats);
 
 /* PauliNoise step: c *= factor_b[b] for the Pauli codes set in damp_codes
  * (bit v <=> code v) on `qubit`, then truncate every term.

@@ -41,6 +41,8 @@ def lib():
             ("ppo_sorted", ctypes.c_int, [_P]),
             ("ppo_get", None, [_P, _U64P, _DP]),
             ("ppo_rotation", ctypes.c_long, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_double]),
+            ("ppo_imaginary_rotation", ctypes.c_long, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_int,
+                                                       ctypes.c_double]),
             ("ppo_pauli_noise", None, [_P, ctypes.c_int, ctypes.c_uint, _DP, ctypes.c_int, ctypes.c_double]),
             ("ppo_amplitude_damping", ctypes.c_long, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                                       ctypes.c_double]),
@@ -145,6 +147,14 @@ class OracleSum:
         s = np.ascontiguousarray(np.broadcast_to(sinv, (self.B,)), dtype=np.float64)
         return lib().ppo_rotation(self._s, u64p(m), dp(c), dp(s), _mw(max_weight), float(min_abs))
 
+    def imaginary_rotation(self, mask_limbs, coshv, sinhv, normalize=True, max_weight=math.inf, min_abs=0.0):
+        m = np.zeros(4, dtype=np.uint64)
+        m[:len(mask_limbs)] = mask_limbs
+        c = np.ascontiguousarray(np.broadcast_to(coshv, (self.B,)), dtype=np.float64)
+        s = np.ascontiguousarray(np.broadcast_to(sinhv, (self.B,)), dtype=np.float64)
+        return lib().ppo_imaginary_rotation(self._s, u64p(m), dp(c), dp(s), int(bool(normalize)), _mw(max_weight),
+                                            float(min_abs))
+
     def pauli_noise(self, qubit0, damp_codes, factor, max_weight=math.inf, min_abs=0.0):
         f = np.ascontiguousarray(np.broadcast_to(factor, (self.B,)), dtype=np.float64)
         lib().ppo_pauli_noise(self._s, int(qubit0), int(damp_codes), dp(f), _mw(max_weight), float(min_abs))
@@ -191,22 +201,23 @@ class OracleSum:
 
 
 def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
-                     batch: int = 1, W: int | None = None):
-    """Propagate a Schrodinger-ordered circuit on the oracle; returns
-    (OracleSum, per-gate n_in counts). thetas: P or P x B."""
+                     batch: int = 1, W: int | None = None, heisenberg: bool = True, normalize_coeffs: bool = True):
+    """Propagate a circuit (given in Schrodinger order) on the oracle; returns
+    (OracleSum, per-gate n_in counts). thetas: P or P x B. heisenberg=False
+    propagates a state in circuit order with transposed gates."""
     import sys
     pkg = os.path.join(os.path.dirname(HERE), "paulipropagation.jl_amd")
     if pkg not in sys.path:
         sys.path.insert(0, pkg)
-    from pauliprop_hip.gates import countparameters, pack_circuit, toheisenberg
+    from pauliprop_hip.gates import countparameters, pack_circuit, toheisenberg, toschrodinger
     from pauliprop_hip.paulis import key_words, int_to_limbs
 
     W = W or key_words(nqubits)
     P = countparameters(circuit)
     th = np.zeros((0, batch)) if thetas is None else np.asarray(thetas, dtype=np.float64).reshape(P, -1)
-    hc, hth = toheisenberg(circuit, list(th))
+    hc, hth = (toheisenberg if heisenberg else toschrodinger)(circuit, list(th))
     hth = np.array(hth).reshape(P, -1) if P else np.zeros((0, batch))
-    gates, lc, ls, _ = pack_circuit(hc, nqubits, W)
+    gates, lc, ls, _ = pack_circuit(hc, nqubits, W, normalize_coeffs=normalize_coeffs)
     s = OracleSum(W, batch)
     keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
     coeffs = np.array([np.broadcast_to(c, (batch,)) for c in init_terms.values()], dtype=np.float64).reshape(-1, batch)

@@ -436,6 +436,61 @@ long ppo_rotation(ppo_sum *s, const uint64_t *mask, const double *cosv, const do
     }
 }
 
+/* ImaginaryPauliRotation (src/Propagation/specializations.jl:86-133): a term
+ * that COMMUTES with the generator keeps c * cosh and branches into
+ * (c * sinh) * sign at term ^ mask (_imaginaryrotationrule :120-131, the sign
+ * of paulirotationproduct); the sum is merged without truncation (xormerge!);
+ * with `normalize` every coefficient is multiplied by 1 / (the identity's
+ * coefficient after the merge), mult!(cache, 1 / getcoeff(sum, 0)) (:94-98);
+ * then every term is truncated (truncate!, :100). */
+long ppo_imaginary_rotation(ppo_sum *s, const uint64_t *mask, const double *coshv, const double *sinhv,
+                            int normalize, int max_weight, double min_abs) {
+    int W = s->W, B = s->B;
+    size_t n = s->n, nc = 0;
+    for (size_t i = 0; i < n; ++i) nc += !anticommutes_w(mask, s->keys + i * W, W);
+    uint64_t *tk = xmalloc((nc ? nc : 1) * W * sizeof(uint64_t));
+    double *tc = xmalloc((nc ? nc : 1) * B * sizeof(double));
+    size_t j = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        if (anticommutes_w(mask, k, W)) continue;
+        int e = sign_exponent_w(mask, k, W);
+        double sign = (double)((e & 2) - 1);
+        for (int l = 0; l < W; ++l) tk[j * W + l] = k[l] ^ mask[l];
+        double *c = s->coeffs + i * B;
+        for (int b = 0; b < B; ++b) {
+            tc[j * B + b] = (c[b] * sinhv[b]) * sign;
+            c[b] = c[b] * coshv[b];
+        }
+        ++j;
+    }
+    finish_branch(s, tk, tc, nc, mask, -1, 0.0);
+    free(tk);
+    free(tc);
+    if (normalize) {
+        uint64_t zero[4] = {0, 0, 0, 0};
+        double *c0 = xmalloc(B * sizeof(double));
+        ppo_get_coeff(s, zero, c0);
+        for (int b = 0; b < B; ++b) c0[b] = 1 / c0[b];
+        for (size_t i = 0; i < s->n; ++i)
+            for (int b = 0; b < B; ++b) s->coeffs[i * B + b] = s->coeffs[i * B + b] * c0[b];
+        free(c0);
+    }
+    size_t o = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        double *c = s->coeffs + i * B;
+        if (truncated(k, c, W, B, max_weight, min_abs)) continue;
+        if (o != i) {
+            memmove(s->keys + o * W, k, W * sizeof(uint64_t));
+            memmove(s->coeffs + o * B, c, B * sizeof(double));
+        }
+        ++o;
+    }
+    s->n = o;
+    return (long)nc;
+}
+
 void ppo_pauli_noise(ppo_sum *s, int qubit, unsigned damp_codes, const double *factor,
                      int max_weight, double min_abs) {
     int W = s->W, B = s->B;
@@ -596,6 +651,14 @@ static int apply_gate(ppo_sum *s, const ppo_gate *g, const double *theta, const 
             sv[b] = sin(t);
         }
         ppo_rotation(s, g->mask, cv, sv, max_weight, min_abs);
+        return 0;
+    case PPO_IMAG_ROTATION:
+        for (int b = 0; b < B; ++b) {
+            double t = gate_value(g, theta, B, b);
+            cv[b] = cosh(t);
+            sv[b] = sinh(t);
+        }
+        ppo_imaginary_rotation(s, g->mask, cv, sv, g->flags & 1, max_weight, min_abs);
         return 0;
     case PPO_PAULI_NOISE:
         for (int b = 0; b < B; ++b) cv[b] = 1 - gate_value(g, theta, B, b);

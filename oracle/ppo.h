@@ -23,7 +23,7 @@ extern "C" {
 typedef struct ppo_sum ppo_sum;
 
 /* gate kinds of a gate list (same numbering as include/pphip.h) */
-enum { PPO_ROTATION = 0, PPO_PAULI_NOISE = 1, PPO_AMP_DAMPING = 2, PPO_CLIFFORD = 3 };
+enum { PPO_ROTATION = 0, PPO_PAULI_NOISE = 1, PPO_AMP_DAMPING = 2, PPO_CLIFFORD = 3, PPO_IMAG_ROTATION = 4 };
 
 typedef struct ppo_gate {
     uint64_t mask[4];     /* rotation generator / unused */
@@ -35,6 +35,7 @@ typedef struct ppo_gate {
     int32_t nq;           /* Clifford: 1 or 2 qubits */
     int32_t qubits[2];    /* Clifford qubits (0-based) */
     int32_t lut;          /* Clifford: index of its table */
+    int32_t flags;        /* ImaginaryPauliRotation: bit 0 = normalize by the identity's coefficient */
 } ppo_gate;
 
 ppo_sum *ppo_create(int W, int B);
@@ -52,6 +53,11 @@ void ppo_get(const ppo_sum *s, uint64_t *keys, double *coeffs);
  * anticommuting terms (0 = nothing touched, nothing truncated). */
 long ppo_rotation(ppo_sum *s, const uint64_t *mask, const double *cosv, const double *sinv,
                   int max_weight, double min_abs);
+/* ImaginaryPauliRotation exp(-tau/2 P) . exp(-tau/2 P): terms commuting with P
+ * branch into cosh and sinh parts; merge; optionally divide by the identity's
+ * coefficient; truncate all. Returns the number of branched terms. */
+long ppo_imaginary_rotation(ppo_sum *s, const uint64_t *mask, const double *coshv, const double *sinhv,
+                            int normalize, int max_weight, double min_abs);
 /* PauliNoise: damp the codes in damp_codes on `qubit` by factor[b]; truncate all. */
 void ppo_pauli_noise(ppo_sum *s, int qubit, unsigned damp_codes, const double *factor,
                      int max_weight, double min_abs);

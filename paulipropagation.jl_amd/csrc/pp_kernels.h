@@ -309,11 +309,11 @@ __device__ __forceinline__ void branch_finish(Ctl* ctl, const Launch& L, const B
 }
 
 // Compaction of the terms that anticommute with `mask` (DAMP: that hold Z
-// where the mask does) into tail[0], in parent order, with SRC_NEG where the
+// where the mask does; COMM: that commute with it) into tail[0], in parent order, with SRC_NEG where the
 // created term's coefficient is negated, and the terms themselves into
 // tl_keys / tl_coef; tiles of K1_TILE terms through the look-back. fin(count)
 // runs once, on the last tile, after every earlier tile's count is in.
-template <int W, bool DAMP, typename Fin>
+template <int W, bool DAMP, bool COMM, typename Fin>
 __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Launch& L, const Key<W>& mask, int p,
                                             uint32_t n, uint32_t cur, Fin fin) {
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
@@ -365,7 +365,8 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
 #pragma unroll
         for (int k = 0; k < K1_ITEMS; ++k) {
             const uint32_t i = base + k * WG + t;
-            const bool f = (i < n) && ((antibits >> k) & 1u);
+            // COMM (imaginary rotations): the commuting terms branch
+            const bool f = (i < n) && (((antibits >> k) & 1u) != (COMM ? 1u : 0u));
             if (!DAMP && f && (ex[k] & 2) == 0) negbits |= 1u << k;
             ballots[k] = __ballot(f);
             if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(ballots[k]);
@@ -413,7 +414,11 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
 // src/Propagation/specializations.jl:258-276): the listed terms are those with
 // Z there (they branch into I with gamma * c), and any term with X, Y or Z
 // there sets st[p].run, since its coefficient changes.
-template <int W, bool DAMP>
+//
+// COMM: an imaginary rotation (src/Propagation/specializations.jl:111-131):
+// the terms that commute with the generator are listed, and st[p].run is set
+// so that the merge runs (it normalizes and truncates) even when none does.
+template <int W, bool DAMP, bool COMM = false>
 __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<W> mask) {
     // every argument the early exits read, loaded with the control block
 #pragma unroll
@@ -433,17 +438,18 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
         }
         return;
     }
+    if (COMM && blockIdx.x == 0 && t == 0) ctl->st[p].run = 1u;
     // the touched terms may already be listed, by the previous gate's merge
     // or (when that gate touched nothing) by its permutation kernel
-    bool listed = !DAMP && c.tail_ready && c.tail_seq == L.seq;
+    bool listed = !DAMP && !COMM && c.tail_ready && c.tail_seq == L.seq;
 #pragma unroll
     for (int l = 0; l < W; ++l) listed = listed && c.tail_mask[l] == mask.l[l];
     if (listed) {
         if (blockIdx.x == 0 && t == 0) branch_finish(ctl, L, bf, p, n, cur, c.s.na, c.s, c.ssamp_for);
         return;
     }
-    branch_scan<W, DAMP>(bf, ctl, L, mask, p, n, cur,
-                         [&](uint32_t na) { branch_finish(ctl, L, bf, p, n, cur, na, c.s, c.ssamp_for); });
+    branch_scan<W, DAMP, COMM>(bf, ctl, L, mask, p, n, cur,
+                               [&](uint32_t na) { branch_finish(ctl, L, bf, p, n, cur, na, c.s, c.ssamp_for); });
 }
 
 // ============================================================ k_xorperm
@@ -603,7 +609,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             if (n == 0) {
                 if (blockIdx.x == 0 && threadIdx.x == 0) fin(0u);
             } else {
-                branch_scan<W, false>(bf, ctl, L, nmask, p, n, cur, fin);
+                branch_scan<W, false, false>(bf, ctl, L, nmask, p, n, cur, fin);
             }
             return;
         }
@@ -826,6 +832,11 @@ struct RotParams {
     double min_abs;
     int damp_q = -1;
     double fxy = 1.0;
+    // imaginary rotation: commuting heads take cosv (cosh), and every output
+    // is multiplied by *norm (1 / the identity's merged coefficient) when
+    // norm is set, before it is truncated
+    int comm = 0;
+    const double* norm = nullptr;
 };
 
 // number of head entries among the first d merged entries; heads go first on ties
@@ -966,7 +977,7 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
                                             uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
                                             double sinv, uint64_t (*s_key)[TILE + 2], double* s_val,
                                             const uint32_t* __restrict__ tpj, uint32_t* s_pj, int damp_q,
-                                            double fxy, const double* __restrict__ tcoef) {
+                                            double fxy, const double* __restrict__ tcoef, int comm = 0) {
     const int t = threadIdx.x;
     constexpr int R = (TILE + 1 + WG - 1) / WG;
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
@@ -988,7 +999,7 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
                 const uint32_t code = code_at<W>(k, damp_q);
                 v = code == 3u ? c * cosv : (code ? c * fxy : c);
             } else {
-                v = anticommutes<W>(mask, k) ? c * cosv : c;
+                v = (anticommutes<W>(mask, k) != (comm != 0)) ? c * cosv : c;
             }
             s_val[x] = v;
         }
@@ -1043,6 +1054,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     const double* __restrict__ coeffs = bf.coeffs[cur];
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
     double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
+    const double inv = rp.norm ? *rp.norm : 1.0;
     const size_t stride = bf.cap;
     const uint32_t ntot = n + na;
     const uint32_t ntiles = (ntot + TILE - 1) / TILE;
@@ -1076,7 +1088,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             nt = 0;
         }
         merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
-                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef);
+                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef,
+                             rp.comm);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;
@@ -1134,6 +1147,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                     if (REC && live) who[k] = (uint32_t)(y + 1) << 16;
                     ++y;
                 }
+                if (rp.norm) oc[k] = oc[k] * inv;
                 if (live && !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs)) {
                     keep |= 1u << k;
                     if (EMIT && anticommutes<W>(nmask, ko)) {
@@ -1250,7 +1264,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
 template <int W, int TB>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
-                                               int damp_q, double fxy) {
+                                               int damp_q, double fxy, int comm, const double* __restrict__ norm) {
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
@@ -1347,7 +1361,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     const uint32_t code = code_at<W>(okey, damp_q);
                     if (code == 3u) head = -(head + 2);
                     xy = code == 1u || code == 2u;
-                } else if (anticommutes<W>(mask, okey)) {
+                } else if (anticommutes<W>(mask, okey) != (comm != 0)) {
                     head = -(head + 2);
                     rh |= REC_FLAG;
                 }
@@ -1397,6 +1411,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     if (tl & SRC_NEG) tc = -tc;
                     c = have ? c + tc : tc;
                 }
+                if (norm) c = c * norm[b];
                 s_rows[(size_t)e * B + b] = c;
                 m = fmax(m, fabs(c));
             }
@@ -1657,6 +1672,59 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
                 out_coeffs[pos * B + b] = c;
             }
         }
+    }
+}
+
+// ============================================================ imaginary rotation normalization
+
+// The identity's coefficient after an imaginary rotation's merge, per column,
+// computed as the merge computes it: the identity head (it commutes with every
+// generator) times cosh, plus the term created from the generator's own
+// string (P P = I): (c sinh), negated for a negative sign. out[b] = 1 / that,
+// the reference's mult!(cache, 1 / getcoeff(sum, 0))
+// (src/Propagation/specializations.jl:94-98). One workgroup; trig holds cosh_b
+// then sinh_b for B > 1.
+template <int W>
+__global__ __launch_bounds__(WG) void k_imag_norm(Bufs bf, const Ctl* ctl, uint32_t seq, Key<W> mask, int B,
+                                                  double cosh1, double sinh1, const double* __restrict__ trig,
+                                                  double* __restrict__ out) {
+    __shared__ uint32_t s_p;
+    if (ctl->overflow) return;
+    const int p = seq & 1;
+    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint64_t* __restrict__ keys = bf.keys[cur];
+    const double* __restrict__ coeffs = bf.coeffs[cur];
+    if (threadIdx.x < 64) {
+        const uint32_t x =
+            wave_first_true(0u, n, [&](uint32_t i) { return key_le<W>(mask, load_key<W>(keys, bf.cap, i)); });
+        if (threadIdx.x == 0) s_p = (x < n && key_eq<W>(load_key<W>(keys, bf.cap, x), mask)) ? x : 0xffffffffu;
+    }
+    __syncthreads();
+    const uint32_t xp = s_p;
+    // the identity is the smallest key: present when entry 0 is zero
+    bool has_i = false;
+    if (n > 0) {
+        const Key<W> k0 = load_key<W>(keys, bf.cap, 0);
+        uint64_t any = 0;
+#pragma unroll
+        for (int l = 0; l < W; ++l) any |= k0.l[l];
+        has_i = any == 0;
+    }
+    const bool neg = (sign_exponent<W>(mask, mask) & 2) == 0;
+    for (int b = threadIdx.x; b < B; b += WG) {
+        const double ch = B > 1 ? trig[b] : cosh1, sh = B > 1 ? trig[B + b] : sinh1;
+        double c = 0.0;
+        bool have = false;
+        if (has_i) {
+            c = coeffs[b] * ch;
+            have = true;
+        }
+        if (xp != 0xffffffffu) {
+            double tc = coeffs[(size_t)xp * B + b] * sh;
+            if (neg) tc = -tc;
+            c = have ? c + tc : tc;
+        }
+        out[b] = 1.0 / c;
     }
 }
 

@@ -103,6 +103,7 @@ struct pp_handle {
     double* d_partial = nullptr;
     size_t partial_cap = 0;
     double* d_out = nullptr;
+    double* d_norm = nullptr;         // B doubles: 1 / the identity's coefficient (imaginary rotations)
     uint64_t* d_pres_part = nullptr;  // presence partials, gmax x 12 words, then the 12 result words
     uint64_t* h_pres = nullptr;       // pinned copy of the result
     // profile
@@ -533,7 +534,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 0, nullptr)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -544,6 +545,65 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
 #undef LAUNCH_MB
     }
     h->prev_lists = chained && next_mask;
+    h->seq += 1;
+}
+
+// ImaginaryPauliRotation (src/Propagation/specializations.jl:86-133): the
+// rotation's kernels with the commuting terms listed, cosh / sinh values,
+// and (normalize) every output divided by the identity's coefficient before it
+// is truncated. Never chained: the next rotation lists its own terms.
+template <int W>
+void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double sh, const double* d_trig,
+                       bool normalize, int max_weight, double min_abs, uint32_t* glog, hipEvent_t* ev) {
+    const Key<W> mask = to_key<W>(mask_in);
+    Levels lv;
+    mask_levels(mask_in, W, &lv);
+    const bool all = h->prof_mode == 2 && ev;
+    const bool mrg = h->prof_mode >= 1 && ev;
+    auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
+    launch(h, E(0, all), E(1, all), k_branch<W, false, true>,
+           lb_grid(h, (const void*)k_branch<W, false, true>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
+           lb_launch(h, glog), mask);
+    Launch L2{};
+    L2.seq = h->seq;
+    L2.glog = glog;
+    const bool part = h->B == 1;
+    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
+           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
+    if (normalize)
+        hipLaunchKernelGGL(k_imag_norm<W>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, mask, h->B,
+                           ch, sh, d_trig, h->d_norm);
+    const double* norm = normalize ? h->d_norm : nullptr;
+    const size_t expect = h->cap + h->cap / 2;
+    if (part) {
+        RotParams rp{ch, sh, max_weight, min_abs};
+        rp.comm = 1;
+        rp.norm = norm;
+        constexpr int TILE = K3Cfg<W>::TILE;
+        Launch LP{};
+        LP.seq = h->seq;
+        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
+               h->d_ctl, LP);
+        launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
+               lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+               lb_launch(h, glog), mask, rp, mask);
+    } else {
+        const int B = h->B;
+        const int TB = tile_rows(B);
+        const size_t shm = (size_t)TB * B * sizeof(double);
+        const Launch L3 = lb_launch(h, glog);
+#define LAUNCH_MB(TBV)                                                                                         \
+    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 1, norm)
+        switch (TB) {
+        case 256: LAUNCH_MB(256); break;
+        case 128: LAUNCH_MB(128); break;
+        case 64: LAUNCH_MB(64); break;
+        case 32: LAUNCH_MB(32); break;
+        default: LAUNCH_MB(16); break;
+        }
+#undef LAUNCH_MB
+    }
     h->seq += 1;
 }
 
@@ -588,7 +648,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy, 0, nullptr)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -624,6 +684,7 @@ struct HostGate {
     uint32_t damp_codes;
     double a, b;           // B = 1: cos, sin (rotation), factor (noise); gamma, sqrt(1 - gamma) (damping)
     int nq = 0, q1 = 0, lut = 0;  // Clifford: qubits (qubit, q1) and table slot
+    int flags = 0;                 // imaginary rotation: bit 0 = normalize by the identity's coefficient
 };
 
 // Radix-sort buffers for `cap` terms.
@@ -734,6 +795,13 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
         case 1: enq_damping<1>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
         case 2: enq_damping<2>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
         default: enq_damping<4>(h, g.qubit, g.a, g.b, dt, max_weight, min_abs, glog, ev); break;
+        }
+    } else if (g.kind == PP_GATE_IMAG_ROTATION) {
+        const bool nrm = (g.flags & 1) != 0;
+        switch (h->W) {
+        case 1: enq_imag_rotation<1>(h, g.mask, g.a, g.b, dt, nrm, max_weight, min_abs, glog, ev); break;
+        case 2: enq_imag_rotation<2>(h, g.mask, g.a, g.b, dt, nrm, max_weight, min_abs, glog, ev); break;
+        default: enq_imag_rotation<4>(h, g.mask, g.a, g.b, dt, nrm, max_weight, min_abs, glog, ev); break;
         }
     } else if (g.kind == PP_GATE_ROTATION) {
         switch (h->W) {
@@ -913,6 +981,8 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
                         skipped[i - first] = 1;
                         continue;
                     }
+                    widen_rotation(P, q.mask, h->W);
+                } else if (q.kind == PP_GATE_IMAG_ROTATION) {
                     widen_rotation(P, q.mask, h->W);
                 } else if (q.kind == PP_GATE_CLIFFORD) {
                     widen_qubit(P, q.qubit);
@@ -1173,6 +1243,17 @@ int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const doubl
                     (*atrig)[g * 2 * B + b] = c;
                     (*atrig)[g * 2 * B + B + b] = sn;
                 }
+            }
+        } else if (gt.kind == PP_GATE_IMAG_ROTATION) {
+            if (int rc = check_mask(h, gt.mask)) return rc;
+            if (atrig) return fail(PP_ERR_UNSUPPORTED, "the adjoint sweep does not take imaginary rotations");
+            hg.mask = gt.mask;
+            hg.flags = gt.flags;
+            hg.a = std::cosh(val(0));
+            hg.b = std::sinh(val(0));
+            for (int b = 0; b < B && B > 1; ++b) {
+                h->h_trig[g * 2 * B + b] = std::cosh(val(b));
+                h->h_trig[g * 2 * B + B + b] = std::sinh(val(b));
             }
         } else if (gt.kind == PP_GATE_PAULI_NOISE) {
             if (gt.qubit < 0 || gt.qubit >= h->nqubits) return fail(PP_ERR_ARG, "noise qubit out of range");
@@ -1542,6 +1623,7 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
     if (hipMalloc(&h->d_ctl, sizeof(Ctl)) != hipSuccess ||
         hipHostMalloc(&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess ||
+        hipMalloc(&h->d_norm, batch * sizeof(double)) != hipSuccess ||
         hipMalloc(&h->d_pres_part, ((size_t)h->gmax + 1) * PRES_WORDS * sizeof(uint64_t)) != hipSuccess ||
         hipHostMalloc(&h->h_pres, PRES_WORDS * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
         return bail(fail(PP_ERR_NOMEM, "device allocation failed"));
@@ -1565,6 +1647,7 @@ int pp_destroy(pp_handle* h) {
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
     if (h->d_out) hipFree(h->d_out);
+    if (h->d_norm) hipFree(h->d_norm);
     if (h->d_pres_part) hipFree(h->d_pres_part);
     if (h->h_pres) hipHostFree(h->h_pres);
     if (h->d_tpj) hipFree(h->d_tpj);
@@ -1698,6 +1781,17 @@ int pp_apply_rotation(pp_handle* h, const uint64_t* mask, const double* cos_b, c
     if (int rc = check_mask(h, m4)) return rc;
     HostGate g{PP_GATE_ROTATION, m4, 0, 0, cos_b[0], sin_b[0]};
     return single_gate(h, g, cos_b, sin_b, max_weight, min_abs_coeff, stats);
+}
+
+int pp_apply_imaginary_rotation(pp_handle* h, const uint64_t* mask, const double* cosh_b, const double* sinh_b,
+                                int normalize, int max_weight, double min_abs_coeff, pp_gate_stats* stats) {
+    if (!h || !mask || !cosh_b || !sinh_b) return fail(PP_ERR_ARG, "NULL argument");
+    uint64_t m4[4] = {0, 0, 0, 0};
+    for (int l = 0; l < h->W; ++l) m4[l] = mask[l];
+    if (int rc = check_mask(h, m4)) return rc;
+    HostGate g{PP_GATE_IMAG_ROTATION, m4, 0, 0, cosh_b[0], sinh_b[0]};
+    g.flags = normalize ? 1 : 0;
+    return single_gate(h, g, cosh_b, sinh_b, max_weight, min_abs_coeff, stats);
 }
 
 int pp_apply_pauli_noise(pp_handle* h, int qubit, uint32_t damp_codes, const double* factor_b, int max_weight,

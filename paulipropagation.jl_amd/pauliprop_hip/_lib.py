@@ -57,6 +57,8 @@ SIGNATURES = [
                                          ctypes.POINTER(pp_gate_stats)]),
     ("pp_apply_pauli_noise", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, _DP, ctypes.c_int, ctypes.c_double,
                                             ctypes.POINTER(pp_gate_stats)]),
+    ("pp_apply_imaginary_rotation", ctypes.c_int, [_P, _U64P, _DP, _DP, ctypes.c_int, ctypes.c_int,
+                                                    ctypes.c_double, ctypes.POINTER(pp_gate_stats)]),
     ("pp_apply_amplitude_damping", ctypes.c_int, [_P, ctypes.c_int, _DP, ctypes.c_int, ctypes.c_double,
                                                    ctypes.POINTER(pp_gate_stats)]),
     ("pp_apply_clifford", ctypes.c_int, [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _U64P, _DP, ctypes.c_int,

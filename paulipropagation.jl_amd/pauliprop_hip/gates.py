@@ -5,16 +5,19 @@ Reference: src/Gates/paulirotations.jl:11-70 (PauliRotation),
 src/Gates/cliffordgates.jl:4-72 (CliffordGate and the default Clifford map),
 src/Gates/noisechannels.jl:24-278 (Pauli noise channels, AmplitudeDampingNoise,
 the strength check), src/Gates/frozengates.jl:12-45 (FrozenGate, freeze),
-src/Gates/heisenbergschrodinger.jl:20-49 (toheisenberg).
+src/Gates/heisenbergschrodinger.jl:20-165 (toheisenberg, toschrodinger),
+src/Gates/imaginarytime.jl (ImaginaryPauliRotation), src/Gates/miscgates.jl:10-33
+(TGate).
 """
 from __future__ import annotations
 
 import ctypes
+import math
 
 from .paulis import SYMBOLS, symboltoint
 
 # pp_gate kinds (include/pphip.h)
-ROTATION, PAULI_NOISE, AMP_DAMPING, CLIFFORD = 0, 1, 2, 3
+ROTATION, PAULI_NOISE, AMP_DAMPING, CLIFFORD, IMAG_ROTATION = 0, 1, 2, 3, 4
 
 
 class pp_gate(ctypes.Structure):
@@ -30,6 +33,7 @@ class pp_gate(ctypes.Structure):
         ("nq", ctypes.c_int32),
         ("qubits", ctypes.c_int32 * 2),
         ("lut", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
     ]
 
 
@@ -62,6 +66,15 @@ class PauliRotation(Gate):
         return f"PauliRotation({self.symbols}, {self.qinds})"
 
 
+class ImaginaryPauliRotation(PauliRotation):
+    """exp(-tau/2 P) . exp(-tau/2 P) on states (src/Gates/imaginarytime.jl):
+    terms commuting with P branch into cosh(tau) and sinh(tau) parts. Defined
+    in the Schrodinger picture only (heisenbergschrodinger.jl:75-79)."""
+
+    def __repr__(self):
+        return f"ImaginaryPauliRotation({self.symbols}, {self.qinds})"
+
+
 # The default Clifford map: for each gathered Pauli index (qubit 1 in the low
 # two bits) the new partial Pauli string and the sign
 # (src/Gates/cliffordgates.jl:47-59). The table is data the gate semantics
@@ -84,6 +97,15 @@ CLIFFORD_MAP = {
                  (0x7, -1), (0x9, 1), (0xA, 1), (0x4, -1), (0xC, 1), (0x2, 1), (0x1, -1), (0xF, 1)],
 }
 CLIFFORD_MAP["SZ"] = CLIFFORD_MAP["S"]
+
+
+def transposecliffordmap(table):
+    """The transposed map: entry q is (p, s) where the map sends p to (q, s)
+    (src/Gates/cliffordgates.jl:94-98)."""
+    out = [None] * len(table)
+    for p, (q, sgn) in enumerate(table):
+        out[q] = (p, sgn)
+    return out
 
 
 class CliffordGate(Gate):
@@ -162,6 +184,19 @@ class AmplitudeDampingNoise(Gate):
         return f"AmplitudeDampingNoise({self.qind})"
 
 
+class TGate(Gate):
+    """The T gate on qubit `qind`: a PauliRotation(:Z, qind) with angle pi/4
+    (src/Gates/miscgates.jl:10-33, src/Propagation/specializations.jl:293-308)."""
+
+    def __init__(self, qind: int):
+        self.qind = int(qind)
+        if self.qind < 1:
+            raise ValueError("qubit index must be >= 1")
+
+    def __repr__(self):
+        return f"TGate({self.qind})"
+
+
 class FrozenGate(Gate):
     """A parametrized gate with its parameter fixed (src/Gates/frozengates.jl:12)."""
 
@@ -185,13 +220,53 @@ def countparameters(circuit) -> int:
     return sum(1 for g in circuit if g.parametrized)
 
 
+def _inner(gate):
+    return gate.gate if isinstance(gate, FrozenGate) else gate
+
+
 def toheisenberg(circuit, thetas):
     """Reverse gates and parameters; angles are unchanged
-    (src/Gates/heisenbergschrodinger.jl:20-49)."""
+    (src/Gates/heisenbergschrodinger.jl:20-49). ImaginaryPauliRotation has no
+    Heisenberg form (:75-79)."""
     n = countparameters(circuit)
     if thetas is not None and len(thetas) != n:
         raise ValueError(f"the circuit has {n} parameters, got {len(thetas)}")
+    for g in circuit:
+        if isinstance(_inner(g), ImaginaryPauliRotation):
+            raise RuntimeError("ImaginaryPauliRotation gates are currently not defined in the Heisenberg picture.")
     return list(reversed(circuit)), (None if thetas is None else list(reversed(list(thetas))))
+
+
+def toschrodinger(circuit, thetas):
+    """Schrodinger-picture gates in circuit order (src/Gates/heisenbergschrodinger.jl:87-165):
+    a PauliRotation's angle changes sign, a Clifford takes its transposed map,
+    Pauli noise, amplitude damping and ImaginaryPauliRotation are unchanged;
+    a TGate has no Schrodinger form here, as in the reference."""
+    n = countparameters(circuit)
+    if thetas is not None and len(thetas) != n:
+        raise ValueError(f"the circuit has {n} parameters, got {len(thetas)}")
+    out, th, k = [], [], 0
+    for g in circuit:
+        inner = _inner(g)
+        if isinstance(inner, TGate):
+            raise NotImplementedError("toschrodinger is not implemented for TGate")
+        if isinstance(inner, CliffordGate):
+            sym = inner.symbol + "_TRANSPOSE"
+            if sym not in CLIFFORD_MAP:
+                CLIFFORD_MAP[sym] = transposecliffordmap(CLIFFORD_MAP[inner.symbol])
+            out.append(CliffordGate(sym, inner.qinds))
+            continue
+        flip = isinstance(inner, PauliRotation) and not isinstance(inner, ImaginaryPauliRotation)
+        if isinstance(g, FrozenGate):
+            out.append(FrozenGate(inner, -g.parameter if flip else g.parameter))
+        else:
+            out.append(g)
+            if g.parametrized:
+                if thetas is not None:
+                    v = thetas[k]
+                    th.append(-v if flip else v)
+                k += 1
+    return out, (None if thetas is None else th)
 
 
 def _qubit0(nqubits: int, q: int) -> int:
@@ -223,8 +298,10 @@ def clifford_table(symbol: str, qubits0, W: int):
     return changes, signs
 
 
-def pack_circuit(circuit, nqubits: int, W: int, param_offset=None):
-    """Pack a Heisenberg-ordered gate list into pp_gate descriptors.
+def pack_circuit(circuit, nqubits: int, W: int, param_offset=None, normalize_coeffs: bool = True):
+    """Pack a gate list in propagation order (Heisenberg or Schrodinger) into
+    pp_gate descriptors; normalize_coeffs sets the flag of every
+    ImaginaryPauliRotation.
 
     Parametrized gates read parameter rows in order (row k is the k-th
     parametrized gate of the list). Returns (gates array, Clifford table
@@ -239,13 +316,16 @@ def pack_circuit(circuit, nqubits: int, W: int, param_offset=None):
         d = arr[gi]
         d.param = -1
         inner, frozen = (gate.gate, gate.parameter) if isinstance(gate, FrozenGate) else (gate, None)
+        if isinstance(inner, TGate):
+            inner, frozen = PauliRotation("Z", inner.qind), math.pi / 4
         if frozen is None and inner.parametrized:
             d.param = k
             k += 1
         else:
             d.value = 0.0 if frozen is None else frozen
         if isinstance(inner, PauliRotation):
-            d.kind = ROTATION
+            d.kind = IMAG_ROTATION if isinstance(inner, ImaginaryPauliRotation) else ROTATION
+            d.flags = 1 if (d.kind == IMAG_ROTATION and normalize_coeffs) else 0
             m = inner.mask(nqubits)
             for l in range(W):
                 d.mask[l] = (m >> (64 * l)) & ((1 << 64) - 1)

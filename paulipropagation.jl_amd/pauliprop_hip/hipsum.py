@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .gates import countparameters, pack_circuit, toheisenberg
+from .gates import countparameters, pack_circuit, toheisenberg, toschrodinger
 from .paulis import PauliString, PauliSum, int_to_limbs, key_words, limbs_to_int
 
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -126,6 +126,20 @@ class HIPPauliSum:
         st = _lib.pp_gate_stats()
         _lib.check(_lib.lib().pp_apply_rotation(self._h, _u64p(m), _dp(c), _dp(s), _max_weight_arg(max_weight),
                                                 float(min_abs_coeff), ctypes.byref(st)))
+        return st
+
+    def apply_imaginary_rotation(self, mask: int, tau, normalize_coeffs: bool = True, max_weight=math.inf,
+                                 min_abs_coeff=1e-10):
+        """applymergetruncate!(ImaginaryPauliRotation, cache, tau; normalize_coeffs) for an
+        integer generator mask (src/Propagation/specializations.jl:86-104)."""
+        tu = np.broadcast_to(np.asarray(tau, dtype=np.float64), (self.batch,))
+        c = np.ascontiguousarray([math.cosh(t) for t in tu], dtype=np.float64)
+        s = np.ascontiguousarray([math.sinh(t) for t in tu], dtype=np.float64)
+        m = np.array([(mask >> (64 * l)) & ((1 << 64) - 1) for l in range(self.W)], dtype=np.uint64)
+        st = _lib.pp_gate_stats()
+        _lib.check(_lib.lib().pp_apply_imaginary_rotation(self._h, _u64p(m), _dp(c), _dp(s), int(bool(normalize_coeffs)),
+                                                          _max_weight_arg(max_weight), float(min_abs_coeff),
+                                                          ctypes.byref(st)))
         return st
 
     def apply_pauli_noise(self, qubit: int, damp_codes: int, lam, max_weight=math.inf, min_abs_coeff=1e-10):
@@ -300,14 +314,15 @@ class PackedCircuit:
 
 
 def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, min_abs_coeff=1e-10,
-                   counts: bool = False):
-    """Run an already Heisenberg-ordered circuit; hthetas is P x B (or P) in that order."""
+                   counts: bool = False, normalize_coeffs: bool = True):
+    """Run a circuit already in propagation order (Heisenberg, or Schrodinger
+    from toschrodinger); hthetas is P x B (or P) in that order."""
     P = countparameters(hcircuit)
     B = psum.batch
     th = np.zeros((max(P, 1), B), dtype=np.float64)
     if P:
         th[:P] = np.asarray(hthetas, dtype=np.float64).reshape(P, -1)
-    gates, lut_c, lut_s, nluts = pack_circuit(hcircuit, psum.nqubits, psum.W)
+    gates, lut_c, lut_s, nluts = pack_circuit(hcircuit, psum.nqubits, psum.W, normalize_coeffs=normalize_coeffs)
     lc = np.ascontiguousarray(lut_c or [0], dtype=np.uint64)
     ls = np.ascontiguousarray(lut_s or [0.0], dtype=np.float64)
     cnt = np.zeros(max(len(hcircuit), 1), dtype=np.int64)
@@ -318,14 +333,17 @@ def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, mi
 
 
 def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-10, batch: int = None,
-              counts: bool = False):
-    """propagate(circuit, psum, thetas; max_weight, min_abs_coeff) on the GPU.
+              counts: bool = False, heisenberg: bool = True, normalize_coeffs: bool = True):
+    """propagate(circuit, psum, thetas; max_weight, min_abs_coeff, heisenberg) on the GPU.
 
     `psum` is a PauliString, a PauliSum or a HIPPauliSum (propagated in place
     and returned). `thetas` holds one parameter per parametrized gate in
     Schrodinger order, or a P x B array for B batch columns. The reference's
     default min_abs_coeff is 1e-10 and max_weight Inf
-    (src/Propagation/generics.jl:54).
+    (src/Propagation/generics.jl:54). heisenberg=False propagates a state in
+    the Schrodinger picture (circuit order, transposed gates,
+    src/Propagation/generics.jl:78-86); normalize_coeffs applies to
+    ImaginaryPauliRotation gates (src/Propagation/specializations.jl:86).
     """
     P = countparameters(circuit)
     th = np.zeros((0, 1)) if thetas is None else np.asarray(thetas, dtype=np.float64)
@@ -338,9 +356,11 @@ def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-
         psum = HIPPauliSum.from_paulisum(psum, batch=B)
     if P and th.shape[1] != psum.batch:
         raise ValueError(f"thetas have {th.shape[1]} columns, the sum has {psum.batch}")
-    hcirc, hth = toheisenberg(circuit, list(th) if P else [])
+    order = toheisenberg if heisenberg else toschrodinger
+    hcirc, hth = order(circuit, list(th) if P else [])
     hth = np.array(hth) if P else None
-    cnt = run_heisenberg(psum, hcirc, hth, max_weight=max_weight, min_abs_coeff=min_abs_coeff, counts=counts)
+    cnt = run_heisenberg(psum, hcirc, hth, max_weight=max_weight, min_abs_coeff=min_abs_coeff, counts=counts,
+                         normalize_coeffs=normalize_coeffs)
     return (psum, cnt) if counts else psum
 
 
