@@ -219,6 +219,32 @@ int pp_get_coeff(pp_handle *h, const uint64_t *key, double *out_b);
  * overlapwithpaulisum(rho, psum) is 2^n times it (:102-104). */
 int pp_scalar_product(pp_handle *h1, pp_handle *h2, double *out_b);
 
+/* Resampling methods of pp_resample (src/Base/MonteCarlo/resample.jl). */
+enum pp_resample_method {
+    PP_RESAMPLE_SEMIDETERMINISTIC = 0, /* semideterministic_systematic_resample! (:139-170), the default */
+    PP_RESAMPLE_SYSTEMATIC = 1,        /* systematic_resample! (:96-112), calibrated when `calibrate` */
+    PP_RESAMPLE_MULTINOMIAL = 2        /* multinomial_resample! (:64-78), the default with squared */
+};
+
+/* Monte Carlo resampling of a B = 1 sum down to about target_size terms.
+ * Every term lies on [0, total) on an interval as wide as its weight (|c|, or
+ * c^2 with `squared`; zero for the terms the semideterministic method keeps as
+ * they are, those with |c| > sum|c| / target_size), in key order. A term gets
+ * the comb's teeth in its interval -- floor(position / spacing - offset)
+ * counts them, `offset` the caller's uniform draw in [0, 1) -- or, for the
+ * multinomial method, the draws (target_size uniforms in [0, 1), ascending,
+ * times the total weight); its new coefficient is teeth * spacing with its
+ * sign (the square root with `squared`); terms left at zero are dropped and
+ * the others keep their order. The systematic method's spacing is total /
+ * target_size, or with `calibrate` the Newton iteration toward (1 - rtol)
+ * target_size - atol expected survivors. *spacing_out (may be NULL) receives
+ * the spacing (the weight per draw). Random numbers come from the caller.
+ * Replaces: resample!(cache, target_size; resample_func, squared)
+ *           (src/Base/MonteCarlo/resample.jl:24-60), the step of mcpropagate!
+ *           (src/Base/MonteCarlo/mcpropagate.jl:42-52). */
+int pp_resample(pp_handle *h, int method, size_t target_size, int squared, double offset, const double *draws,
+                int calibrate, double rtol, double atol, double *spacing_out);
+
 /* Wait for all work on the handle's stream. */
 int pp_synchronize(pp_handle *h);
 

@@ -50,6 +50,8 @@ def lib():
                                     ctypes.c_double]),
             ("ppo_overlap_zero", None, [_P, _DP]),
             ("ppo_overlap_kind", None, [_P, ctypes.c_int, _U64P, _DP]),
+            ("ppo_resample", ctypes.c_double, [_P, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_double,
+                                               _DP, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             ("ppo_get_coeff", None, [_P, _U64P, _DP]),
             ("ppo_scalar_product", ctypes.c_int, [_P, _P, _DP]),
             ("ppo_run", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
@@ -173,6 +175,17 @@ class OracleSum:
         out = np.zeros(self.B, dtype=np.float64)
         lib().ppo_overlap_kind(self._s, int(kind), u64p(o), dp(out))
         return out
+
+    def resample(self, method: int, target: int, squared=False, offset=0.0, draws=None, calibrate=True, rtol=0.01,
+                 atol=1.0):
+        """method 0 semideterministic systematic, 1 systematic, 2 multinomial (draws ascending in [0, 1));
+        returns the comb spacing (weight per draw)."""
+        d = np.ascontiguousarray(draws if draws is not None else np.zeros(1), dtype=np.float64)
+        sp = lib().ppo_resample(self._s, int(method), int(target), int(bool(squared)), float(offset), dp(d),
+                                int(bool(calibrate)), float(rtol), float(atol))
+        if sp < 0:
+            raise ValueError("bad resampling arguments")
+        return sp
 
     def get_coeff(self, key_limbs):
         k = np.ascontiguousarray(key_limbs, dtype=np.uint64)

@@ -633,6 +633,117 @@ int ppo_scalar_product(const ppo_sum *a, const ppo_sum *b, double *out) {
 }
 
 /* ------------------------------------------------------------------ */
+/* resampling (src/Base/MonteCarlo/resample.jl)                        */
+/* ------------------------------------------------------------------ */
+
+static double rs_weight(double c, int squared) { return squared ? c * c : fabs(c); }
+
+/* The comb spacing that keeps `target` terms in expectation, Newton steps from
+ * total / target (_calibrate_comb_spacing, resample.jl:118-130). */
+static double rs_calibrate(const ppo_sum *s, double total, size_t target, int squared, double rtol, double atol) {
+    double lowest = (1 - rtol) * (double)target - atol;
+    double spacing = total / (double)target;
+    for (int it = 0; it < 10; ++it) {
+        size_t n_heavy = 0;
+        double light = 0.0;
+        for (size_t i = 0; i < s->n; ++i) {
+            double w = rs_weight(s->coeffs[i], squared);
+            if (w >= spacing) ++n_heavy;
+            else light += w;
+        }
+        if ((double)n_heavy + light / spacing >= lowest || light == 0.0) return spacing;
+        spacing = light / (double)(target - n_heavy);
+    }
+    return spacing;
+}
+
+/* Resampling of a B = 1 sum down to about `target` terms (resample.jl):
+ * method 0 semideterministic_systematic_resample! (:139-170), 1
+ * systematic_resample! (:96-112, calibrated when `calibrate`), 2
+ * multinomial_resample! (:64-78) with `draws` = target uniforms in [0, 1),
+ * ascending. `offset` is the comb's random offset in [0, 1). Every term lies
+ * on [0, total) on an interval as wide as its weight; a term gets the teeth
+ * (comb: floor(position / spacing - offset) counts; multinomial: draws
+ * * total) in its interval, its new coefficient is copysign(teeth * spacing,
+ * c) (squared: sqrt(teeth * spacing)), and terms left at zero are dropped
+ * in order (_set_coeffs_and_drop_zeros!, :250-300). Returns the spacing used,
+ * or -1 on bad arguments. */
+double ppo_resample(ppo_sum *s, int method, size_t target, int squared, double offset, const double *draws,
+                    int calibrate, double rtol, double atol) {
+    if (s->B != 1 || target == 0) return -1.0;
+    if (method == 0 && squared) return -1.0;
+    if (method != 2 && target > s->n) return -1.0;
+    int W = s->W;
+    size_t n = s->n;
+    double total = 0.0, thr = 0.0, spacing = 0.0;
+    size_t n_kept = 0;
+    if (method == 0) {
+        double sabs = 0.0;
+        for (size_t i = 0; i < n; ++i) sabs += fabs(s->coeffs[i]);
+        thr = sabs / (double)target;
+        for (size_t i = 0; i < n; ++i) {
+            double c = s->coeffs[i];
+            if (fabs(c) > thr) ++n_kept;
+            else total += fabs(c);
+        }
+        spacing = target > n_kept ? total / (double)(target - n_kept) : 0.0;
+    } else {
+        for (size_t i = 0; i < n; ++i) total += rs_weight(s->coeffs[i], squared);
+        if (method == 1)
+            spacing = calibrate ? rs_calibrate(s, total, target, squared, rtol, atol) : total / (double)target;
+        else
+            spacing = total / (double)target;
+    }
+    double inv = spacing == 0.0 ? 0.0 : 1 / spacing;
+    double *scaled = NULL;
+    if (method == 2) {
+        scaled = xmalloc((target ? target : 1) * sizeof(double));
+        for (size_t k = 0; k < target; ++k) scaled[k] = draws[k] * total;
+    }
+    double pos = 0.0;
+    size_t o = 0;
+    double below_end = 0.0;
+    if (method == 2) {
+        size_t k = 0;
+        while (k < target && scaled[k] < pos) ++k;
+        below_end = (double)k;
+    } else {
+        below_end = floor(pos * inv - offset);
+    }
+    for (size_t i = 0; i < n; ++i) {
+        double c = s->coeffs[i];
+        int kept = method == 0 && fabs(c) > thr;
+        double w = method == 0 ? (kept ? 0.0 : fabs(c)) : rs_weight(c, squared);
+        double below_start = below_end;
+        pos += w;
+        if (method == 2) {
+            /* number of draws below the position: searchsortedfirst(draws, pos) - 1 */
+            size_t lo = 0, hi = target;
+            while (lo < hi) {
+                size_t mid = lo + (hi - lo) / 2;
+                if (scaled[mid] < pos) lo = mid + 1;
+                else hi = mid;
+            }
+            below_end = (double)lo;
+        } else {
+            below_end = floor(pos * inv - offset);
+        }
+        double teeth = below_end - below_start;
+        double nc;
+        if (kept) nc = c;
+        else if (squared) nc = sqrt(teeth * spacing) * (c == 0.0 ? 0.0 : 1.0);
+        else nc = copysign(teeth * spacing, c);
+        if (nc == 0.0) continue;
+        if (o != i) memmove(s->keys + o * W, s->keys + i * W, W * sizeof(uint64_t));
+        s->coeffs[o] = nc;
+        ++o;
+    }
+    s->n = o;
+    free(scaled);
+    return spacing;
+}
+
+/* ------------------------------------------------------------------ */
 /* gate lists                                                          */
 /* ------------------------------------------------------------------ */
 

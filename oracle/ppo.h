@@ -72,6 +72,12 @@ void ppo_overlap_zero(const ppo_sum *s, double *out);
 /* overlap with a product state, per column: kind 0 |0..0>, 1 |+..+>, 2 the
  * computational basis state with ones where bit 2q of `ones` is set */
 void ppo_overlap_kind(const ppo_sum *s, int kind, const uint64_t *ones, double *out);
+/* Monte Carlo resampling of a B = 1 sum to about `target` terms: method 0
+ * semideterministic systematic, 1 systematic (calibrated when `calibrate`),
+ * 2 multinomial with `draws` (target uniforms, ascending); `offset` is the
+ * comb offset in [0, 1). Returns the comb spacing (weight per draw), or -1. */
+double ppo_resample(ppo_sum *s, int method, size_t target, int squared, double offset, const double *draws,
+                    int calibrate, double rtol, double atol);
 /* the coefficients of `key` (zeros when absent) */
 void ppo_get_coeff(const ppo_sum *s, const uint64_t *key, double *out);
 /* sum over common strings of coefficient products; -1 on mismatched W or B */

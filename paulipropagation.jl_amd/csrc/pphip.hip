@@ -32,6 +32,7 @@
 #include "../../include/pphip.h"
 #include "pp_kernels.h"
 #include "pp_sort.h"
+#include "pp_mc.h"
 
 using namespace pp;
 
@@ -104,6 +105,14 @@ struct pp_handle {
     size_t partial_cap = 0;
     double* d_out = nullptr;
     double* d_norm = nullptr;         // B doubles: 1 / the identity's coefficient (imaginary rotations)
+    // resampling: per chunk sums, counts, interval starts and output offsets; the draws
+    double* d_mc_sum = nullptr;
+    uint32_t* d_mc_cnt = nullptr;
+    double* d_mc_start = nullptr;
+    uint32_t* d_mc_out = nullptr;
+    size_t mc_chunks = 0;
+    double* d_mc_draws = nullptr;
+    size_t mc_draws_cap = 0;
     uint64_t* d_pres_part = nullptr;  // presence partials, gmax x 12 words, then the 12 result words
     uint64_t* h_pres = nullptr;       // pinned copy of the result
     // profile
@@ -1558,6 +1567,148 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     return rc;
 }
 
+// Chunk buffers of the resampling kernels for `nch` chunks.
+int ensure_mc(pp_handle* h, size_t nch) {
+    if (nch <= h->mc_chunks) return PP_OK;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->d_mc_sum) hipFree(h->d_mc_sum);
+    if (h->d_mc_cnt) hipFree(h->d_mc_cnt);
+    if (h->d_mc_start) hipFree(h->d_mc_start);
+    if (h->d_mc_out) hipFree(h->d_mc_out);
+    h->d_mc_sum = nullptr;
+    h->d_mc_cnt = nullptr;
+    h->d_mc_start = nullptr;
+    h->d_mc_out = nullptr;
+    h->mc_chunks = 0;
+    HIP_TRY(hipMalloc(&h->d_mc_sum, nch * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_mc_cnt, nch * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&h->d_mc_start, nch * sizeof(double)));
+    HIP_TRY(hipMalloc(&h->d_mc_out, nch * sizeof(uint32_t)));
+    h->mc_chunks = nch;
+    return PP_OK;
+}
+
+// One k_mc_weigh pass: per-chunk sums (kept in `sums`) and their total and
+// count, added chunk by chunk in order.
+int mc_weigh(pp_handle* h, size_t nch, int mode, double p, int squared, std::vector<double>& sums, double* total,
+             uint64_t* count) {
+    hipLaunchKernelGGL(k_mc_weigh, dim3((unsigned)nch), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, mode, p,
+                       squared, h->d_mc_sum, h->d_mc_cnt);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> cnts(nch);
+    sums.resize(nch);
+    HIP_TRY(hipMemcpyAsync(sums.data(), h->d_mc_sum, nch * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(cnts.data(), h->d_mc_cnt, nch * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double t = 0.0;
+    uint64_t c = 0;
+    for (size_t k = 0; k < nch; ++k) {
+        t += sums[k];
+        c += cnts[k];
+    }
+    *total = t;
+    *count = c;
+    return PP_OK;
+}
+
+// Resampling of a B = 1 sum (src/Base/MonteCarlo/resample.jl): the totals and
+// the comb spacing on the host from per-chunk sums, then the new coefficients
+// and the survivors' compaction on the device (pp_mc.h).
+template <int W>
+int resample_impl(pp_handle* h, int method, size_t target, int squared, double offset, const double* draws,
+                  int calibrate, double rtol, double atol, double* spacing_out) {
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    if (method != PP_RESAMPLE_MULTINOMIAL && target > n)
+        return fail(PP_ERR_ARG, "this resampler cannot grow " + std::to_string(n) + " terms to target_size " +
+                                    std::to_string(target) + "; use the multinomial one");
+    if (n == 0) {
+        if (spacing_out) *spacing_out = 0.0;
+        return PP_OK;
+    }
+    const size_t nch = ((size_t)n + MC_CHUNK - 1) / MC_CHUNK;
+    if (int rc = ensure_mc(h, nch)) return rc;
+    std::vector<double> sums;
+    double total = 0.0, thr = 0.0, spacing = 0.0;
+    uint64_t cnt = 0;
+    if (method == PP_RESAMPLE_SEMIDETERMINISTIC) {
+        // keep the terms heavier than the mean weight per target term; a comb over the others
+        // (semideterministic_systematic_resample!, resample.jl:139-170)
+        double sabs = 0.0;
+        if (int rc = mc_weigh(h, nch, MC_ABS, 0.0, 0, sums, &sabs, &cnt)) return rc;
+        thr = sabs / (double)target;
+        uint64_t n_kept = 0;
+        if (int rc = mc_weigh(h, nch, MC_COMB, thr, 0, sums, &total, &n_kept)) return rc;
+        spacing = target > n_kept ? total / (double)(target - n_kept) : 0.0;
+    } else {
+        if (int rc = mc_weigh(h, nch, MC_W, 0.0, squared, sums, &total, &cnt)) return rc;
+        spacing = total / (double)target;
+        if (method == PP_RESAMPLE_SYSTEMATIC && calibrate) {
+            // Newton steps from total / target toward the expected survivor count
+            // (_calibrate_comb_spacing, resample.jl:118-130)
+            const double lowest = (1 - rtol) * (double)target - atol;
+            std::vector<double> hs;
+            for (int it = 0; it < 10; ++it) {
+                double light = 0.0;
+                uint64_t n_heavy = 0;
+                if (int rc = mc_weigh(h, nch, MC_HEAVY, spacing, squared, hs, &light, &n_heavy)) return rc;
+                if ((double)n_heavy + light / spacing >= lowest || light == 0.0) break;
+                spacing = light / (double)(target - n_heavy);
+            }
+        }
+    }
+    // chunk starts: the chunks' weights added in order
+    std::vector<double> starts(nch);
+    double r = 0.0;
+    for (size_t k = 0; k < nch; ++k) {
+        starts[k] = r;
+        r += sums[k];
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_mc_start, starts.data(), nch * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    McParams m{};
+    m.method = method;
+    m.squared = squared;
+    m.thr = thr;
+    m.spacing = spacing;
+    m.inv = spacing == 0.0 ? 0.0 : 1 / spacing;
+    m.offset = offset;
+    if (method == PP_RESAMPLE_MULTINOMIAL) {
+        if (target > h->mc_draws_cap) {
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            if (h->d_mc_draws) hipFree(h->d_mc_draws);
+            h->d_mc_draws = nullptr;
+            h->mc_draws_cap = 0;
+            HIP_TRY(hipMalloc(&h->d_mc_draws, target * sizeof(double)));
+            h->mc_draws_cap = target;
+        }
+        std::vector<double> scaled(target);
+        for (size_t k = 0; k < target; ++k) scaled[k] = draws[k] * total;
+        HIP_TRY(hipMemcpyAsync(h->d_mc_draws, scaled.data(), target * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        m.draws = h->d_mc_draws;
+        m.ndraws = (uint32_t)target;
+    }
+    hipLaunchKernelGGL(k_mc_apply, dim3((unsigned)nch), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, m,
+                       h->d_mc_start, h->bufs.tcoef, h->d_mc_cnt);
+    HIP_TRY(hipGetLastError());
+    std::vector<uint32_t> keeps(nch), outs(nch);
+    HIP_TRY(hipMemcpyAsync(keeps.data(), h->d_mc_cnt, nch * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint64_t kept = 0;
+    for (size_t k = 0; k < nch; ++k) {
+        outs[k] = (uint32_t)kept;
+        kept += keeps[k];
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_mc_out, outs.data(), nch * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_mc_scatter<W>, dim3((unsigned)nch), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
+                       h->bufs.tcoef, h->d_mc_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (spacing_out) *spacing_out = spacing;
+    return reset_ctl(h, (uint32_t)kept, cur ^ 1);
+}
+
 // Terms in any order with repeats: upload as given, sort on the device (stable,
 // limb by limb), sum every run of equal strings in input order, truncate.
 template <int W>
@@ -1648,6 +1799,11 @@ int pp_destroy(pp_handle* h) {
     if (h->d_partial) hipFree(h->d_partial);
     if (h->d_out) hipFree(h->d_out);
     if (h->d_norm) hipFree(h->d_norm);
+    if (h->d_mc_sum) hipFree(h->d_mc_sum);
+    if (h->d_mc_cnt) hipFree(h->d_mc_cnt);
+    if (h->d_mc_start) hipFree(h->d_mc_start);
+    if (h->d_mc_out) hipFree(h->d_mc_out);
+    if (h->d_mc_draws) hipFree(h->d_mc_draws);
     if (h->d_pres_part) hipFree(h->d_pres_part);
     if (h->h_pres) hipHostFree(h->h_pres);
     if (h->d_tpj) hipFree(h->d_tpj);
@@ -1716,6 +1872,31 @@ int pp_upload(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n
                                h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     return reset_ctl(h, (uint32_t)n, 0);
+}
+
+int pp_resample(pp_handle* h, int method, size_t target_size, int squared, double offset, const double* draws,
+                int calibrate, double rtol, double atol, double* spacing_out) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    if (h->B != 1) return fail(PP_ERR_UNSUPPORTED, "resampling takes one coefficient column");
+    if (method != PP_RESAMPLE_SEMIDETERMINISTIC && method != PP_RESAMPLE_SYSTEMATIC &&
+        method != PP_RESAMPLE_MULTINOMIAL)
+        return fail(PP_ERR_ARG, "unknown resampling method");
+    if (target_size == 0 || target_size > MAX_TERMS) return fail(PP_ERR_ARG, "target_size must be positive");
+    if (method == PP_RESAMPLE_SEMIDETERMINISTIC && squared)
+        return fail(PP_ERR_ARG, "semideterministic_systematic_resample! does not support squared=true");
+    if (method != PP_RESAMPLE_MULTINOMIAL && !(offset >= 0.0 && offset < 1.0))
+        return fail(PP_ERR_ARG, "the comb offset must lie in [0, 1)");
+    if (method == PP_RESAMPLE_MULTINOMIAL) {
+        if (!draws) return fail(PP_ERR_ARG, "multinomial resampling needs target_size draws");
+        for (size_t k = 0; k < target_size; ++k)
+            if (!(draws[k] >= 0.0 && draws[k] < 1.0) || (k && draws[k] < draws[k - 1]))
+                return fail(PP_ERR_ARG, "draws must lie in [0, 1) in ascending order");
+    }
+    switch (h->W) {
+    case 1: return resample_impl<1>(h, method, target_size, squared, offset, draws, calibrate, rtol, atol, spacing_out);
+    case 2: return resample_impl<2>(h, method, target_size, squared, offset, draws, calibrate, rtol, atol, spacing_out);
+    default: return resample_impl<4>(h, method, target_size, squared, offset, draws, calibrate, rtol, atol, spacing_out);
+    }
 }
 
 int pp_load_merge(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n, int max_weight,

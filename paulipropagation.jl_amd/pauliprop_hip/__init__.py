@@ -14,6 +14,7 @@ from .circuits import (bricklayertopology, staircasetopology, rectangletopology,
                        hardwareefficientcircuit, efficientsu2circuit)
 from .hipsum import (HIPPauliSum, PackedCircuit, propagate, overlapwithzero, overlapwithplus,
                      overlapwithcomputational, overlapwithmaxmixed, overlapwithpaulisum, getcoeff, scalarproduct,
-                     run_heisenberg, adjointgradient, rewindgradient)
+                     run_heisenberg, adjointgradient, rewindgradient, mcpropagate, resample,
+                     semideterministic_systematic_resample, systematic_resample, multinomial_resample)
 
 __version__ = "0.1.0"

@@ -71,6 +71,8 @@ SIGNATURES = [
     ("pp_overlap", ctypes.c_int, [_P, ctypes.c_int, _U64P, _DP]),
     ("pp_get_coeff", ctypes.c_int, [_P, _U64P, _DP]),
     ("pp_scalar_product", ctypes.c_int, [_P, _P, _DP]),
+    ("pp_resample", ctypes.c_int, [_P, ctypes.c_int, _SZ, ctypes.c_int, ctypes.c_double, _DP, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, _DP]),
     ("pp_synchronize", ctypes.c_int, [_P]),
     ("pp_profile_control", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     ("pp_profile_read", ctypes.c_int, [_P, ctypes.POINTER(pp_profile)]),
@@ -79,6 +81,7 @@ SIGNATURES = [
 
 PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS, PP_TUNE_SKIP_IDENTITY = 1, 2, 3, 4
 PP_OVERLAP_ZERO, PP_OVERLAP_PLUS, PP_OVERLAP_COMPUTATIONAL = 0, 1, 2
+PP_RESAMPLE_SEMIDETERMINISTIC, PP_RESAMPLE_SYSTEMATIC, PP_RESAMPLE_MULTINOMIAL = 0, 1, 2
 
 
 def lib():

@@ -231,6 +231,29 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_scalar_product(self._h, other._h, _dp(out)))
         return self._column(out)
 
+    def resample(self, target_size: int, resample_func=None, squared: bool = False, rng=None,
+                 calibrate: bool = True, rtol: float = 0.01, atol: float = 1.0, offset=None, draws=None) -> float:
+        """resample!(psum, target_size; resample_func, squared) on the device
+        (src/Base/MonteCarlo/resample.jl:24-60). resample_func is one of
+        semideterministic_systematic_resample (the default), systematic_resample,
+        multinomial_resample (the default with squared=True). The random numbers
+        -- the comb offset, or the multinomial draws -- come from `rng` (a numpy
+        Generator) unless given. Returns the comb spacing (weight per draw)."""
+        method = _resample_method(resample_func, squared)
+        rng = np.random.default_rng() if rng is None else rng
+        dp_draws = None
+        if method == _lib.PP_RESAMPLE_MULTINOMIAL:
+            d = np.sort(rng.random(int(target_size))) if draws is None else np.asarray(draws, dtype=np.float64)
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            dp_draws = _dp(d)
+            off = 0.0
+        else:
+            off = float(rng.random()) if offset is None else float(offset)
+        sp = ctypes.c_double()
+        _lib.check(_lib.lib().pp_resample(self._h, method, int(target_size), int(bool(squared)), off, dp_draws,
+                                          int(bool(calibrate)), float(rtol), float(atol), ctypes.byref(sp)))
+        return sp.value
+
     def synchronize(self) -> None:
         _lib.check(_lib.lib().pp_synchronize(self._h))
 
@@ -404,6 +427,79 @@ def overlapwithzero(psum):
     raise TypeError("overlapwithzero runs on a HIPPauliSum; upload the sum first")
 
 
+def semideterministic_systematic_resample(*args, **kwargs):
+    """Name of the semideterministic systematic resampler (src/Base/MonteCarlo/resample.jl:139-170);
+    pass it as resample_func."""
+    raise TypeError("pass semideterministic_systematic_resample as resample_func of resample or mcpropagate")
+
+
+def systematic_resample(*args, **kwargs):
+    """Name of the systematic resampler (src/Base/MonteCarlo/resample.jl:96-112); pass it as resample_func."""
+    raise TypeError("pass systematic_resample as resample_func of resample or mcpropagate")
+
+
+def multinomial_resample(*args, **kwargs):
+    """Name of the multinomial resampler (src/Base/MonteCarlo/resample.jl:64-78); pass it as resample_func."""
+    raise TypeError("pass multinomial_resample as resample_func of resample or mcpropagate")
+
+
+def _resample_method(resample_func, squared):
+    if resample_func is None:
+        return _lib.PP_RESAMPLE_MULTINOMIAL if squared else _lib.PP_RESAMPLE_SEMIDETERMINISTIC
+    table = {semideterministic_systematic_resample: _lib.PP_RESAMPLE_SEMIDETERMINISTIC,
+             systematic_resample: _lib.PP_RESAMPLE_SYSTEMATIC, multinomial_resample: _lib.PP_RESAMPLE_MULTINOMIAL}
+    if resample_func not in table:
+        raise ValueError(f"unknown resample_func {resample_func!r}")
+    return table[resample_func]
+
+
+def resample(psum, target_size, **kwargs):
+    """resample!(psum, target_size; resample_func, squared) in place on a HIPPauliSum; returns it."""
+    _needs_device(psum, "resample").resample(target_size, **kwargs)
+    return psum
+
+
+def mcpropagate(circuit, psum, thetas=None, max_size=None, resampling_size=None, resample_func=None,
+                squared: bool = False, rng=None, max_weight=math.inf, min_abs_coeff=1e-10, heisenberg: bool = True,
+                **resample_kwargs):
+    """mcpropagate(circuit, psum, thetas; max_size, resampling_size, resample_func)
+    (src/Base/MonteCarlo/mcpropagate.jl): propagate gate by gate and, whenever the
+    sum grows past max_size, resample it to resampling_size (default
+    round(max_size / 2)). Returns the HIPPauliSum."""
+    if max_size is None:
+        raise TypeError("mcpropagate needs max_size")
+    resampling_size = round(max_size / 2) if resampling_size is None else int(resampling_size)
+    P = countparameters(circuit)
+    th = np.zeros((0, 1)) if thetas is None else np.asarray(thetas, dtype=np.float64)
+    if th.ndim == 1:
+        th = th.reshape(-1, 1)
+    if th.shape[0] != P:
+        raise ValueError(f"the circuit has {P} parameters, got {th.shape[0]}")
+    if not isinstance(psum, HIPPauliSum):
+        psum = HIPPauliSum.from_paulisum(psum, batch=1)
+    if psum.batch != 1:
+        raise ValueError("mcpropagate takes one coefficient column")
+    order = toheisenberg if heisenberg else toschrodinger
+    hcirc, hth = order(circuit, list(th) if P else [])
+    rng = np.random.default_rng() if rng is None else rng
+    pth = np.zeros((max(P, 1), 1))
+    if P:
+        pth[:P] = np.asarray(hth, dtype=np.float64).reshape(P, 1)
+    gates, lut_c, lut_s, nluts = pack_circuit(hcirc, psum.nqubits, psum.W)
+    lc = np.ascontiguousarray(lut_c or [0], dtype=np.uint64)
+    ls = np.ascontiguousarray(lut_s or [0.0], dtype=np.float64)
+    cnt = np.zeros(1, dtype=np.int64)
+    size = ctypes.sizeof(gates._type_)
+    for g in range(len(hcirc)):
+        g0 = ctypes.cast(ctypes.addressof(gates) + g * size, ctypes.POINTER(type(gates[0])))
+        _lib.check(_lib.lib().pp_run_circuit(psum._h, g0, 1, _dp(pth), P, _u64p(lc), _dp(ls), nluts,
+                                             _max_weight_arg(max_weight), float(min_abs_coeff),
+                                             cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        if len(psum) > max_size:
+            psum.resample(resampling_size, resample_func=resample_func, squared=squared, rng=rng, **resample_kwargs)
+    return psum
+
+
 def overlapwithplus(psum):
     """<+|psum|+> (src/stateoverlap.jl:44)."""
     return _needs_device(psum, "overlapwithplus").overlapwithplus()
@@ -446,5 +542,6 @@ def key_to_int(row) -> int:
     return limbs_to_int(row)
 
 
-__all__ = ["HIPPauliSum", "propagate", "overlapwithzero", "overlapwithplus", "overlapwithcomputational",
+__all__ = ["HIPPauliSum", "propagate", "mcpropagate", "resample", "semideterministic_systematic_resample",
+           "systematic_resample", "multinomial_resample", "overlapwithzero", "overlapwithplus", "overlapwithcomputational",
            "overlapwithmaxmixed", "overlapwithpaulisum", "getcoeff", "scalarproduct", "run_heisenberg", "key_words"]
