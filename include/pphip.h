@@ -205,6 +205,18 @@ enum pp_overlap_kind {
  * and overlapwithzero (:35) on a VectorPauliSum. */
 int pp_overlap(pp_handle *h, int kind, const uint64_t *ones, double *out_b);
 
+/* The largest |c| of every column (0 for an empty sum).
+ * Replaces: maxabscoeff(cache) (src/Base/Primitives/maxabscoeff.jl:7). */
+int pp_max_abs_coeff(pp_handle *h, double *out_b);
+
+/* Truncate every term: weight above max_weight (-1: none), or coefficient
+ * below the threshold -- min_abs_coeff, or with min_rel_coeff >= 0 (B = 1)
+ * max(min_rel_coeff * maxabscoeff, min_abs_coeff); batched sums drop a term
+ * when max_b |c_b| is below it. *threshold_out (may be NULL) receives it.
+ * Replaces: truncate!(cache; min_abs_coeff, max_weight, min_rel_coeff)
+ *           (src/Propagation/generics.jl:192-250). */
+int pp_truncate(pp_handle *h, int max_weight, double min_abs_coeff, double min_rel_coeff, double *threshold_out);
+
 /* The B coefficients of one Pauli string (key_words limbs), zeros when it is
  * absent. overlapwithmaxmixed is pp_get_coeff of the identity (key 0).
  * Replaces: getcoeff(::VectorPauliSum, trm) (src/Base/termsum.jl:104-124) and

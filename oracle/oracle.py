@@ -50,6 +50,8 @@ def lib():
                                     ctypes.c_double]),
             ("ppo_overlap_zero", None, [_P, _DP]),
             ("ppo_overlap_kind", None, [_P, ctypes.c_int, _U64P, _DP]),
+            ("ppo_max_abs", None, [_P, _DP]),
+            ("ppo_truncate", ctypes.c_double, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             ("ppo_resample", ctypes.c_double, [_P, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_double,
                                                _DP, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
             ("ppo_get_coeff", None, [_P, _U64P, _DP]),
@@ -187,6 +189,14 @@ class OracleSum:
             raise ValueError("bad resampling arguments")
         return sp
 
+    def max_abs(self):
+        out = np.zeros(self.B, dtype=np.float64)
+        lib().ppo_max_abs(self._s, dp(out))
+        return out
+
+    def truncate(self, max_weight=math.inf, min_abs=0.0, min_rel=None):
+        return lib().ppo_truncate(self._s, _mw(max_weight), float(min_abs), -1.0 if min_rel is None else float(min_rel))
+
     def get_coeff(self, key_limbs):
         k = np.ascontiguousarray(key_limbs, dtype=np.uint64)
         out = np.zeros(self.B, dtype=np.float64)
@@ -214,7 +224,8 @@ class OracleSum:
 
 
 def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
-                     batch: int = 1, W: int | None = None, heisenberg: bool = True, normalize_coeffs: bool = True):
+                     batch: int = 1, W: int | None = None, heisenberg: bool = True, normalize_coeffs: bool = True,
+                     min_rel_coeff=None):
     """Propagate a circuit (given in Schrodinger order) on the oracle; returns
     (OracleSum, per-gate n_in counts). thetas: P or P x B. heisenberg=False
     propagates a state in circuit order with transposed gates."""
@@ -227,7 +238,7 @@ def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math
 
     W = W or key_words(nqubits)
     P = countparameters(circuit)
-    th = np.zeros((0, batch)) if thetas is None else np.asarray(thetas, dtype=np.float64).reshape(P, -1)
+    th = np.zeros((0, batch)) if thetas is None or P == 0 else np.asarray(thetas, dtype=np.float64).reshape(P, -1)
     hc, hth = (toheisenberg if heisenberg else toschrodinger)(circuit, list(th))
     hth = np.array(hth).reshape(P, -1) if P else np.zeros((0, batch))
     gates, lc, ls, _ = pack_circuit(hc, nqubits, W, normalize_coeffs=normalize_coeffs)
@@ -235,7 +246,29 @@ def oracle_propagate(circuit, nqubits, thetas, init_terms: dict, max_weight=math
     keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
     coeffs = np.array([np.broadcast_to(c, (batch,)) for c in init_terms.values()], dtype=np.float64).reshape(-1, batch)
     s.set(keys, coeffs)
-    counts = s.run(gates, len(hc), hth, P, lc, ls, max_weight, min_abs_coeff)
+    if min_rel_coeff is None:
+        counts = s.run(gates, len(hc), hth, P, lc, ls, max_weight, min_abs_coeff)
+        return s, counts
+    # a relative threshold: every gate but a Clifford branches and merges without
+    # truncating, then truncate! runs with max(min_rel * maxabscoeff, min_abs)
+    # (src/Propagation/specializations.jl:367-377, :225-229); a Clifford truncates
+    # as without it (:154-177)
+    counts = np.zeros(len(hc), dtype=np.int64)
+    size = ctypes.sizeof(gates._type_)
+    k = 0
+    for g in range(len(hc)):
+        one = ctypes.cast(ctypes.addressof(gates) + g * size, ctypes.POINTER(type(gates[0])))
+        npar = 1 if gates[g].param >= 0 else 0
+        row = hth[gates[g].param:gates[g].param + 1] if npar else np.zeros((1, batch))
+        save = gates[g].param
+        if npar:
+            gates[g].param = 0
+        if gates[g].kind == 3:
+            counts[g] = s.run(one, 1, row, 1, lc, ls, max_weight, min_abs_coeff)[0]
+        else:
+            counts[g] = s.run(one, 1, row, 1, lc, ls, math.inf, 0.0)[0]
+            s.truncate(max_weight, min_abs_coeff, min_rel_coeff)
+        gates[g].param = save
     return s, counts
 
 

@@ -632,6 +632,41 @@ int ppo_scalar_product(const ppo_sum *a, const ppo_sum *b, double *out) {
     return 0;
 }
 
+/* maxabscoeff (src/Base/Primitives/maxabscoeff.jl:7): per column */
+void ppo_max_abs(const ppo_sum *s, double *out) {
+    int B = s->B;
+    for (int b = 0; b < B; ++b) out[b] = 0.0;
+    for (size_t i = 0; i < s->n; ++i)
+        for (int b = 0; b < B; ++b) out[b] = fmax(out[b], fabs(s->coeffs[i * B + b]));
+}
+
+/* truncate! (src/Propagation/generics.jl:192-250): drop terms over max_weight
+ * or under the coefficient threshold max(min_rel * maxabscoeff, min_abs) (a
+ * negative min_rel: min_abs alone); B = 1 for a relative threshold. Returns
+ * the threshold used. */
+double ppo_truncate(ppo_sum *s, int max_weight, double min_abs, double min_rel) {
+    int W = s->W, B = s->B;
+    double thr = min_abs;
+    if (min_rel >= 0) {
+        double m;
+        ppo_max_abs(s, &m);
+        thr = fmax(min_rel * m, min_abs);
+    }
+    size_t o = 0;
+    for (size_t i = 0; i < s->n; ++i) {
+        uint64_t *k = s->keys + i * W;
+        double *c = s->coeffs + i * B;
+        if (truncated(k, c, W, B, max_weight, thr)) continue;
+        if (o != i) {
+            memmove(s->keys + o * W, k, W * sizeof(uint64_t));
+            memmove(s->coeffs + o * B, c, B * sizeof(double));
+        }
+        ++o;
+    }
+    s->n = o;
+    return thr;
+}
+
 /* ------------------------------------------------------------------ */
 /* resampling (src/Base/MonteCarlo/resample.jl)                        */
 /* ------------------------------------------------------------------ */

@@ -78,6 +78,10 @@ void ppo_overlap_kind(const ppo_sum *s, int kind, const uint64_t *ones, double *
  * comb offset in [0, 1). Returns the comb spacing (weight per draw), or -1. */
 double ppo_resample(ppo_sum *s, int method, size_t target, int squared, double offset, const double *draws,
                     int calibrate, double rtol, double atol);
+/* per-column largest |c| */
+void ppo_max_abs(const ppo_sum *s, double *out);
+/* truncate! with max(min_rel * maxabscoeff, min_abs) (min_rel < 0: min_abs); returns the threshold */
+double ppo_truncate(ppo_sum *s, int max_weight, double min_abs, double min_rel);
 /* the coefficients of `key` (zeros when absent) */
 void ppo_get_coeff(const ppo_sum *s, const uint64_t *key, double *out);
 /* sum over common strings of coefficient products; -1 on mismatched W or B */

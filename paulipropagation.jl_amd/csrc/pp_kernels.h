@@ -1822,6 +1822,30 @@ __global__ __launch_bounds__(WG) void k_overlapB(Bufs bf, const Ctl* ctl, uint32
     }
 }
 
+// maxabscoeff (src/Base/Primitives/maxabscoeff.jl:7) per column: a wave per
+// row, lanes over columns; partial is gridDim x B.
+__global__ __launch_bounds__(WG) void k_maxabs(Bufs bf, const Ctl* ctl, uint32_t seq, int B,
+                                               double* __restrict__ partial) {
+    extern __shared__ double s_m[];  // NWAVES x B
+    const int p = seq & 1;
+    const uint32_t n = ctl->st[p].n;
+    const double* __restrict__ coeffs = bf.coeffs[ctl->st[p].cur];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    for (int b0 = 0; b0 < B; b0 += 64) {
+        const int b = b0 + lane;
+        double m = 0.0;
+        for (uint32_t i = blockIdx.x * NWAVES + wave; i < n; i += gridDim.x * NWAVES)
+            if (b < B) m = fmax(m, fabs(coeffs[(size_t)i * B + b]));
+        if (b < B) s_m[wave * B + b] = m;
+    }
+    __syncthreads();
+    for (int b = t; b < B; b += WG) {
+        double m = 0.0;
+        for (int w = 0; w < NWAVES; ++w) m = fmax(m, s_m[w * B + b]);
+        partial[(size_t)blockIdx.x * B + b] = m;
+    }
+}
+
 // First position in the sorted sum whose term is not below `key` (n if none).
 template <int W>
 __device__ __forceinline__ uint32_t sum_lower_bound(const uint64_t* __restrict__ keys, size_t stride, uint32_t n,

@@ -1157,6 +1157,27 @@ int overlap_impl(pp_handle* h, double* out, const Readout& rd = Readout{}) {
     return PP_OK;
 }
 
+// Per-column largest |c| into out (B doubles).
+int maxabs_impl(pp_handle* h, double* out) {
+    const int B = h->B;
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    for (int b = 0; b < B; ++b) out[b] = 0.0;
+    if (n == 0) return PP_OK;
+    const unsigned g = std::min<unsigned>(1024u, (n + NWAVES - 1) / NWAVES);
+    if (int rc = ensure_partial(h, (size_t)g * B)) return rc;
+    hipLaunchKernelGGL(k_maxabs, dim3(g), dim3(WG), (size_t)NWAVES * B * sizeof(double), h->stream, h->bufs,
+                       h->d_ctl, h->seq, B, h->d_partial);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)g * B);
+    HIP_TRY(hipMemcpyAsync(part.data(), h->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (unsigned k = 0; k < g; ++k)
+        for (int b = 0; b < B; ++b) out[b] = std::fmax(out[b], part[(size_t)k * B + b]);
+    return PP_OK;
+}
+
 template <int W>
 int getcoeff_impl(pp_handle* h, const uint64_t* key, double* out) {
     if (int rc = sync_ctl(h)) return rc;
@@ -2071,6 +2092,27 @@ int pp_overlap(pp_handle* h, int kind, const uint64_t* ones, double* out_b) {
     case 2: return overlap_impl<2>(h, out_b, rd);
     default: return overlap_impl<4>(h, out_b, rd);
     }
+}
+
+int pp_max_abs_coeff(pp_handle* h, double* out_b) {
+    if (!h || !out_b) return fail(PP_ERR_ARG, "NULL argument");
+    return maxabs_impl(h, out_b);
+}
+
+int pp_truncate(pp_handle* h, int max_weight, double min_abs_coeff, double min_rel_coeff, double* threshold_out) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    double thr = min_abs_coeff;
+    if (min_rel_coeff >= 0) {
+        if (h->B != 1) return fail(PP_ERR_UNSUPPORTED, "a relative threshold takes one coefficient column");
+        double m = 0.0;
+        if (int rc = maxabs_impl(h, &m)) return rc;
+        thr = std::fmax(min_rel_coeff * m, min_abs_coeff);
+    }
+    if (threshold_out) *threshold_out = thr;
+    // Pauli noise that damps nothing: every term truncated, the rest compacted
+    std::vector<double> one((size_t)h->B, 1.0);
+    HostGate g{PP_GATE_PAULI_NOISE, nullptr, 0, 0u, 1.0, 0.0};
+    return single_gate(h, g, one.data(), nullptr, max_weight, thr, nullptr);
 }
 
 int pp_get_coeff(pp_handle* h, const uint64_t* key, double* out_b) {

@@ -14,7 +14,8 @@ from .circuits import (bricklayertopology, staircasetopology, rectangletopology,
                        hardwareefficientcircuit, efficientsu2circuit)
 from .hipsum import (HIPPauliSum, PackedCircuit, propagate, overlapwithzero, overlapwithplus,
                      overlapwithcomputational, overlapwithmaxmixed, overlapwithpaulisum, getcoeff, scalarproduct,
-                     run_heisenberg, adjointgradient, rewindgradient, mcpropagate, resample,
+                     run_heisenberg, adjointgradient, rewindgradient, mcpropagate, resample, truncate,
+                     maxabscoeff,
                      semideterministic_systematic_resample, systematic_resample, multinomial_resample)
 
 __version__ = "0.1.0"

@@ -71,6 +71,8 @@ SIGNATURES = [
     ("pp_overlap", ctypes.c_int, [_P, ctypes.c_int, _U64P, _DP]),
     ("pp_get_coeff", ctypes.c_int, [_P, _U64P, _DP]),
     ("pp_scalar_product", ctypes.c_int, [_P, _P, _DP]),
+    ("pp_max_abs_coeff", ctypes.c_int, [_P, _DP]),
+    ("pp_truncate", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, _DP]),
     ("pp_resample", ctypes.c_int, [_P, ctypes.c_int, _SZ, ctypes.c_int, ctypes.c_double, _DP, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, _DP]),
     ("pp_synchronize", ctypes.c_int, [_P]),

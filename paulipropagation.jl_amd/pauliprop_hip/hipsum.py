@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from . import _lib
-from .gates import countparameters, pack_circuit, toheisenberg, toschrodinger
+from .gates import CLIFFORD, countparameters, pack_circuit, toheisenberg, toschrodinger
 from .paulis import PauliString, PauliSum, int_to_limbs, key_words, limbs_to_int
 
 _U64P = ctypes.POINTER(ctypes.c_uint64)
@@ -231,6 +231,21 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_scalar_product(self._h, other._h, _dp(out)))
         return self._column(out)
 
+    def maxabscoeff(self):
+        """The largest |c| (per column) (src/Base/Primitives/maxabscoeff.jl:7)."""
+        out = np.zeros(self.batch, dtype=np.float64)
+        _lib.check(_lib.lib().pp_max_abs_coeff(self._h, _dp(out)))
+        return self._column(out)
+
+    def truncate(self, max_weight=math.inf, min_abs_coeff=1e-10, min_rel_coeff=None) -> float:
+        """truncate!(psum; max_weight, min_abs_coeff, min_rel_coeff) on the device
+        (src/Propagation/generics.jl:192-250); returns the coefficient threshold used."""
+        thr = ctypes.c_double()
+        rel = -1.0 if min_rel_coeff is None else float(min_rel_coeff)
+        _lib.check(_lib.lib().pp_truncate(self._h, _max_weight_arg(max_weight), float(min_abs_coeff), rel,
+                                          ctypes.byref(thr)))
+        return thr.value
+
     def resample(self, target_size: int, resample_func=None, squared: bool = False, rng=None,
                  calibrate: bool = True, rtol: float = 0.01, atol: float = 1.0, offset=None, draws=None) -> float:
         """resample!(psum, target_size; resample_func, squared) on the device
@@ -356,7 +371,7 @@ def run_heisenberg(psum: HIPPauliSum, hcircuit, hthetas, max_weight=math.inf, mi
 
 
 def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-10, batch: int = None,
-              counts: bool = False, heisenberg: bool = True, normalize_coeffs: bool = True):
+              counts: bool = False, heisenberg: bool = True, normalize_coeffs: bool = True, min_rel_coeff=None):
     """propagate(circuit, psum, thetas; max_weight, min_abs_coeff, heisenberg) on the GPU.
 
     `psum` is a PauliString, a PauliSum or a HIPPauliSum (propagated in place
@@ -367,6 +382,9 @@ def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-
     the Schrodinger picture (circuit order, transposed gates,
     src/Propagation/generics.jl:78-86); normalize_coeffs applies to
     ImaginaryPauliRotation gates (src/Propagation/specializations.jl:86).
+    min_rel_coeff (B = 1) truncates after every gate but a Clifford below
+    max(min_rel_coeff * maxabscoeff, min_abs_coeff), gate by gate
+    (src/Propagation/specializations.jl:367-377, :225-229).
     """
     P = countparameters(circuit)
     th = np.zeros((0, 1)) if thetas is None else np.asarray(thetas, dtype=np.float64)
@@ -382,9 +400,38 @@ def propagate(circuit, psum, thetas=None, max_weight=math.inf, min_abs_coeff=1e-
     order = toheisenberg if heisenberg else toschrodinger
     hcirc, hth = order(circuit, list(th) if P else [])
     hth = np.array(hth) if P else None
+    if min_rel_coeff is not None:
+        cnt = _run_relative(psum, hcirc, hth, max_weight, min_abs_coeff, min_rel_coeff, normalize_coeffs)
+        return (psum, cnt) if counts else psum
     cnt = run_heisenberg(psum, hcirc, hth, max_weight=max_weight, min_abs_coeff=min_abs_coeff, counts=counts,
                          normalize_coeffs=normalize_coeffs)
     return (psum, cnt) if counts else psum
+
+
+def _run_relative(psum, hcirc, hth, max_weight, min_abs_coeff, min_rel_coeff, normalize_coeffs):
+    """Gate by gate: a Clifford truncates as usual; any other gate applies and
+    merges without truncation, then truncate! with the relative threshold."""
+    if psum.batch != 1:
+        raise ValueError("min_rel_coeff takes one coefficient column")
+    P = countparameters(hcirc)
+    th = np.zeros((max(P, 1), 1))
+    if P:
+        th[:P] = np.asarray(hth, dtype=np.float64).reshape(P, 1)
+    gates, lut_c, lut_s, nluts = pack_circuit(hcirc, psum.nqubits, psum.W, normalize_coeffs=normalize_coeffs)
+    lc = np.ascontiguousarray(lut_c or [0], dtype=np.uint64)
+    ls = np.ascontiguousarray(lut_s or [0.0], dtype=np.float64)
+    cnt = np.zeros(max(len(hcirc), 1), dtype=np.int64)
+    size = ctypes.sizeof(gates._type_)
+    for g in range(len(hcirc)):
+        one = ctypes.cast(ctypes.addressof(gates) + g * size, ctypes.POINTER(type(gates[0])))
+        clifford = gates[g].kind == CLIFFORD
+        mw, mac = (max_weight, min_abs_coeff) if clifford else (math.inf, 0.0)
+        _lib.check(_lib.lib().pp_run_circuit(psum._h, one, 1, _dp(th), P, _u64p(lc), _dp(ls), nluts,
+                                             _max_weight_arg(mw), float(mac),
+                                             cnt[g:].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        if not clifford:
+            psum.truncate(max_weight, min_abs_coeff, min_rel_coeff)
+    return cnt[:len(hcirc)]
 
 
 def adjointgradient(circuit, psum, thetas, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0):
@@ -451,6 +498,17 @@ def _resample_method(resample_func, squared):
     if resample_func not in table:
         raise ValueError(f"unknown resample_func {resample_func!r}")
     return table[resample_func]
+
+
+def maxabscoeff(psum):
+    """maxabscoeff(psum) (src/Base/Primitives/maxabscoeff.jl:7)."""
+    return _needs_device(psum, "maxabscoeff").maxabscoeff()
+
+
+def truncate(psum, max_weight=math.inf, min_abs_coeff=1e-10, min_rel_coeff=None):
+    """truncate!(psum; max_weight, min_abs_coeff, min_rel_coeff) in place; returns the sum."""
+    _needs_device(psum, "truncate").truncate(max_weight, min_abs_coeff, min_rel_coeff)
+    return psum
 
 
 def resample(psum, target_size, **kwargs):
@@ -542,6 +600,6 @@ def key_to_int(row) -> int:
     return limbs_to_int(row)
 
 
-__all__ = ["HIPPauliSum", "propagate", "mcpropagate", "resample", "semideterministic_systematic_resample",
+__all__ = ["HIPPauliSum", "propagate", "mcpropagate", "resample", "truncate", "maxabscoeff", "semideterministic_systematic_resample",
            "systematic_resample", "multinomial_resample", "overlapwithzero", "overlapwithplus", "overlapwithcomputational",
            "overlapwithmaxmixed", "overlapwithpaulisum", "getcoeff", "scalarproduct", "run_heisenberg", "key_words"]
