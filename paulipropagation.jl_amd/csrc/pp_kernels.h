@@ -559,18 +559,10 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
 template <int W, bool BRANCH>
 __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask, Key<W> nmask,
                                                 uint32_t has_next, uint32_t check) {
-    // the first tile's entries (tile = blockIdx.x) go out with the control
-    // block; indices are clamped into the buffer and the values sanitized below
-    uint32_t psrc[K2_ITEMS];
-    Key<W> pkey[K2_ITEMS];
-    double pcf[K2_ITEMS];
-#pragma unroll
-    for (int i = 0; i < K2_ITEMS; ++i) {
-        const uint32_t x = min(blockIdx.x * (uint32_t)K2_TILE + threadIdx.x * K2_ITEMS + i, bf.cap - 1);
-        psrc[i] = bf.tail[0][x];
-        pkey[i] = load_key<W>(bf.tl_keys, bf.cap, x);
-        pcf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
-    }
+    // The list is read only once its length is known: a first tile loaded
+    // with the control block, as the merge loads its splits, makes every
+    // workgroup of the capacity-sized grid read a full tile, which on the
+    // many gates with short lists is tens of MB of stale entries.
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (c.overflow) return;
     const int p = L.seq & 1;
@@ -652,22 +644,12 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         double cf[K2_ITEMS];  // the source coefficients the merge will scale
         const uint32_t j0 = base + t * K2_ITEMS;
         int nvalid = 0;
-        if (tile == blockIdx.x) {
-            // prefetched (entries past the list are stale and never written out)
 #pragma unroll
-            for (int i = 0; i < K2_ITEMS; ++i) {
-                src[i] = psrc[i];
-                key[i] = pkey[i];
-                cf[i] = pcf[i];
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < K2_ITEMS; ++i) {
-                const uint32_t x = min(j0 + i, last);
-                src[i] = tail_in[x];
-                key[i] = load_key<W>(lkeys, stride, x);
-                cf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
-            }
+        for (int i = 0; i < K2_ITEMS; ++i) {
+            const uint32_t x = min(j0 + i, last);
+            src[i] = tail_in[x];
+            key[i] = load_key<W>(lkeys, stride, x);
+            cf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
         }
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) nvalid += (j0 + i <= last) ? 1 : 0;
