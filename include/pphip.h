@@ -302,7 +302,7 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * skips its own pass over the sum; 0 turns this off (results are identical).
  * PP_TUNE_SKIP_IDENTITY != 0 (the default) lets pp_run_circuit leave out the
  * rotations that touch no term, found from which Paulis occur on each qubit
- * (read from the device every 64 gates and widened gate by gate); they count
+ * (read from the device every 128 gates and widened gate by gate); they count
  * in pp_profile.identity_gates. 0 turns this off (results are identical). */
 enum pp_tune_key {
     PP_TUNE_SPIN_LIMIT = 1,

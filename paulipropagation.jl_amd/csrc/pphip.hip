@@ -52,7 +52,7 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr size_t MAX_TERMS = 0x7fffffffu;  // indices carry a sign bit
-constexpr size_t RUN_CHUNK = 64;           // gates enqueued between overflow checks
+constexpr size_t RUN_CHUNK = 128;          // gates enqueued between overflow checks
 constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4 limbs
 
 }  // namespace
