@@ -616,9 +616,9 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
     const int64_t const_off = (int64_t)lv.top_runs * na;
     const uint32_t ntiles = (na + K2_TILE - 1) / K2_TILE;
     __shared__ Key<W> s_last[WG];
-    __shared__ Key<W> s_kb, s_kl;
+    __shared__ Key<W> s_kb, s_kl, s_kp, s_kn;
     __shared__ int s_edge_s[MAX_LEVELS], s_edge_e[MAX_LEVELS];
-    __shared__ int s_wagg[NWAVES];
+    __shared__ int s_wmax[MAX_LEVELS][NWAVES], s_wmin[MAX_LEVELS][NWAVES];  // per level and wave
     __shared__ int s_lvbit[MAX_LEVELS], s_lvsgn[MAX_LEVELS];
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     // the levels go to shared memory with constant indices: indexing the kernel
@@ -651,6 +651,11 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             key[i] = load_key<W>(lkeys, stride, x);
             cf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
         }
+        // the entries just before and after the tile, loaded with it for every level's edges
+        const bool has_prev = base > 0, has_after = last + 1 < na;
+        Key<W> kout{};
+        if (t == 0 && has_prev) kout = key_at(base - 1);
+        else if (t == WG - 1 && has_after) kout = key_at(last + 1);
 #pragma unroll
         for (int i = 0; i < K2_ITEMS; ++i) nvalid += (j0 + i <= last) ? 1 : 0;
         // the last valid item, selected with constant indices (a variable index
@@ -663,6 +668,8 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         // tile edges: the block around `base` and the block around `last`, per level
         if (t == 0) s_kb = key[0];
         if (nvalid > 0 && j0 + nvalid - 1 == last) s_kl = klast;
+        if (t == 0 && has_prev) s_kp = kout;
+        if (t == WG - 1 && has_after) s_kn = kout;
         __syncthreads();
         const Key<W> kb = s_kb, kl = s_kl;
         for (int task = wave; task < 2 * lv.n; task += NWAVES) {
@@ -670,16 +677,16 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             const int bit = s_lvbit[v];
             if ((task & 1) == 0) {
                 uint32_t s = 0;
-                if (base > 0) {
-                    const Key<W> kp = key_at(base - 1);
+                if (has_prev) {
+                    const Key<W> kp = s_kp;
                     if (!eq_from<W>(kp, kb, bit)) s = base;
                     else s = wave_first_true_back(0u, base, [&](uint32_t x) { return eq_from<W>(key_at(x), kb, bit); });
                 }
                 if (lane == 0) s_edge_s[v] = (int)s;
             } else {
                 uint32_t e = na;
-                if (last + 1 < na) {
-                    const Key<W> kn = key_at(last + 1);
+                if (has_after) {
+                    const Key<W> kn = s_kn;
                     if (!eq_from<W>(kn, kl, bit)) e = last + 1;
                     else e = wave_first_true_fwd(last + 1, na, [&](uint32_t x) { return !eq_from<W>(key_at(x), kl, bit); });
                 }
@@ -689,7 +696,7 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         // previous entry of this thread's first item
         Key<W> prev;
         if (t > 0) prev = s_last[t - 1];
-        else if (base > 0) prev = key_at(base - 1);
+        else if (has_prev) prev = s_kp;
         else prev = key[0];
         __syncthreads();
 
@@ -720,23 +727,22 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 el[i] = nxt;
                 if (h[i]) nxt = (int)(j0 + i);
             }
-            // exclusive max-scan of `run` over threads, exclusive suffix-min of `nxt`
+            // exclusive max-scan of `run` over threads, exclusive suffix-min of
+            // `nxt`: both waves' totals go out together, in this level's own
+            // slots, so one barrier serves both and none is needed after
             const int incl = wave_incl_max(run);
             int wex = __shfl_up(incl, 1, 64);
             if (lane == 0) wex = -1;
-            if (lane == 63) s_wagg[wave] = incl;
-            __syncthreads();
-            int pre = wex;
-            for (int w = 0; w < wave; ++w) pre = max(pre, s_wagg[w]);
-            __syncthreads();
             const int sincl = wave_suffix_min(nxt);
             int sex = __shfl_down(sincl, 1, 64);
             if (lane == 63) sex = INF_POS;
-            if (lane == 0) s_wagg[wave] = sincl;
+            if (lane == 63) s_wmax[v][wave] = incl;
+            if (lane == 0) s_wmin[v][wave] = sincl;
             __syncthreads();
+            int pre = wex;
+            for (int w = 0; w < wave; ++w) pre = max(pre, s_wmax[v][w]);
             int suf = sex;
-            for (int w = wave + 1; w < NWAVES; ++w) suf = min(suf, s_wagg[w]);
-            __syncthreads();
+            for (int w = wave + 1; w < NWAVES; ++w) suf = min(suf, s_wmin[v][w]);
             const int es = s_edge_s[v], ee = s_edge_e[v];
             const int sg = s_lvsgn[v];
 #pragma unroll
