@@ -40,7 +40,7 @@ struct Bufs {
     double* coeffs[2];   // cap x B
     uint32_t* tail[2];   // cap entries each
     uint64_t* tkeys;     // W planes of cap words: the permuted tail's keys (term ^ mask)
-    double* tcoef;       // B = 1: the permuted tail's source coefficients (null for B > 1)
+    double* tcoef;       // B = 1: the permuted tail's values, source coefficient x tscale x sign (null for B > 1)
     uint64_t* tl_keys;   // W planes of cap words: keys of the terms listed in tail[0], in list order
     double* tl_coef;     // B = 1: their coefficients (null for B > 1)
     uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile
@@ -546,7 +546,10 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
 }
 
 // The permutation of the touched terms into ascending order of term ^ mask
-// (rank formula above), the tail keys, coefficients and partition samples.
+// (rank formula above), the tail keys, values and partition samples. With one
+// coefficient column a created term's value (source coefficient times tscale,
+// the gate's sin, sinh or gamma, negated for a negative sign) is stored in
+// tcoef and its index is not needed; with B columns the index goes to tail[1].
 //
 // BRANCH: the gate's k_branch was not launched, because the touched terms are
 // listed already (by the previous gate's merge, or by this kernel of the
@@ -558,7 +561,7 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
 // gate's.
 template <int W, bool BRANCH>
 __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask, Key<W> nmask,
-                                                uint32_t has_next, uint32_t check) {
+                                                uint32_t has_next, uint32_t check, double tscale) {
     // The list is read only once its length is known: a first tile loaded
     // with the control block, as the merge loads its splits, makes every
     // workgroup of the capacity-sized grid read a full tile, which on the
@@ -757,8 +760,15 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             if (i < nvalid) {
                 // a rank outside [0, na) would be a bug; it is reported, never written
                 if (dest[i] >= 0 && dest[i] < (int64_t)na) {
-                    tail_out[dest[i]] = src[i];
-                    if (bf.tcoef) bf.tcoef[dest[i]] = cf[i];
+                    if (bf.tcoef) {
+                        // one column: the created term's value itself, so the
+                        // merge reads neither the index nor the sign
+                        double v = cf[i] * tscale;
+                        if (src[i] & SRC_NEG) v = -v;
+                        bf.tcoef[dest[i]] = v;
+                    } else {
+                        tail_out[dest[i]] = src[i];
+                    }
                     if (bf.tpj) bf.tpj[dest[i]] = j0 + i;
 #pragma unroll
                     for (int l = 0; l < W; ++l) bf.tkeys[l * stride + dest[i]] = key[i].l[l] ^ mask.l[l];
@@ -911,7 +921,6 @@ __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
     static_assert(TILE % 64 == 0, "tile boundaries must fall on sampled diagonals");
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (c.overflow) return;
-    const int p = L.seq & 1;
     const uint32_t na = c.s.na;
     if (!(c.s.na || c.s.run)) return;
     const uint32_t n = c.s.n;
@@ -955,15 +964,15 @@ __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
 
 // Loads a tile into shared memory with its merge values: slot 0 = head entry
 // i0-1, slots 1..ns = head entries i0..i1-1, slots ns+1..ns+1+nt = tail entries
-// j0..j1. A head value is the coefficient times cos when the term anticommutes,
-// a tail value the source coefficient times sin, negated for a negative sign
-// (src/Propagation/specializations.jl:40); the merge then only adds head + tail.
+// j0..j1. A head value is the coefficient times cos when the term anticommutes;
+// a tail value, the source coefficient times sin and negated for a negative
+// sign (src/Propagation/specializations.jl:40), was stored by the permutation
+// kernel. The merge then only adds head + tail.
 template <int W, int TILE>
 __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, const double* __restrict__ coeffs,
-                                            size_t stride, const uint32_t* __restrict__ tail,
-                                            const uint64_t* __restrict__ tkeys, uint32_t na, uint32_t i0,
-                                            uint32_t j0, int ns, int nt, const Key<W>& mask, double cosv,
-                                            double sinv, uint64_t (*s_key)[TILE + 2], double* s_val,
+                                            size_t stride, const uint64_t* __restrict__ tkeys, uint32_t na,
+                                            uint32_t i0, uint32_t j0, int ns, int nt, const Key<W>& mask,
+                                            double cosv, uint64_t (*s_key)[TILE + 2], double* s_val,
                                             const uint32_t* __restrict__ tpj, uint32_t* s_pj, int damp_q,
                                             double fxy, const double* __restrict__ tcoef, int comm = 0) {
     const int t = threadIdx.x;
@@ -1001,10 +1010,8 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
         Key<W> k;
 #pragma unroll
         for (int l = 0; l < W; ++l) k.l[l] = tkeys[l * stride + gj];
-        const uint32_t src = tail[gj];
         const uint32_t pj = tpj ? tpj[gj] : 0u;
-        double c = tcoef[gj] * sinv;
-        if (src & SRC_NEG) c = -c;
+        const double c = tcoef[gj];
         if (y < nt + 1 && j0 + (uint32_t)y < na) {
 #pragma unroll
             for (int l = 0; l < W; ++l) s_key[l][ns + 1 + y] = k.l[l];
@@ -1075,9 +1082,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             ns = 0;
             nt = 0;
         }
-        merge_stage<W, TILE>(keys, coeffs, stride, bf.tail[1], bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv,
-                             rp.sinv, s_key, s_val, REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef,
-                             rp.comm);
+        merge_stage<W, TILE>(keys, coeffs, stride, bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv, s_key, s_val,
+                             REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef, rp.comm);
         __syncthreads();
         auto S = [&](int x) {
             Key<W> k;

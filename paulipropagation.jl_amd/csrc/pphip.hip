@@ -4,7 +4,8 @@
 //   keys[2]    W limb planes of cap uint64 each (limb l of term i at l*cap + i)
 //   coeffs[2]  cap x B doubles, row-major
 //   tail[2]    cap uint32 entries: index of an anticommuting term, sign in bit 31
-//   tkeys      W limb planes of cap words: permuted tail keys (term ^ mask)
+//   tkeys      W limb planes of cap words: permuted tail keys (term ^ mask);
+//              tcoef (B = 1) their values, tail[1] (B > 1) their indices
 //   tl_keys    W limb planes of cap words: the terms listed in tail[0], in list
 //              order (tl_coef: their coefficients, B = 1)
 //   states     one look-back word per tile of the largest compaction launch
@@ -511,10 +512,10 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         launch(h, E(2, all), nullptr, k_xorperm<W, true>,
                lb_grid(h, (const void*)k_xorperm<W, true>, 0, h->cap, K2_TILE), 0, h->bufs, h->d_ctl,
                lb_launch(h, glog), lv, mask, next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u,
-               listed ? 1u : 0u);
+               listed ? 1u : 0u, s1);
     else
         launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-               h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
+               h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, s1);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
@@ -578,7 +579,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
     L2.glog = glog;
     const bool part = h->B == 1;
     launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
+           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, sh);
     if (normalize)
         hipLaunchKernelGGL(k_imag_norm<W>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, mask, h->B,
                            ch, sh, d_trig, h->d_norm);
@@ -638,7 +639,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
     L2.glog = glog;
     const bool part = h->B == 1;
     launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u);
+           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, gamma);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{1 - gamma, gamma, max_weight, min_abs, qubit, fxy};
