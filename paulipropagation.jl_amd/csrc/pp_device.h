@@ -172,28 +172,6 @@ __device__ __forceinline__ double wave_max_f64(double v) {
     return v;
 }
 
-// inclusive scan over the wave of an int with max / min
-__device__ __forceinline__ int wave_incl_max(int v) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int u = __shfl_up(v, o, 64);
-        if (lane >= o) v = max(v, u);
-    }
-    return v;
-}
-
-// inclusive suffix scan (lane l gets the min over lanes >= l)
-__device__ __forceinline__ int wave_suffix_min(int v) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        int u = __shfl_down(v, o, 64);
-        if (lane + o < 64) v = min(v, u);
-    }
-    return v;
-}
-
 __device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
     const int lane = lane_id();
 #pragma unroll

@@ -270,6 +270,36 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* s_w, ui
     return pre + incl - v;
 }
 
+// block_excl_sum for a packed pair of small counts (each below 1 << BITS:
+// low half and high half of v): the wave's prefix is read off one ballot per
+// bit instead of a shuffle scan.
+template <int BITS>
+__device__ __forceinline__ uint32_t block_excl_sum_small(uint32_t v, uint32_t* s_w, uint32_t& total) {
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int b = 0; b < BITS; ++b) {
+            const uint64_t m = __ballot((v >> (16 * h + b)) & 1u);
+            pre += (uint32_t)__popcll(m & lt) << (16 * h + b);
+            tot += (uint32_t)__popcll(m) << (16 * h + b);
+        }
+    }
+    if (lane == 0) s_w[wave] = tot;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; ++w) {
+        base += (w < wave) ? s_w[w] : 0u;
+        all += s_w[w];
+    }
+    total = all;
+    __syncthreads();
+    return base + pre;
+}
+
 // ============================================================ k_branch
 
 constexpr int K1_ITEMS = 16;
@@ -731,16 +761,28 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
                 if (h[i]) nxt = (int)(j0 + i);
             }
             // exclusive max-scan of `run` over threads, exclusive suffix-min of
-            // `nxt`: both waves' totals go out together, in this level's own
-            // slots, so one barrier serves both and none is needed after
-            const int incl = wave_incl_max(run);
-            int wex = __shfl_up(incl, 1, 64);
-            if (lane == 0) wex = -1;
-            const int sincl = wave_suffix_min(nxt);
-            int sex = __shfl_down(sincl, 1, 64);
-            if (lane == 63) sex = INF_POS;
-            if (lane == 63) s_wmax[v][wave] = incl;
-            if (lane == 0) s_wmin[v][wave] = sincl;
+            // `nxt`. Head positions grow with the lane, so the max over earlier
+            // lanes is the `run` of the nearest earlier lane holding a head, and
+            // the min over later lanes the `nxt` of the nearest later one: one
+            // ballot and one lane read each. Both waves' totals go out together,
+            // in this level's own slots, so one barrier serves both and none is
+            // needed after.
+            const uint64_t hm = __ballot(run >= 0);
+            const uint64_t below = hm & ((1ull << lane) - 1ull);
+            const uint64_t above = lane == 63 ? 0ull : (hm & (~0ull << (lane + 1)));
+            const int lo_src = below ? 63 - __clzll((long long)below) : 0;
+            const int hi_src = above ? __ffsll((long long)above) - 1 : 0;
+            int wex = __shfl(run, lo_src, 64);
+            int sex = __shfl(nxt, hi_src, 64);
+            if (!below) wex = -1;
+            if (!above) sex = INF_POS;
+            // the wave's totals, read with every lane active
+            const int wmax = __shfl(run, hm ? 63 - __clzll((long long)hm) : 0, 64);
+            const int wmin = __shfl(nxt, hm ? __ffsll((long long)hm) - 1 : 0, 64);
+            if (lane == 0) {
+                s_wmax[v][wave] = hm ? wmax : -1;
+                s_wmin[v][wave] = hm ? wmin : INF_POS;
+            }
             __syncthreads();
             int pre = wex;
             for (int w = 0; w < wave; ++w) pre = max(pre, s_wmax[v][w]);
@@ -1154,7 +1196,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         // kept count in the low half, touched count in the high half
         uint32_t agg;
         const uint32_t mine = (uint32_t)__popc(keep) | ((uint32_t)__popc(touch) << 16);
-        const uint32_t pre = block_excl_sum(mine, s_w, agg);
+        static_assert(ITEMS < 8, "per-thread counts fit 3 bits");
+        const uint32_t pre = block_excl_sum_small<3>(mine, s_w, agg);
         if (wave == 0) {
             if (EMIT) {
                 const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
@@ -1415,7 +1458,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         __syncthreads();
         const uint32_t mine = (t < TB) ? s_keep[t] : 0u;
         uint32_t agg;
-        const uint32_t pre = block_excl_sum(mine, s_w, agg);
+        const uint32_t pre = block_excl_sum_small<1>(mine, s_w, agg);
         if (wave == 0) {
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
@@ -1626,7 +1669,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         __syncthreads();
         const uint32_t mine = (t < KNB_ROWS) ? s_keep[t] : 0u;
         uint32_t agg;
-        const uint32_t pre = block_excl_sum(mine, s_w, agg);
+        const uint32_t pre = block_excl_sum_small<1>(mine, s_w, agg);
         if (t < KNB_ROWS) s_pos[t] = pre;
         if (wave == 0) {
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);

@@ -220,7 +220,8 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
         }
         const uint32_t mine = (uint32_t)__popc(keep) | ((uint32_t)__popc(segs) << 16);
         uint32_t agg;
-        const uint32_t pre = block_excl_sum(mine, s_w, agg);
+        static_assert(KC_ITEMS < 8, "per-thread counts fit 3 bits");
+        const uint32_t pre = block_excl_sum_small<3>(mine, s_w, agg);
         if (wave == 0) {
             const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
             uint64_t e2[2];
@@ -350,7 +351,8 @@ __global__ __launch_bounds__(WG) void k_reduce_runs(Bufs bf, Ctl* ctl, Launch L,
             if (!((max_weight >= 0 && weight<W>(kk[k]) > max_weight) || m < min_abs)) keep |= 1u << k;
         }
         uint32_t agg;
-        const uint32_t pre = block_excl_sum((uint32_t)__popc(keep), s_w, agg);
+        static_assert(KR_ITEMS < 8, "per-thread counts fit 3 bits");
+        const uint32_t pre = block_excl_sum_small<3>((uint32_t)__popc(keep), s_w, agg);
         if (wave == 0) {
             const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
             if (lane == 0) {
