@@ -160,16 +160,58 @@ __device__ __forceinline__ uint32_t code_at(const Key<W>& k, int q) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+// Sum over a full wave by data-parallel primitives: prefix sums inside each
+// row of 16 lanes (row_shr steps, zeros shifted in at the row start), then
+// the four row totals read as scalars. No LDS traffic, unlike shuffles.
+__device__ __forceinline__ uint64_t wave_sum_u64_dpp(uint64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#define PP_ROW_STEP(CTRL)                                                                            \
+    {                                                                                                \
+        const uint32_t tl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, CTRL, 0xF, 0xF, true); \
+        const uint32_t th = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, CTRL, 0xF, 0xF, true); \
+        const uint64_t sum = (((uint64_t)hi << 32) | lo) + (((uint64_t)th << 32) | tl);             \
+        lo = (uint32_t)sum;                                                                          \
+        hi = (uint32_t)(sum >> 32);                                                                  \
+    }
+    PP_ROW_STEP(0x111)  // row_shr:1
+    PP_ROW_STEP(0x112)  // row_shr:2
+    PP_ROW_STEP(0x114)  // row_shr:4
+    PP_ROW_STEP(0x118)  // row_shr:8
+#undef PP_ROW_STEP
+    uint64_t total = 0;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t rl = (uint32_t)__builtin_amdgcn_readlane((int)lo, 16 * r + 15);
+        const uint32_t rh = (uint32_t)__builtin_amdgcn_readlane((int)hi, 16 * r + 15);
+        total += ((uint64_t)rh << 32) | rl;
+    }
+    return total;
 }
 
-__device__ __forceinline__ double wave_max_f64(double v) {
+// Maximum over a full wave of values >= 0 (zeros shifted in are neutral), by
+// the same row steps and row reads as wave_sum_u64_dpp.
+__device__ __forceinline__ double wave_max_nonneg_f64(double v) {
+#define PP_ROW_STEP(CTRL)                                                                              \
+    {                                                                                                  \
+        const uint64_t u = (uint64_t)__double_as_longlong(v);                                          \
+        const uint32_t tl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, true); \
+        const uint32_t th = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true); \
+        v = fmax(v, __longlong_as_double((long long)(((uint64_t)th << 32) | tl)));                     \
+    }
+    PP_ROW_STEP(0x111)  // row_shr:1
+    PP_ROW_STEP(0x112)  // row_shr:2
+    PP_ROW_STEP(0x114)  // row_shr:4
+    PP_ROW_STEP(0x118)  // row_shr:8
+#undef PP_ROW_STEP
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    double m = 0.0;
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t rl = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 16 * r + 15);
+        const uint32_t rh = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 16 * r + 15);
+        m = fmax(m, __longlong_as_double((long long)(((uint64_t)rh << 32) | rl)));
+    }
+    return m;
 }
 
 __device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
@@ -281,7 +323,7 @@ __device__ __forceinline__ bool lb_window(const uint64_t* words, size_t cstride,
         const uint64_t pm = __ballot(st == 2u);
         const int P = pm ? (__ffsll((unsigned long long)pm) - 1) : count;
         ok = ok && (!active[c] || __ballot(lane < P && st == 0u) == 0);
-        s[c] = wave_sum_u64(lane <= P && valid ? (w[c] & LB_VAL_MASK) : 0ull);
+        s[c] = wave_sum_u64_dpp(lane <= P && valid ? (w[c] & LB_VAL_MASK) : 0ull);
         f[c] = P < count;
     }
     if (!ok) return false;

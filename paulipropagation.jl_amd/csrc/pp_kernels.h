@@ -1452,7 +1452,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                 s_rows[(size_t)e * B + b] = c;
                 m = fmax(m, fabs(c));
             }
-            m = wave_max_f64(m);
+            m = wave_max_nonneg_f64(m);
             if (lane == 0 && m < min_abs) s_keep[e] = 0u;
         }
         __syncthreads();
@@ -1661,7 +1661,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
                     if (damped) c = c * factor[b];
                     m = fmax(m, fabs(c));
                 }
-                m = wave_max_f64(m);
+                m = wave_max_nonneg_f64(m);
                 keep = !((max_weight >= 0 && weight<W>(kk) > max_weight) || m < min_abs);
             }
             if (lane == 0) s_keep[r] = keep;
