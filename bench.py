@@ -62,6 +62,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="kicked_ising_heavyhex_127q",
                     choices=["tfim_brick_32q", "kicked_ising_heavyhex_127q", "heisenberg_square_64q_b64"])
+    ap.add_argument("--layers", type=int, default=None,
+                    help="heisenberg_square_64q_b64 only: its first LAYERS steps (the full ten outgrow one GPU)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the single-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
@@ -90,7 +92,10 @@ def main() -> int:
     from pauliprop_hip import workloads
     from pauliprop_hip.hipsum import PackedCircuit
 
-    wl = workloads.WORKLOADS[args.workload]()
+    if args.layers is not None and args.workload != "heisenberg_square_64q_b64":
+        ap.error("--layers applies to heisenberg_square_64q_b64 only")
+    wl = (workloads.WORKLOADS[args.workload]() if args.layers is None
+          else workloads.heisenberg_square_64q_b64(nsteps=args.layers))
     nq, B = wl["nqubits"], wl.get("batch", 1)
     mw, mac = wl["max_weight"], wl["min_abs_coeff"]
     pc = PackedCircuit(wl["circuit"], wl["thetas"], nq, batch=B)
@@ -179,7 +184,7 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload_text(args.workload, wl), "nqubits": nq, "key_bits": 64 * pc.W, "batch": B,
+        "config": {"workload": workload_text(args.workload, wl, args.layers), "nqubits": nq, "key_bits": 64 * pc.W, "batch": B,
                    "gates": pc.ngates, "max_weight": mw, "min_abs_coeff": mac},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -289,7 +294,10 @@ def adjoint_detail(pp, PackedCircuit):
     return out
 
 
-def workload_text(name, wl) -> str:
+def workload_text(name, wl, layers=None) -> str:
+    if layers is not None:
+        return (f"Heisenberg on the 8x8 periodic square lattice (64 qubits), first {layers} of 10 steps, B=64, "
+                "sum ZZ, UInt128 keys")
     return {
         "tfim_brick_32q": "TFIM on a 32-qubit periodic brick, 32 layers, theta 0.1, Z16, UInt64 keys",
         "kicked_ising_heavyhex_127q": "kicked Ising on the 127-qubit heavy-hex graph, 20 x (RX(U[0,pi/2]) on 127 qubits + RZZ(-pi/2) on 144 "
