@@ -91,3 +91,26 @@ def test_propagate_with_relative_threshold_matches_oracle(nq, rel):
         if rel == 0.0:
             plain = pp.propagate(circ, obs, th, max_weight=mw, min_abs_coeff=mac)
             assert_bitwise(*dev.download(), *plain.download(), "min_rel_coeff=0 vs none")
+
+
+def test_schrodinger_circuit_with_relative_threshold_matches_oracle():
+    # heisenberg=False: gates in circuit order, imaginary rotations normalized,
+    # every gate but a Clifford followed by truncate! with min_rel_coeff
+    print("Schrodinger circuit with min_rel_coeff", flush=True)
+    nq = 12
+    circ = []
+    for layer in range(3):
+        pp.rxlayer(circ, nq)
+        for q in (2, 6, 9):
+            circ.append(pp.ImaginaryPauliRotation(["Z", "Z"], [q, q + 1]))
+        circ.append(pp.CliffordGate("CNOT", [layer + 3, layer + 4]))
+        pp.rzzlayer(circ, pp.bricklayertopology(nq))
+        pp.depolarizinglayer(circ, nq, 0.02)
+    th = np.random.default_rng(31).uniform(-0.5, 0.5, size=pp.countparameters(circ))
+    rho = pp.PauliSum(nq, {0: 1.0, symboltoint(nq, ["Z"], [4]): 0.3})
+    for rel in (1e-5, 1e-4, 1e-3):
+        dev = pp.propagate(circ, rho, th, max_weight=6, min_abs_coeff=1e-7, min_rel_coeff=rel, heisenberg=False)
+        ref, _ = oracle_propagate(circ, nq, th, rho.terms, max_weight=6, min_abs_coeff=1e-7, min_rel_coeff=rel,
+                                  heisenberg=False)
+        assert_bitwise(*dev.download(), *ref.get(), f"Schrodinger min_rel_coeff={rel}")
+        assert len(dev) > 100
