@@ -484,7 +484,7 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
 
 // ============================================================ k_xorperm
 
-constexpr int K2_ITEMS = 4;
+constexpr int K2_ITEMS = 2;  // 1, 3, 4 and 8 measured slower on the kicked Ising workload
 constexpr int K2_TILE = WG * K2_ITEMS;
 constexpr int INF_POS = 0x7fffffff;
 
