@@ -302,7 +302,7 @@ __device__ __forceinline__ uint32_t block_excl_sum_small(uint32_t v, uint32_t* s
 
 // ============================================================ k_branch
 
-constexpr int K1_ITEMS = 16;
+constexpr int K1_ITEMS = 4;  // 8 and 16 measured slower (most on batched sums, where every rotation branches)
 constexpr int K1_TILE = WG * K1_ITEMS;
 
 // End of the branch step, once the touched count na is known: check room for
@@ -425,8 +425,7 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
         __syncthreads();
         const uint64_t excl = s_excl;
         const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-#pragma unroll
-        for (int k = 0; k < K1_ITEMS; ++k) {
+        for (int k = 0; k < K1_ITEMS; ++k) {  // unrolled by the compiler (no scratch)
             if ((ballots[k] >> lane) & 1ull) {
                 const uint32_t i = base + k * WG + t;
                 const uint32_t pos = (uint32_t)excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);

@@ -165,6 +165,8 @@ int tile_rows(int B) {
 }
 
 // look-back words for the largest compaction launch over up to cap entries
+// (every look-back kernel's tiles are at least this wide)
+static_assert(K1_TILE >= K3Cfg<4>::TILE, "branch tiles narrower than the merge's would outnumber the look-back words");
 size_t tiles_needed(size_t cap, int B) {
     const size_t tile = B > 1 ? (size_t)std::min(KNB_ROWS, tile_rows(B)) : (size_t)K3Cfg<4>::TILE;
     return cap / tile + 64;
