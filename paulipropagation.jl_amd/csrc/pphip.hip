@@ -158,8 +158,7 @@ void free_terms(pp_handle* h) {
 
 int tile_rows(int B) {
     if (B <= 16) return 256;
-    if (B <= 32) return 128;
-    if (B <= 64) return 64;
+    if (B <= 64) return 64;  // 128 rows for B = 32 measured 7 % slower (a noisy forward run), 32 rows for B = 64 20 % slower
     if (B <= 128) return 32;
     return 16;
 }
