@@ -2150,4 +2150,21 @@ __global__ __launch_bounds__(WG) void k_adj_final(const double* __restrict__ gpa
     }
 }
 
+// Every gate of a segment of the sweep: its blocks' partial gradients summed
+// per column in block order, as k_adj_final sums one gate's (one workgroup per
+// gate; a gate with no blocks keeps its zero).
+__global__ __launch_bounds__(WG) void k_adj_final_all(const double* __restrict__ gpart,
+                                                      const uint32_t* __restrict__ nbs, uint32_t gblocks, int B,
+                                                      double* __restrict__ out) {
+    const uint32_t s = blockIdx.x;
+    const int nb = (int)nbs[s];
+    if (nb == 0) return;
+    const double* __restrict__ p = gpart + (size_t)s * gblocks * B;
+    for (int b = threadIdx.x; b < B; b += WG) {
+        double acc = 0.0;
+        for (int k = 0; k < nb; ++k) acc += p[(size_t)k * B + b];
+        out[(size_t)s * B + b] = acc;
+    }
+}
+
 }  // namespace pp

@@ -1458,6 +1458,7 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     double* d_a[2] = {nullptr, nullptr};
     double* d_gpart = nullptr;
     double* d_dgrad = nullptr;
+    uint32_t* d_nbs = nullptr;
     std::vector<Snapshot> snaps;
     Snapshot final_sum;
     int rc = PP_OK;
@@ -1467,6 +1468,7 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
         if (d_a[1]) hipFree(d_a[1]);
         if (d_gpart) hipFree(d_gpart);
         if (d_dgrad) hipFree(d_dgrad);
+        if (d_nbs) hipFree(d_nbs);
         for (auto& sn : snaps) free_snapshot(sn);
         free_snapshot(final_sum);
     };
@@ -1475,11 +1477,20 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     };
     const int lgBP = B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1));
     const unsigned gblocks = h->gmax;
+    // The blocks' partial gradients of every gate of a segment are kept and
+    // summed by one launch after the segment's sweep (one workgroup per gate,
+    // blocks in order: the sums a launch per gate would give), unless that
+    // takes more than ADJ_PART_BYTES; then each gate sums its own.
+    constexpr size_t ADJ_PART_BYTES = (size_t)2 << 30;
+    const bool deferred = ngates && seg * (size_t)gblocks * B * sizeof(double) <= ADJ_PART_BYTES;
+    const size_t pslots = deferred ? seg : 1;
+    std::vector<uint32_t> nbs(deferred ? seg : 0, 0u);
     do {
         if (ngates) {
             if (hipMalloc(&d_atrig, atrig.size() * sizeof(double)) != hipSuccess ||
                 hipMalloc(&d_dgrad, ngates * (size_t)B * sizeof(double)) != hipSuccess ||
-                hipMalloc(&d_gpart, (size_t)gblocks * B * sizeof(double)) != hipSuccess) {
+                hipMalloc(&d_gpart, pslots * gblocks * B * sizeof(double)) != hipSuccess ||
+                (deferred && hipMalloc(&d_nbs, seg * sizeof(uint32_t)) != hipSuccess)) {
                 rc = fail(PP_ERR_NOMEM, "adjoint: device allocation failed");
                 break;
             }
@@ -1537,6 +1548,7 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
             }
             // the sweep through this segment, last gate first
             const size_t g0 = kk * seg, g1 = std::min(ngates, (kk + 1) * seg);
+            if (deferred) std::fill(nbs.begin(), nbs.end(), 0u);
             for (size_t g = g1; g-- > g0;) {
                 const uint32_t* lg = h->h_log + LOG_WORDS * g;
                 const uint32_t nin = lg[0], na = lg[1], nout = lg[2], roff = lg[3], coff = lg[4];
@@ -1549,11 +1561,15 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
                     if (na == 0 || nin == 0) continue;  // identity
                     const unsigned rows = WG >> lgBP;
                     const unsigned nb = (unsigned)std::min<size_t>(gblocks, ((size_t)nin + na + rows - 1) / rows);
+                    double* part = d_gpart + (deferred ? (g - g0) * (size_t)gblocks * B : 0);
                     hipLaunchKernelGGL(k_adj_rotation, dim3(nb), dim3(WG), 0, h->stream, d_a[acur], d_a[acur ^ 1],
                                        h->d_rec + roff, nin, na, h->d_rcoef + (size_t)coff * B, B, lgBP, tr,
-                                       d_gpart);
-                    hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(WG), 0, h->stream, d_gpart, (int)nb, B,
-                                       d_dgrad + g * (size_t)B);
+                                       part);
+                    if (deferred)
+                        nbs[g - g0] = nb;
+                    else
+                        hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(WG), 0, h->stream, d_gpart, (int)nb, B,
+                                           d_dgrad + g * (size_t)B);
                 } else {
                     if (nin == 0) continue;
                     hipLaunchKernelGGL(k_adj_noise, dim3(grid_for(h, (size_t)nin * B, WG * 4)), dim3(WG), 0,
@@ -1566,6 +1582,20 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
                 acur ^= 1;
             }
             if (rc) break;
+            if (deferred && g1 > g0) {
+                if (hipMemcpyAsync(d_nbs, nbs.data(), (g1 - g0) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                   h->stream) != hipSuccess) {
+                    rc = fail(PP_ERR_HIP, "adjoint: staging failed");
+                    break;
+                }
+                hipLaunchKernelGGL(k_adj_final_all, dim3((unsigned)(g1 - g0)), dim3(WG), 0, h->stream, d_gpart,
+                                   d_nbs, gblocks, B, d_dgrad + g0 * (size_t)B);
+                // nbs is refilled for the next segment only once the copy has been read
+                if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
+                    rc = fail(PP_ERR_HIP, "adjoint: gradient sums failed");
+                    break;
+                }
+            }
         }
         if (rc) break;
         // per-gate gradients, summed into parameters in gate order
