@@ -1424,6 +1424,59 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             s_rt[t] = ~0u;
         }
         __syncthreads();
+        if (B <= 64) {
+            // one column per lane: the next row's coefficients are loaded
+            // while this row is finished, so a wave keeps two rows' loads in
+            // flight (same values and operations as the loop below)
+            const int b = lane;
+            const bool col = b < B;
+            auto load_row = [&](int e, double& hv, double& tv) {
+                hv = 0.0;
+                tv = 0.0;
+                if (e < TB && s_keep[e] && col) {
+                    const int head = s_head[e];
+                    const uint32_t tl = s_tl[e];
+                    if (head != -1) hv = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + b];
+                    if (tl != ~0u) tv = coeffs[(size_t)(tl & SRC_MASK) * B + b];
+                }
+            };
+            double ch, ct;
+            load_row(wave, ch, ct);
+            for (int e = wave; e < TB; e += NWAVES) {
+                double nh, nt;
+                load_row(e + NWAVES, nh, nt);
+                if (s_keep[e]) {
+                    const int head = s_head[e];
+                    const uint32_t tl = s_tl[e];
+                    double m = 0.0;
+                    if (col) {
+                        double c = 0.0;
+                        bool have = false;
+                        if (head != -1) {
+                            if (head >= 0) {
+                                c = ch;
+                                if (s_hxy[e]) c = c * fxy;
+                            } else {
+                                c = ch * cosb[b];
+                            }
+                            have = true;
+                        }
+                        if (tl != ~0u) {
+                            double tc = ct * sinb[b];
+                            if (tl & SRC_NEG) tc = -tc;
+                            c = have ? c + tc : tc;
+                        }
+                        if (norm) c = c * norm[b];
+                        s_rows[(size_t)e * B + b] = c;
+                        m = fabs(c);
+                    }
+                    m = wave_max_nonneg_f64(m);
+                    if (lane == 0 && m < min_abs) s_keep[e] = 0u;
+                }
+                ch = nh;
+                ct = nt;
+            }
+        } else
         for (int e = wave; e < TB; e += NWAVES) {
             if (!s_keep[e]) continue;
             const int head = s_head[e];
