@@ -1701,6 +1701,43 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)KNB_ROWS;
+        if (B <= 64) {
+            // one column per lane: the next row's key and coefficient are
+            // loaded while this row is decided (same operations as below)
+            const int b = lane;
+            const bool col = b < B;
+            auto load_row = [&](int r, Key<W>& k, double& c) {
+                const uint32_t i = base + r;
+                if (r < KNB_ROWS && i < n) {
+                    k = load_key<W>(keys, stride, i);
+                    c = col ? coeffs[(size_t)i * B + b] : 0.0;
+                }
+            };
+            Key<W> ck{};
+            double cc = 0.0;
+            load_row(wave, ck, cc);
+            for (int r = wave; r < KNB_ROWS; r += NWAVES) {
+                Key<W> nk{};
+                double nc = 0.0;
+                load_row(r + NWAVES, nk, nc);
+                const uint32_t i = base + r;
+                uint32_t keep = 0;
+                if (i < n) {
+                    const bool damped = (damp_codes >> code_at<W>(ck, qubit)) & 1u;
+                    double m = 0.0;
+                    if (col) {
+                        double c = cc;
+                        if (damped) c = c * factor[b];
+                        m = fabs(c);
+                    }
+                    m = wave_max_nonneg_f64(m);
+                    keep = !((max_weight >= 0 && weight<W>(ck) > max_weight) || m < min_abs);
+                }
+                if (lane == 0) s_keep[r] = keep;
+                ck = nk;
+                cc = nc;
+            }
+        } else
         for (int r = wave; r < KNB_ROWS; r += NWAVES) {
             const uint32_t i = base + r;
             uint32_t keep = 0;
