@@ -7,6 +7,14 @@ knows only the Pauli matrices and the gate definitions
                                                heisenbergschrodinger.jl:131-136)
   PauliNoise on qubit q: the Pauli channel whose Heisenberg action damps the
   listed Paulis by (1 - lambda) (src/Gates/noisechannels.jl:24-50).
+  CliffordGate: the unitary of each symbol as the reference's own Yao test
+  builds it (ext/PauliPropagationYao.jl:8-60, test/test_gates_against_yao.jl:
+  117-160): H, S = diag(1, i), X, Y, Z, SX = Rx(pi/2), SY = Ry(pi/2), CNOT and
+  CZ with the first qubit as control, SWAP, ZZpihalf = rot(Z (x) Z, -pi/2) =
+  exp(+i pi/4 Z (x) Z); Heisenberg action O -> U^dag O U.
+  AmplitudeDampingNoise(gamma): the Kraus channel K0 = diag(1, sqrt(1-gamma)),
+  K1 = sqrt(gamma) |0><1| (Yao's AmplitudeDampingError, used by the same
+  test); Heisenberg action O -> sum_k K_k^dag O K_k.
 Qubit q (1-based) is tensor factor q counted from the least significant bit of
 the computational-basis index.
 """
@@ -75,6 +83,77 @@ def _qubit_component(op, q, code, n):
     return np.moveaxis(comp, (0, 1), (ax_r, ax_c)).reshape(dim, dim)
 
 
+def embed(G, qs, n: int) -> np.ndarray:
+    """The 2^n x 2^n matrix of a gate G on the 1-based qubits qs; G's basis
+    index holds qubit qs[a] at bit a."""
+    dim = 2 ** n
+    k = len(qs)
+    M = np.zeros((dim, dim), dtype=complex)
+    for i in range(dim):
+        for j in range(dim):
+            rest = all(((i >> (q - 1)) & 1) == ((j >> (q - 1)) & 1) for q in range(1, n + 1) if q not in qs)
+            if not rest:
+                continue
+            si = sum(((i >> (q - 1)) & 1) << a for a, q in enumerate(qs))
+            sj = sum(((j >> (q - 1)) & 1) << a for a, q in enumerate(qs))
+            M[i, j] = G[si, sj]
+    assert k == int(round(np.log2(G.shape[0])))
+    return M
+
+
+def _rot(P, theta):
+    return np.cos(theta / 2) * np.eye(P.shape[0]) - 1j * np.sin(theta / 2) * P
+
+
+def _cnot():
+    G = np.zeros((4, 4), dtype=complex)
+    for c in range(2):
+        for t in range(2):
+            G[c | ((t ^ c) << 1), c | (t << 1)] = 1
+    return G
+
+
+def _swap():
+    G = np.zeros((4, 4), dtype=complex)
+    for a in range(2):
+        for b in range(2):
+            G[b | (a << 1), a | (b << 1)] = 1
+    return G
+
+
+ZZ = np.kron(Z, Z)
+CLIFFORD_UNITARIES = {
+    "H": np.array([[1, 1], [1, -1]], dtype=complex) / np.sqrt(2),
+    "X": X, "Y": Y, "Z": Z,
+    "S": np.diag([1, 1j]).astype(complex),
+    "SX": _rot(X, np.pi / 2),
+    "SY": _rot(Y, np.pi / 2),
+    "CNOT": _cnot(),
+    "CZ": np.diag([1, 1, 1, -1]).astype(complex),
+    "SWAP": _swap(),
+    "ZZPIHALF": _rot(ZZ, -np.pi / 2),
+}
+CLIFFORD_UNITARIES["SZ"] = CLIFFORD_UNITARIES["S"]
+
+
+def heisenberg_step_unitary(op, U):
+    return U.conj().T @ op @ U
+
+
+def damping_kraus(gamma: float):
+    K0 = np.diag([1.0, np.sqrt(1 - gamma)]).astype(complex)
+    K1 = np.array([[0.0, np.sqrt(gamma)], [0.0, 0.0]], dtype=complex)
+    return K0, K1
+
+
+def heisenberg_step_damping(op, q, gamma, n):
+    out = np.zeros_like(op)
+    for K in damping_kraus(gamma):
+        Kf = embed(K, [q], n)
+        out += Kf.conj().T @ op @ Kf
+    return out
+
+
 def pauli_expand(op, n, tol=1e-15) -> dict:
     """Coefficients c_t = tr(P_t op) / 2^n for all 4^n Pauli strings above tol."""
     res = {}
@@ -96,14 +175,18 @@ def zero_state_expectation(op) -> float:
 
 
 def dense_propagate(heisenberg_gates, n, init_terms: dict):
-    """heisenberg_gates: list of ("rot", codes, theta) or ("noise", q, damped_codes, lam),
-    in propagation (Heisenberg) order."""
+    """heisenberg_gates: list of ("rot", codes, theta), ("noise", q, damped_codes, lam),
+    ("clifford", symbol, qs) or ("damp", q, gamma), in propagation (Heisenberg) order."""
     op = operator_from_terms(init_terms, n)
     for g in heisenberg_gates:
         if g[0] == "rot":
             op = heisenberg_step_rotation(op, g[1], g[2], n)
         elif g[0] == "noise":
             op = heisenberg_step_noise(op, g[1], g[2], g[3], n)
+        elif g[0] == "clifford":
+            op = heisenberg_step_unitary(op, embed(CLIFFORD_UNITARIES[g[1]], list(g[2]), n))
+        elif g[0] == "damp":
+            op = heisenberg_step_damping(op, g[1], g[2], n)
         else:
             raise ValueError(g[0])
     return op

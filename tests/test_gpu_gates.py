@@ -127,7 +127,9 @@ def test_tile_claim_modes_are_bit_identical(mode):
     ref.pauli_noise(6, 0b1110, 0.8, max_weight=10, min_abs=1e-3)
     assert_bitwise(*dev.download(), *ref.get(), f"{mode} noise")
     if mode == "tiny_spin_limit":
-        print("fallbacks", dev.profile()["fallbacks"], flush=True)
+        fb = dev.profile()["fallbacks"]
+        print("fallbacks", fb, flush=True)
+        assert fb > 0, "a one-poll wait limit never sent a gate to the atomic-claim rerun"
 
 
 @pytest.mark.parametrize("nq,qubit", [(32, 1), (32, 17), (64, 33), (127, 64), (127, 127)])

@@ -113,3 +113,139 @@ def test_batched_adjoint_columns_equal_single_runs():
         l1, g1 = oracle_adjoint(circ, n, th[:, b], obs, min_abs_coeff=0.0)
         assert loss[b] == pytest.approx(l1[0], abs=1e-15)
         np.testing.assert_allclose(grad[:, b], g1[:, 0], atol=1e-15)
+
+
+# ------------------------------------------------------------------ Clifford tables and amplitude damping
+# The Clifford tables (src/Gates/cliffordgates.jl:47-59) and the amplitude-damping
+# rule (src/Propagation/specializations.jl:258-276) checked against the gates'
+# matrices: U^dag P U for every Pauli P, and sum_k K_k^dag P K_k.
+
+BASE_CLIFFORDS = ["H", "X", "Y", "Z", "SX", "SY", "S", "SZ", "CNOT", "CZ", "SWAP", "ZZPIHALF"]
+
+
+def _single(expansion):
+    assert len(expansion) == 1, expansion
+    (q, v), = expansion.items()
+    assert abs(abs(v) - 1.0) < 1e-12, expansion
+    return q, int(round(v))
+
+
+@pytest.mark.parametrize("symbol", BASE_CLIFFORDS)
+def test_clifford_table_is_the_conjugation(symbol):
+    table = pp.CLIFFORD_MAP[symbol]
+    nq = 1 if len(table) == 4 else 2
+    U = dense.embed(dense.CLIFFORD_UNITARIES[symbol], list(range(1, nq + 1)), nq)
+    transposed = pp.transposecliffordmap(table)
+    for p in range(4 ** nq):
+        P = dense.pauli_matrix(dense.term_codes(p, nq), nq)
+        # Heisenberg action (the table itself)
+        assert _single(dense.pauli_expand(U.conj().T @ P @ U, nq, tol=1e-12)) == (table[p][0], table[p][1]), \
+            (symbol, p)
+        # Schrodinger action (the transposed table, src/Gates/cliffordgates.jl:94-98)
+        assert _single(dense.pauli_expand(U @ P @ U.conj().T, nq, tol=1e-12)) == (transposed[p][0], transposed[p][1]), \
+            (symbol, p)
+
+
+def test_clifford_table_check_catches_a_flipped_sign():
+    # a wrong but self-consistent entry (sign of S on X) is caught by the conjugation check
+    table = list(pp.CLIFFORD_MAP["S"])
+    table[1] = (table[1][0], -table[1][1])
+    U = dense.CLIFFORD_UNITARIES["S"]
+    P = dense.pauli_matrix({1: 1}, 1)
+    assert _single(dense.pauli_expand(U.conj().T @ P @ U, 1, tol=1e-12)) != table[1]
+    assert pp.transposecliffordmap(pp.transposecliffordmap(table)) == table  # the old check passes it
+
+
+@pytest.mark.parametrize("n", [1, 2, 3])
+@pytest.mark.parametrize("gamma", [0.0, 0.3, 1.0])
+def test_amplitude_damping_against_kraus_channel(n, gamma):
+    from oracle import OracleSum
+    from pauliprop_hip.paulis import int_to_limbs
+    rng = np.random.default_rng(100 * n + int(10 * gamma))
+    terms = list(range(4 ** n))
+    coeffs = rng.uniform(-1, 1, size=len(terms))
+    for q in range(1, n + 1):
+        s = OracleSum(1, 1)
+        s.set(np.array([int_to_limbs(t, 1) for t in terms], dtype=np.uint64), coeffs)
+        s.amplitude_damping(q - 1, gamma, max_weight=math.inf, min_abs=0.0)
+        op = dense.heisenberg_step_damping(dense.operator_from_terms(dict(zip(terms, coeffs)), n), q, gamma, n)
+        want = dense.pauli_expand(op, n, tol=0.0)
+        keys, c = s.get()
+        got = {int(k[0]): float(v[0]) for k, v in zip(keys, c)}
+        for t in set(want) | set(got):
+            assert abs(got.get(t, 0.0) - want.get(t, 0.0)) < 1e-12, (n, q, gamma, t)
+
+
+def test_amplitude_damping_closed_form():
+    # I -> I, X -> sqrt(1-g) X, Y -> sqrt(1-g) Y, Z -> (1-g) Z + g I, from the Kraus operators
+    g = 0.37
+    for code in range(4):
+        op = dense.heisenberg_step_damping(dense.pauli_matrix({1: code} if code else {}, 1), 1, g, 1)
+        got = dense.pauli_expand(op, 1, tol=1e-15)
+        want = {0: {0: 1.0}, 1: {1: math.sqrt(1 - g)}, 2: {2: math.sqrt(1 - g)}, 3: {3: 1 - g, 0: g}}[code]
+        assert set(got) == set(want) and all(abs(got[k] - want[k]) < 1e-14 for k in want), (code, got)
+
+
+def _mixed_circuit(n, depth, rng):
+    circ = []
+    syms1 = ["H", "X", "Y", "Z", "SX", "SY", "S"]
+    syms2 = ["CNOT", "CZ", "SWAP", "ZZPIHALF"]
+    for _ in range(depth):
+        kind = rng.integers(0, 6)
+        q = int(rng.integers(1, n + 1))
+        if kind == 0:
+            circ.append(pp.PauliRotation(str("XYZ"[rng.integers(0, 3)]), q))
+        elif kind == 1 and n >= 2:
+            a, b = rng.choice(np.arange(1, n + 1), size=2, replace=False)
+            circ.append(pp.PauliRotation([str("XYZ"[rng.integers(0, 3)]), str("XYZ"[rng.integers(0, 3)])],
+                                         [int(a), int(b)]))
+        elif kind == 2:
+            circ.append(pp.CliffordGate(syms1[rng.integers(0, len(syms1))], q))
+        elif kind == 3 and n >= 2:
+            a, b = rng.choice(np.arange(1, n + 1), size=2, replace=False)
+            circ.append(pp.CliffordGate(syms2[rng.integers(0, len(syms2))], [int(a), int(b)]))
+        elif kind == 4:
+            circ.append(pp.freeze(pp.AmplitudeDampingNoise(q), float(rng.uniform(0, 0.4))))
+        else:
+            circ.append(pp.DepolarizingNoise(q, float(rng.uniform(0, 0.2))))
+    return circ
+
+
+def _dense_mixed(hcirc, hth):
+    out, k = [], 0
+    for g in hcirc:
+        inner = g.gate if isinstance(g, pp.FrozenGate) else g
+        if isinstance(inner, pp.PauliRotation):
+            th = g.parameter if isinstance(g, pp.FrozenGate) else hth[k]
+            if not isinstance(g, pp.FrozenGate):
+                k += 1
+            out.append(("rot", {q: pp.paulis.SYMBOLS[s] for s, q in zip(inner.symbols, inner.qinds)}, th))
+        elif isinstance(inner, pp.CliffordGate):
+            out.append(("clifford", inner.symbol, inner.qinds))
+        elif isinstance(inner, pp.AmplitudeDampingNoise):
+            out.append(("damp", inner.qind, g.parameter))
+        else:
+            out.append(("noise", inner.qind, set(inner.damped), g.parameter))
+    return out
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_mixed_circuit_with_cliffords_and_damping_against_dense(seed):
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(2, 6))
+    circ = _mixed_circuit(n, 18, rng)
+    P = pp.countparameters(circ)
+    th = rng.uniform(-np.pi, np.pi, size=P)
+    obs = {pp.symboltoint(n, "Z", 1): 1.0, pp.symboltoint(n, ["X", "Y"], [1, n]): 0.5,
+           pp.symboltoint(n, "Y", n): -0.25}
+    s, _ = oracle_propagate(circ, n, th, obs, min_abs_coeff=0.0)
+    hc, hth = pp.toheisenberg(circ, list(th))
+    op = dense.dense_propagate(_dense_mixed(hc, hth), n, obs)
+    want = dense.pauli_expand(op, n, tol=1e-13)
+    keys, coeffs = s.get()
+    got = {}
+    for k, c in zip(keys, coeffs):
+        got[int(k[0])] = got.get(int(k[0]), 0.0) + float(c[0])
+    for t in set(want) | set(got):
+        assert abs(got.get(t, 0.0) - want.get(t, 0.0)) < 1e-12, (seed, t, got.get(t), want.get(t))
+    assert s.overlap_zero()[0] == pytest.approx(dense.zero_state_expectation(op), abs=1e-12)
