@@ -167,21 +167,49 @@ int pp_run_circuit(pp_handle *h, const pp_gate *gates, size_t ngates, const doub
                    const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
                    double min_abs_coeff, int64_t *counts);
 
-/* Loss and gradient by the adjoint sweep: loss_b = overlapwithzero of column
- * b after the Heisenberg-ordered gate list (as pp_run_circuit), and
- * grad[p * B + b] = d loss_b / d theta[p * B + b]. The forward run records,
- * per gate, where every input term went and the touched terms' coefficients;
- * the sweep then applies the transpose of each gate's 1 / cos / +-sin map to
- * the coefficient adjoints, last gate first, with every truncation held as it
- * happened (noise gates are linear maps without parameters). checkpoint_gates
- * = 0 records the whole run; k > 0 records k gates at a time, each segment run
- * again from a snapshot of its input sum (one extra forward run), so records
- * need room for k gates only. On return the handle holds the propagated sum.
- * Replaces: rewindgradient(circuit, psum, params, overlapwithzero; kwargs...)
- *           (src/gradient.jl:26-82), which rejects noise; here noise is allowed. */
+/* Loss and gradient by the adjoint sweep: loss_b = the overlap `overlap_kind`
+ * (enum pp_overlap_kind; `ones` as for pp_overlap) of column b after the
+ * Heisenberg-ordered gate list (as pp_run_circuit), and grad[p * B + b] =
+ * d loss_b / d theta[p * B + b]. The forward run records, per gate, where
+ * every input term went and the touched terms' coefficients; the sweep then
+ * applies the transpose of each gate's linear map to the coefficient adjoints,
+ * last gate first, with every truncation held as it happened: a rotation's
+ * 1 / cos / +-sin branch, a Pauli noise gate's damping factors, a Clifford's
+ * signed permutation. It is the exact derivative of the truncated forward map
+ * (wherever no coefficient crosses the threshold), which is what central
+ * differences of pp_run_circuit measure. checkpoint_gates = 0 records the
+ * whole run; k > 0 records k gates at a time, each segment run again from a
+ * snapshot of its input sum (one extra forward run), so records need room
+ * for k gates only. On return the handle holds the propagated sum.
+ * Amplitude damping and imaginary rotations are not taken (PP_ERR_UNSUPPORTED).
+ * Counterpart: rewindgradient(circuit, psum, params, overlapfunc; kwargs...)
+ *   (src/gradient.jl:26-108) computes the gradient of the same loss, equal to
+ *   this one when nothing is truncated; under truncation it differs, since it
+ *   truncates the operator again while undoing the gates. pp_rewind is that
+ *   algorithm. The reference rejects noise; here noise is allowed. */
 int pp_adjoint(pp_handle *h, const pp_gate *gates, size_t ngates, const double *theta, size_t nparams,
                const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
-               double min_abs_coeff, size_t checkpoint_gates, double *loss, double *grad);
+               double min_abs_coeff, size_t checkpoint_gates, int overlap_kind, const uint64_t *ones, double *loss,
+               double *grad);
+
+/* The reference gradient, rewindgradient!(circuit, psum, params, overlapfunc;
+ * max_weight, min_abs_coeff) (src/gradient.jl:40-108), on a B = 1 handle: the
+ * Heisenberg-ordered gate list (rotations, frozen rotations, Cliffords; noise
+ * is rejected with PP_ERR_UNSUPPORTED, as the reference rejects it) runs
+ * forward with truncation and *loss receives the overlap `overlap_kind` of the
+ * result; a dual sum starts on the result's strings with each string's own
+ * overlap; then the gates are undone last to first in their Schrodinger form
+ * (rotation angle negated, Clifford table transposed), the operator truncated
+ * as in the forward run and the dual not, the dual capped to the operator's
+ * strings after every gate, and for every parametrized rotation, before it is
+ * undone, grad[param] = sum over operator terms t anticommuting with its
+ * generator P of ((e & 2) - 1) c_t d(P xor t), P t = i^e (P xor t). Under
+ * truncation this differs from pp_adjoint's gradient (which holds the forward
+ * run's truncations fixed); without truncation both are exact. On return the
+ * handle holds the operator after the undo sweep, as the reference's cache. */
+int pp_rewind(pp_handle *h, const pp_gate *gates, size_t ngates, const double *theta, size_t nparams,
+              const uint64_t *lut_changes, const double *lut_signs, size_t nluts, int max_weight,
+              double min_abs_coeff, int overlap_kind, const uint64_t *ones, double *loss, double *grad);
 
 /* overlapwithzero for every column: sum of the coefficients of terms with no
  * X or Y, accumulated in double-double and rounded once.
@@ -192,7 +220,8 @@ int pp_overlap_zero(pp_handle *h, double *out_b);
 enum pp_overlap_kind {
     PP_OVERLAP_ZERO = 0,          /* |0...0>: overlapwithzero (src/stateoverlap.jl:35) */
     PP_OVERLAP_PLUS = 1,          /* |+...+>: overlapwithplus (src/stateoverlap.jl:44) */
-    PP_OVERLAP_COMPUTATIONAL = 2  /* a computational basis state: overlapwithcomputational (src/stateoverlap.jl:59-80) */
+    PP_OVERLAP_COMPUTATIONAL = 2, /* a computational basis state: overlapwithcomputational (src/stateoverlap.jl:59-80) */
+    PP_OVERLAP_MAXMIXED = 3       /* the maximally mixed state: overlapwithmaxmixed (src/stateoverlap.jl:87-93) */
 };
 
 /* Overlap of the sum with a product state, for every column: the sum over
@@ -201,8 +230,9 @@ enum pp_overlap_kind {
  * key layout): bit 2q set for every qubit q (0-based) in state |1>; terms with
  * X or Y vanish and the others count (-1)^(number of Z on those qubits).
  * `ones` may be NULL for the other kinds.
- * Replaces: overlapwithplus / overlapwithcomputational (src/stateoverlap.jl:44,59-80)
- * and overlapwithzero (:35) on a VectorPauliSum. */
+ * PP_OVERLAP_MAXMIXED reads the identity's coefficient.
+ * Replaces: overlapwithplus / overlapwithcomputational (src/stateoverlap.jl:44,59-80),
+ * overlapwithmaxmixed (:87-93) and overlapwithzero (:35) on a VectorPauliSum. */
 int pp_overlap(pp_handle *h, int kind, const uint64_t *ones, double *out_b);
 
 /* The largest |c| of every column (0 for an empty sum).
@@ -303,12 +333,17 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * PP_TUNE_SKIP_IDENTITY != 0 (the default) lets pp_run_circuit leave out the
  * rotations that touch no term, found from which Paulis occur on each qubit
  * (read from the device every 128 gates and widened gate by gate); they count
- * in pp_profile.identity_gates. 0 turns this off (results are identical). */
+ * in pp_profile.identity_gates. 0 turns this off (results are identical).
+ * PP_TUNE_ADJ_DEFER_BYTES bounds the device memory pp_adjoint may take to keep
+ * every rotation's per-block partial gradients of a segment and sum them in
+ * one launch (default 2 GiB); above it, or when that allocation fails, each
+ * rotation sums its own (results are identical). */
 enum pp_tune_key {
     PP_TUNE_SPIN_LIMIT = 1,
     PP_TUNE_DYNAMIC_TILES = 2,
     PP_TUNE_CHAIN_TAILS = 3,
-    PP_TUNE_SKIP_IDENTITY = 4
+    PP_TUNE_SKIP_IDENTITY = 4,
+    PP_TUNE_ADJ_DEFER_BYTES = 5
 };
 
 /* Set a tuning value. No counterpart in the reference. */

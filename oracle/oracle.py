@@ -58,8 +58,11 @@ def lib():
             ("ppo_scalar_product", ctypes.c_int, [_P, _P, _DP]),
             ("ppo_run", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
                                        ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_int64)]),
-            ("ppo_adjoint", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, ctypes.c_int,
-                                           ctypes.c_double, _DP, _DP]),
+            ("ppo_adjoint", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_int, _U64P, _DP, _DP]),
+            ("ppo_rewind", ctypes.c_int, [_P, ctypes.c_void_p, ctypes.c_size_t, _DP, ctypes.c_size_t, _U64P, _DP,
+                                          ctypes.c_int, ctypes.c_double, ctypes.c_int, _U64P, _P, ctypes.c_double,
+                                          _DP, _DP]),
             ("ppo_anticommutes", ctypes.c_int, [_U64P, _U64P, ctypes.c_int]),
             ("ppo_sign_exponent", ctypes.c_int, [_U64P, _U64P, ctypes.c_int]),
             ("ppo_weight", ctypes.c_int, [_U64P, ctypes.c_int]),
@@ -318,9 +321,11 @@ class MultiOracleSum:
 
 
 def oracle_adjoint(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
-                   batch: int = 1):
-    """(loss[B], grad[P x B]) of overlapwithzero(propagate(...)) w.r.t. the
-    Schrodinger-ordered parameters, by the oracle's reverse sweep."""
+                   batch: int = 1, overlap: str = "zero", onebitinds=()):
+    """(loss[B], grad[P x B]) of overlap(propagate(...)) w.r.t. the
+    Schrodinger-ordered parameters, by the oracle's reverse sweep with every
+    truncation held fixed; overlap "zero", "plus", "computational" (ones at
+    `onebitinds`) or "maxmixed"."""
     import sys
     pkg = os.path.join(os.path.dirname(HERE), "paulipropagation.jl_amd")
     if pkg not in sys.path:
@@ -333,16 +338,73 @@ def oracle_adjoint(circuit, nqubits, thetas, init_terms: dict, max_weight=math.i
     th = np.asarray(thetas, dtype=np.float64).reshape(P, -1)
     hc, hth = toheisenberg(circuit, list(th))
     hth = np.ascontiguousarray(np.array(hth).reshape(P, -1))
-    gates, _, _, _ = pack_circuit(hc, nqubits, W)
+    gates, lc, ls, _ = pack_circuit(hc, nqubits, W)
     s = OracleSum(W, batch)
     keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
     coeffs = np.array([np.broadcast_to(c, (batch,)) for c in init_terms.values()], dtype=np.float64).reshape(-1, batch)
     s.set(keys, coeffs)
     loss = np.zeros(batch)
     grad = np.zeros((max(P, 1), batch))
-    rc = lib().ppo_adjoint(s._s, ctypes.cast(gates, ctypes.c_void_p), len(hc), dp(hth), P, _mw(max_weight),
-                           float(min_abs_coeff), dp(loss), dp(grad))
+    ones_int = 0
+    for q in onebitinds:
+        ones_int |= 1 << (2 * (int(q) - 1))
+    ones = np.ascontiguousarray(int_to_limbs(ones_int, W), dtype=np.uint64)
+    lcv = np.ascontiguousarray(lc if len(lc) else [0], dtype=np.uint64)
+    lsv = np.ascontiguousarray(ls if len(ls) else [0.0], dtype=np.float64)
+    rc = lib().ppo_adjoint(s._s, ctypes.cast(gates, ctypes.c_void_p), len(hc), dp(hth), P, u64p(lcv), dp(lsv),
+                           _mw(max_weight), float(min_abs_coeff), READOUT_KINDS[overlap], u64p(ones), dp(loss),
+                           dp(grad))
     if rc != 0:
         raise ValueError("oracle adjoint rejected a gate")
     # Heisenberg parameter rows run in reverse Schrodinger order
     return loss, grad[:P][::-1].copy()
+
+
+READOUT_KINDS = {"zero": 0, "plus": 1, "computational": 2, "maxmixed": 3, "paulisum": 4}
+
+
+def oracle_rewind(circuit, nqubits, thetas, init_terms: dict, max_weight=math.inf, min_abs_coeff=1e-10,
+                  overlap: str = "zero", onebitinds=(), rho: dict | None = None):
+    """(expectation, grad[P]) of rewindgradient(circuit, psum, params, overlapfunc;
+    max_weight, min_abs_coeff) (src/gradient.jl:26-108), restated on the CPU.
+    overlap: "zero", "plus", "computational" (ones at the 1-based `onebitinds`),
+    "maxmixed", or "paulisum" (overlapwithpaulisum(rho, .), rho a dict of
+    terms). The gradient is in Schrodinger parameter order, as the reference
+    returns it."""
+    import sys
+    pkg = os.path.join(os.path.dirname(HERE), "paulipropagation.jl_amd")
+    if pkg not in sys.path:
+        sys.path.insert(0, pkg)
+    from pauliprop_hip.gates import countparameters, pack_circuit, toheisenberg
+    from pauliprop_hip.paulis import key_words, int_to_limbs
+
+    W = key_words(nqubits)
+    P = countparameters(circuit)
+    th = np.asarray(thetas, dtype=np.float64).reshape(P)
+    hc, hth = toheisenberg(circuit, list(th))
+    hth = np.ascontiguousarray(np.array(hth, dtype=np.float64).reshape(-1) if P else np.zeros(1))
+    gates, lc, ls, _ = pack_circuit(hc, nqubits, W)
+    s = OracleSum(W, 1)
+    keys = np.array([int_to_limbs(t, W) for t in init_terms], dtype=np.uint64).reshape(-1, W)
+    s.set(keys, np.array(list(init_terms.values()), dtype=np.float64))
+    kind = READOUT_KINDS[overlap]
+    ones_int = 0
+    for q in onebitinds:
+        ones_int |= 1 << (2 * (int(q) - 1))
+    ones = np.ascontiguousarray(int_to_limbs(ones_int, W), dtype=np.uint64)
+    rs, scale = None, 1.0
+    if kind == 4:
+        rs = OracleSum(W, 1)
+        rk = np.array([int_to_limbs(t, W) for t in rho], dtype=np.uint64).reshape(-1, W)
+        rs.set(rk, np.array(list(rho.values()), dtype=np.float64))
+        scale = 2.0 ** nqubits
+    lcv = np.ascontiguousarray(lc if len(lc) else [0], dtype=np.uint64)
+    lsv = np.ascontiguousarray(ls if len(ls) else [0.0], dtype=np.float64)
+    loss = np.zeros(1)
+    grad = np.zeros(max(P, 1))
+    rc = lib().ppo_rewind(s._s, ctypes.cast(gates, ctypes.c_void_p), len(hc), dp(hth), P, u64p(lcv), dp(lsv),
+                          _mw(max_weight), float(min_abs_coeff), kind, u64p(ones), rs._s if rs else None,
+                          scale, dp(loss), dp(grad))
+    if rc != 0:
+        raise ValueError("rewindgradient takes rotations and Clifford gates only (no noise)")
+    return float(loss[0]), grad[:P][::-1].copy()

@@ -583,9 +583,15 @@ static unsigned code_of(const uint64_t *key, int q) { return (unsigned)(key[q >>
  * (src/stateoverlap.jl:12-80): kind 0 |0..0> (X or Y anywhere -> 0), kind 1
  * |+..+> (Y or Z anywhere -> 0, src/stateoverlap.jl:44), kind 2 the basis
  * state with qubit q in |1> when bit 2q of `ones` is set (X or Y -> 0, else
- * (-1)^(number of Z on those qubits), _calcsignwithones :70-80). */
+ * (-1)^(number of Z on those qubits), _calcsignwithones :70-80); kind 3 the
+ * maximally mixed state (1 for the identity, else 0; overlapwithmaxmixed :87-93). */
 static int state_factor(const uint64_t *key, int W, int kind, const uint64_t *ones) {
     int f = 1;
+    if (kind == 3) { /* the maximally mixed state: only the identity counts (src/stateoverlap.jl:87-93) */
+        for (int l = 0; l < W; ++l)
+            if (key[l]) return 0;
+        return 1;
+    }
     for (int q = 0; q < 32 * W; ++q) {
         unsigned c = code_of(key, q);
         if (kind == 1) {
@@ -870,31 +876,59 @@ static ppo_sum *clone_sum(const ppo_sum *a) {
     return s;
 }
 
+static void merge_only(ppo_sum *s);
+
 int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const double *theta,
-                size_t nparams, int max_weight, double min_abs, double *loss, double *grad) {
+                size_t nparams, const uint64_t *lut_changes, const double *lut_signs, int max_weight, double min_abs,
+                int kind, const uint64_t *ones, double *loss, double *grad) {
     int W = s0->W, B = s0->B;
-    if (!s0->sorted) return -1;
+    if (!s0->sorted || kind < 0 || kind > 3) return -1;
     for (size_t g = 0; g < ngates; ++g)
-        if (gates[g].kind != PPO_ROTATION && gates[g].kind != PPO_PAULI_NOISE) return -1;
-    ppo_sum **st = xmalloc((ngates + 1) * sizeof(ppo_sum *));
+        if (gates[g].kind != PPO_ROTATION && gates[g].kind != PPO_PAULI_NOISE && gates[g].kind != PPO_CLIFFORD)
+            return -1;
+    const uint64_t zero_ones[4] = {0, 0, 0, 0};
+    if (!ones) ones = zero_ones;
     double *cv = xmalloc(B * sizeof(double));
     double *sv = xmalloc(B * sizeof(double));
-    st[0] = clone_sum(s0);
+    /* The sum before every S-th gate is kept (S ~ sqrt(ngates)); a segment's
+     * intermediate sums are computed again from its checkpoint when the sweep
+     * reaches it, so memory holds about 2 sqrt(ngates) sums. A Clifford leaves
+     * its output unsorted; it is sorted here for the lookups (the next
+     * branching gate sorts it anyway, with the same sums). */
+    size_t S = 1;
+    while (S * S < ngates) ++S;
+    const size_t nseg = (ngates + S - 1) / S;
+    ppo_sum **ck = xmalloc((nseg ? nseg : 1) * sizeof(ppo_sum *));
+    ppo_sum *fin = clone_sum(s0);
     for (size_t g = 0; g < ngates; ++g) {
-        st[g + 1] = clone_sum(st[g]);
-        apply_gate(st[g + 1], gates + g, theta, NULL, NULL, max_weight, min_abs, cv, sv);
+        if (g % S == 0) ck[g / S] = clone_sum(fin);
+        apply_gate(fin, gates + g, theta, lut_changes, lut_signs, max_weight, min_abs, cv, sv);
+        merge_only(fin);
     }
-    ppo_overlap_zero(st[ngates], loss);
+    for (int b = 0; b < B; ++b) loss[b] = 0.0;
+    for (size_t i = 0; i < fin->n; ++i) {
+        const int f = state_factor(fin->keys + i * W, W, kind, ones);
+        if (f)
+            for (int b = 0; b < B; ++b) loss[b] += f * fin->coeffs[i * B + b];
+    }
     memset(grad, 0, nparams * B * sizeof(double));
-    /* adjoint of the final sum: d overlap / d c_t */
-    ppo_sum *fin = st[ngates];
+    /* adjoint of the final sum: d overlap / d c_t, the readout's weight of t */
     double *adj = xmalloc((fin->n ? fin->n : 1) * B * sizeof(double));
     for (size_t i = 0; i < fin->n; ++i)
-        for (int b = 0; b < B; ++b) adj[i * B + b] = contains_xy_w(fin->keys + i * W, W) ? 0.0 : 1.0;
+        for (int b = 0; b < B; ++b) adj[i * B + b] = (double)state_factor(fin->keys + i * W, W, kind, ones);
     uint64_t pk[4];
-    for (size_t gg = ngates; gg-- > 0;) {
+    ppo_sum **st = xmalloc((S + 1) * sizeof(ppo_sum *));
+    for (size_t seg = nseg; seg-- > 0;) {
+      const size_t g0 = seg * S, g1 = g0 + S < ngates ? g0 + S : ngates;
+      st[0] = clone_sum(ck[seg]);
+      for (size_t g = g0; g < g1; ++g) {
+          st[g - g0 + 1] = clone_sum(st[g - g0]);
+          apply_gate(st[g - g0 + 1], gates + g, theta, lut_changes, lut_signs, max_weight, min_abs, cv, sv);
+          merge_only(st[g - g0 + 1]);
+      }
+      for (size_t gg = g1; gg-- > g0;) {
         const ppo_gate *g = gates + gg;
-        ppo_sum *in = st[gg], *out = st[gg + 1];
+        ppo_sum *in = st[gg - g0], *out = st[gg - g0 + 1];
         double *nadj = xmalloc((in->n ? in->n : 1) * B * sizeof(double));
         if (g->kind == PPO_ROTATION) {
             size_t na = 0;
@@ -927,6 +961,18 @@ int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const d
                         grad[(size_t)g->param * B + b] += as * (-c[b] * sv[b]) + ap * (c[b] * cv[b] * sign);
                 }
             }
+        } else if (g->kind == PPO_CLIFFORD) { /* a signed permutation */
+            const size_t base = (size_t)g->lut * 16;
+            const uint64_t *ch = lut_changes + base * W;
+            const double *sg = lut_signs + base;
+            for (size_t i = 0; i < in->n; ++i) {
+                const uint64_t *k = in->keys + i * W;
+                int idx = 0;
+                for (int a = 0; a < g->nq; ++a) idx |= get_pauli(k, g->qubits[a]) << (2 * a);
+                for (int l = 0; l < W; ++l) pk[l] = k[l] ^ ch[idx * W + l];
+                long ps = find_key(out->keys, out->n, pk, W);
+                for (int b = 0; b < B; ++b) nadj[i * B + b] = ps >= 0 ? adj[ps * B + b] * sg[idx] : 0.0;
+            }
         } else { /* PauliNoise: diagonal */
             for (size_t i = 0; i < in->n; ++i) {
                 const uint64_t *k = in->keys + i * W;
@@ -942,12 +988,161 @@ int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const d
         }
         free(adj);
         adj = nadj;
+      }
+      for (size_t j = 0; j <= g1 - g0; ++j) ppo_destroy(st[j]);
     }
     free(adj);
-    for (size_t g = 0; g <= ngates; ++g) ppo_destroy(st[g]);
+    for (size_t k = 0; k < nseg; ++k) ppo_destroy(ck[k]);
+    ppo_destroy(fin);
+    free(ck);
     free(st);
     free(cv);
     free(sv);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* rewindgradient: the reference gradient (src/gradient.jl)            */
+/* ------------------------------------------------------------------ */
+
+/* merge! on a sum the last step may have left unsorted: sort, sum equal
+ * strings, truncate nothing (src/Base/Merge/mergeandtruncate.jl:13-18). */
+static void merge_only(ppo_sum *s) {
+    if (!s->sorted) sort_reduce_truncate(s, NULL, NULL, 0, -1, 0.0);
+}
+
+/* _generatorcommutatordot (src/gradient.jl:137-153, 190-241): the sum over
+ * operator terms t that anticommute with the generator P of
+ * -0.5 * imag(comm) * c_t * dual(P xor t), where commutator(P, t) = 2 i^e
+ * (P xor t) (src/PauliAlgebra/paulioperations.jl:414-424) with e odd, so that
+ * -0.5 * imag(comm) = (e & 2) - 1. Both sums sorted; terms in operator order. */
+static double commutator_dot(const ppo_sum *op, const ppo_sum *dual, const uint64_t *mask) {
+    const int W = op->W;
+    double total = 0.0;
+    uint64_t pk[4];
+    for (size_t i = 0; i < op->n; ++i) {
+        const uint64_t *k = op->keys + i * W;
+        if (!anticommutes_w(mask, k, W)) continue;
+        for (int l = 0; l < W; ++l) pk[l] = k[l] ^ mask[l];
+        long j = find_key(dual->keys, dual->n, pk, W);
+        if (j < 0) continue;
+        double sgn = (double)((sign_exponent_w(mask, k, W) & 2) - 1);
+        total += (sgn * op->coeffs[i]) * dual->coeffs[j];
+    }
+    return total;
+}
+
+/* _intersectfilter! (src/gradient.jl:278-350): keep the dual terms whose
+ * strings the (sorted) operator sum still holds, in order. */
+static void intersect_filter(ppo_sum *dual, const ppo_sum *op) {
+    const int W = dual->W, B = dual->B;
+    size_t j = 0, o = 0;
+    for (size_t i = 0; i < dual->n; ++i) {
+        const uint64_t *dk = dual->keys + i * W;
+        while (j < op->n && key_cmp(op->keys + j * W, dk, W) < 0) ++j;
+        if (j < op->n && key_eq(op->keys + j * W, dk, W)) {
+            if (o != i) {
+                memmove(dual->keys + o * W, dk, W * sizeof(uint64_t));
+                memmove(dual->coeffs + o * B, dual->coeffs + i * B, B * sizeof(double));
+            }
+            ++o;
+        }
+    }
+    dual->n = o;
+}
+
+/* transposecliffordmap (src/Gates/cliffordgates.jl:94-98) on a packed table:
+ * entry idx sends pattern idx to idx' = idx xor (the change on the gate's
+ * qubits) with sign s; the transposed table sends idx' back with the same
+ * change and sign. */
+static void transpose_lut(int nq, const int *qubits, const uint64_t *change, const double *sign, int W,
+                          uint64_t *tchange, double *tsign) {
+    memset(tchange, 0, 16 * W * sizeof(uint64_t));
+    for (int idx = 0; idx < 16; ++idx) tsign[idx] = 1.0;
+    const int nent = 1 << (2 * nq);
+    for (int idx = 0; idx < nent; ++idx) {
+        int moved = 0;
+        for (int a = 0; a < nq; ++a) moved |= get_pauli(change + idx * W, qubits[a]) << (2 * a);
+        const int to = idx ^ moved;
+        memcpy(tchange + to * W, change + idx * W, W * sizeof(uint64_t));
+        tsign[to] = sign[idx];
+    }
+}
+
+/* rewindgradient!(circuit, cache, params, overlapfunc; kwargs...)
+ * (src/gradient.jl:40-108) on a B = 1 sum, for a gate list in Heisenberg
+ * order (rotations, frozen rotations and Cliffords; anything else returns
+ * -1, as the reference rejects noise, :42-43):
+ *   forward  the list as propagate runs it, truncating (max_weight, min_abs);
+ *   dual     the final operator's strings, each with overlapfunc of that
+ *            string alone (_dualcache, :116-124): kinds 0..3 of state_factor,
+ *            or kind 4, scale * (the string's coefficient in rho)
+ *            (overlapwithpaulisum(rho, .) = 2^n scalarproduct, scale = 2^n);
+ *   undo     the list backwards (the circuit's own order), each gate in its
+ *            Schrodinger form (rotation angle negated, Clifford table
+ *            transposed; src/Gates/heisenbergschrodinger.jl:134-161): a
+ *            parametrized rotation first records grad[param] =
+ *            commutator_dot(operator, dual); the operator takes the gate
+ *            with truncation, the dual without, then the dual keeps only
+ *            strings the operator holds (_undostep!, :84-108).
+ * grad has one entry per Heisenberg parameter row; *loss is overlapfunc of
+ * the forward result, in term order. */
+int ppo_rewind(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const double *theta, size_t nparams,
+               const uint64_t *lut_changes, const double *lut_signs, int max_weight, double min_abs, int kind,
+               const uint64_t *ones, const ppo_sum *rho, double scale, double *loss, double *grad) {
+    const int W = s0->W;
+    if (s0->B != 1 || (kind == 4 && (!rho || rho->W != W || rho->B != 1)) || kind < 0 || kind > 4) return -1;
+    for (size_t g = 0; g < ngates; ++g) {
+        if (gates[g].kind != PPO_ROTATION && gates[g].kind != PPO_CLIFFORD) return -1;
+        if (gates[g].param >= 0 && (size_t)gates[g].param >= nparams) return -1;
+    }
+    const uint64_t zero_ones[4] = {0, 0, 0, 0};
+    if (!ones) ones = zero_ones;
+    double cv, sv;
+    ppo_sum *op = clone_sum(s0);
+    for (size_t g = 0; g < ngates; ++g)
+        apply_gate(op, gates + g, theta, lut_changes, lut_signs, max_weight, min_abs, &cv, &sv);
+    /* the dual sum: the operator's strings with their own overlaps */
+    ppo_sum *dual = clone_sum(op);
+    double ex = 0.0;
+    for (size_t i = 0; i < dual->n; ++i) {
+        const uint64_t *k = dual->keys + i * W;
+        double d;
+        if (kind == 4) {
+            ppo_get_coeff(rho, k, &d);
+            d = scale * d;
+        } else {
+            d = (double)state_factor(k, W, kind, ones);
+        }
+        dual->coeffs[i] = d;
+        ex += op->coeffs[i] * d;
+    }
+    *loss = ex;
+    memset(grad, 0, nparams * sizeof(double));
+    uint64_t tch[16 * 4];
+    double tsg[16];
+    for (size_t gg = ngates; gg-- > 0;) {
+        const ppo_gate *g = gates + gg;
+        merge_only(op);
+        merge_only(dual);
+        if (g->kind == PPO_ROTATION) {
+            const double t = -gate_value(g, theta, 1, 0);
+            const double c = cos(t), sn = sin(t);
+            if (g->param >= 0) grad[g->param] = commutator_dot(op, dual, g->mask);
+            ppo_rotation(op, g->mask, &c, &sn, max_weight, min_abs);
+            ppo_rotation(dual, g->mask, &c, &sn, -1, 0.0);
+        } else {
+            const size_t base = (size_t)g->lut * 16;
+            transpose_lut(g->nq, g->qubits, lut_changes + base * W, lut_signs + base, W, tch, tsg);
+            ppo_clifford(op, g->nq, g->qubits, tch, tsg, max_weight, min_abs);
+            ppo_clifford(dual, g->nq, g->qubits, tch, tsg, -1, 0.0);
+            merge_only(op);
+            merge_only(dual);
+        }
+        intersect_filter(dual, op);
+    }
+    ppo_destroy(op);
+    ppo_destroy(dual);
     return 0;
 }
 

@@ -70,7 +70,8 @@ void ppo_clifford(ppo_sum *s, int nq, const int *qubits, const uint64_t *changes
 /* sum of coefficients over terms without X or Y, in array order, per column */
 void ppo_overlap_zero(const ppo_sum *s, double *out);
 /* overlap with a product state, per column: kind 0 |0..0>, 1 |+..+>, 2 the
- * computational basis state with ones where bit 2q of `ones` is set */
+ * computational basis state with ones where bit 2q of `ones` is set, 3 the
+ * maximally mixed state (the identity's coefficient) */
 void ppo_overlap_kind(const ppo_sum *s, int kind, const uint64_t *ones, double *out);
 /* Monte Carlo resampling of a B = 1 sum to about `target` terms: method 0
  * semideterministic systematic, 1 systematic (calibrated when `calibrate`),
@@ -93,12 +94,27 @@ int ppo_run(ppo_sum *s, const ppo_gate *gates, size_t ngates, const double *thet
             const uint64_t *lut_changes, const double *lut_signs, int max_weight, double min_abs,
             int64_t *counts);
 
-/* Reverse-mode gradient of overlapwithzero(propagate(gates, s0, theta)) for
- * every column; every intermediate sum is kept. grad is P x B row-major and
- * is overwritten; loss (B entries) receives the forward overlap. Rotations
- * and PauliNoise only. Returns 0, or -1 on an unsupported gate. */
+/* Reverse-mode gradient of overlap(propagate(gates, s0, theta)) for every
+ * column, every truncation held as the forward run made it; every
+ * intermediate sum is kept. The overlap is `kind` of ppo_overlap_kind (0..3).
+ * grad is P x B row-major and is overwritten; loss (B entries) receives the
+ * forward overlap. Rotations, PauliNoise and Cliffords (tables as ppo_run).
+ * Returns 0, or -1 on an unsupported gate. */
 int ppo_adjoint(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const double *theta,
-                size_t nparams, int max_weight, double min_abs, double *loss, double *grad);
+                size_t nparams, const uint64_t *lut_changes, const double *lut_signs, int max_weight,
+                double min_abs, int kind, const uint64_t *ones, double *loss, double *grad);
+
+/* rewindgradient (src/gradient.jl:40-108), the reference gradient: forward
+ * propagation with truncation, then an undo sweep that re-applies each gate's
+ * Schrodinger form to the truncated operator and carries a dual sum. B = 1;
+ * gates are rotations (parametrized or frozen) and Cliffords, in Heisenberg
+ * order; grad (nparams entries, Heisenberg rows) is overwritten. kind: 0
+ * overlapwithzero, 1 overlapwithplus, 2 overlapwithcomputational (`ones`), 3
+ * overlapwithmaxmixed, 4 scale * scalarproduct with rho (overlapwithpaulisum
+ * with scale 2^n). Returns 0, or -1 on an unsupported gate or argument. */
+int ppo_rewind(const ppo_sum *s0, const ppo_gate *gates, size_t ngates, const double *theta, size_t nparams,
+               const uint64_t *lut_changes, const double *lut_signs, int max_weight, double min_abs, int kind,
+               const uint64_t *ones, const ppo_sum *rho, double scale, double *loss, double *grad);
 
 /* bit-algebra helpers, exposed for the brute-force tests */
 int ppo_anticommutes(const uint64_t *a, const uint64_t *b, int W);

@@ -1863,7 +1863,9 @@ __global__ __launch_bounds__(WG) void k_imag_norm(Bufs bf, const Ctl* ctl, uint3
 //   RD_COMP   a computational basis state: terms holding X or Y vanish, the
 //             others count (-1)^(number of Z on the qubits in state 1); `ones`
 //             has bit 2q set for every such qubit q (the key layout)
-constexpr int RD_ZERO = 0, RD_PLUS = 1, RD_COMP = 2;
+//   RD_MAXMIX the maximally mixed state: the identity counts 1, every other
+//             string 0 (overlapwithmaxmixed, src/stateoverlap.jl:87-93)
+constexpr int RD_ZERO = 0, RD_PLUS = 1, RD_COMP = 2, RD_MAXMIX = 3;
 
 struct Readout {
     int kind;
@@ -1875,6 +1877,11 @@ template <int W>
 __device__ __forceinline__ int readout_factor(const Readout& r, const Key<W>& k) {
     uint64_t orth = 0;
     int nz = 0;
+    if (r.kind == RD_MAXMIX) {
+#pragma unroll
+        for (int l = 0; l < W; ++l) orth |= k.l[l];
+        return orth ? 0 : 1;
+    }
 #pragma unroll
     for (int l = 0; l < W; ++l) {
         const uint64_t lo = k.l[l] & ALT, hi = (k.l[l] >> 1) & ALT;  // X 01, Y 10, Z 11
@@ -2066,6 +2073,155 @@ __global__ __launch_bounds__(WG) void k_overlap_final(const double* __restrict__
     }
 }
 
+// ============================================================ rewindgradient (reference gradient)
+//
+// The reference's gradient (src/gradient.jl:40-108) carries, beside the
+// operator, a dual sum on the same strings whose coefficients are the loss's
+// weight of each string, both undone gate by gate. Three steps of it are not
+// gates: seeding the dual, the commutator dot of a rotation, and capping the
+// dual to the operator's strings. All on B = 1 sums.
+
+// The dual's coefficients: the readout factor of every string, in place
+// (_dualcache, src/gradient.jl:116-124).
+template <int W>
+__global__ __launch_bounds__(WG) void k_dual_seed(Bufs bf, const Ctl* ctl, uint32_t seq, Readout rd) {
+    const int p = seq & 1;
+    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint64_t* __restrict__ keys = bf.keys[cur];
+    double* __restrict__ c = bf.coeffs[cur];
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG)
+        c[i] = (double)readout_factor<W>(rd, load_key<W>(keys, bf.cap, i));
+}
+
+// _generatorcommutatordot (src/gradient.jl:137-241): over operator terms t
+// that anticommute with the generator P, (s c_t) d(P xor t) with s = (e & 2) - 1
+// (-0.5 times the imaginary part of commutator(P, t) = 2 i^e (P xor t),
+// src/PauliAlgebra/paulioperations.jl:414-424) and d the dual's coefficient
+// (0 when absent), found by binary search; per-block double-double partials
+// as k_overlap1 writes them.
+template <int W>
+__global__ __launch_bounds__(WG) void k_comm_dot(Bufs ob, const Ctl* octl, uint32_t oseq, Bufs db, const Ctl* dctl,
+                                                 uint32_t dseq, Key<W> mask, double* __restrict__ partial) {
+    __shared__ double s_hi[NWAVES], s_lo[NWAVES];
+    const int op = oseq & 1, dp = dseq & 1;
+    const uint32_t n = octl->st[op].n, nd = dctl->st[dp].n;
+    const uint64_t* __restrict__ keys = ob.keys[octl->st[op].cur];
+    const double* __restrict__ coeffs = ob.coeffs[octl->st[op].cur];
+    const uint64_t* __restrict__ dkeys = db.keys[dctl->st[dp].cur];
+    const double* __restrict__ dcoef = db.coeffs[dctl->st[dp].cur];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    DD acc{0.0, 0.0};
+    for (uint32_t i = blockIdx.x * WG + t; i < n; i += gridDim.x * WG) {
+        const Key<W> k = load_key<W>(keys, ob.cap, i);
+        if (!anticommutes<W>(mask, k)) continue;
+        Key<W> pk;
+#pragma unroll
+        for (int l = 0; l < W; ++l) pk.l[l] = k.l[l] ^ mask.l[l];
+        const uint32_t j = sum_lower_bound<W>(dkeys, db.cap, nd, pk);
+        if (j >= nd || !key_eq<W>(load_key<W>(dkeys, db.cap, j), pk)) continue;
+        const double sg = (sign_exponent<W>(mask, k) & 2) ? 1.0 : -1.0;
+        acc = dd_add_d(acc, (sg * coeffs[i]) * dcoef[j]);
+    }
+    acc = wave_sum_dd(acc);
+    if (lane == 0) {
+        s_hi[wave] = acc.hi;
+        s_lo[wave] = acc.lo;
+    }
+    __syncthreads();
+    if (t == 0) {
+        DD r{0.0, 0.0};
+        for (int w = 0; w < NWAVES; ++w) r = dd_add(r, DD{s_hi[w], s_lo[w]});
+        partial[2 * blockIdx.x] = r.hi;
+        partial[2 * blockIdx.x + 1] = r.lo;
+    }
+}
+
+// _intersectfilter! (src/gradient.jl:278-350): the dual (this launch's sum)
+// keeps, in order, the terms whose strings the operator (ob at its state oseq)
+// holds; both sorted. A look-back compaction into the dual's other buffer, as
+// k_noise1 compacts.
+constexpr int KI_ITEMS = 4;
+constexpr int KI_TILE = WG * KI_ITEMS;
+
+template <int W>
+__global__ __launch_bounds__(WG) void k_intersect(Bufs bf, Ctl* ctl, Launch L, Bufs ob, const Ctl* octl,
+                                                  uint32_t oseq) {
+    if (!lb_start(ctl, L)) return;
+    const int p = L.seq & 1;
+    const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
+    const uint32_t on = octl->st[oseq & 1].n;
+    const uint64_t* __restrict__ okeys = ob.keys[octl->st[oseq & 1].cur];
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    if (n == 0) {
+        if (blockIdx.x == 0 && t == 0) set_state(ctl, p ^ 1, 0u, cur);
+        return;
+    }
+    const uint64_t* __restrict__ keys = bf.keys[cur];
+    const double* __restrict__ coeffs = bf.coeffs[cur];
+    uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
+    double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
+    const size_t stride = bf.cap;
+    const uint32_t ntiles = (n + KI_TILE - 1) / KI_TILE;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_cnt[KI_ITEMS][NWAVES];
+    __shared__ uint64_t s_excl;
+    TileSource ts;
+    while (true) {
+        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
+        if (tile >= ntiles) break;
+        const uint32_t base = tile * (uint32_t)KI_TILE;
+        uint64_t ballots[KI_ITEMS];
+        Key<W> kv[KI_ITEMS];
+        double cv[KI_ITEMS];
+#pragma unroll
+        for (int k = 0; k < KI_ITEMS; ++k) {
+            const uint32_t i = min(base + k * WG + t, n - 1);
+            kv[k] = load_key<W>(keys, stride, i);
+            cv[k] = coeffs[i];
+        }
+#pragma unroll
+        for (int k = 0; k < KI_ITEMS; ++k) {
+            const uint32_t i = base + k * WG + t;
+            bool keep = false;
+            if (i < n) {
+                const uint32_t j = sum_lower_bound<W>(okeys, ob.cap, on, kv[k]);
+                keep = j < on && key_eq<W>(load_key<W>(okeys, ob.cap, j), kv[k]);
+            }
+            ballots[k] = __ballot(keep);
+            if (lane == 0) s_cnt[k][wave] = (uint32_t)__popcll(ballots[k]);
+        }
+        __syncthreads();
+        uint32_t agg = 0, before[KI_ITEMS];
+#pragma unroll
+        for (int k = 0; k < KI_ITEMS; ++k) {
+            uint32_t rb = 0, row = 0;
+#pragma unroll
+            for (int w = 0; w < NWAVES; ++w) {
+                rb += (w < wave) ? s_cnt[k][w] : 0u;
+                row += s_cnt[k][w];
+            }
+            before[k] = agg + rb;
+            agg += row;
+        }
+        if (wave == 0) {
+            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
+            if (lane == 0) {
+                s_excl = excl;
+                if (tile == ntiles - 1) set_state(ctl, p ^ 1, (uint32_t)(excl + agg), cur ^ 1);
+            }
+        }
+        __syncthreads();
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+        for (int k = 0; k < KI_ITEMS; ++k) {
+            if (!((ballots[k] >> lane) & 1ull)) continue;
+            const uint64_t pos = s_excl + before[k] + (uint32_t)__popcll(ballots[k] & lt);
+            store_key<W>(out_keys, stride, pos, kv[k]);
+            out_coeffs[pos] = cv[k];
+        }
+    }
+}
+
 // ============================================================ Pauli presence
 //
 // Which Paulis occur on each qubit anywhere in the sum: three bit planes in
@@ -2154,20 +2310,21 @@ __global__ __launch_bounds__(WG) void k_presence_final(const uint64_t* __restric
 // Threads form rows of BP = 2^lgBP threads, thread b of a row handling
 // column b < B; coefficient rows are B doubles.
 
-// Adjoint of the final sum: 1 for strings without X or Y, else 0
-// (src/stateoverlap.jl:12-35).
+// Adjoint of the final sum: the loss's weight of every string, the readout
+// factor of `rd` (overlapwithzero: 1 without X or Y, else 0; src/stateoverlap.jl:12-93).
 template <int W>
 __global__ __launch_bounds__(WG) void k_adj_seed(const uint64_t* __restrict__ keys, size_t stride, uint32_t n, int B,
-                                                 double* __restrict__ a) {
+                                                 Readout rd, double* __restrict__ a) {
     const uint64_t total = (uint64_t)n * B;
     for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < total; e += (uint64_t)gridDim.x * WG) {
         const uint32_t i = (uint32_t)(e / B);
-        a[e] = contains_xy<W>(load_key<W>(keys, stride, i)) ? 0.0 : 1.0;
+        a[e] = (double)readout_factor<W>(rd, load_key<W>(keys, stride, i));
     }
 }
 
-// Sum of a block's per-thread values of each column, in a fixed order.
-__device__ __forceinline__ void adj_block_columns(double g, int lgBP, int B, double* __restrict__ out) {
+// Sum of a block's per-thread values of each column, in a fixed order; the
+// block's columns are c0 + b for the BP lanes b of a row.
+__device__ __forceinline__ void adj_block_columns(double g, int lgBP, int B, int c0, double* __restrict__ out) {
     __shared__ double s_g[WG];
     const int t = threadIdx.x, BP = 1 << lgBP, r = t >> lgBP, b = t & (BP - 1);
     s_g[t] = g;
@@ -2176,44 +2333,57 @@ __device__ __forceinline__ void adj_block_columns(double g, int lgBP, int B, dou
         if (r < k) s_g[t] += s_g[t + k * BP];
         __syncthreads();
     }
-    if (r == 0 && b < B) out[b] = s_g[t];
+    if (r == 0 && c0 + b < B) out[c0 + b] = s_g[t];
+    __syncthreads();  // s_g is reused by the next column group
 }
+
+// Column groups a row of BP = 2^lgBP threads covers: one for B <= WG (BP the
+// next power of two of B), else ceil(B / WG) groups of WG columns.
+constexpr int ADJ_MAX_GROUPS = 4;  // B <= 1024
 
 // One rotation backwards. rec points at the gate's records, rcoef at its
 // coefficient rows, trig at cos_b then sin_b. Items [0, n) are the gate's input
 // terms (those that commute are done here), items [n, n + na) its touched terms.
+// Thread b of a row handles columns b, b + BP, ... below B.
 __global__ __launch_bounds__(WG) void k_adj_rotation(const double* __restrict__ a_out, double* __restrict__ a_in,
                                                      const uint32_t* __restrict__ rec, uint32_t n, uint32_t na,
                                                      const double* __restrict__ rcoef, int B, int lgBP,
                                                      const double* __restrict__ trig, double* __restrict__ gpart) {
     const int t = threadIdx.x, BP = 1 << lgBP, rows = WG >> lgBP, r = t >> lgBP, b = t & (BP - 1);
-    const bool col = b < B;
-    const double cb = col ? trig[b] : 0.0, sb = col ? trig[B + b] : 0.0;
+    const int groups = (B + BP - 1) / BP;
     const uint32_t* __restrict__ rsrc = rec + n;
     const uint32_t* __restrict__ rpt = rec + n + na;
     const uint32_t items = n + na;
-    double g = 0.0;
-    if (col) {
-        for (uint32_t it = blockIdx.x * rows + r; it < items; it += gridDim.x * rows) {
+    double g[ADJ_MAX_GROUPS];
+#pragma unroll
+    for (int k = 0; k < ADJ_MAX_GROUPS; ++k) g[k] = 0.0;
+    for (uint32_t it = blockIdx.x * rows + r; it < items; it += gridDim.x * rows) {
+#pragma unroll
+        for (int k = 0; k < ADJ_MAX_GROUPS; ++k) {
+            const int col = b + k * BP;
+            if (k >= groups || col >= B) continue;
+            const double cb = trig[col], sb = trig[B + col];
             if (it < n) {
                 const uint32_t ph = rec[it];
                 if (ph & REC_FLAG) continue;  // touched: done with its created term
                 const uint32_t at = ph & SRC_MASK;
-                a_in[(size_t)it * B + b] = at != REC_NONE ? a_out[(size_t)at * B + b] : 0.0;
+                a_in[(size_t)it * B + col] = at != REC_NONE ? a_out[(size_t)at * B + col] : 0.0;
             } else {
                 const uint32_t j = it - n;
                 const uint32_t src = rsrc[j], i = src & SRC_MASK;
                 const uint32_t hp = rec[i] & SRC_MASK, tp = rpt[j];
-                const double ah = hp != REC_NONE ? a_out[(size_t)hp * B + b] : 0.0;
-                double at = tp != REC_NONE ? a_out[(size_t)tp * B + b] : 0.0;
+                const double ah = hp != REC_NONE ? a_out[(size_t)hp * B + col] : 0.0;
+                double at = tp != REC_NONE ? a_out[(size_t)tp * B + col] : 0.0;
                 if (src & SRC_NEG) at = -at;
-                a_in[(size_t)i * B + b] = ah * cb + at * sb;
-                const double c = rcoef[(size_t)j * B + b];
-                g += c * (at * cb) - c * (ah * sb);
+                a_in[(size_t)i * B + col] = ah * cb + at * sb;
+                const double c = rcoef[(size_t)j * B + col];
+                g[k] += c * (at * cb) - c * (ah * sb);
             }
         }
     }
-    adj_block_columns(g, lgBP, B, gpart + (size_t)blockIdx.x * B);
+#pragma unroll
+    for (int k = 0; k < ADJ_MAX_GROUPS; ++k)
+        if (k < groups) adj_block_columns(g[k], lgBP, B, k * BP, gpart + (size_t)blockIdx.x * B);
 }
 
 // Pauli noise backwards: a_in[i] = a[pos] times the factor when i was damped.
@@ -2240,20 +2410,23 @@ __global__ __launch_bounds__(WG) void k_adj_final(const double* __restrict__ gpa
     }
 }
 
-// Every gate of a segment of the sweep: its blocks' partial gradients summed
-// per column in block order, as k_adj_final sums one gate's (one workgroup per
-// gate; a gate with no blocks keeps its zero).
+// Every rotation of a segment of the sweep: slot s (one workgroup each) holds
+// the partial gradients of nbs[s] blocks of the rotation at gate offset
+// gidx[s]; they are summed per column in block order, as k_adj_final sums one
+// gate's.
 __global__ __launch_bounds__(WG) void k_adj_final_all(const double* __restrict__ gpart,
-                                                      const uint32_t* __restrict__ nbs, uint32_t gblocks, int B,
+                                                      const uint32_t* __restrict__ nbs,
+                                                      const uint32_t* __restrict__ gidx, uint32_t gblocks, int B,
                                                       double* __restrict__ out) {
     const uint32_t s = blockIdx.x;
     const int nb = (int)nbs[s];
     if (nb == 0) return;
     const double* __restrict__ p = gpart + (size_t)s * gblocks * B;
+    double* __restrict__ o = out + (size_t)gidx[s] * B;
     for (int b = threadIdx.x; b < B; b += WG) {
         double acc = 0.0;
         for (int k = 0; k < nb; ++k) acc += p[(size_t)k * B + b];
-        out[(size_t)s * B + b] = acc;
+        o[b] = acc;
     }
 }
 

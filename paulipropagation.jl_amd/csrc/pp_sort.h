@@ -177,6 +177,13 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
         }
         return;
     }
+    // recording (adjoint sweep): term i's output position and sign, one word
+    // each (the positions are those of the mapped sum until the sort is done)
+    const uint32_t roff = ctl->st[p].roff, coff = ctl->st[p].coff;
+    if (bf.rec && (uint64_t)roff + n > bf.rec_words) {
+        if (blockIdx.x == 0 && t == 0) record_overflow(ctl, L, (uint64_t)roff + n, coff);
+        return;
+    }
     const uint64_t* __restrict__ keys = bf.keys[cur];
     const double* __restrict__ coeffs = bf.coeffs[cur];
     uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
@@ -232,7 +239,16 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(e2[0] + a2[0]);
                     set_state(ctl, p ^ 1, nout, cur ^ 1);
-                    carry_records(ctl, L, p);
+                    if (bf.rec) {
+                        ctl->st[p ^ 1].roff = roff + n;
+                        ctl->st[p ^ 1].coff = coff;
+                        if (L.glog) {
+                            L.glog[3] = roff;
+                            L.glog[4] = coff;
+                        }
+                    } else {
+                        carry_records(ctl, L, p);
+                    }
                     if (L.glog) {
                         L.glog[0] = n;
                         L.glog[1] = n;
@@ -247,8 +263,11 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
 #pragma unroll
         for (int k = 0; k < KC_ITEMS; ++k) {
             seg += (segs >> k) & 1u;
-            if (!((keep >> k) & 1u)) continue;
             const uint32_t i = i0 + k;
+            const bool kept = (keep >> k) & 1u;
+            if (bf.rec && i < n)
+                bf.rec[(size_t)roff + i] = (a.sign[idxs[k]] < 0.0 ? REC_FLAG : 0u) | (kept ? (uint32_t)pos : REC_NONE);
+            if (!kept) continue;
             store_key<W>(out_keys, stride, pos, kv[k]);
             for (int b = 0; b < B; ++b) out_coeffs[pos * B + b] = coeffs[(size_t)i * B + b] * a.sign[idxs[k]];
             if (a.composite) {
@@ -284,12 +303,26 @@ __global__ __launch_bounds__(WG) void k_clifford_gather(Bufs bf, Ctl* ctl, int p
     uint64_t* __restrict__ dkeys = bf.keys[dst];
     double* __restrict__ dcoef = bf.coeffs[dst];
     const size_t stride = bf.cap;
+    const bool rec = bf.rec && bf.tpj;
     for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
         const uint32_t s = perm[i];
         store_key<W>(dkeys, stride, i, load_key<W>(skeys, stride, s));
         for (int b = 0; b < B; ++b) dcoef[(size_t)i * B + b] = scoef[(size_t)s * B + b];
+        if (rec) bf.tpj[s] = i;  // mapped term s ends at i
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->st[p ^ 1].cur = dst;
+}
+
+// Recording only: the records of a Clifford's input terms point at the mapped
+// sum; after the sort they are moved to the sorted positions (tpj, from the
+// gather). The sign stays in REC_FLAG.
+__global__ __launch_bounds__(WG) void k_clifford_rec_fix(Bufs bf, const Ctl* ctl, int p) {
+    if (ctl->overflow) return;
+    const uint32_t n = ctl->st[p].n, roff = ctl->st[p].roff;
+    for (uint32_t i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+        const uint32_t w = bf.rec[(size_t)roff + i], at = w & SRC_MASK;
+        if (at != REC_NONE) bf.rec[(size_t)roff + i] = (w & REC_FLAG) | bf.tpj[at];
+    }
 }
 
 // ============================================================ k_reduce_runs

@@ -62,6 +62,7 @@ struct pp_handle {
     int nqubits = 0, W = 1, B = 1;
     size_t cap = 0;
     hipStream_t stream = nullptr;
+    bool own_stream = true;        // false: a work handle on its parent's stream
     Bufs bufs{};
     size_t lb_cap = 0;
     unsigned gmax = 2048;          // persistent grid size
@@ -75,6 +76,8 @@ struct pp_handle {
     int64_t identity_gates = 0;      // rotations left out that way
     uint32_t safe_until = 0;         // gates before this sequence number use dynamic tiles
     int64_t fallbacks = 0;           // gates run again with dynamic tiles
+    size_t adj_defer_bytes = (size_t)2 << 30;  // largest store of per-rotation partial gradients (adjoint)
+    bool last_adj_deferred = false;  // the last adjoint sweep kept its partials and summed them per segment
     Ctl* d_ctl = nullptr;
     Ctl* h_ctl = nullptr;          // pinned mirror
     uint32_t seq = 0;              // next gate's sequence number; the sum's state is st[seq & 1]
@@ -784,6 +787,9 @@ void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_ab
     }
     hipLaunchKernelGGL(k_clifford_gather<W>, dim3(grid_for(h, h->cap, WG * 4)), dim3(WG), 0, h->stream, h->bufs,
                        h->d_ctl, p, h->d_rs_v[c], h->B);
+    if (h->recording)
+        hipLaunchKernelGGL(k_clifford_rec_fix, dim3(grid_for(h, h->cap, WG * 4)), dim3(WG), 0, h->stream, h->bufs,
+                           h->d_ctl, p);
     h->seq += 1;
 }
 
@@ -1327,7 +1333,9 @@ int resolve_gates(pp_handle* h, const pp_gate* gates, size_t ngates, const doubl
             hg.q1 = gt.nq == 2 ? gt.qubits[1] : gt.qubits[0];
             hg.nq = gt.nq;
             hg.lut = gt.lut;
-            if (atrig) return fail(PP_ERR_UNSUPPORTED, "the adjoint sweep does not take Clifford gates yet");
+            // the sweep multiplies the adjoint of a term whose sign flips by -1
+            if (atrig)
+                for (int b = 0; b < B; ++b) (*atrig)[g * 2 * B + b] = -1.0;
         } else {
             return fail(PP_ERR_UNSUPPORTED, "gate kind " + std::to_string(gt.kind) + " is not implemented");
         }
@@ -1429,22 +1437,46 @@ int restore_snapshot(pp_handle* h, const Snapshot& sn) {
 }
 
 template <int W>
-int seed_adjoint(pp_handle* h, double* a) {
+int seed_adjoint(pp_handle* h, const Readout& rd, double* a) {
     uint32_t n = 0, cur = 0;
     if (int rc = live(h, &n, &cur)) return rc;
     if (n == 0) return PP_OK;
     hipLaunchKernelGGL(k_adj_seed<W>, dim3(grid_for(h, (size_t)n * h->B, WG * 4)), dim3(WG), 0, h->stream,
-                       h->bufs.keys[cur], (size_t)h->cap, n, h->B, a);
+                       h->bufs.keys[cur], (size_t)h->cap, n, h->B, rd, a);
     HIP_TRY(hipGetLastError());
     return PP_OK;
 }
 
-// Loss = overlapwithzero of the propagated sum, and its gradient, by a forward
-// run that records every gate (per segment of checkpoint_gates gates, from a
-// snapshot of the segment's input sum) and a reverse sweep over the records.
+// The readout of an overlap kind (enum pp_overlap_kind), checking `ones`.
+int make_readout(const pp_handle* h, int kind, const uint64_t* ones, Readout* rd) {
+    *rd = Readout{};
+    if (kind != PP_OVERLAP_ZERO && kind != PP_OVERLAP_PLUS && kind != PP_OVERLAP_COMPUTATIONAL &&
+        kind != PP_OVERLAP_MAXMIXED)
+        return fail(PP_ERR_ARG, "unknown overlap kind");
+    rd->kind = kind == PP_OVERLAP_PLUS ? RD_PLUS
+               : kind == PP_OVERLAP_COMPUTATIONAL ? RD_COMP
+               : kind == PP_OVERLAP_MAXMIXED ? RD_MAXMIX
+                                             : RD_ZERO;
+    if (kind == PP_OVERLAP_COMPUTATIONAL) {
+        if (!ones) return fail(PP_ERR_ARG, "computational overlap needs the one-bit mask");
+        for (int l = 0; l < h->W; ++l) {
+            if (ones[l] & ~0x5555555555555555ull)
+                return fail(PP_ERR_ARG, "one-bit mask: only bit 2q of a qubit q may be set");
+            rd->ones[l] = ones[l];
+        }
+        for (int q = h->nqubits; q < 32 * h->W; ++q)
+            if ((rd->ones[q >> 5] >> (2 * (q & 31))) & 1ull) return fail(PP_ERR_ARG, "one-bit index out of range");
+    }
+    return PP_OK;
+}
+
+// Loss = the readout `rd` of the propagated sum, and its gradient, by a
+// forward run that records every gate (per segment of checkpoint_gates gates,
+// from a snapshot of the segment's input sum) and a reverse sweep over the
+// records.
 int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
                  const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight, double min_abs,
-                 size_t checkpoint_gates, double* loss, double* grad) {
+                 size_t checkpoint_gates, const Readout& rd, double* loss, double* grad) {
     const int B = h->B;
     std::vector<HostGate> gs;
     std::vector<double> atrig;
@@ -1458,7 +1490,7 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     double* d_a[2] = {nullptr, nullptr};
     double* d_gpart = nullptr;
     double* d_dgrad = nullptr;
-    uint32_t* d_nbs = nullptr;
+    uint32_t* d_slots = nullptr;  // per rotation slot of a segment: its block count, then its gate offset
     std::vector<Snapshot> snaps;
     Snapshot final_sum;
     int rc = PP_OK;
@@ -1468,29 +1500,48 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
         if (d_a[1]) hipFree(d_a[1]);
         if (d_gpart) hipFree(d_gpart);
         if (d_dgrad) hipFree(d_dgrad);
-        if (d_nbs) hipFree(d_nbs);
+        if (d_slots) hipFree(d_slots);
         for (auto& sn : snaps) free_snapshot(sn);
         free_snapshot(final_sum);
     };
     auto run_segment = [&](size_t k) -> int {
         return run_resolved(h, gs, k * seg, std::min(ngates, (k + 1) * seg), max_weight, min_abs, nullptr);
     };
-    const int lgBP = B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1));
+    // rows of BP = 2^lgBP threads, one column per thread (B <= WG), or rows
+    // of WG threads covering B in groups of WG columns
+    const int lgBP = B <= 1 ? 0 : std::min(8, 32 - __builtin_clz((unsigned)(B - 1)));
+    static_assert(ADJ_MAX_GROUPS * WG >= 1024, "the column groups cover the largest batch");
     const unsigned gblocks = h->gmax;
-    // The blocks' partial gradients of every gate of a segment are kept and
-    // summed by one launch after the segment's sweep (one workgroup per gate,
-    // blocks in order: the sums a launch per gate would give), unless that
-    // takes more than ADJ_PART_BYTES; then each gate sums its own.
-    constexpr size_t ADJ_PART_BYTES = (size_t)2 << 30;
-    const bool deferred = ngates && seg * (size_t)gblocks * B * sizeof(double) <= ADJ_PART_BYTES;
-    const size_t pslots = deferred ? seg : 1;
-    std::vector<uint32_t> nbs(deferred ? seg : 0, 0u);
+    // The blocks' partial gradients of every rotation of a segment are kept and
+    // summed by one launch after the segment's sweep (one workgroup per
+    // rotation, blocks in order: the sums a launch per gate would give), unless
+    // that takes more than adj_defer_bytes or the allocation fails; then each
+    // rotation sums its own.
+    size_t maxrot = 0;
+    for (size_t k = 0; k < nseg; ++k) {
+        size_t r = 0;
+        for (size_t g = k * seg; g < std::min(ngates, (k + 1) * seg); ++g) r += gs[g].kind == PP_GATE_ROTATION;
+        maxrot = std::max(maxrot, r);
+    }
+    bool deferred = maxrot > 0 && maxrot * (size_t)gblocks * B * sizeof(double) <= h->adj_defer_bytes;
+    std::vector<uint32_t> slots(deferred ? 2 * maxrot : 0, 0u);
     do {
         if (ngates) {
             if (hipMalloc(&d_atrig, atrig.size() * sizeof(double)) != hipSuccess ||
-                hipMalloc(&d_dgrad, ngates * (size_t)B * sizeof(double)) != hipSuccess ||
-                hipMalloc(&d_gpart, pslots * gblocks * B * sizeof(double)) != hipSuccess ||
-                (deferred && hipMalloc(&d_nbs, seg * sizeof(uint32_t)) != hipSuccess)) {
+                hipMalloc(&d_dgrad, ngates * (size_t)B * sizeof(double)) != hipSuccess) {
+                rc = fail(PP_ERR_NOMEM, "adjoint: device allocation failed");
+                break;
+            }
+            if (deferred && (hipMalloc(&d_gpart, maxrot * gblocks * B * sizeof(double)) != hipSuccess ||
+                             hipMalloc(&d_slots, 2 * maxrot * sizeof(uint32_t)) != hipSuccess)) {
+                // no room for every rotation's partials: sum them gate by gate
+                (void)hipGetLastError();
+                if (d_gpart) hipFree(d_gpart);
+                d_gpart = nullptr;
+                deferred = false;
+                slots.clear();
+            }
+            if (!d_gpart && hipMalloc(&d_gpart, (size_t)gblocks * B * sizeof(double)) != hipSuccess) {
                 rc = fail(PP_ERR_NOMEM, "adjoint: device allocation failed");
                 break;
             }
@@ -1534,21 +1585,21 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
                     break;
                 }
                 switch (h->W) {
-                case 1: rc = overlap_impl<1>(h, loss); break;
-                case 2: rc = overlap_impl<2>(h, loss); break;
-                default: rc = overlap_impl<4>(h, loss); break;
+                case 1: rc = overlap_impl<1>(h, loss, rd); break;
+                case 2: rc = overlap_impl<2>(h, loss, rd); break;
+                default: rc = overlap_impl<4>(h, loss, rd); break;
                 }
                 if (rc) break;
                 switch (h->W) {
-                case 1: rc = seed_adjoint<1>(h, d_a[0]); break;
-                case 2: rc = seed_adjoint<2>(h, d_a[0]); break;
-                default: rc = seed_adjoint<4>(h, d_a[0]); break;
+                case 1: rc = seed_adjoint<1>(h, rd, d_a[0]); break;
+                case 2: rc = seed_adjoint<2>(h, rd, d_a[0]); break;
+                default: rc = seed_adjoint<4>(h, rd, d_a[0]); break;
                 }
                 if (rc) break;
             }
             // the sweep through this segment, last gate first
             const size_t g0 = kk * seg, g1 = std::min(ngates, (kk + 1) * seg);
-            if (deferred) std::fill(nbs.begin(), nbs.end(), 0u);
+            uint32_t nslot = 0;
             for (size_t g = g1; g-- > g0;) {
                 const uint32_t* lg = h->h_log + LOG_WORDS * g;
                 const uint32_t nin = lg[0], na = lg[1], nout = lg[2], roff = lg[3], coff = lg[4];
@@ -1561,16 +1612,21 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
                     if (na == 0 || nin == 0) continue;  // identity
                     const unsigned rows = WG >> lgBP;
                     const unsigned nb = (unsigned)std::min<size_t>(gblocks, ((size_t)nin + na + rows - 1) / rows);
-                    double* part = d_gpart + (deferred ? (g - g0) * (size_t)gblocks * B : 0);
+                    double* part = d_gpart + (deferred ? (size_t)nslot * gblocks * B : 0);
                     hipLaunchKernelGGL(k_adj_rotation, dim3(nb), dim3(WG), 0, h->stream, d_a[acur], d_a[acur ^ 1],
                                        h->d_rec + roff, nin, na, h->d_rcoef + (size_t)coff * B, B, lgBP, tr,
                                        part);
-                    if (deferred)
-                        nbs[g - g0] = nb;
-                    else
+                    if (deferred) {
+                        slots[nslot] = nb;
+                        slots[maxrot + nslot] = (uint32_t)(g - g0);
+                        ++nslot;
+                    } else {
                         hipLaunchKernelGGL(k_adj_final, dim3(1), dim3(WG), 0, h->stream, d_gpart, (int)nb, B,
                                            d_dgrad + g * (size_t)B);
+                    }
                 } else {
+                    // Pauli noise (factor on damped terms) or a Clifford (factor -1 on
+                    // terms whose sign flips): a_in[i] = a_out[its position] x factor
                     if (nin == 0) continue;
                     hipLaunchKernelGGL(k_adj_noise, dim3(grid_for(h, (size_t)nin * B, WG * 4)), dim3(WG), 0,
                                        h->stream, d_a[acur], d_a[acur ^ 1], h->d_rec + roff, nin, B, tr);
@@ -1582,15 +1638,15 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
                 acur ^= 1;
             }
             if (rc) break;
-            if (deferred && g1 > g0) {
-                if (hipMemcpyAsync(d_nbs, nbs.data(), (g1 - g0) * sizeof(uint32_t), hipMemcpyHostToDevice,
+            if (deferred && nslot > 0) {
+                if (hipMemcpyAsync(d_slots, slots.data(), 2 * maxrot * sizeof(uint32_t), hipMemcpyHostToDevice,
                                    h->stream) != hipSuccess) {
                     rc = fail(PP_ERR_HIP, "adjoint: staging failed");
                     break;
                 }
-                hipLaunchKernelGGL(k_adj_final_all, dim3((unsigned)(g1 - g0)), dim3(WG), 0, h->stream, d_gpart,
-                                   d_nbs, gblocks, B, d_dgrad + g0 * (size_t)B);
-                // nbs is refilled for the next segment only once the copy has been read
+                hipLaunchKernelGGL(k_adj_final_all, dim3(nslot), dim3(WG), 0, h->stream, d_gpart, d_slots,
+                                   d_slots + maxrot, gblocks, B, d_dgrad + g0 * (size_t)B);
+                // slots are refilled for the next segment only once the copy has been read
                 if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) {
                     rc = fail(PP_ERR_HIP, "adjoint: gradient sums failed");
                     break;
@@ -1617,6 +1673,7 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
         if (nseg > 1) rc = restore_snapshot(h, final_sum);
     } while (false);
     cleanup();
+    h->last_adj_deferred = deferred;
     return rc;
 }
 
@@ -1790,6 +1847,154 @@ int load_merge_impl(pp_handle* h, int max_weight, double min_abs) {
 
 // ============================================================ C ABI
 
+namespace {
+int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStream_t shared, pp_handle** out);
+
+// The transposed table of a packed Clifford table (transposecliffordmap,
+// src/Gates/cliffordgates.jl:94-98): entry idx moves the pattern idx to
+// idx ^ (the change on the gate's qubits) with a sign; the transpose sends that
+// pattern back with the same change and sign.
+void transpose_table(int nq, const int* qubits, const uint64_t* change, const double* sign, int W, uint64_t* tch,
+                     double* tsg) {
+    std::fill(tch, tch + 16 * W, 0ull);
+    std::fill(tsg, tsg + 16, 1.0);
+    const int nent = 1 << (2 * nq);
+    for (int idx = 0; idx < nent; ++idx) {
+        int moved = 0;
+        for (int a = 0; a < nq; ++a)
+            moved |= (int)((change[idx * W + (qubits[a] >> 5)] >> (2 * (qubits[a] & 31))) & 3u) << (2 * a);
+        const int to = idx ^ moved;
+        for (int l = 0; l < W; ++l) tch[to * W + l] = change[idx * W + l];
+        tsg[to] = sign[idx];
+    }
+}
+
+// Cap the dual (hd) to the strings of the operator (h), both current; on a
+// look-back wait that ran out the compaction runs again with atomic tile claims.
+template <int W>
+int intersect(pp_handle* hd, pp_handle* h) {
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (int rc = sync_ctl(hd)) return rc;
+        const GateState in = hd->h_ctl->st[hd->seq & 1];
+        Launch L = lb_launch(hd, nullptr);
+        if (attempt) L.dynamic = 1;
+        hipLaunchKernelGGL(k_intersect<W>, dim3(lb_grid(hd, (const void*)k_intersect<W>, 0, hd->cap, KI_TILE)),
+                           dim3(WG), 0, hd->stream, hd->bufs, hd->d_ctl, L, h->bufs, h->d_ctl, h->seq);
+        HIP_TRY(hipGetLastError());
+        if (int rc = sync_ctl(hd)) return rc;
+        if (hd->h_ctl->overflow == 0) {
+            hd->seq += 1;
+            return PP_OK;
+        }
+        hd->fallbacks += 1;
+        if (int rc = reset_ctl(hd, in)) return rc;
+    }
+    return fail(PP_ERR_STATE, "rewind: the dual's compaction did not complete");
+}
+
+// rewindgradient (src/gradient.jl:40-108) on a B = 1 handle: forward run,
+// loss, the dual sum on a work handle that shares the stream, then the undo
+// sweep (the gate list backwards, each gate's Schrodinger form) with the
+// commutator dot of every parametrized rotation taken before it is undone.
+template <int W>
+int rewind_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
+                const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight, double min_abs,
+                const Readout& rd, double* loss, double* grad) {
+    if (h->B != 1) return fail(PP_ERR_UNSUPPORTED, "rewindgradient takes one coefficient column");
+    for (size_t g = 0; g < ngates; ++g)
+        if (gates[g].kind != PP_GATE_ROTATION && gates[g].kind != PP_GATE_CLIFFORD)
+            return fail(PP_ERR_UNSUPPORTED, "rewindgradient: gate " + std::to_string(g) +
+                                                " is not a rotation or a Clifford (noise channels cannot be undone)");
+    std::vector<HostGate> gs;
+    if (int rc = resolve_gates(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, gs, nullptr))
+        return rc;
+    // the undo list: the circuit's own order, rotations at -theta, transposed tables
+    std::vector<pp_gate> ug(ngates);
+    for (size_t k = 0; k < ngates; ++k) {
+        ug[k] = gates[ngates - 1 - k];
+        if (ug[k].kind == PP_GATE_ROTATION) {
+            ug[k].value = -(ug[k].param >= 0 ? theta[(size_t)ug[k].param] : ug[k].value);
+            ug[k].param = -1;
+        }
+    }
+    std::vector<uint64_t> tch(std::max<size_t>(nluts, 1) * 16 * W, 0ull);
+    std::vector<double> tsg(std::max<size_t>(nluts, 1) * 16, 1.0);
+    std::vector<uint8_t> done(nluts, 0);
+    for (size_t g = 0; g < ngates; ++g) {
+        if (gates[g].kind != PP_GATE_CLIFFORD || done[gates[g].lut]) continue;
+        const size_t sl = gates[g].lut;
+        transpose_table(gates[g].nq, gates[g].qubits, lut_changes + sl * 16 * W, lut_signs + sl * 16, W,
+                        &tch[sl * 16 * W], &tsg[sl * 16]);
+        done[sl] = 1;
+    }
+    if (int rc = run_resolved(h, gs, 0, ngates, max_weight, min_abs, nullptr)) return rc;
+    if (int rc = overlap_impl<W>(h, loss, rd)) return rc;
+    for (size_t q = 0; q < nparams; ++q) grad[q] = 0.0;
+    if (ngates == 0) return PP_OK;
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    pp_handle* hd = nullptr;
+    double* d_g = nullptr;
+    int rc = PP_OK;
+    do {
+        if ((rc = create_handle(h->nqubits, W, 1, std::max<size_t>(h->cap, 1024), h->stream, &hd))) break;
+        hd->skip_identity = false;
+        for (int l = 0; l < W && n; ++l)
+            if (hipMemcpyAsync(hd->bufs.keys[0] + (size_t)l * hd->cap, h->bufs.keys[cur] + (size_t)l * h->cap,
+                               n * sizeof(uint64_t), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
+                rc = fail(PP_ERR_HIP, "rewind: dual copy failed");
+                break;
+            }
+        if (rc) break;
+        if ((rc = reset_ctl(hd, n, 0))) break;
+        hipLaunchKernelGGL(k_dual_seed<W>, dim3(grid_for(hd, n, WG * 4)), dim3(WG), 0, h->stream, hd->bufs,
+                           hd->d_ctl, hd->seq, rd);
+        std::vector<HostGate> gu, gd;
+        if ((rc = resolve_gates(h, ug.data(), ngates, nullptr, 0, tch.data(), tsg.data(), nluts, gu, nullptr))) break;
+        if ((rc = resolve_gates(hd, ug.data(), ngates, nullptr, 0, tch.data(), tsg.data(), nluts, gd, nullptr))) break;
+        if (hipMalloc(&d_g, ngates * sizeof(double)) != hipSuccess ||
+            hipMemsetAsync(d_g, 0, ngates * sizeof(double), h->stream) != hipSuccess) {
+            rc = fail(PP_ERR_NOMEM, "rewind: device allocation failed");
+            break;
+        }
+        for (size_t k = 0; k < ngates && !rc; ++k) {
+            const pp_gate& og = gates[ngates - 1 - k];
+            if (og.kind == PP_GATE_ROTATION && og.param >= 0) {
+                // _generatorcommutatordot before the gate is undone
+                const unsigned gb = std::min<unsigned>(2048u, (unsigned)std::max<size_t>(1, (h->cap + WG - 1) / WG));
+                if ((rc = ensure_partial(h, (size_t)gb * 2))) break;
+                hipLaunchKernelGGL(k_comm_dot<W>, dim3(gb), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
+                                   hd->bufs, hd->d_ctl, hd->seq, to_key<W>(og.mask), h->d_partial);
+                hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, h->d_partial, (int)gb, 1,
+                                   d_g + k);
+                if (hipGetLastError() != hipSuccess) {
+                    rc = fail(PP_ERR_HIP, "rewind: kernel launch failed");
+                    break;
+                }
+            }
+            // the operator truncates; the dual does not (_undostep!, :84-108)
+            if ((rc = run_resolved(h, gu, k, k + 1, max_weight, min_abs, nullptr))) break;
+            if ((rc = run_resolved(hd, gd, k, k + 1, -1, 0.0, nullptr))) break;
+            rc = intersect<W>(hd, h);
+        }
+        if (rc) break;
+        std::vector<double> hg(ngates);
+        if (hipMemcpyAsync(hg.data(), d_g, ngates * sizeof(double), hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            rc = fail(PP_ERR_HIP, "rewind: download failed");
+            break;
+        }
+        for (size_t k = 0; k < ngates; ++k) {
+            const pp_gate& og = gates[ngates - 1 - k];
+            if (og.kind == PP_GATE_ROTATION && og.param >= 0) grad[og.param] = hg[k];
+        }
+    } while (false);
+    if (d_g) hipFree(d_g);
+    if (hd) pp_destroy(hd);
+    return rc;
+}
+}  // namespace
+
 extern "C" {
 
 int pp_version(void) { return 200; }
@@ -1797,6 +2002,15 @@ int pp_version(void) { return 200; }
 const char* pp_last_error(void) { return g_err.c_str(); }
 
 int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle** out) {
+    return create_handle(nqubits, key_words, batch, capacity, nullptr, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// A handle, on its own new stream or (work handles) on `shared`.
+int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStream_t shared, pp_handle** out) {
     if (!out) return fail(PP_ERR_ARG, "out is NULL");
     *out = nullptr;
     if (nqubits < 1 || nqubits > 128) return fail(PP_ERR_ARG, "nqubits must be in 1..128");
@@ -1822,8 +2036,12 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     h->gmax = (unsigned)std::max(64, 4 * cus);
     h->cus = std::max(1, cus);
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+    if (shared) {
+        h->stream = shared;
+        h->own_stream = false;
+    } else if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
         return bail(fail(PP_ERR_HIP, "hipStreamCreate failed"));
+    }
     if (hipMalloc(&h->d_ctl, sizeof(Ctl)) != hipSuccess ||
         hipHostMalloc(&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess ||
@@ -1836,6 +2054,10 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle*
     *out = h;
     return PP_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int pp_destroy(pp_handle* h) {
     if (!h) return PP_OK;
@@ -1869,7 +2091,7 @@ int pp_destroy(pp_handle* h) {
     if (h->d_rs_hist) hipFree(h->d_rs_hist);
     if (h->d_lut_c) hipFree(h->d_lut_c);
     if (h->d_lut_s) hipFree(h->d_lut_s);
-    if (h->stream) hipStreamDestroy(h->stream);
+    if (h->stream && h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return PP_OK;
 }
@@ -2081,16 +2303,39 @@ int pp_run_circuit(pp_handle* h, const pp_gate* gates, size_t ngates, const doub
 
 int pp_adjoint(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
                const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight,
-               double min_abs_coeff, size_t checkpoint_gates, double* loss, double* grad) {
+               double min_abs_coeff, size_t checkpoint_gates, int overlap_kind, const uint64_t* ones, double* loss,
+               double* grad) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (!loss || (nparams > 0 && !grad)) return fail(PP_ERR_ARG, "loss or grad is NULL");
+    Readout rd;
+    if (int rc = make_readout(h, overlap_kind, ones, &rd)) return rc;
     const bool chain = h->chain_tails;
     h->chain_tails = false;
     const int rc = adjoint_impl(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
-                                min_abs_coeff, checkpoint_gates, loss, grad);
+                                min_abs_coeff, checkpoint_gates, rd, loss, grad);
     h->chain_tails = chain;
     set_recording(h, false);
     return rc;
+}
+
+int pp_rewind(pp_handle* h, const pp_gate* gates, size_t ngates, const double* theta, size_t nparams,
+              const uint64_t* lut_changes, const double* lut_signs, size_t nluts, int max_weight,
+              double min_abs_coeff, int overlap_kind, const uint64_t* ones, double* loss, double* grad) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    if (!loss || (nparams > 0 && !grad)) return fail(PP_ERR_ARG, "loss or grad is NULL");
+    Readout rd;
+    if (int rc = make_readout(h, overlap_kind, ones, &rd)) return rc;
+    switch (h->W) {
+    case 1:
+        return rewind_impl<1>(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
+                              min_abs_coeff, rd, loss, grad);
+    case 2:
+        return rewind_impl<2>(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
+                              min_abs_coeff, rd, loss, grad);
+    default:
+        return rewind_impl<4>(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
+                              min_abs_coeff, rd, loss, grad);
+    }
 }
 
 int pp_overlap_zero(pp_handle* h, double* out_b) {
@@ -2104,21 +2349,8 @@ int pp_overlap_zero(pp_handle* h, double* out_b) {
 
 int pp_overlap(pp_handle* h, int kind, const uint64_t* ones, double* out_b) {
     if (!h || !out_b) return fail(PP_ERR_ARG, "NULL argument");
-    if (kind != PP_OVERLAP_ZERO && kind != PP_OVERLAP_PLUS && kind != PP_OVERLAP_COMPUTATIONAL)
-        return fail(PP_ERR_ARG, "unknown overlap kind");
-    Readout rd{};
-    rd.kind = kind == PP_OVERLAP_PLUS ? RD_PLUS : (kind == PP_OVERLAP_COMPUTATIONAL ? RD_COMP : RD_ZERO);
-    if (kind == PP_OVERLAP_COMPUTATIONAL) {
-        if (!ones) return fail(PP_ERR_ARG, "computational overlap needs the one-bit mask");
-        for (int l = 0; l < h->W; ++l) {
-            if (ones[l] & ~0x5555555555555555ull)
-                return fail(PP_ERR_ARG, "one-bit mask: only bit 2q of a qubit q may be set");
-            rd.ones[l] = ones[l];
-        }
-        const int nq = h->nqubits;
-        for (int q = nq; q < 32 * h->W; ++q)
-            if ((rd.ones[q >> 5] >> (2 * (q & 31))) & 1ull) return fail(PP_ERR_ARG, "one-bit index out of range");
-    }
+    Readout rd;
+    if (int rc = make_readout(h, kind, ones, &rd)) return rc;
     switch (h->W) {
     case 1: return overlap_impl<1>(h, out_b, rd);
     case 2: return overlap_impl<2>(h, out_b, rd);
@@ -2203,6 +2435,10 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         return PP_OK;
     case PP_TUNE_SKIP_IDENTITY:
         h->skip_identity = value != 0;
+        return PP_OK;
+    case PP_TUNE_ADJ_DEFER_BYTES:
+        if (value < 0) return fail(PP_ERR_ARG, "the partial-gradient store size must be non-negative");
+        h->adj_defer_bytes = (size_t)value;
         return PP_OK;
     default:
         return fail(PP_ERR_ARG, "unknown tuning key");

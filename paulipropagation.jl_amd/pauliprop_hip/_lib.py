@@ -66,7 +66,9 @@ SIGNATURES = [
     ("pp_run_circuit", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
                                       ctypes.c_double, _I64P]),
     ("pp_adjoint", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
-                                  ctypes.c_double, _SZ, _DP, _DP]),
+                                  ctypes.c_double, _SZ, ctypes.c_int, _U64P, _DP, _DP]),
+    ("pp_rewind", ctypes.c_int, [_P, ctypes.POINTER(pp_gate), _SZ, _DP, _SZ, _U64P, _DP, _SZ, ctypes.c_int,
+                                 ctypes.c_double, ctypes.c_int, _U64P, _DP, _DP]),
     ("pp_overlap_zero", ctypes.c_int, [_P, _DP]),
     ("pp_overlap", ctypes.c_int, [_P, ctypes.c_int, _U64P, _DP]),
     ("pp_get_coeff", ctypes.c_int, [_P, _U64P, _DP]),
@@ -82,7 +84,8 @@ SIGNATURES = [
 ]
 
 PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS, PP_TUNE_SKIP_IDENTITY = 1, 2, 3, 4
-PP_OVERLAP_ZERO, PP_OVERLAP_PLUS, PP_OVERLAP_COMPUTATIONAL = 0, 1, 2
+PP_TUNE_ADJ_DEFER_BYTES = 5
+PP_OVERLAP_ZERO, PP_OVERLAP_PLUS, PP_OVERLAP_COMPUTATIONAL, PP_OVERLAP_MAXMIXED = 0, 1, 2, 3
 PP_RESAMPLE_SEMIDETERMINISTIC, PP_RESAMPLE_SYSTEMATIC, PP_RESAMPLE_MULTINOMIAL = 0, 1, 2
 
 

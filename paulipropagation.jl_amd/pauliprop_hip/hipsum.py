@@ -277,11 +277,14 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
     def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None,
-             skip_identity: bool = None) -> None:
+             skip_identity: bool = None, adj_defer_bytes: int = None) -> None:
         """Bound of a look-back wait (polls); whether tiles are always claimed
         atomically; whether a merge lists the next rotation's touched terms;
-        whether circuits leave out the rotations that touch no term."""
+        whether circuits leave out the rotations that touch no term; the device
+        memory the adjoint sweep may keep per-rotation partial gradients in."""
         L = _lib.lib()
+        if adj_defer_bytes is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_ADJ_DEFER_BYTES, int(adj_defer_bytes)))
         if skip_identity is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SKIP_IDENTITY, int(bool(skip_identity))))
         if spin_limit is not None:
@@ -335,19 +338,22 @@ class PackedCircuit:
         return self.counts[first:last]
 
 
-    def adjoint(self, psum: HIPPauliSum, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0):
-        """(loss[B], grad[P x B]) of overlapwithzero after the whole circuit, by
-        the device adjoint sweep; grad rows are in Heisenberg parameter order.
-        psum holds the observable and is left holding the propagated sum."""
+    def adjoint(self, psum: HIPPauliSum, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0,
+                overlapfunc=None, onebitinds=()):
+        """(loss[B], grad[P x B]) of the overlap `overlapfunc` (default
+        overlapwithzero) after the whole circuit, by the device adjoint sweep;
+        grad rows are in Heisenberg parameter order. psum holds the observable
+        and is left holding the propagated sum."""
         B = psum.batch
         if self.theta.shape[1] != B:
             raise ValueError(f"the circuit has {self.theta.shape[1]} parameter columns, the sum {B}")
+        kind, ones = overlap_spec(overlapfunc, onebitinds, psum.nqubits, psum.W)
         loss = np.zeros(B)
         grad = np.zeros((max(self.nparams, 1), B))
         _lib.check(_lib.lib().pp_adjoint(
             psum._h, self.gates, self.ngates, _dp(self.theta), self.nparams, _u64p(self.lut_c), _dp(self.lut_s),
-            self.nluts, _max_weight_arg(max_weight), float(min_abs_coeff), int(checkpoint_gates), _dp(loss),
-            _dp(grad)))
+            self.nluts, _max_weight_arg(max_weight), float(min_abs_coeff), int(checkpoint_gates), kind, _u64p(ones),
+            _dp(loss), _dp(grad)))
         return loss, grad[:self.nparams]
 
 
@@ -434,14 +440,20 @@ def _run_relative(psum, hcirc, hth, max_weight, min_abs_coeff, min_rel_coeff, no
     return cnt[:len(hcirc)]
 
 
-def adjointgradient(circuit, psum, thetas, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0):
-    """(expectation, gradient) of overlapwithzero(propagate(circuit, psum, thetas))
+def adjointgradient(circuit, psum, thetas, max_weight=math.inf, min_abs_coeff=1e-10, checkpoint_gates: int = 0,
+                    overlapfunc=None, onebitinds=()):
+    """(expectation, gradient) of overlapfunc(propagate(circuit, psum, thetas))
     on the GPU by the adjoint sweep, with truncations held fixed as the forward
-    run made them. thetas is P (or P x B) in Schrodinger order and the gradient
-    comes back in the same shape and order. Noise gates are allowed.
+    run made them: the exact derivative of the truncated propagation, which
+    central differences of propagate() measure. overlapfunc is overlapwithzero
+    (the default), overlapwithplus, overlapwithmaxmixed or
+    overlapwithcomputational with `onebitinds`. thetas is P (or P x B) in
+    Schrodinger order and the gradient comes back in the same shape and order.
+    Noise gates and Cliffords are allowed.
 
-    The reference's counterpart is rewindgradient(circuit, psum, params,
-    overlapwithzero; kwargs...) (src/gradient.jl:26-82)."""
+    This is not the reference's rewindgradient (src/gradient.jl:26-108), which
+    truncates the operator again while it undoes the gates; both agree when
+    nothing is truncated. `rewindgradient` below runs that algorithm."""
     P = countparameters(circuit)
     th = np.asarray(thetas, dtype=np.float64)
     vec = th.ndim == 1
@@ -451,19 +463,66 @@ def adjointgradient(circuit, psum, thetas, max_weight=math.inf, min_abs_coeff=1e
         psum = HIPPauliSum.from_paulisum(psum, batch=B)
     pc = PackedCircuit(circuit, th, psum.nqubits, batch=B)
     loss, grad = pc.adjoint(psum, max_weight=max_weight, min_abs_coeff=min_abs_coeff,
-                            checkpoint_gates=checkpoint_gates)
+                            checkpoint_gates=checkpoint_gates, overlapfunc=overlapfunc, onebitinds=onebitinds)
     grad = grad[::-1].copy()  # Heisenberg parameter rows run in reverse Schrodinger order
     if vec:
         return float(loss[0]), grad[:, 0].copy()
     return loss, grad
 
 
-def rewindgradient(circuit, psum, thetas, overlapfunc=None, max_weight=math.inf, min_abs_coeff=1e-10):
-    """rewindgradient(circuit, psum, params, overlapwithzero; kwargs...) -> (expec, grad)
-    (src/gradient.jl:26-30), for the loss overlapwithzero."""
-    if overlapfunc is not None and overlapfunc is not overlapwithzero:
-        raise NotImplementedError("the device gradient supports overlapwithzero only")
-    return adjointgradient(circuit, psum, thetas, max_weight=max_weight, min_abs_coeff=min_abs_coeff)
+def rewindgradient(circuit, psum, thetas, overlapfunc=None, max_weight=math.inf, min_abs_coeff=1e-10,
+                   onebitinds=()):
+    """rewindgradient(circuit, psum, params, overlapfunc; max_weight, min_abs_coeff)
+    -> (expec, grad) (src/gradient.jl:26-108) on the GPU: the reference's own
+    algorithm -- forward propagation, then the gates undone in their
+    Schrodinger form on the truncated operator and on an untruncated dual sum,
+    a commutator dot per parametrized rotation. overlapfunc is overlapwithzero
+    (the default), overlapwithplus, overlapwithmaxmixed or
+    overlapwithcomputational with `onebitinds`. psum is copied (the reference
+    deep-copies it too); params in Schrodinger order, one column. Noise
+    channels raise, as in the reference."""
+    P = countparameters(circuit)
+    th = np.asarray(thetas, dtype=np.float64).reshape(-1)
+    if th.shape[0] != P:
+        raise ValueError(f"the circuit has {P} parameters, got {th.shape[0]}")
+    if isinstance(psum, HIPPauliSum):
+        if psum.batch != 1:
+            raise ValueError("rewindgradient takes one coefficient column")
+        keys, coeffs = psum.download()
+        work = HIPPauliSum(psum.nqubits, batch=1, capacity=max(1024, 2 * len(keys)))
+        work.upload(keys, coeffs)
+    else:
+        work = HIPPauliSum.from_paulisum(psum, batch=1)
+    pc = PackedCircuit(circuit, th.reshape(P, 1), work.nqubits, batch=1)
+    kind, ones = overlap_spec(overlapfunc, onebitinds, work.nqubits, work.W)
+    loss = np.zeros(1)
+    grad = np.zeros(max(P, 1))
+    _lib.check(_lib.lib().pp_rewind(
+        work._h, pc.gates, pc.ngates, _dp(pc.theta), pc.nparams, _u64p(pc.lut_c), _dp(pc.lut_s), pc.nluts,
+        _max_weight_arg(max_weight), float(min_abs_coeff), kind, _u64p(ones), _dp(loss), _dp(grad)))
+    return float(loss[0]), grad[:P][::-1].copy()
+
+
+def overlap_spec(overlapfunc, onebitinds, nqubits: int, W: int):
+    """(pp_overlap_kind, ones limbs) of an overlap function of this module."""
+    ones_int = 0
+    if overlapfunc is None or overlapfunc is overlapwithzero or overlapfunc == "zero":
+        kind = _lib.PP_OVERLAP_ZERO
+    elif overlapfunc is overlapwithplus or overlapfunc == "plus":
+        kind = _lib.PP_OVERLAP_PLUS
+    elif overlapfunc is overlapwithmaxmixed or overlapfunc == "maxmixed":
+        kind = _lib.PP_OVERLAP_MAXMIXED
+    elif overlapfunc is overlapwithcomputational or overlapfunc == "computational":
+        kind = _lib.PP_OVERLAP_COMPUTATIONAL
+        for q in onebitinds:
+            q = int(q)
+            if not 1 <= q <= nqubits:
+                raise ValueError(f"qubit index {q} out of range 1..{nqubits}")
+            ones_int |= 1 << (2 * (q - 1))
+    else:
+        raise NotImplementedError("the device gradients take overlapwithzero, overlapwithplus, "
+                                  "overlapwithcomputational and overlapwithmaxmixed")
+    return kind, np.ascontiguousarray(int_to_limbs(ones_int, W), dtype=np.uint64)
 
 
 def overlapwithzero(psum):

@@ -236,3 +236,75 @@ def test_clifford_circuit(B):
         ref, _ = oracle_propagate(circ, nq, th if B > 1 else th[:, 0], obs.terms, max_weight=mw,
                                   min_abs_coeff=1e-6, batch=B)
         assert_bitwise(*dev.download(), *ref.get_sorted(), f"Clifford circuit B={B} max_weight={mw}")
+
+
+@pytest.mark.parametrize("B", [65, 128, 300])
+def test_wide_batches_rotation_and_noise_bitwise(B):
+    # the batched merge's and noise pass's loops over columns past 64 and the
+    # 32- and 16-row merge tiles that batches above 64 take
+    print(f"wide batch B={B}", flush=True)
+    nq = 12
+    circ = []
+    for _ in range(2):
+        pp.rylayer(circ, nq)
+        pp.rzzlayer(circ, pp.bricklayertopology(nq))
+        pp.depolarizinglayer(circ, nq, 0.05)
+        pp.rxlayer(circ, nq)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(B).uniform(-0.8, 0.8, size=(P, B))
+    obs = pp.PauliSum(nq)
+    for i in range(1, nq):
+        obs.add(["Z", "Z"], [i, i + 1], 1.0)
+    dev = pp.propagate(circ, obs, th, max_weight=6, min_abs_coeff=1e-4)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=6, min_abs_coeff=1e-4, batch=B)
+    assert_bitwise(*dev.download(), *ref.get(), f"wide batch B={B}")
+
+
+def test_wide_batch_amplitude_damping_bitwise():
+    print("wide batch amplitude damping B=65", flush=True)
+    nq, B = 10, 65
+    circ = []
+    for q in range(1, nq + 1):
+        circ.append(pp.PauliRotation("X", q))
+        circ.append(pp.freeze(pp.AmplitudeDampingNoise(q), 0.1 + 0.02 * q))
+    pp.rzzlayer(circ, pp.bricklayertopology(nq))
+    for q in range(1, nq + 1, 2):
+        circ.append(pp.freeze(pp.AmplitudeDampingNoise(q), 0.3))
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(7).uniform(-1, 1, size=(P, B))
+    obs = pp.PauliSum(nq)
+    for i in range(1, nq):
+        obs.add(["Z", "Z"], [i, i + 1], 1.0)
+    dev = pp.propagate(circ, obs, th, max_weight=5, min_abs_coeff=1e-5)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=5, min_abs_coeff=1e-5, batch=B)
+    assert_bitwise(*dev.download(), *ref.get(), "wide batch amplitude damping")
+
+
+@pytest.mark.timeout(600)
+def test_heisenberg_64q_b64_first_step_matches_zone_oracle():
+    # the batched workload (8x8 periodic Heisenberg, 64 qubits, UInt128 keys,
+    # B = 64 columns, max_weight 8, min_abs_coeff 1e-6 on the batch maximum) at
+    # its own size for its first Trotter step (448 gates, 1,455,699 terms):
+    # bit for bit against the zone-parallel oracle on the host's cores
+    import ctypes
+    import os
+    from oracle import MultiOracleSum
+    from pauliprop_hip import workloads
+    from pauliprop_hip.hipsum import PackedCircuit
+    print("64-qubit B=64 Heisenberg, first step", flush=True)
+    wl = workloads.heisenberg_square_64q_b64(nsteps=1)
+    B = wl["batch"]
+    pc = PackedCircuit(wl["circuit"], wl["thetas"], 64, batch=B)
+    keys0, coeffs0 = wl["obs"].to_arrays(pc.W, B)
+    dev = pp.HIPPauliSum(64, batch=B, capacity=1 << 20)
+    dev.upload(keys0, coeffs0)
+    counts = pc.run(dev, max_weight=wl["max_weight"], min_abs_coeff=wl["min_abs_coeff"]).copy()
+    print(f"device: {len(dev)} terms", flush=True)
+    m = MultiOracleSum(pc.W, B, zbits=4)
+    m.set(keys0, coeffs0)
+    rcounts = np.zeros(pc.ngates, dtype=np.int64)
+    m.run(ctypes.cast(pc.gates, ctypes.c_void_p), pc.ngates, pc.theta, pc.nparams, wl["max_weight"],
+          wl["min_abs_coeff"], min(16, os.cpu_count() or 1), rcounts)
+    assert np.array_equal(counts, rcounts)
+    assert len(dev) == 1455699
+    assert_bitwise(*dev.download(), *m.get_sorted(), "64-qubit B=64 first step")
