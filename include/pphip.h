@@ -235,6 +235,17 @@ enum pp_overlap_kind {
  * overlapwithmaxmixed (:87-93) and overlapwithzero (:35) on a VectorPauliSum. */
 int pp_overlap(pp_handle *h, int kind, const uint64_t *ones, double *out_b);
 
+/* pp_overlap without any device allocation: the same kernels, with their
+ * per-block partial sums and results in a static device array of the library
+ * (about 1 MiB on each GPU it runs on, held until the process ends) instead
+ * of memory the handle allocates on demand. At most 64 coefficient columns
+ * (B > 64 is rejected with PP_ERR_ARG). Every call on one GPU shares that
+ * array, which keeps the last call's contents: calls that overlap in time on
+ * one GPU race (calls on one handle never overlap, as for every entry point).
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). */
+int pp_overlap_noalloc(pp_handle *h, int kind, const uint64_t *ones, double *out_b);
+
 /* The largest |c| of every column (0 for an empty sum).
  * Replaces: maxabscoeff(cache) (src/Base/Primitives/maxabscoeff.jl:7). */
 int pp_max_abs_coeff(pp_handle *h, double *out_b);

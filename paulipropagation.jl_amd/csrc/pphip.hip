@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -36,6 +37,10 @@
 #include "pp_mc.h"
 
 using namespace pp;
+
+// 1024 blocks x 64 columns x 2 (double-double partials) + 64 results
+constexpr size_t SCRATCH_DOUBLES_DECL = 1024 * 64 * 2 + 64;
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES_DECL];
 
 namespace {
 
@@ -53,6 +58,24 @@ int fail(int code, const std::string& msg) {
     } while (0)
 
 constexpr size_t MAX_TERMS = 0x7fffffffu;  // indices carry a sign bit
+
+// Scratch of pp_overlap_noalloc: the overlap kernels' per-block double-double
+// partials (at most SCRATCH_BLOCKS blocks of SCRATCH_COLS columns) and the
+// SCRATCH_COLS results, about 1 MiB per GPU, held until the process ends.
+// Host code never passes the array's name to a kernel: each device's address
+// comes from hipGetSymbolAddress, taken once per device by pp_create with that
+// device current (scratch_on).
+constexpr int SCRATCH_COLS = 64;
+constexpr unsigned SCRATCH_BLOCKS = 1024;
+constexpr size_t SCRATCH_DOUBLES = (size_t)SCRATCH_BLOCKS * SCRATCH_COLS * 2 + SCRATCH_COLS;
+static_assert(SCRATCH_DOUBLES == SCRATCH_DOUBLES_DECL, "scratch layout and declaration agree");
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[MAX_DEVICES] = {};
+std::string g_scratch_err[MAX_DEVICES];
+
+// The device address of `scratch` on device `dev` (current), taken once.
+int scratch_setup(int dev, double** out);
 constexpr size_t RUN_CHUNK = 128;          // gates enqueued between overflow checks
 constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4 limbs
 
@@ -60,6 +83,8 @@ constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4
 
 struct pp_handle {
     int nqubits = 0, W = 1, B = 1;
+    int device = 0;                // HIP device of the handle's stream and buffers
+    double* scratch = nullptr;     // that device's address of the file-scope scratch array
     size_t cap = 0;
     hipStream_t stream = nullptr;
     bool own_stream = true;        // false: a work handle on its parent's stream
@@ -157,6 +182,24 @@ void free_terms(pp_handle* h) {
     h->bufs.states = nullptr;
     h->bufs.tkeys = nullptr;
     h->bufs.splits = nullptr;
+}
+
+int scratch_setup(int dev, double** out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(PP_ERR_HIP, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev] && g_scratch_err[dev].empty()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            g_scratch_err[dev] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        } else {
+            g_scratch[dev] = static_cast<double*>(p);
+        }
+    }
+    *out = g_scratch[dev];
+    if (!*out) return fail(PP_ERR_HIP, g_scratch_err[dev]);
+    return PP_OK;
 }
 
 int tile_rows(int B) {
@@ -1135,9 +1178,12 @@ int ensure_partial(pp_handle* h, size_t need) {
     return PP_OK;
 }
 
-// Double-double partials of `rd`'s readout, finished into B doubles at out.
+// Double-double partials of `rd`'s readout in `partial` (max_blocks x B x 2
+// doubles; nullptr: the handle's own, grown as needed), finished into B doubles
+// at d_res on the device and copied to out.
 template <int W>
-int overlap_impl(pp_handle* h, double* out, const Readout& rd = Readout{}) {
+int overlap_impl(pp_handle* h, double* out, const Readout& rd = Readout{}, double* partial = nullptr,
+                 double* d_res = nullptr, unsigned max_blocks = 2048u) {
     const int B = h->B;
     uint32_t n = 0, cur = 0;
     if (int rc = live(h, &n, &cur)) return rc;
@@ -1146,19 +1192,22 @@ int overlap_impl(pp_handle* h, double* out, const Readout& rd = Readout{}) {
         return PP_OK;
     }
     const unsigned per = B == 1 ? WG : NWAVES;
-    const unsigned g = std::min<unsigned>(2048u, (n + per - 1) / per);
-    if (int rc = ensure_partial(h, (size_t)g * B * 2)) return rc;
+    const unsigned g = std::min<unsigned>(max_blocks, (n + per - 1) / per);
+    if (!partial) {
+        if (int rc = ensure_partial(h, (size_t)g * B * 2)) return rc;
+        partial = h->d_partial;
+        d_res = h->d_out;
+    }
     if (B == 1) {
-        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, rd,
-                           h->d_partial);
+        hipLaunchKernelGGL(k_overlap1<W>, dim3(g), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, rd, partial);
     } else {
         hipLaunchKernelGGL(k_overlapB<W>, dim3(g), dim3(WG), (size_t)NWAVES * B * 2 * sizeof(double), h->stream,
-                           h->bufs, h->d_ctl, h->seq, rd, B, h->d_partial);
+                           h->bufs, h->d_ctl, h->seq, rd, B, partial);
     }
     HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, h->d_partial, (int)g, B, h->d_out);
+    hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(WG), 0, h->stream, partial, (int)g, B, d_res);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, h->d_out, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(out, d_res, B * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->launches[PP_KERNEL_OVERLAP] += 1;
     h->alg_bytes[PP_KERNEL_OVERLAP] += (8.0 * h->W + 8.0 * B) * (double)n;
@@ -2034,6 +2083,10 @@ int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStr
     };
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    h->device = dev;
+    // a device without the scratch address still gets its handle; only
+    // pp_overlap_noalloc reports the failure
+    (void)scratch_setup(dev, &h->scratch);
     h->gmax = (unsigned)std::max(64, 4 * cus);
     h->cus = std::max(1, cus);
     if (shared) {
@@ -2355,6 +2408,28 @@ int pp_overlap(pp_handle* h, int kind, const uint64_t* ones, double* out_b) {
     case 1: return overlap_impl<1>(h, out_b, rd);
     case 2: return overlap_impl<2>(h, out_b, rd);
     default: return overlap_impl<4>(h, out_b, rd);
+    }
+}
+
+int pp_overlap_noalloc(pp_handle* h, int kind, const uint64_t* ones, double* out_b) {
+    if (!h || !out_b) return fail(PP_ERR_ARG, "NULL argument");
+    if (h->B > SCRATCH_COLS)
+        return fail(PP_ERR_ARG, "pp_overlap_noalloc takes at most " + std::to_string(SCRATCH_COLS) +
+                                    " coefficient columns (its static scratch); this sum has " +
+                                    std::to_string(h->B));
+    if (!h->scratch) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        return fail(PP_ERR_HIP, "no scratch address on the handle's device: " +
+                                    g_scratch_err[h->device < MAX_DEVICES && h->device >= 0 ? h->device : 0]);
+    }
+    Readout rd;
+    if (int rc = make_readout(h, kind, ones, &rd)) return rc;
+    double* part = h->scratch;
+    double* res = h->scratch + (size_t)SCRATCH_BLOCKS * SCRATCH_COLS * 2;
+    switch (h->W) {
+    case 1: return overlap_impl<1>(h, out_b, rd, part, res, SCRATCH_BLOCKS);
+    case 2: return overlap_impl<2>(h, out_b, rd, part, res, SCRATCH_BLOCKS);
+    default: return overlap_impl<4>(h, out_b, rd, part, res, SCRATCH_BLOCKS);
     }
 }
 

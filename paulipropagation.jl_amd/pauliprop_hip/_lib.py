@@ -71,6 +71,7 @@ SIGNATURES = [
                                  ctypes.c_double, ctypes.c_int, _U64P, _DP, _DP]),
     ("pp_overlap_zero", ctypes.c_int, [_P, _DP]),
     ("pp_overlap", ctypes.c_int, [_P, ctypes.c_int, _U64P, _DP]),
+    ("pp_overlap_noalloc", ctypes.c_int, [_P, ctypes.c_int, _U64P, _DP]),
     ("pp_get_coeff", ctypes.c_int, [_P, _U64P, _DP]),
     ("pp_scalar_product", ctypes.c_int, [_P, _P, _DP]),
     ("pp_max_abs_coeff", ctypes.c_int, [_P, _DP]),
