@@ -89,6 +89,13 @@ int pp_create(int nqubits, int key_words, int batch, size_t capacity, pp_handle 
 /* Free every device buffer of the handle. */
 int pp_destroy(pp_handle *h);
 
+/* A new handle (its own stream and buffers) holding a copy of h's sum, with
+ * the same qubits, key width and batch, room for its live terms, and h's
+ * tuning settings. The two are independent afterwards.
+ * Replaces: deepcopy of the sum that propagate and rewindgradient take
+ *           (src/Base/propagate.jl:3, src/gradient.jl:27). */
+int pp_clone(pp_handle *h, pp_handle **out);
+
 /* nqubits, key_words, batch, live terms, capacity (any pointer may be NULL). */
 int pp_info(pp_handle *h, int *nqubits, int *key_words, int *batch, size_t *n, size_t *capacity);
 

@@ -2149,6 +2149,39 @@ int pp_destroy(pp_handle* h) {
     return PP_OK;
 }
 
+int pp_clone(pp_handle* h, pp_handle** out) {
+    if (!h || !out) return fail(PP_ERR_ARG, "NULL argument");
+    *out = nullptr;
+    uint32_t n = 0, cur = 0;
+    if (int rc = live(h, &n, &cur)) return rc;
+    pp_handle* c = nullptr;
+    if (int rc = create_handle(h->nqubits, h->W, h->B, std::max<size_t>(n, 1024), nullptr, &c)) return rc;
+    c->spin_limit = h->spin_limit;
+    c->force_dynamic = h->force_dynamic;
+    c->chain_tails = h->chain_tails;
+    c->skip_identity = h->skip_identity;
+    c->adj_defer_bytes = h->adj_defer_bytes;
+    // the copies run on the source's stream, which is idle once they are done
+    bool ok = true;
+    for (int l = 0; l < h->W && n; ++l)
+        ok = ok && hipMemcpyAsync(c->bufs.keys[0] + (size_t)l * c->cap, h->bufs.keys[cur] + (size_t)l * h->cap,
+                                  (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+    if (n)
+        ok = ok && hipMemcpyAsync(c->bufs.coeffs[0], h->bufs.coeffs[cur], (size_t)n * h->B * sizeof(double),
+                                  hipMemcpyDeviceToDevice, h->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(h->stream) == hipSuccess;
+    if (!ok) {
+        pp_destroy(c);
+        return fail(PP_ERR_HIP, "pp_clone: device copy failed");
+    }
+    if (int rc = reset_ctl(c, n, 0)) {
+        pp_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return PP_OK;
+}
+
 int pp_info(pp_handle* h, int* nqubits, int* key_words, int* batch, size_t* n, size_t* capacity) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (nqubits) *nqubits = h->nqubits;

@@ -48,6 +48,7 @@ SIGNATURES = [
     ("pp_last_error", ctypes.c_char_p, []),
     ("pp_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _SZ, ctypes.POINTER(_P)]),
     ("pp_destroy", ctypes.c_int, [_P]),
+    ("pp_clone", ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     ("pp_info", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     ("pp_upload", ctypes.c_int, [_P, _U64P, _DP, _SZ]),

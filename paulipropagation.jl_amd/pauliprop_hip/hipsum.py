@@ -90,6 +90,18 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_download(self._h, _u64p(keys), _dp(coeffs), max(n, 1), ctypes.byref(got)))
         return keys[:n], coeffs[:n]
 
+    def clone(self) -> "HIPPauliSum":
+        """An independent device copy (pp_clone): what deepcopy of the sum gives,
+        as propagate and rewindgradient take it (src/Base/propagate.jl:3)."""
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib().pp_clone(self._h, ctypes.byref(h)))
+        c = HIPPauliSum.__new__(HIPPauliSum)
+        c.nqubits, c.batch, c.W, c._h = self.nqubits, self.batch, self.W, h
+        return c
+
+    def __deepcopy__(self, memo):
+        return self.clone()
+
     def to_paulisum(self) -> PauliSum:
         keys, coeffs = self.download()
         return PauliSum.from_arrays(self.nqubits, keys, coeffs)
@@ -488,9 +500,7 @@ def rewindgradient(circuit, psum, thetas, overlapfunc=None, max_weight=math.inf,
     if isinstance(psum, HIPPauliSum):
         if psum.batch != 1:
             raise ValueError("rewindgradient takes one coefficient column")
-        keys, coeffs = psum.download()
-        work = HIPPauliSum(psum.nqubits, batch=1, capacity=max(1024, 2 * len(keys)))
-        work.upload(keys, coeffs)
+        work = psum.clone()
     else:
         work = HIPPauliSum.from_paulisum(psum, batch=1)
     pc = PackedCircuit(circuit, th.reshape(P, 1), work.nqubits, batch=1)

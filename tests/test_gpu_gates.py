@@ -231,3 +231,24 @@ def test_load_merge_with_repeats(nq, n, B):
         dev.load_merge(keys, coeffs, max_weight=mw, min_abs_coeff=mac)
         ref = OracleSum(W, B).load_merge(keys, coeffs, max_weight=mw, min_abs=mac)
         assert_bitwise(*dev.download(), *ref.get(), f"max_weight {mw} min_abs {mac}")
+
+
+def test_clone_is_an_independent_copy():
+    # pp_clone: what deepcopy of the sum gives (src/Base/propagate.jl:3); the
+    # copy and the original evolve separately, and the copy outlives the original
+    import copy
+    print("pp_clone", flush=True)
+    nq = 64
+    dev, ref = _pair(nq, 50000, seed=3)
+    W = key_words(nq)
+    c = copy.deepcopy(dev)
+    assert_bitwise(*c.download(), *dev.download(), "clone")
+    m = mask_of(nq, {10: 1, 11: 2})
+    c.apply_rotation(m, 0.4, max_weight=9, min_abs_coeff=1e-5)
+    assert_bitwise(*dev.download(), *ref.get(), "original after the copy changed")
+    ref.rotation(int_to_limbs(m, W), math.cos(0.4), math.sin(0.4), max_weight=9, min_abs=1e-5)
+    dev.close()
+    assert_bitwise(*c.download(), *ref.get(), "copy after a gate")
+    c2 = c.clone()
+    c.close()
+    assert len(c2) == len(ref)
