@@ -67,6 +67,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the single-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
+    ap.add_argument("--size-log", default=None,
+                    help="run the workload in chunks of 64 gates and write every gate's input size (and capacity) to "
+                         "this file until it ends or a device allocation fails; no timing")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device three times (scales the HBM counters)")
     args = ap.parse_args()
@@ -106,6 +109,9 @@ def main() -> int:
         psum.upload(keys0, coeffs0)
         pc.run(psum, max_weight=mw, min_abs_coeff=mac)
         psum.synchronize()
+
+    if args.size_log:
+        return size_log(args.size_log, pp, pc, psum, keys0, coeffs0, mw, mac, B, args.workload)
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -165,6 +171,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(pp, pc, psum, keys0, coeffs0, counts, wl, args.cpu_seconds)
+        cpu["published_workload"] = published_anchor(pp, PackedCircuit)
 
     adj = None
     if not args.no_adjoint:
@@ -291,6 +298,116 @@ def adjoint_detail(pp, PackedCircuit):
                      "max_abs_diff_vs_fd": float(max(diffs)), "fd_params_checked": len(diffs)}
         log(f"adjoint {name}: {t_adj * 1e3:.1f} ms per sweep of {B} gradient(s), forward {t_fwd * 1e3:.1f} ms, "
             f"x{adj_rate / fd_rate:.0f} over FD, max |adj - fd| {max(diffs):.2e}")
+    return out
+
+
+def size_log(path, pp, pc, psum, keys0, coeffs0, mw, mac, B, workload):
+    """Every gate's input size, chunk by chunk, until the run ends or fails
+    (e.g. a device allocation the sum outgrows); one line per gate
+    (gate index, input terms, capacity after the chunk), then a JSON summary."""
+    psum.upload(keys0, coeffs0)
+    status, err, g = "complete", None, 0
+    bytes_per_term = 8 * pc.W + 8 * B
+    with open(path, "w") as f:
+        f.write(f"# {workload}: gate n_in capacity (B = {B}, {bytes_per_term} B of key and coefficients per term)\n")
+        while g < pc.ngates:
+            last = min(pc.ngates, g + 64)
+            try:
+                cnt = pc.run(psum, max_weight=mw, min_abs_coeff=mac, first=g, last=last)
+            except pp._lib.PPError as e:
+                status, err = "stopped", str(e)
+                f.write(f"# stopped in gates {g}..{last - 1}: {err}\n")
+                break
+            cap = psum.capacity
+            for i, c in enumerate(cnt):
+                f.write(f"{g + i} {int(c)} {cap}\n")
+            f.flush()
+            log(f"gates {g}..{last - 1}: {int(cnt[-1])} terms entering the last, capacity {cap}")
+            g = last
+    n_end = len(psum) if status == "complete" else None
+    print(json.dumps({"size_log": path, "workload": workload, "status": status, "gates_done": g,
+                      "gates": pc.ngates, "terms_at_end": n_end, "error": err, "capacity": psum.capacity,
+                      "bytes_per_term_key_and_coefficients": bytes_per_term}))
+    return 0
+
+
+def published_anchor(pp, PackedCircuit):
+    """The workload the reference publishes CPU numbers for
+    (examples/advanced_performance.ipynb:92-118 of the reference): the RNG-free
+    6x6 tilted-field Ising Trotter circuit (36 qubits, UInt128 keys; per layer
+    RX and RZ on every qubit and RZZ on rectangletopology(6, 6); dt 0.05, h 1,
+    J 0.5), 14 layers, Z21 Z22, min_abs_coeff 2^-20. Timed here on the GPU,
+    on one host thread (the C oracle) and on all host threads (the zone-parallel
+    C restatement), all bit-identical; rates under both counting conventions:
+    exact (sum of every gate's input terms) and the notebook's (gates per layer
+    times the terms at each layer's start, :457), beside its published
+    218 M/s (one thread, :307-308) and 372-403 M/s (8 threads, :348, :391)."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    nq = 36
+    layer = []
+    pp.rxlayer(layer, nq)
+    pp.rzlayer(layer, nq)
+    pp.rzzlayer(layer, pp.rectangletopology(6, 6))
+    th = np.ones(len(layer)) * 0.05 * 2
+    th[2 * nq:] *= 0.5
+    nlay, L = 14, len(layer)
+    pc = PackedCircuit(layer * nlay, np.tile(th, nlay), nq)
+    obs = pp.PauliSum.from_string(pp.PauliString(nq, ["Z", "Z"], [21, 22]))
+    k0, c0 = obs.to_arrays(pc.W, 1)
+    mac = 2.0 ** -20
+    dev = pp.HIPPauliSum(nq, capacity=1 << 21)
+
+    def gpu_run():
+        dev.upload(k0, c0)
+        pc.run(dev, min_abs_coeff=mac)
+        dev.synchronize()
+
+    gpu_run()
+    reps = 5
+    t = time.perf_counter()
+    for _ in range(reps):
+        gpu_run()
+    gpu_s = (time.perf_counter() - t) / reps
+    counts = pc.counts[:pc.ngates].copy()
+    exact = int(counts.sum())
+    notebook = int(sum(L * counts[i * L] for i in range(nlay)))
+    gk, gc = dev.download()
+    ov = dev.overlapwithzero()
+    th_flat = np.ascontiguousarray(pc.theta.reshape(-1))
+    o = orc.OracleSum(pc.W, 1).set(k0, c0)
+    t = time.perf_counter()
+    o.run(pc.gates, pc.ngates, pc.theta, pc.nparams, [], [], None, mac)
+    one_s = time.perf_counter() - t
+    ok, oc = o.get()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    zones = threads if threads & (threads - 1) == 0 else 2 * (1 << (threads - 1).bit_length())
+    m = orc.MultiOracleSum(pc.W, 1, zones.bit_length() - 1).set(k0, c0)
+    t = time.perf_counter()
+    m.run(ctypes.cast(pc.gates, ctypes.c_void_p), pc.ngates, th_flat, pc.nparams, -1, mac, threads)
+    all_s = time.perf_counter() - t
+    mk, mc = m.get_sorted()
+    same = bool(np.array_equal(gk, ok) and np.array_equal(gc.view(np.uint64), oc.view(np.uint64)) and
+                np.array_equal(gk, mk) and np.array_equal(gc.view(np.uint64), mc.view(np.uint64)))
+
+    def rates(sec):
+        return {"seconds": round(sec, 4), "Mps_exact": round(exact / sec / 1e6, 1),
+                "Mps_notebook": round(notebook / sec / 1e6, 1)}
+
+    out = {"workload": "6x6 tilted-field Ising, 14 layers, Z21 Z22, min_abs_coeff 2^-20 (36 qubits, UInt128)",
+           "terms": len(gk), "overlapwithzero": ov, "applications_exact": exact, "applications_notebook": notebook,
+           "bitwise_equal_gpu_oracle_zones": same,
+           "gpu": rates(gpu_s), "one_thread": rates(one_s), "all_threads": {**rates(all_s), "threads": threads},
+           "cpu_model": cpu_model(),
+           "published": {"one_thread_Mps": 218.0, "eight_threads_Mps": [372.0, 403.2],
+                         "source": "examples/advanced_performance.ipynb:307-308,348,391"}}
+    out["gpu_over_all_threads"] = round(all_s / gpu_s, 2)
+    out["gpu_over_published_8_threads"] = round(notebook / gpu_s / 1e6 / 403.2, 2)
+    out["one_thread_over_published"] = round(notebook / one_s / 1e6 / 218.0, 3)
+    log(f"6x6 TFIM: {len(gk)} terms, <Z21Z22> {ov!r}; GPU {gpu_s * 1e3:.1f} ms, one thread {one_s:.2f} s, "
+        f"{threads} threads {all_s:.2f} s; bit-identical {same}")
     return out
 
 

@@ -46,6 +46,12 @@ struct ppo_sum {
     uint64_t *keys;
     double *coeffs;
     int sorted; /* the whole active set is ascending and duplicate-free */
+    /* working buffers kept between gates (a gate allocates nothing once they
+     * are large enough): the merge output, swapped with keys/coeffs; the
+     * created terms of a branching gate; the XOR sort's second buffer */
+    uint64_t *akeys, *tkeys, *xkeys;
+    double *acoeffs, *tcoeffs, *xcoeffs;
+    size_t acap, tcap, xcap;
 };
 
 /* ------------------------------------------------------------------ */
@@ -160,7 +166,26 @@ void ppo_destroy(ppo_sum *s) {
     if (!s) return;
     free(s->keys);
     free(s->coeffs);
+    free(s->akeys);
+    free(s->acoeffs);
+    free(s->tkeys);
+    free(s->tcoeffs);
+    free(s->xkeys);
+    free(s->xcoeffs);
     free(s);
+}
+
+/* room for `need` rows in a working buffer pair (grown by half again) */
+static void ensure_buf(uint64_t **k, double **c, size_t *cap, size_t need, int W, int B) {
+    if (need <= *cap) return;
+    size_t nc = *cap ? *cap : 16;
+    while (nc < need) nc = nc + nc / 2 + 16;
+    free(*k);
+    free(*c);
+    *k = malloc(nc * W * sizeof(uint64_t));
+    *c = malloc(nc * B * sizeof(double));
+    if (!*k || !*c) abort();
+    *cap = nc;
 }
 
 size_t ppo_size(const ppo_sum *s) { return s->n; }
@@ -256,12 +281,14 @@ static void xor_pass(const uint64_t *ik, const double *ic, uint64_t *ok, double 
     }
 }
 
-int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, const uint64_t *mask) {
+/* The XOR passes over (keys, coeffs), ping-ponging with (tk, tc); *rk, *rc
+ * receive whichever pair holds the sorted tail. Returns the pass count, or
+ * -1 for more than 256 groups. */
+static int xor_sort_buf(uint64_t *keys, double *coeffs, uint64_t *tk, double *tc, size_t n, int W, int B,
+                        const uint64_t *mask, uint64_t **rk, double **rc) {
     int lo[256], hi[256];
     int g = mask_groups(mask, W, lo, hi, 256);
     if (g > 256) return -1;
-    uint64_t *tk = xmalloc(n * W * sizeof(uint64_t));
-    double *tc = xmalloc(n * B * sizeof(double));
     uint64_t *ak = keys, *bk = tk;
     double *ac = coeffs, *bc = tc;
     for (int i = 0; i < g; ++i) {
@@ -269,9 +296,20 @@ int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, co
         uint64_t *t1 = ak; ak = bk; bk = t1;
         double *t2 = ac; ac = bc; bc = t2;
     }
-    if (ak != keys) {
-        memcpy(keys, ak, n * W * sizeof(uint64_t));
-        memcpy(coeffs, ac, n * B * sizeof(double));
+    *rk = ak;
+    *rc = ac;
+    return g;
+}
+
+int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, const uint64_t *mask) {
+    uint64_t *tk = xmalloc(n * W * sizeof(uint64_t));
+    double *tc = xmalloc(n * B * sizeof(double));
+    uint64_t *rk;
+    double *rc;
+    int g = xor_sort_buf(keys, coeffs, tk, tc, n, W, B, mask, &rk, &rc);
+    if (g >= 0 && rk != keys) {
+        memcpy(keys, rk, n * W * sizeof(uint64_t));
+        memcpy(coeffs, rc, n * B * sizeof(double));
     }
     free(tk);
     free(tc);
@@ -285,13 +323,93 @@ int ppo_xor_sort_tail(uint64_t *keys, double *coeffs, size_t n, int W, int B, co
 /* Two-pointer merge of the sorted head (the sum itself) with a sorted,
  * duplicate-free tail. Equal keys merge as head + tail; every output term is
  * truncated. The result replaces the sum. */
+/* One column, key width W known at compile time: the same merge without a
+ * call per copied key or coefficient. */
+/* a < b on keys of W limbs (W = 2 as one 128-bit compare) */
+INLINE int key_lt(const uint64_t *a, const uint64_t *b, const int W) {
+    if (W == 1) return a[0] < b[0];
+    if (W == 2) {
+        const unsigned __int128 x = ((unsigned __int128)a[1] << 64) | a[0];
+        const unsigned __int128 y = ((unsigned __int128)b[1] << 64) | b[0];
+        return x < y;
+    }
+    return key_cmp(a, b, W) < 0;
+}
+
+INLINE size_t merge_head_tail_1(const uint64_t *hk, const double *hc, size_t nh, const uint64_t *tk,
+                                const double *tc, size_t nt, uint64_t *ok, double *oc, int max_weight,
+                                double min_abs, const int W) {
+    size_t i = 0, j = 0, o = 0;
+    while (i < nh || j < nt) {
+        const uint64_t *key;
+        double v;
+        if (j < nt) {
+            /* the heads below the next tail entry go out as they are (each still truncated) */
+            const uint64_t *tj = tk + j * W;
+            while (i < nh && key_lt(hk + i * W, tj, W)) {
+                const uint64_t *kk = hk + i * W;
+                const double hv = hc[i];
+                ++i;
+                if ((max_weight >= 0 && weight_w(kk, W) > max_weight) || fabs(hv) < min_abs) continue;
+                for (int l = 0; l < W; ++l) ok[o * W + l] = kk[l];
+                oc[o] = hv;
+                ++o;
+            }
+        }
+        int c = (i < nh && j < nt) ? key_cmp(hk + i * W, tk + j * W, W) : (i < nh ? -1 : 1);
+        if (c == 0) {
+            key = hk + i * W;
+            v = hc[i] + tc[j];
+            ++i;
+            ++j;
+        } else if (c < 0) {
+            key = hk + i * W;
+            v = hc[i];
+            ++i;
+        } else {
+            key = tk + j * W;
+            v = tc[j];
+            ++j;
+        }
+        if ((max_weight >= 0 && weight_w(key, W) > max_weight) || fabs(v) < min_abs) continue;
+        for (int l = 0; l < W; ++l) ok[o * W + l] = key[l];
+        oc[o] = v;
+        ++o;
+    }
+    return o;
+}
+
 static void merge_head_tail(ppo_sum *s, const uint64_t *tk, const double *tc, size_t nt,
                             int max_weight, double min_abs) {
     int W = s->W, B = s->B;
+    if (B == 1) {
+        const size_t nh = s->n;
+        ensure_buf(&s->akeys, &s->acoeffs, &s->acap, nh + nt, W, B);
+        size_t o;
+        switch (W) {
+        case 1: o = merge_head_tail_1(s->keys, s->coeffs, nh, tk, tc, nt, s->akeys, s->acoeffs, max_weight, min_abs, 1); break;
+        case 2: o = merge_head_tail_1(s->keys, s->coeffs, nh, tk, tc, nt, s->akeys, s->acoeffs, max_weight, min_abs, 2); break;
+        default: o = merge_head_tail_1(s->keys, s->coeffs, nh, tk, tc, nt, s->akeys, s->acoeffs, max_weight, min_abs, 4); break;
+        }
+        uint64_t *k = s->keys;
+        double *c = s->coeffs;
+        s->keys = s->akeys;
+        s->coeffs = s->acoeffs;
+        s->akeys = k;
+        s->acoeffs = c;
+        const size_t oldcap = s->cap;
+        s->cap = s->acap;
+        s->acap = oldcap;
+        s->n = o;
+        s->sorted = 1;
+        return;
+    }
     size_t nh = s->n;
-    uint64_t *ok = xmalloc((nh + nt) * W * sizeof(uint64_t));
-    double *oc = xmalloc((nh + nt) * B * sizeof(double));
-    double *tmp = xmalloc(B * sizeof(double));
+    ensure_buf(&s->akeys, &s->acoeffs, &s->acap, nh + nt, W, B);
+    uint64_t *ok = s->akeys;
+    double *oc = s->acoeffs;
+    double tmp_small[64];
+    double *tmp = B <= 64 ? tmp_small : xmalloc(B * sizeof(double));
     size_t i = 0, j = 0, o = 0;
     while (i < nh || j < nt) {
         const uint64_t *key;
@@ -315,12 +433,15 @@ static void merge_head_tail(ppo_sum *s, const uint64_t *tk, const double *tc, si
         memcpy(oc + o * B, tmp, B * sizeof(double));
         ++o;
     }
-    free(tmp);
-    free(s->keys);
-    free(s->coeffs);
+    if (tmp != tmp_small) free(tmp);
+    /* the output becomes the sum; the old arrays are the next merge's output */
+    s->akeys = s->keys;
+    s->acoeffs = s->coeffs;
     s->keys = ok;
     s->coeffs = oc;
-    s->cap = nh + nt;
+    const size_t oldcap = s->cap;
+    s->cap = s->acap;
+    s->acap = oldcap;
     s->n = o;
     s->sorted = 1;
 }
@@ -391,22 +512,25 @@ int ppo_load_merge(ppo_sum *s, const uint64_t *keys, const double *coeffs, size_
 static void finish_branch(ppo_sum *s, uint64_t *tk, double *tc, size_t nt, const uint64_t *mask,
                           int max_weight, double min_abs) {
     if (s->sorted) {
-        ppo_xor_sort_tail(tk, tc, nt, s->W, s->B, mask);
-        merge_head_tail(s, tk, tc, nt, max_weight, min_abs);
+        uint64_t *rk;
+        double *rc;
+        ensure_buf(&s->xkeys, &s->xcoeffs, &s->xcap, nt, s->W, s->B);
+        xor_sort_buf(tk, tc, s->xkeys, s->xcoeffs, nt, s->W, s->B, mask, &rk, &rc);
+        merge_head_tail(s, rk, rc, nt, max_weight, min_abs);
     } else {
         sort_reduce_truncate(s, tk, tc, nt, max_weight, min_abs);
     }
 }
 
-INLINE long rotation_impl(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv,
-                          int max_weight, double min_abs, const int W) {
+/* The branch half of a rotation: anticommuting terms keep c * cos in place;
+ * their created terms, (c * sin) * sign at term ^ mask, go in parent order to
+ * the sum's tail buffer (src/Base/xorbranch.jl:214-240). Returns the count. */
+INLINE size_t branch_into_tail(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv, const int W) {
     const int B = s->B;
     const size_t n = s->n;
-    size_t na = 0;
-    for (size_t i = 0; i < n; ++i) na += anticommutes_w(mask, s->keys + i * W, W);
-    if (na == 0) return 0;
-    uint64_t *tk = xmalloc(na * W * sizeof(uint64_t));
-    double *tc = xmalloc(na * B * sizeof(double));
+    ensure_buf(&s->tkeys, &s->tcoeffs, &s->tcap, n, W, B);
+    uint64_t *tk = s->tkeys;
+    double *tc = s->tcoeffs;
     size_t j = 0;
     for (size_t i = 0; i < n; ++i) {
         uint64_t *k = s->keys + i * W;
@@ -421,9 +545,14 @@ INLINE long rotation_impl(ppo_sum *s, const uint64_t *mask, const double *cv, co
         }
         ++j;
     }
-    finish_branch(s, tk, tc, na, mask, max_weight, min_abs);
-    free(tk);
-    free(tc);
+    return j;
+}
+
+INLINE long rotation_impl(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv,
+                          int max_weight, double min_abs, const int W) {
+    const size_t na = branch_into_tail(s, mask, cv, sv, W);
+    if (na == 0) return 0;
+    finish_branch(s, s->tkeys, s->tcoeffs, na, mask, max_weight, min_abs);
     return (long)na;
 }
 
@@ -1245,34 +1374,6 @@ void ppo_multi_get(const ppo_multi *m, uint64_t *keys, double *coeffs) {
     }
 }
 
-/* The branch half of a rotation on one zone: anticommuting terms keep c * cos
- * in place and their created terms (parent order) go to a new tail. */
-static size_t branch_zone(ppo_sum *s, const uint64_t *mask, const double *cv, const double *sv, uint64_t **tko,
-                          double **tco) {
-    const int W = s->W, B = s->B;
-    size_t na = 0;
-    for (size_t i = 0; i < s->n; ++i) na += anticommutes_w(mask, s->keys + i * W, W);
-    uint64_t *tk = xmalloc((na ? na : 1) * W * sizeof(uint64_t));
-    double *tc = xmalloc((na ? na : 1) * B * sizeof(double));
-    size_t j = 0;
-    for (size_t i = 0; i < s->n && j < na; ++i) {
-        uint64_t *k = s->keys + i * W;
-        if (!anticommutes_w(mask, k, W)) continue;
-        int e = sign_exponent_w(mask, k, W);
-        double sign = (double)((e & 2) - 1);
-        for (int l = 0; l < W; ++l) tk[j * W + l] = k[l] ^ mask[l];
-        double *c = s->coeffs + i * B;
-        for (int b = 0; b < B; ++b) {
-            tc[j * B + b] = (c[b] * sv[b]) * sign;
-            c[b] = c[b] * cv[b];
-        }
-        ++j;
-    }
-    *tko = tk;
-    *tco = tc;
-    return na;
-}
-
 /* Run rotations and Pauli noise over the zones on nthreads threads: a rotation
  * branches every zone in parallel, then every zone merges the tail of its
  * partner zone (zone ^ zone(mask)), also in parallel. Other gate kinds are
@@ -1300,26 +1401,29 @@ int ppo_multi_run(ppo_multi *m, const ppo_gate *gates, size_t ngates, const doub
             }
             const int zm = zone_of(m, gt->mask);
             size_t total = 0;
+            /* every zone branches into its own tail buffer; a zone's tail is
+             * sorted in its own XOR buffer and merged by its partner zone
+             * z ^ zone(mask), which writes its own merge buffer */
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) reduction(+ : total)
             for (int z = 0; z < Z; ++z) {
-                nt[z] = branch_zone(m->zones[z], gt->mask, cv, sv, &tk[z], &tc[z]);
+                ppo_sum *zs = m->zones[z];
+                switch (W) {
+                case 1: nt[z] = branch_into_tail(zs, gt->mask, cv, sv, 1); break;
+                case 2: nt[z] = branch_into_tail(zs, gt->mask, cv, sv, 2); break;
+                default: nt[z] = branch_into_tail(zs, gt->mask, cv, sv, 4); break;
+                }
+                if (nt[z]) {
+                    ensure_buf(&zs->xkeys, &zs->xcoeffs, &zs->xcap, nt[z], W, B);
+                    xor_sort_buf(zs->tkeys, zs->tcoeffs, zs->xkeys, zs->xcoeffs, nt[z], W, B, gt->mask, &tk[z], &tc[z]);
+                }
                 total += nt[z];
             }
             if (total > 0) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
                 for (int z = 0; z < Z; ++z) {
                     const int src = z ^ zm;
-                    ppo_xor_sort_tail(tk[src], tc[src], nt[src], W, B, gt->mask);
-                }
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
-                for (int z = 0; z < Z; ++z) {
-                    const int src = z ^ zm;
                     merge_head_tail(m->zones[z], tk[src], tc[src], nt[src], max_weight, min_abs);
                 }
-            }
-            for (int z = 0; z < Z; ++z) {
-                free(tk[z]);
-                free(tc[z]);
             }
         } else if (gt->kind == PPO_PAULI_NOISE) {
             for (int b = 0; b < B; ++b) fv[b] = 1.0 - gate_value(gt, theta, B, b);
