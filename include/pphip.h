@@ -355,17 +355,42 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * PP_TUNE_ADJ_DEFER_BYTES bounds the device memory pp_adjoint may take to keep
  * every rotation's per-block partial gradients of a segment and sum them in
  * one launch (default 2 GiB); above it, or when that allocation fails, each
- * rotation sums its own (results are identical). */
+ * rotation sums its own (results are identical).
+ * Launch shapes (any value gives identical results): PP_TUNE_MERGE_ITEMS the
+ * one-column merge's tile of 256 x 2, 3 or 4 diagonals (0: 3 for 256-bit
+ * keys, 4 otherwise); PP_TUNE_PERM_ITEMS the tail permutation's list entries
+ * per thread (1, 2, 4; default 2); PP_TUNE_BATCH_ROWS the batched merge's term
+ * rows per tile (a power of two 16..256 with rows x B x 8 bytes <= 64 KiB; 0:
+ * by B); PP_TUNE_RADIX_BITS the radix sort's digit width (4..8, default 8);
+ * PP_TUNE_GRID_PER_CU the persistent grids' workgroups per compute unit
+ * (1..8, default 4); PP_TUNE_RUN_CHUNK the gates pp_run_circuit enqueues
+ * between overflow checks (16..1024, default 128). */
 enum pp_tune_key {
     PP_TUNE_SPIN_LIMIT = 1,
     PP_TUNE_DYNAMIC_TILES = 2,
     PP_TUNE_CHAIN_TAILS = 3,
     PP_TUNE_SKIP_IDENTITY = 4,
-    PP_TUNE_ADJ_DEFER_BYTES = 5
+    PP_TUNE_ADJ_DEFER_BYTES = 5,
+    PP_TUNE_MERGE_ITEMS = 6,
+    PP_TUNE_PERM_ITEMS = 7,
+    PP_TUNE_BATCH_ROWS = 8,
+    PP_TUNE_RADIX_BITS = 9,
+    PP_TUNE_GRID_PER_CU = 10,
+    PP_TUNE_RUN_CHUNK = 11
 };
 
 /* Set a tuning value. No counterpart in the reference. */
 int pp_tune(pp_handle *h, int key, int64_t value);
+
+/* Apply a tuning table: lines "<key_words> <max_batch> <name> <value>" ('#'
+ * starts a comment), name one of merge_items, perm_items, batch_rows,
+ * radix_bits, grid_per_cu, run_chunk. The handle takes the lines of its key
+ * width with the smallest max_batch not below its batch. path NULL: the file
+ * named by the environment variable PPHIP_TUNE_FILE, else pphip_tune.txt in
+ * the library's directory if it exists (pp_create applies that one). The
+ * table is written by the autotuner (python -m pauliprop_hip.autotune). No
+ * counterpart in the reference. */
+int pp_load_tuning(pp_handle *h, const char *path);
 
 #ifdef __cplusplus
 }

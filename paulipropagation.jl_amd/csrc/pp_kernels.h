@@ -483,7 +483,10 @@ __global__ __launch_bounds__(WG) void k_branch(Bufs bf, Ctl* ctl, Launch L, Key<
 
 // ============================================================ k_xorperm
 
-constexpr int K2_ITEMS = 2;  // 1, 3, 4 and 8 measured slower on the kicked Ising workload
+// list entries per thread of the tail permutation: the default 2 measured
+// fastest on the kicked Ising workload (1, 3, 4 and 8 slower); 1 and 4 are
+// compiled too, for the tuning table
+constexpr int K2_ITEMS = 2;
 constexpr int K2_TILE = WG * K2_ITEMS;
 constexpr int INF_POS = 0x7fffffff;
 
@@ -588,9 +591,10 @@ __device__ __forceinline__ uint32_t wave_first_true_fwd(uint32_t lo, uint32_t hi
 // kernel lists that rotation's touched terms instead, so that its k_branch
 // is not needed either. check: report (err bit 4) a list that is not this
 // gate's.
-template <int W, bool BRANCH>
+template <int W, bool BRANCH, int K2_ITEMS = pp::K2_ITEMS>
 __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Levels lv, Key<W> mask, Key<W> nmask,
                                                 uint32_t has_next, uint32_t check, double tscale) {
+    constexpr int K2_TILE = WG * K2_ITEMS;
     // The list is read only once its length is known: a first tile loaded
     // with the control block, as the merge loads its splits, makes every
     // workgroup of the capacity-sized grid read a full tile, which on the
@@ -939,13 +943,16 @@ __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, si
     }
 }
 
-// Tile of the one-column merge: 768 diagonals for 256-bit keys, 1024 otherwise,
-// so that the tile's keys and values fit shared memory five times per CU.
+// Tile of the one-column merge: by default 768 diagonals for 256-bit keys,
+// 1024 otherwise, so that the tile's keys and values fit shared memory four
+// times per CU; 512, 768 and 1024 are compiled for every key width, for the
+// tuning table.
 template <int W>
 struct K3Cfg {
     static constexpr int ITEMS = W == 4 ? 3 : 4;
     static constexpr int TILE = WG * ITEMS;
 };
+constexpr int MIN_MERGE_ITEMS = 2;  // the narrowest merge tile any table may choose
 
 // Merge-path split of every tile boundary of this gate's merge, one wave per
 // boundary: splits[b] = head entries among the first min(b * TILE, n + na).
@@ -1070,10 +1077,10 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
 //
 // With REC, every head term and every touched term's created term records its
 // output position (REC_NONE when truncated, merged away or absent).
-template <int W, bool EMIT, bool REC>
+template <int W, bool EMIT, bool REC, int ITEMS = K3Cfg<W>::ITEMS>
 __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
                                                Key<W> nmask) {
-    constexpr int ITEMS = K3Cfg<W>::ITEMS, TILE = K3Cfg<W>::TILE;
+    constexpr int TILE = WG * ITEMS;
     static_assert(TILE < 65536, "kept and touched counts of a tile are packed in 16 bits each");
     // the first static tile's splits go out with the control block (tile =
     // blockIdx.x; the index is clamped into the array, and used only if the

@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -76,7 +78,17 @@ std::string g_scratch_err[MAX_DEVICES];
 
 // The device address of `scratch` on device `dev` (current), taken once.
 int scratch_setup(int dev, double** out);
-constexpr size_t RUN_CHUNK = 128;          // gates enqueued between overflow checks
+// Launch shapes the tuning table may set (pp_tune, pp_load_tuning); every
+// choice gives bit-identical results.
+struct Tuning {
+    int merge_items = 0;   // one-column merge tile = 256 x this (2, 3, 4); 0: 3 for 256-bit keys, else 4
+    int perm_items = 2;    // tail permutation: list entries per thread (1, 2, 4)
+    int batch_rows = 0;    // batched merge: term rows per tile (16 .. 256); 0: by B
+    int radix_bits = 8;    // Clifford / load-merge radix sort: digit width (4 .. 8)
+    int grid_per_cu = 4;   // persistent grids: at most this many workgroups per compute unit (1 .. 8)
+    int run_chunk = 128;   // gates enqueued between overflow checks (16 .. 1024)
+};
+constexpr int MAX_GRID_PER_CU = 8;
 constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4 limbs
 
 }  // namespace
@@ -90,8 +102,9 @@ struct pp_handle {
     bool own_stream = true;        // false: a work handle on its parent's stream
     Bufs bufs{};
     size_t lb_cap = 0;
-    unsigned gmax = 2048;          // persistent grid size
+    unsigned gmax = 2048;          // persistent grid size (tun.grid_per_cu x compute units)
     int cus = 256;                 // compute units
+    Tuning tun;                    // launch shapes
     std::unordered_map<const void*, unsigned> resident;  // co-resident workgroups per kernel
     uint32_t spin_limit = 1u << 16;  // bound of one look-back wait with static tiles
     bool force_dynamic = false;      // take tiles from the atomic counter always
@@ -209,11 +222,18 @@ int tile_rows(int B) {
     return 16;
 }
 
+// the batched merge's rows per tile: the tuning table's, or by B
+int batch_rows(const pp_handle* h) { return h->tun.batch_rows ? h->tun.batch_rows : tile_rows(h->B); }
+
+// the batched merge's dynamic shared memory limit (rows x B doubles)
+constexpr size_t MAX_ROWS_LDS = 64 * 1024;
+
 // look-back words for the largest compaction launch over up to cap entries
-// (every look-back kernel's tiles are at least this wide)
-static_assert(K1_TILE >= K3Cfg<4>::TILE, "branch tiles narrower than the merge's would outnumber the look-back words");
+// (every look-back kernel's tiles are at least this wide: 512 terms for one
+// column, 16 rows for batches, whatever the tuning table picks)
+static_assert(K1_TILE >= WG * MIN_MERGE_ITEMS, "branch tiles narrower than the merge's would outnumber the look-back words");
 size_t tiles_needed(size_t cap, int B) {
-    const size_t tile = B > 1 ? (size_t)std::min(KNB_ROWS, tile_rows(B)) : (size_t)K3Cfg<4>::TILE;
+    const size_t tile = B > 1 ? (size_t)16 : (size_t)WG * MIN_MERGE_ITEMS;
     return cap / tile + 64;
 }
 
@@ -524,6 +544,61 @@ void launch(pp_handle* h, hipEvent_t e0, hipEvent_t e1, F kernel, unsigned grid,
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(WG), shm, h->stream, args...);
 }
 
+// The tail permutation with the tuning table's list entries per thread; on a
+// look-back launch (lbgrid) the grid is capped at the kernel's residency.
+template <int W, bool BR, int PI>
+void enq_perm_pi(pp_handle* h, hipEvent_t e0, hipEvent_t e1, const Launch& L, bool lbgrid, const Levels& lv,
+                 const Key<W>& mask, const Key<W>& nmask, uint32_t has_next, uint32_t check, double tscale) {
+    constexpr int T2 = WG * PI;
+    const unsigned grid = lbgrid ? lb_grid(h, (const void*)k_xorperm<W, BR, PI>, 0, h->cap, T2)
+                                 : grid_for(h, h->cap, T2);
+    launch(h, e0, e1, k_xorperm<W, BR, PI>, grid, 0, h->bufs, h->d_ctl, L, lv, mask, nmask, has_next, check, tscale);
+}
+
+template <int W, bool BR>
+void enq_perm(pp_handle* h, hipEvent_t e0, hipEvent_t e1, const Launch& L, bool lbgrid, const Levels& lv,
+              const Key<W>& mask, const Key<W>& nmask, uint32_t has_next, uint32_t check, double tscale) {
+    switch (h->tun.perm_items) {
+    case 1: enq_perm_pi<W, BR, 1>(h, e0, e1, L, lbgrid, lv, mask, nmask, has_next, check, tscale); break;
+    case 4: enq_perm_pi<W, BR, 4>(h, e0, e1, L, lbgrid, lv, mask, nmask, has_next, check, tscale); break;
+    default: enq_perm_pi<W, BR, 2>(h, e0, e1, L, lbgrid, lv, mask, nmask, has_next, check, tscale); break;
+    }
+}
+
+// The one-column merge with the tuning table's tile (256 x MI diagonals): its
+// partition, then the merge; variant 0 plain, 1 listing the next rotation's
+// touched terms (nmask), 2 recording for the adjoint sweep.
+template <int W, int MI>
+void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int variant, const Key<W>& mask,
+                   const RotParams& rp, const Key<W>& nmask, uint32_t* glog) {
+    constexpr int TILE = WG * MI;
+    const size_t expect = h->cap + h->cap / 2;
+    Launch LP{};
+    LP.seq = h->seq;
+    launch(h, nullptr, e3, k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs, h->d_ctl,
+           LP);
+    if (variant == 2)
+        launch(h, e4, e5, k_merge1<W, false, true, MI>, lb_grid(h, (const void*)k_merge1<W, false, true, MI>, 0, expect, TILE),
+               0, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+    else if (variant == 1)
+        launch(h, e4, e5, k_merge1<W, true, false, MI>, lb_grid(h, (const void*)k_merge1<W, true, false, MI>, 0, expect, TILE),
+               0, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, nmask);
+    else
+        launch(h, e4, e5, k_merge1<W, false, false, MI>,
+               lb_grid(h, (const void*)k_merge1<W, false, false, MI>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
+               lb_launch(h, glog), mask, rp, mask);
+}
+
+template <int W>
+void enq_merge1(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int variant, const Key<W>& mask,
+                const RotParams& rp, const Key<W>& nmask, uint32_t* glog) {
+    switch (h->tun.merge_items ? h->tun.merge_items : K3Cfg<W>::ITEMS) {
+    case 2: enq_merge1_mi<W, 2>(h, e3, e4, e5, variant, mask, rp, nmask, glog); break;
+    case 3: enq_merge1_mi<W, 3>(h, e3, e4, e5, variant, mask, rp, nmask, glog); break;
+    default: enq_merge1_mi<W, 4>(h, e3, e4, e5, variant, mask, rp, nmask, glog); break;
+    }
+}
+
 // next_mask: the mask of the rotation enqueued right after this one, if any;
 // the merge then lists that gate's touched terms (B = 1).
 // Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
@@ -556,37 +631,18 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
                h->d_ctl, L2, h->B);
     const bool part = h->B == 1;
     if (chained)
-        launch(h, E(2, all), nullptr, k_xorperm<W, true>,
-               lb_grid(h, (const void*)k_xorperm<W, true>, 0, h->cap, K2_TILE), 0, h->bufs, h->d_ctl,
-               lb_launch(h, glog), lv, mask, next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u,
-               listed ? 1u : 0u, s1);
+        enq_perm<W, true>(h, E(2, all), nullptr, lb_launch(h, glog), true, lv, mask,
+                          next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u, listed ? 1u : 0u, s1);
     else
-        launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-               h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, s1);
+        enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, s1);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
-        constexpr int TILE = K3Cfg<W>::TILE;
-        Launch LP{};
-        LP.seq = h->seq;
-        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
-               h->d_ctl, LP);
-        if (h->recording) {
-            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, true>,
-                   lb_grid(h, (const void*)k_merge1<W, false, true>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-                   lb_launch(h, glog), mask, rp, mask);
-        } else if (next_mask) {
-            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, true, false>,
-                   lb_grid(h, (const void*)k_merge1<W, true, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-                   lb_launch(h, glog), mask, rp, to_key<W>(next_mask));
-        } else {
-            launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
-                   lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-                   lb_launch(h, glog), mask, rp, mask);
-        }
+        enq_merge1<W>(h, E(3, all), E(4, mrg), E(5, mrg), h->recording ? 2 : (next_mask ? 1 : 0), mask, rp,
+                      next_mask ? to_key<W>(next_mask) : mask, glog);
     } else {
         const int B = h->B;
-        const int TB = tile_rows(B);
+        const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
@@ -625,8 +681,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
     L2.seq = h->seq;
     L2.glog = glog;
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, sh);
+    enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, sh);
     if (normalize)
         hipLaunchKernelGGL(k_imag_norm<W>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, mask, h->B,
                            ch, sh, d_trig, h->d_norm);
@@ -636,17 +691,10 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
         RotParams rp{ch, sh, max_weight, min_abs};
         rp.comm = 1;
         rp.norm = norm;
-        constexpr int TILE = K3Cfg<W>::TILE;
-        Launch LP{};
-        LP.seq = h->seq;
-        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
-               h->d_ctl, LP);
-        launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
-               lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-               lb_launch(h, glog), mask, rp, mask);
+        enq_merge1<W>(h, E(3, all), E(4, mrg), E(5, mrg), 0, mask, rp, mask, glog);
     } else {
         const int B = h->B;
-        const int TB = tile_rows(B);
+        const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
@@ -685,22 +733,14 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
     L2.seq = h->seq;
     L2.glog = glog;
     const bool part = h->B == 1;
-    launch(h, E(2, all), part ? nullptr : E(3, all), k_xorperm<W, false>, grid_for(h, h->cap, K2_TILE), 0,
-           h->bufs, h->d_ctl, L2, lv, mask, mask, 0u, 0u, gamma);
+    enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, gamma);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{1 - gamma, gamma, max_weight, min_abs, qubit, fxy};
-        constexpr int TILE = K3Cfg<W>::TILE;
-        Launch LP{};
-        LP.seq = h->seq;
-        launch(h, nullptr, E(3, all), k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
-               h->d_ctl, LP);
-        launch(h, E(4, mrg), E(5, mrg), k_merge1<W, false, false>,
-               lb_grid(h, (const void*)k_merge1<W, false, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-               lb_launch(h, glog), mask, rp, mask);
+        enq_merge1<W>(h, E(3, all), E(4, mrg), E(5, mrg), 0, mask, rp, mask, glog);
     } else {
         const int B = h->B;
-        const int TB = tile_rows(B);
+        const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = lb_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
@@ -773,8 +813,9 @@ int ensure_sort(pp_handle* h) {
 // the buffer index holding the result. n is read on the device at n_ptr.
 int enq_radix_sort(pp_handle* h, const uint32_t* n_ptr, int c, int nbits) {
     const unsigned grid = grid_for(h, h->cap, RS_TILE);
-    for (int shift = 0; shift < nbits; shift += 8) {
-        const int bits = std::min(8, nbits - shift);
+    const int step = h->tun.radix_bits;
+    for (int shift = 0; shift < nbits; shift += step) {
+        const int bits = std::min(step, nbits - shift);
         hipLaunchKernelGGL(k_rs_upsweep, dim3(grid), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_k[c], shift,
                            bits, h->d_rs_hist, (uint32_t)h->rs_tiles);
         hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(WG), 0, h->stream, h->d_ctl, n_ptr, h->d_rs_hist,
@@ -972,7 +1013,7 @@ void widen_qubit(Presence& P, int q) {
 // the next stream synchronization).
 int enq_presence(pp_handle* h) {
     const unsigned grid = grid_for(h, h->cap, (size_t)WG * 16);
-    uint64_t* res = h->d_pres_part + (size_t)h->gmax * PRES_WORDS;
+    uint64_t* res = h->d_pres_part + (size_t)std::max(64, MAX_GRID_PER_CU * h->cus) * PRES_WORDS;
     switch (h->W) {
     case 1:
         hipLaunchKernelGGL(k_presence<1>, dim3(grid), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq,
@@ -1021,7 +1062,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
     const bool skip = h->skip_identity && !h->recording && last - first > 1;
     std::vector<uint8_t> skipped(last - first, 0);
     std::vector<size_t> order;  // gates of the chunk that are enqueued
-    order.reserve(RUN_CHUNK);
+    order.reserve(h->tun.run_chunk);
     bool pres_ok = false;
     if (skip && first < last) {
         if (int rc = enq_presence(h)) return rc;
@@ -1029,7 +1070,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         pres_ok = h->h_ctl->overflow == 0;
     }
     while (g < ng) {
-        const size_t end = std::min(ng, g + RUN_CHUNK);
+        const size_t end = std::min(ng, g + (size_t)h->tun.run_chunk);
         const uint32_t seq0 = h->seq;
         order.clear();
         if (pres_ok) {
@@ -1988,6 +2029,8 @@ int rewind_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double*
     do {
         if ((rc = create_handle(h->nqubits, W, 1, std::max<size_t>(h->cap, 1024), h->stream, &hd))) break;
         hd->skip_identity = false;
+        hd->tun = h->tun;
+        hd->gmax = h->gmax;
         for (int l = 0; l < W && n; ++l)
             if (hipMemcpyAsync(hd->bufs.keys[0] + (size_t)l * hd->cap, h->bufs.keys[cur] + (size_t)l * h->cap,
                                n * sizeof(uint64_t), hipMemcpyDeviceToDevice, h->stream) != hipSuccess) {
@@ -2087,8 +2130,8 @@ int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStr
     // a device without the scratch address still gets its handle; only
     // pp_overlap_noalloc reports the failure
     (void)scratch_setup(dev, &h->scratch);
-    h->gmax = (unsigned)std::max(64, 4 * cus);
     h->cus = std::max(1, cus);
+    h->gmax = (unsigned)std::max(64, h->tun.grid_per_cu * h->cus);
     if (shared) {
         h->stream = shared;
         h->own_stream = false;
@@ -2099,11 +2142,15 @@ int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStr
         hipHostMalloc(&h->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess ||
         hipMalloc(&h->d_out, batch * sizeof(double)) != hipSuccess ||
         hipMalloc(&h->d_norm, batch * sizeof(double)) != hipSuccess ||
-        hipMalloc(&h->d_pres_part, ((size_t)h->gmax + 1) * PRES_WORDS * sizeof(uint64_t)) != hipSuccess ||
+        hipMalloc(&h->d_pres_part, ((size_t)std::max(64, MAX_GRID_PER_CU * h->cus) + 1) * PRES_WORDS *
+                                       sizeof(uint64_t)) != hipSuccess ||
         hipHostMalloc(&h->h_pres, PRES_WORDS * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
         return bail(fail(PP_ERR_NOMEM, "device allocation failed"));
     if (int rc = reset_ctl(h, 0, 0)) return bail(rc);
     if (int rc = grow(h, std::max<size_t>(capacity, 1024))) return bail(rc);
+    // a work handle takes its parent's launch shapes (the caller copies them)
+    if (!shared)
+        if (int rc = pp_load_tuning(h, nullptr)) return bail(rc);
     *out = h;
     return PP_OK;
 }
@@ -2161,6 +2208,8 @@ int pp_clone(pp_handle* h, pp_handle** out) {
     c->chain_tails = h->chain_tails;
     c->skip_identity = h->skip_identity;
     c->adj_defer_bytes = h->adj_defer_bytes;
+    c->tun = h->tun;
+    c->gmax = h->gmax;
     // the copies run on the source's stream, which is idle once they are done
     bool ok = true;
     for (int l = 0; l < h->W && n; ++l)
@@ -2548,9 +2597,104 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         if (value < 0) return fail(PP_ERR_ARG, "the partial-gradient store size must be non-negative");
         h->adj_defer_bytes = (size_t)value;
         return PP_OK;
+    case PP_TUNE_MERGE_ITEMS:
+        if (value != 0 && value != 2 && value != 3 && value != 4)
+            return fail(PP_ERR_ARG, "merge items must be 2, 3 or 4 (0: the key width's default)");
+        h->tun.merge_items = (int)value;
+        return PP_OK;
+    case PP_TUNE_PERM_ITEMS:
+        if (value != 1 && value != 2 && value != 4) return fail(PP_ERR_ARG, "permutation items must be 1, 2 or 4");
+        h->tun.perm_items = (int)value;
+        return PP_OK;
+    case PP_TUNE_BATCH_ROWS:
+        if (value != 0 && !(value >= 16 && value <= 256 && (value & (value - 1)) == 0))
+            return fail(PP_ERR_ARG, "batch rows must be a power of two in 16..256 (0: by the batch)");
+        if ((size_t)value * h->B * sizeof(double) > MAX_ROWS_LDS)
+            return fail(PP_ERR_ARG, "batch rows x columns exceed the merge's shared memory");
+        h->tun.batch_rows = (int)value;
+        return PP_OK;
+    case PP_TUNE_RADIX_BITS:
+        if (value < 4 || value > 8) return fail(PP_ERR_ARG, "radix digits must be 4..8 bits");
+        h->tun.radix_bits = (int)value;
+        return PP_OK;
+    case PP_TUNE_GRID_PER_CU:
+        if (value < 1 || value > MAX_GRID_PER_CU) return fail(PP_ERR_ARG, "grid per compute unit must be 1..8");
+        h->tun.grid_per_cu = (int)value;
+        h->gmax = (unsigned)std::max(64, h->tun.grid_per_cu * h->cus);
+        return PP_OK;
+    case PP_TUNE_RUN_CHUNK:
+        if (value < 16 || value > 1024) return fail(PP_ERR_ARG, "run chunk must be 16..1024 gates");
+        h->tun.run_chunk = (int)value;
+        return PP_OK;
     default:
         return fail(PP_ERR_ARG, "unknown tuning key");
     }
+}
+
+int pp_load_tuning(pp_handle* h, const char* path) {
+    if (!h) return fail(PP_ERR_ARG, "handle is NULL");
+    std::string file;
+    if (path) {
+        file = path;
+    } else if (const char* env = std::getenv("PPHIP_TUNE_FILE")) {
+        file = env;
+    } else {
+        // pphip_tune.txt beside the library, if there is one
+        Dl_info info{};
+        if (dladdr((const void*)&pp_version, &info) && info.dli_fname) {
+            file = info.dli_fname;
+            const size_t slash = file.rfind('/');
+            file = (slash == std::string::npos ? std::string(".") : file.substr(0, slash)) + "/pphip_tune.txt";
+        }
+        FILE* probe = file.empty() ? nullptr : std::fopen(file.c_str(), "r");
+        if (!probe) return PP_OK;
+        std::fclose(probe);
+    }
+    FILE* f = std::fopen(file.c_str(), "r");
+    if (!f) return fail(PP_ERR_ARG, "cannot open the tuning table " + file);
+    struct Entry {
+        int w, bmax;
+        std::string name;
+        long long value;
+    };
+    std::vector<Entry> es;
+    char line[512];
+    int lineno = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        ++lineno;
+        const char* p = line;
+        while (*p == ' ' || *p == '\t') ++p;
+        if (*p == '#' || *p == '\n' || *p == '\0') continue;
+        int w = 0, bmax = 0;
+        char name[64] = {0};
+        long long value = 0;
+        if (std::sscanf(p, "%d %d %63s %lld", &w, &bmax, name, &value) != 4) {
+            std::fclose(f);
+            return fail(PP_ERR_ARG, file + ":" + std::to_string(lineno) + ": expected <key_words> <max_batch> <name> <value>");
+        }
+        es.push_back({w, bmax, name, value});
+    }
+    std::fclose(f);
+    // the class of this handle: its key width and the smallest batch bound that covers B
+    int best = -1;
+    for (const Entry& e : es)
+        if (e.w == h->W && e.bmax >= h->B && (best < 0 || e.bmax < best)) best = e.bmax;
+    if (best < 0) return PP_OK;
+    static const struct {
+        const char* name;
+        int key;
+    } names[] = {{"merge_items", PP_TUNE_MERGE_ITEMS}, {"perm_items", PP_TUNE_PERM_ITEMS},
+                 {"batch_rows", PP_TUNE_BATCH_ROWS},   {"radix_bits", PP_TUNE_RADIX_BITS},
+                 {"grid_per_cu", PP_TUNE_GRID_PER_CU}, {"run_chunk", PP_TUNE_RUN_CHUNK}};
+    for (const Entry& e : es) {
+        if (e.w != h->W || e.bmax != best) continue;
+        int key = 0;
+        for (const auto& n : names)
+            if (e.name == n.name) key = n.key;
+        if (!key) return fail(PP_ERR_ARG, file + ": unknown tuning name " + e.name);
+        if (int rc = pp_tune(h, key, e.value)) return rc;
+    }
+    return PP_OK;
 }
 
 int pp_profile_read(const pp_handle* h, pp_profile* out) {

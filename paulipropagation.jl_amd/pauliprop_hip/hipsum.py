@@ -289,12 +289,19 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
     def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None,
-             skip_identity: bool = None, adj_defer_bytes: int = None) -> None:
+             skip_identity: bool = None, adj_defer_bytes: int = None, **shapes) -> None:
         """Bound of a look-back wait (polls); whether tiles are always claimed
         atomically; whether a merge lists the next rotation's touched terms;
         whether circuits leave out the rotations that touch no term; the device
-        memory the adjoint sweep may keep per-rotation partial gradients in."""
+        memory the adjoint sweep may keep per-rotation partial gradients in.
+        Launch shapes by name (merge_items, perm_items, batch_rows, radix_bits,
+        grid_per_cu, run_chunk; any value the library accepts gives identical
+        results)."""
         L = _lib.lib()
+        for name, v in shapes.items():
+            if name not in _lib.TUNE_NAMES:
+                raise TypeError(f"unknown launch shape {name}")
+            _lib.check(L.pp_tune(self._h, _lib.TUNE_NAMES[name], int(v)))
         if adj_defer_bytes is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_ADJ_DEFER_BYTES, int(adj_defer_bytes)))
         if skip_identity is not None:
@@ -305,6 +312,10 @@ class HIPPauliSum:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_DYNAMIC_TILES, int(bool(dynamic_tiles))))
         if chain_tails is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_CHAIN_TAILS, int(bool(chain_tails))))
+
+    def load_tuning(self, path: str) -> None:
+        """Apply a tuning table (lines "<key_words> <max_batch> <name> <value>")."""
+        _lib.check(_lib.lib().pp_load_tuning(self._h, path.encode()))
 
     def profile(self) -> dict:
         p = _lib.pp_profile()

@@ -149,3 +149,34 @@ def test_product_path_has_no_host_fallback():
         pytest.skip("a GPU is present")
     with pytest.raises(_lib.PPError):
         pp.HIPPauliSum(8)
+
+
+def test_tuning_table_and_autotuner_candidates():
+    # the table pp_create reads: "<key_words> <max_batch> <name> <value>" lines
+    # whose values the library accepts (pp_tune's ranges, include/pphip.h)
+    import os
+    from pauliprop_hip import _lib, autotune
+    ok = {"merge_items": {0, 2, 3, 4}, "perm_items": {1, 2, 4}, "batch_rows": {0, 16, 32, 64, 128, 256},
+          "radix_bits": set(range(4, 9)), "grid_per_cu": set(range(1, 9)), "run_chunk": set(range(16, 1025))}
+    assert set(ok) == set(_lib.TUNE_NAMES) == set(autotune.CANDIDATES) == set(autotune.DEFAULTS)
+    for name, vals in autotune.CANDIDATES.items():
+        assert set(vals) <= ok[name], name
+        assert autotune.DEFAULTS[name] in ok[name]
+    for B in (1, 32, 64, 256):
+        assert all(r * B * 8 <= 64 * 1024 for r in autotune.candidates("batch_rows", B))
+    classes = {(W, b) for W, b, _ in autotune.CLASSES}
+    path = os.path.join(os.path.dirname(_lib.__file__), "pphip_tune.txt")
+    if not os.path.exists(path):
+        return
+    seen = set()
+    for line in open(path):
+        line = line.strip()
+        if not line or line.startswith("#"):
+            continue
+        W, b, name, v = line.split()
+        W, b, v = int(W), int(b), int(v)
+        assert (W, b) in classes and name in ok and v in ok[name], line
+        if name == "batch_rows":
+            assert v * b * 8 <= 64 * 1024, line
+        assert (W, b, name) not in seen, line
+        seen.add((W, b, name))
