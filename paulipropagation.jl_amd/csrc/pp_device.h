@@ -335,26 +335,25 @@ __device__ __forceinline__ bool lb_window(const uint64_t* words, size_t cstride,
     return true;
 }
 
-// Two-level decoupled look-back over NV prefix chains at once (the chains
-// share every poll). Called by all 64 lanes of one wave; publishes agg[c] for
-// `tile` and returns in excl[c] the exclusive prefix of all earlier tiles.
-// Tile words of chain c start at states + c * tstride, group words at
-// gstates + c * gstride.
-//
-// Tiles form groups of 64. A tile first reads its own group's earlier words;
-// when none of them holds an inclusive prefix it adds the group-level words
-// (one per earlier group: the group's aggregate, or its inclusive prefix),
-// 64 groups per poll. The last tile of a group publishes the group's word.
-// A prefix therefore resolves in a few polls however many tiles run at once,
-// where a flat look-back walks back through every running tile, 64 per poll.
-//
-// spin_limit bounds the polls of one wait (0 = unbounded). When it runs out,
-// *aborted is set and the partial sums read so far are returned: they never
-// exceed the true prefixes, so positions built from them stay in range.
+// First half of lb_exclusive: publish this tile's aggregate (tile 0: its
+// inclusive prefix). A workgroup may do other work between this and
+// lb_resolve (its next tile's loads): nothing waits on it until then.
 template <int NV>
-__device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates, size_t gstride, uint32_t tile,
-                             uint32_t epoch, const uint64_t (&agg)[NV], uint32_t spin_limit, bool* aborted,
-                             uint64_t (&excl)[NV]) {
+__device__ __forceinline__ void lb_publish(uint64_t* states, size_t tstride, uint32_t tile, uint32_t epoch,
+                                           const uint64_t (&agg)[NV]) {
+    if (lane_id() != 0) return;
+    const uint64_t ep = (uint64_t)(epoch & LB_EPOCH_MASK) << 40;
+    const uint64_t st = tile == 0 ? (2ull << 62) : (1ull << 62);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) lb_store(states + c * tstride + tile, st | ep | agg[c]);
+}
+
+// Second half of lb_exclusive: the exclusive prefix of `tile`, whose aggregate
+// lb_publish has stored.
+template <int NV>
+__device__ void lb_resolve(uint64_t* states, size_t tstride, uint64_t* gstates, size_t gstride, uint32_t tile,
+                           uint32_t epoch, const uint64_t (&agg)[NV], uint32_t spin_limit, bool* aborted,
+                           uint64_t (&excl)[NV]) {
     const int lane = lane_id();
     const uint32_t ep32 = epoch & LB_EPOCH_MASK;
     const uint64_t ep = (uint64_t)ep32 << 40;
@@ -363,15 +362,7 @@ __device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates
     const bool last_of_group = r == (int)LB_GROUP - 1;
 #pragma unroll
     for (int c = 0; c < NV; ++c) excl[c] = 0;
-    if (tile == 0) {
-        if (lane == 0)
-#pragma unroll
-            for (int c = 0; c < NV; ++c) lb_store(states + c * tstride, (2ull << 62) | ep | agg[c]);
-        return;
-    }
-    if (lane == 0)
-#pragma unroll
-        for (int c = 0; c < NV; ++c) lb_store(states + c * tstride + tile, (1ull << 62) | ep | agg[c]);
+    if (tile == 0) return;
     uint32_t spins = 0;
     bool active[NV], found[NV];
 #pragma unroll
@@ -424,6 +415,30 @@ __device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates
             if (last_of_group) lb_store(gstates + c * gstride + g, (2ull << 62) | ep | (excl[c] + agg[c]));
         }
     }
+}
+
+// Two-level decoupled look-back over NV prefix chains at once (the chains
+// share every poll). Called by all 64 lanes of one wave; publishes agg[c] for
+// `tile` and returns in excl[c] the exclusive prefix of all earlier tiles.
+// Tile words of chain c start at states + c * tstride, group words at
+// gstates + c * gstride.
+//
+// Tiles form groups of 64. A tile first reads its own group's earlier words;
+// when none of them holds an inclusive prefix it adds the group-level words
+// (one per earlier group: the group's aggregate, or its inclusive prefix),
+// 64 groups per poll. The last tile of a group publishes the group's word.
+// A prefix therefore resolves in a few polls however many tiles run at once,
+// where a flat look-back walks back through every running tile, 64 per poll.
+//
+// spin_limit bounds the polls of one wait (0 = unbounded). When it runs out,
+// *aborted is set and the partial sums read so far are returned: they never
+// exceed the true prefixes, so positions built from them stay in range.
+template <int NV>
+__device__ void lb_exclusive(uint64_t* states, size_t tstride, uint64_t* gstates, size_t gstride, uint32_t tile,
+                             uint32_t epoch, const uint64_t (&agg)[NV], uint32_t spin_limit, bool* aborted,
+                             uint64_t (&excl)[NV]) {
+    lb_publish<NV>(states, tstride, tile, epoch, agg);
+    lb_resolve<NV>(states, tstride, gstates, gstride, tile, epoch, agg, spin_limit, aborted, excl);
 }
 
 }  // namespace pp

@@ -43,7 +43,8 @@ struct Bufs {
     double* tcoef;       // B = 1: the permuted tail's values, source coefficient x tscale x sign (null for B > 1)
     uint64_t* tl_keys;   // W planes of cap words: keys of the terms listed in tail[0], in list order
     double* tl_coef;     // B = 1: their coefficients (null for B > 1)
-    uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile
+    uint64_t* states;    // look-back words: two chains of lb_tiles words, one word per tile, then the
+                         // one-column merge's round-window words (lb_tiles more)
     uint64_t* gstates;   // look-back words: two chains of lb_groups words, one per 64 tiles
     uint32_t* splits;    // merge-path split of every merge tile boundary
     uint64_t* ssamp;     // W words per sample: sum entries 0, 64, 128, ...
@@ -185,6 +186,26 @@ __device__ __forceinline__ void lb_prefix_n(Ctl* ctl, const Launch& L, const Buf
     bool aborted = false;
     lb_exclusive<NV>(bf.states, bf.lb_tiles, bf.gstates, bf.lb_groups, tile, L.epoch, agg,
                      L.dynamic ? 0u : L.spin_limit, &aborted, excl);
+    if (aborted && lane_id() == 0) {
+        ctl->overflow_seq = L.seq;
+        __threadfence();
+        atomicMax(&ctl->overflow, 2u);
+    }
+}
+
+// lb_prefix_n in two halves, for a kernel that loads its next tile between them
+template <int NV>
+__device__ __forceinline__ void lb_publish_n(const Launch& L, const Bufs& bf, uint32_t tile,
+                                             const uint64_t (&agg)[NV]) {
+    lb_publish<NV>(bf.states, bf.lb_tiles, tile, L.epoch, agg);
+}
+
+template <int NV>
+__device__ __forceinline__ void lb_resolve_n(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
+                                             const uint64_t (&agg)[NV], uint64_t (&excl)[NV]) {
+    bool aborted = false;
+    lb_resolve<NV>(bf.states, bf.lb_tiles, bf.gstates, bf.lb_groups, tile, L.epoch, agg,
+                   L.dynamic ? 0u : L.spin_limit, &aborted, excl);
     if (aborted && lane_id() == 0) {
         ctl->overflow_seq = L.seq;
         __threadfence();
@@ -1010,61 +1031,84 @@ __global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
     }
 }
 
-// Loads a tile into shared memory with its merge values: slot 0 = head entry
-// i0-1, slots 1..ns = head entries i0..i1-1, slots ns+1..ns+1+nt = tail entries
-// j0..j1. A head value is the coefficient times cos when the term anticommutes;
-// a tail value, the source coefficient times sin and negated for a negative
-// sign (src/Propagation/specializations.jl:40), was stored by the permutation
-// kernel. The merge then only adds head + tail.
-template <int W, int TILE>
-__device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, const double* __restrict__ coeffs,
-                                            size_t stride, const uint64_t* __restrict__ tkeys, uint32_t na,
-                                            uint32_t i0, uint32_t j0, int ns, int nt, const Key<W>& mask,
-                                            double cosv, uint64_t (*s_key)[TILE + 2], double* s_val,
-                                            const uint32_t* __restrict__ tpj, uint32_t* s_pj, int damp_q,
-                                            double fxy, const double* __restrict__ tcoef, int comm = 0) {
+// One merge tile's inputs held in registers between their loads and their
+// shared-memory staging: slot s = threadIdx.x + r * WG of the staging layout
+// (0 = head entry i0-1, 1..ns = head entries i0..i1-1, ns+1..ns+1+nt = tail
+// entries j0..j1).
+template <int W, int TILE, bool REC>
+struct MergeRegs {
+    static constexpr int R = (TILE + 2 + WG - 1) / WG;
+    uint64_t k[R][W];
+    double c[R];
+    uint32_t pj[REC ? R : 1];
+};
+
+// Issues the loads of one tile's head and tail entries (head keys and
+// coefficients of the sum, tail keys and stored values of the permuted list)
+// without waiting for them. Every lane loads from a valid address; slots past
+// the tile read a clamped entry that merge_put ignores.
+template <int W, int TILE, bool REC>
+__device__ __forceinline__ void merge_fetch(const Bufs& bf, const uint64_t* __restrict__ keys,
+                                            const double* __restrict__ coeffs, size_t stride, uint32_t na,
+                                            uint32_t i0, uint32_t j0, int ns, MergeRegs<W, TILE, REC>& f) {
     const int t = threadIdx.x;
-    constexpr int R = (TILE + 1 + WG - 1) / WG;
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
+    const uint32_t tmax = na > 0 ? na - 1 : 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int x = t + r * WG;
-        size_t gi = (size_t)i0 + x;
+    for (int r = 0; r < MergeRegs<W, TILE, REC>::R; ++r) {
+        const int s = t + r * WG;
+        const bool tl = s > ns && na > 0;
+        size_t gi = (size_t)i0 + s;
         gi = gi > 0 ? gi - 1 : 0;
         gi = gi < hmax ? gi : hmax;
-        Key<W> k;
+        const uint32_t gj = tl ? min(j0 + (uint32_t)(s - ns - 1), tmax) : 0u;
+        const uint64_t* __restrict__ kp = tl ? bf.tkeys : keys;
+        const size_t idx = tl ? (size_t)gj : gi;
 #pragma unroll
-        for (int l = 0; l < W; ++l) k.l[l] = keys[l * stride + gi];
-        const double c = coeffs[gi];
-        if (x < ns + 1 && (x > 0 || i0 > 0)) {
-#pragma unroll
-            for (int l = 0; l < W; ++l) s_key[l][x] = k.l[l];
-            double v;
-            if (damp_q >= 0) {
-                const uint32_t code = code_at<W>(k, damp_q);
-                v = code == 3u ? c * cosv : (code ? c * fxy : c);
-            } else {
-                v = (anticommutes<W>(mask, k) != (comm != 0)) ? c * cosv : c;
-            }
-            s_val[x] = v;
-        }
+        for (int l = 0; l < W; ++l) f.k[r][l] = kp[l * stride + idx];
+        f.c[r] = tl ? bf.tcoef[gj] : coeffs[gi];
+        if (REC) f.pj[r] = tl ? bf.tpj[gj] : 0u;
     }
-    if (na == 0) return;  // amplitude damping with no Z on the qubit: nothing created
-    const uint32_t tmax = na - 1;
+}
+
+// Stages fetched entries in shared memory with their merge values: a head
+// value is the coefficient times cos when the term anticommutes (amplitude
+// damping: its own factors); a tail value, the source coefficient times sin
+// and negated for a negative sign (src/Propagation/specializations.jl:40), was
+// stored by the permutation kernel. The merge then only adds head + tail.
+template <int W, int TILE, bool REC>
+__device__ __forceinline__ void merge_put(const MergeRegs<W, TILE, REC>& f, uint32_t na, uint32_t i0, uint32_t j0,
+                                          int ns, int nt, const Key<W>& mask, double cosv, int damp_q, double fxy,
+                                          int comm, uint64_t (*s_key)[TILE + 2], double* s_val, uint32_t* s_pj) {
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int y = t + r * WG;
-        const uint32_t gj = min(j0 + (uint32_t)y, tmax);
+    for (int r = 0; r < MergeRegs<W, TILE, REC>::R; ++r) {
+        const int s = t + r * WG;
         Key<W> k;
 #pragma unroll
-        for (int l = 0; l < W; ++l) k.l[l] = tkeys[l * stride + gj];
-        const uint32_t pj = tpj ? tpj[gj] : 0u;
-        const double c = tcoef[gj];
-        if (y < nt + 1 && j0 + (uint32_t)y < na) {
+        for (int l = 0; l < W; ++l) k.l[l] = f.k[r][l];
+        const double c = f.c[r];
+        if (s <= ns) {
+            if (s > 0 || i0 > 0) {
 #pragma unroll
-            for (int l = 0; l < W; ++l) s_key[l][ns + 1 + y] = k.l[l];
-            s_val[ns + 1 + y] = c;
-            if (tpj) s_pj[y] = pj;
+                for (int l = 0; l < W; ++l) s_key[l][s] = k.l[l];
+                double v;
+                if (damp_q >= 0) {
+                    const uint32_t code = code_at<W>(k, damp_q);
+                    v = code == 3u ? c * cosv : (code ? c * fxy : c);
+                } else {
+                    v = (anticommutes<W>(mask, k) != (comm != 0)) ? c * cosv : c;
+                }
+                s_val[s] = v;
+            }
+        } else {
+            const int y = s - ns - 1;
+            if (y <= nt && j0 + (uint32_t)y < na) {
+#pragma unroll
+                for (int l = 0; l < W; ++l) s_key[l][s] = k.l[l];
+                s_val[s] = c;
+                if (REC) s_pj[y] = f.pj[r];
+            }
         }
     }
 }
@@ -1072,16 +1116,27 @@ __device__ __forceinline__ void merge_stage(const uint64_t* __restrict__ keys, c
 // With EMIT, the kept output terms are also tested against `nmask`, the mask
 // of the next gate (a rotation), and the touched ones are listed in tail[0] in
 // output order, as k_branch of that gate would list them: the next k_branch
-// then only checks room. The list's offsets come from a second look-back
-// chain that shares every poll with the output offsets.
+// then only checks room. The list's offsets come from a second prefix chain
+// that shares every poll with the output offsets.
 //
 // With REC, every head term and every touched term's created term records its
 // output position (REC_NONE when truncated, merged away or absent).
+//
+// Tiles are software-pipelined: a workgroup orders a tile's kept outputs in
+// shared memory, publishes its counts, issues the loads of its next tile, and
+// only then waits for the tile's prefix and stores it, so the wait for the
+// tiles before it overlaps the next tile's loads. With static tiles (tile =
+// blockIdx.x + k * G) the workgroup keeps its inclusive prefix after each
+// tile: the exclusive prefix of its next tile is that plus the counts of the
+// G - 1 tiles in between, which every thread of the workgroup reads at once
+// (up to 8 words each) -- one round trip, where a chained look-back takes one
+// per level. Dynamic tiles (the fallback after a wait runs out) use the
+// two-level look-back.
 template <int W, bool EMIT, bool REC, int ITEMS = K3Cfg<W>::ITEMS>
 __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
                                                Key<W> nmask) {
     constexpr int TILE = WG * ITEMS;
-    static_assert(TILE < 65536, "kept and touched counts of a tile are packed in 16 bits each");
+    static_assert(TILE < 32768, "kept positions carry a sign in bit 15; counts are packed in 16 bits");
     // the first static tile's splits go out with the control block (tile =
     // blockIdx.x; the index is clamped into the array, and used only if the
     // tile is that one)
@@ -1105,34 +1160,83 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ double s_val[TILE + 2];
     __shared__ uint32_t s_pj[REC ? TILE + 2 : 1];
     __shared__ uint16_t s_slot[TILE];
-    __shared__ uint32_t s_tl[EMIT ? TILE : 1];
-    __shared__ uint16_t s_tlp[EMIT ? TILE : 1];
+    __shared__ uint16_t s_tlp[EMIT ? TILE : 1];  // kept position of each touched output, bit 15: negative sign
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
+    __shared__ uint64_t s_wacc[NWAVES];
     __shared__ uint64_t s_excl, s_excl_t;
     uint32_t* __restrict__ rhead = REC ? bf.rec + c.s.roff : nullptr;
     uint32_t* __restrict__ rcreated = REC ? bf.rec + c.s.roff + n + na : nullptr;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     uint32_t* __restrict__ next_tail = bf.tail[0];
-    TileSource ts;
-    while (true) {
-        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
-        if (tile >= ntiles) break;
-        const uint32_t d0 = tile * (uint32_t)TILE;
+    // a tile's bounds from its splits; an inconsistent partition (a bug) is
+    // reported, and the tile then writes nothing but still publishes
+    auto bounds = [&](uint32_t tl, uint32_t a0, uint32_t a1, uint32_t& j0, int& ns, int& nt) {
+        const uint32_t d0 = tl * (uint32_t)TILE;
         const uint32_t d1 = min(d0 + (uint32_t)TILE, ntot);
-        const bool first = tile == pt && !L.dynamic;
-        const uint32_t i0 = first ? pre0 : bf.splits[tile], i1 = first ? pre1 : bf.splits[tile + 1];
-        const uint32_t j0 = d0 - i0, j1 = d1 - i1;
-        int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
-        if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
-            // inconsistent partition (a bug): report it, write nothing, still publish
+        j0 = d0 - a0;
+        const uint32_t j1 = d1 - a1;
+        ns = (int)(a1 - a0);
+        nt = (int)(j1 - j0);
+        if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || a1 > n || j1 > na) {
             if (t == 0) atomicOr(&ctl->err, 2u);
             ns = 0;
             nt = 0;
         }
-        merge_stage<W, TILE>(keys, coeffs, stride, bf.tkeys, na, i0, j0, ns, nt, mask, rp.cosv, s_key, s_val,
-                             REC ? bf.tpj : nullptr, s_pj, rp.damp_q, rp.fxy, bf.tcoef, rp.comm);
+    };
+    uint32_t tile;
+    if (L.dynamic) {
+        if (t == 0) s_tile = __hip_atomic_fetch_add(&ctl->counter[L.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
+        tile = s_tile;
+    } else {
+        tile = blockIdx.x;
+    }
+    if (tile >= ntiles) return;
+    uint32_t i0, j0;
+    int ns, nt;
+    {
+        const bool first = tile == pt && !L.dynamic;
+        i0 = first ? pre0 : bf.splits[tile];
+        const uint32_t i1 = first ? pre1 : bf.splits[tile + 1];
+        bounds(tile, i0, i1, j0, ns, nt);
+    }
+    MergeRegs<W, TILE, REC> F;
+    merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
+    // static tiles: splits loaded two tiles ahead (s1: tile + G), and the
+    // workgroup's inclusive prefix, packed as (touched << 32) | kept
+    const uint32_t G = gridDim.x;
+    uint64_t* __restrict__ win = bf.states + 2 * (size_t)bf.lb_tiles;
+    const uint32_t ep32 = L.epoch & LB_EPOCH_MASK;
+    uint64_t incl = 0;
+    uint32_t s1a = 0, s1b = 0;
+    if (!L.dynamic) {
+        const bool ok1 = tile + G < ntiles;
+        s1a = bf.splits[ok1 ? tile + G : 0u];
+        s1b = bf.splits[ok1 ? tile + G + 1 : 0u];
+    }
+    while (true) {
+        __syncthreads();  // the previous tile's shared memory is free again
+        uint32_t s2a = 0, s2b = 0;
+        if (L.dynamic) {
+            if (t == 0)
+                s_tile = __hip_atomic_fetch_add(&ctl->counter[L.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const bool ok2 = tile + 2 * G < ntiles;
+            s2a = bf.splits[ok2 ? tile + 2 * G : 0u];
+            s2b = bf.splits[ok2 ? tile + 2 * G + 1 : 0u];
+        }
+        merge_put<W, TILE, REC>(F, na, i0, j0, ns, nt, mask, rp.cosv, rp.damp_q, rp.fxy, rp.comm, s_key, s_val,
+                                s_pj);
+        __syncthreads();
+        // the next tile and its splits (dynamic tiles: loaded while this tile merges)
+        const uint32_t nxt = L.dynamic ? s_tile : tile + G;
+        const bool more = nxt < ntiles;
+        uint32_t ni0 = s1a, ni1 = s1b;
+        if (L.dynamic) {
+            ni0 = bf.splits[more ? nxt : 0u];
+            ni1 = bf.splits[more ? nxt + 1 : 0u];
+        }
         auto S = [&](int x) {
             Key<W> k;
 #pragma unroll
@@ -1172,11 +1276,11 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
                 bool live;
                 Key<W> ko;
                 if (take_s) {
-                    double c = s_val[1 + x];
+                    double cv = s_val[1 + x];
                     const bool merged = j0 + y < na && key_eq<W>(kt, ks);
-                    if (merged) c = c + s_val[ns + 1 + y];
+                    if (merged) cv = cv + s_val[ns + 1 + y];
                     slot[k] = 1 + x;
-                    oc[k] = c;
+                    oc[k] = cv;
                     ko = ks;
                     live = true;
                     if (REC) who[k] = (uint32_t)(x + 1) | (merged ? (uint32_t)(y + 1) << 16 : 0u);
@@ -1204,74 +1308,158 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         const uint32_t mine = (uint32_t)__popc(keep) | ((uint32_t)__popc(touch) << 16);
         static_assert(ITEMS < 8, "per-thread counts fit 3 bits");
         const uint32_t pre = block_excl_sum_small<3>(mine, s_w, agg);
-        if (wave == 0) {
-            if (EMIT) {
-                const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
-                uint64_t e2[2];
-                lb_prefix_n<2>(ctl, L, bf, tile, a2, e2);
-                if (lane == 0) {
-                    s_excl = e2[0];
-                    s_excl_t = e2[1];
-                    if (tile == ntiles - 1) {
-                        const uint32_t nout = (uint32_t)(e2[0] + a2[0]);
-                        set_state(ctl, p ^ 1, nout, cur ^ 1);
-                        ctl->st[p ^ 1].na = (uint32_t)(e2[1] + a2[1]);
-#pragma unroll
-                        for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
-                        ctl->tail_seq = L.seq + 1;
-                        ctl->tail_ready = 1u;
-                        ctl->ssamp_for = L.seq + 2u;
-                        if (L.glog) L.glog[2] = nout;
-                    }
-                }
-            } else {
-                const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg & 0xffffu);
-                if (lane == 0) {
-                    s_excl = excl;
-                    if (tile == ntiles - 1) {
-                        const uint32_t nout = (uint32_t)(excl + (agg & 0xffffu));
-                        set_state(ctl, p ^ 1, nout, cur ^ 1);
-                        ctl->ssamp_for = L.seq + 2u;
-                        if (L.glog) L.glog[2] = nout;
-                    }
-                }
-            }
-        }
-        __syncthreads();
         // The kept outputs go to LDS in output order (their slots, and their
-        // values over s_val, whose staged inputs every thread has read by now),
-        // then out in consecutive addresses: a thread's own outputs are ITEMS
-        // apart in the wave, which would split every store over 3x the lines.
-        const uint32_t obase = (uint32_t)s_excl, tbase = EMIT ? (uint32_t)s_excl_t : 0u;
-        uint32_t lp = pre & 0xffffu, lt = pre >> 16;
+        // values over s_val, whose staged inputs every thread has read by the
+        // block sum's barriers), to be stored in consecutive addresses once
+        // the tile's offset is known: a thread's own outputs are ITEMS apart
+        // in the wave, which would split every store over 3x the lines.
+        {
+            uint32_t lp = pre & 0xffffu, lt = pre >> 16;
 #pragma unroll
-        for (int k = 0; k < ITEMS; ++k) {
-            const uint32_t pos = obase + lp;
-            if (REC && who[k]) {
-                const uint32_t at = ((keep >> k) & 1u) ? pos : REC_NONE;
-                const uint32_t hx = who[k] & 0xffffu, ty = who[k] >> 16;
-                if (hx) {
-                    Key<W> hk;
-#pragma unroll
-                    for (int l = 0; l < W; ++l) hk.l[l] = s_key[l][hx];
-                    rhead[i0 + hx - 1] = (anticommutes<W>(mask, hk) ? REC_FLAG : 0u) | at;
+            for (int k = 0; k < ITEMS; ++k) {
+                if ((keep >> k) & 1u) {
+                    s_slot[lp] = (uint16_t)slot[k];
+                    s_val[lp] = oc[k];
+                    if (EMIT && ((touch >> k) & 1u)) {
+                        s_tlp[lt] = (uint16_t)(lp | (((neg >> k) & 1u) ? 0x8000u : 0u));
+                        ++lt;
+                    }
+                    ++lp;
                 }
-                if (ty) rcreated[s_pj[ty - 1]] = at;
             }
-            if ((keep >> k) & 1u) {
-                s_slot[lp] = (uint16_t)slot[k];
-                s_val[lp] = oc[k];
-                if (EMIT && ((touch >> k) & 1u)) {
-                    s_tl[lt] = pos | (((neg >> k) & 1u) ? SRC_NEG : 0u);
-                    s_tlp[lt] = (uint16_t)lp;
-                    ++lt;
+        }
+        const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
+        if (L.dynamic) {
+            if (wave == 0) {
+                if (EMIT) {
+                    lb_publish_n<2>(L, bf, tile, a2);
+                } else {
+                    const uint64_t a1[1] = {a2[0]};
+                    lb_publish_n<1>(L, bf, tile, a1);
                 }
-                ++lp;
+            }
+        } else if (t == 0) {
+            lb_store(win + tile, (1ull << 63) | ((uint64_t)ep32 << 40) | ((uint64_t)(agg >> 16) << 16) | (agg & 0xffffu));
+        }
+        // The next tile's loads go out before this tile's wait (always, from
+        // clamped addresses after the last tile: a conditional fetch would
+        // keep the consumed registers live through the merge).
+        const uint32_t ci0 = i0;
+        if (more) {
+            i0 = ni0;
+            bounds(nxt, ni0, ni1, j0, ns, nt);
+        } else {
+            i0 = 0;
+            j0 = 0;
+            ns = 0;
+            nt = 0;
+        }
+        merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
+        uint64_t ek = 0, et = 0;  // exclusive prefixes: kept outputs, touched outputs
+        if (L.dynamic) {
+            if (wave == 0) {
+                if (EMIT) {
+                    uint64_t e2[2];
+                    lb_resolve_n<2>(ctl, L, bf, tile, a2, e2);
+                    ek = e2[0];
+                    et = e2[1];
+                } else {
+                    const uint64_t a1[1] = {a2[0]};
+                    uint64_t e1[1];
+                    lb_resolve_n<1>(ctl, L, bf, tile, a1, e1);
+                    ek = e1[0];
+                }
+            }
+        } else {
+            // the window [tile - G + 1, tile): four words a thread per poll
+            // (G <= 1025 in one poll, larger grids in 256-word batches after it)
+            const uint32_t base = tile >= G ? tile - G + 1 : 0u;
+            const uint32_t cnt = tile - base;
+            const uint64_t* __restrict__ wb = win + base;
+            uint64_t acc = 0;
+            uint32_t spins = 0;
+            bool stop = false;
+            for (uint32_t b0 = 0; b0 < cnt && !stop; b0 += 4 * WG) {
+                uint32_t pend = 0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pend |= (b0 + t + (uint32_t)u * WG < cnt) ? (1u << u) : 0u;
+                while (true) {
+                    uint64_t w[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) w[u] = ((pend >> u) & 1u) ? lb_load(wb + (b0 + t + (uint32_t)u * WG)) : 0ull;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        if (((pend >> u) & 1u) && (w[u] >> 63) && ((uint32_t)(w[u] >> 40) & LB_EPOCH_MASK) == ep32) {
+                            acc += (((w[u] >> 16) & 0xffffull) << 32) | (w[u] & 0xffffull);
+                            pend &= ~(1u << u);
+                        }
+                    }
+                    if (!__syncthreads_or(pend != 0u)) break;
+                    if (L.spin_limit && ++spins > L.spin_limit) {
+                        // stop; the host runs the gate again with tiles claimed in start order
+                        if (t == 0) {
+                            ctl->overflow_seq = L.seq;
+                            __threadfence();
+                            atomicMax(&ctl->overflow, 2u);
+                        }
+                        stop = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            acc = wave_sum_u64_dpp(acc);
+            if (lane == 0) s_wacc[wave] = acc;
+            __syncthreads();
+            uint64_t tot = 0;
+#pragma unroll
+            for (int v = 0; v < NWAVES; ++v) tot += s_wacc[v];
+            const uint64_t ex = incl + tot;
+            incl = ex + (((uint64_t)(agg >> 16)) << 32) + (agg & 0xffffu);
+            ek = ex & 0xffffffffull;
+            et = ex >> 32;
+        }
+        if (t == 0) {
+            s_excl = ek;
+            s_excl_t = et;
+            if (tile == ntiles - 1) {
+                const uint32_t nout = (uint32_t)(ek + a2[0]);
+                set_state(ctl, p ^ 1, nout, cur ^ 1);
+                if (EMIT) {
+                    ctl->st[p ^ 1].na = (uint32_t)(et + a2[1]);
+#pragma unroll
+                    for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                    ctl->tail_seq = L.seq + 1;
+                    ctl->tail_ready = 1u;
+                }
+                ctl->ssamp_for = L.seq + 2u;
+                if (L.glog) L.glog[2] = nout;
             }
         }
         __syncthreads();
+        const uint32_t obase = (uint32_t)s_excl, tbase = EMIT ? (uint32_t)s_excl_t : 0u;
+        if (REC) {
+            uint32_t lp = pre & 0xffffu;
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                const bool kept = (keep >> k) & 1u;
+                if (who[k]) {
+                    const uint32_t at = kept ? obase + lp : REC_NONE;
+                    const uint32_t hx = who[k] & 0xffffu, ty = who[k] >> 16;
+                    if (hx) {
+                        Key<W> hk;
+#pragma unroll
+                        for (int l = 0; l < W; ++l) hk.l[l] = s_key[l][hx];
+                        rhead[ci0 + hx - 1] = (anticommutes<W>(mask, hk) ? REC_FLAG : 0u) | at;
+                    }
+                    if (ty) rcreated[s_pj[ty - 1]] = at;
+                }
+                lp += kept ? 1u : 0u;
+            }
+        }
         // every 64th output is also the next gate's partition sample
         const uint32_t nkept = agg & 0xffffu;
+#pragma unroll 1
         for (uint32_t j = t; j < nkept; j += WG) {
             const int sl = s_slot[j];
             const uint32_t o = obase + j;
@@ -1286,15 +1474,21 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
         if (EMIT) {
             // the next rotation's list: positions, and the terms themselves
             const uint32_t ntouch = agg >> 16;
+#pragma unroll 1
             for (uint32_t j = t; j < ntouch; j += WG) {
-                const int lpj = s_tlp[j];
+                const uint32_t e = s_tlp[j];
+                const uint32_t lpj = e & 0x7fffu;
                 const int sl = s_slot[lpj];
-                next_tail[tbase + j] = s_tl[j];
+                next_tail[tbase + j] = (obase + lpj) | ((e & 0x8000u) ? SRC_NEG : 0u);
 #pragma unroll
                 for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + tbase + j] = s_key[l][sl];
                 bf.tl_coef[tbase + j] = s_val[lpj];
             }
         }
+        if (!more) break;
+        tile = nxt;
+        s1a = s2a;
+        s1b = s2b;
     }
 }
 

@@ -8,7 +8,8 @@
 //              tcoef (B = 1) their values, tail[1] (B > 1) their indices
 //   tl_keys    W limb planes of cap words: the terms listed in tail[0], in list
 //              order (tl_coef: their coefficients, B = 1)
-//   states     one look-back word per tile of the largest compaction launch
+//   states     one look-back word per tile of the largest compaction launch, two
+//              chains, then one round-window word per tile (one-column merge)
 //   gstates    one look-back word per group of 64 such tiles
 //   splits     merge-path split of every merge tile boundary
 //   ssamp      every 64th sum entry (W words each), written by the merge that
@@ -312,7 +313,7 @@ int grow(pp_handle* h, size_t newcap) {
         ok = ok && hipMalloc(&nc[b], newcap * B * sizeof(double)) == hipSuccess;
         ok = ok && hipMalloc(&nt[b], newcap * sizeof(uint32_t)) == hipSuccess;
     }
-    ok = ok && hipMalloc(&ns, 2 * nlb * sizeof(uint64_t)) == hipSuccess;
+    ok = ok && hipMalloc(&ns, 3 * nlb * sizeof(uint64_t)) == hipSuccess;
     ok = ok && hipMalloc(&ntk, newcap * W * sizeof(uint64_t)) == hipSuccess;
     if (B == 1) ok = ok && hipMalloc(&ntc, newcap * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&nlk, newcap * W * sizeof(uint64_t)) == hipSuccess;
@@ -339,7 +340,7 @@ int grow(pp_handle* h, size_t newcap) {
         (void)hipGetLastError();
         return fail(PP_ERR_NOMEM, "device allocation of " + std::to_string(newcap) + " terms failed");
     }
-    HIP_TRY(hipMemsetAsync(ns, 0, 2 * nlb * sizeof(uint64_t), h->stream));
+    HIP_TRY(hipMemsetAsync(ns, 0, 3 * nlb * sizeof(uint64_t), h->stream));
     HIP_TRY(hipMemsetAsync(ngs, 0, 2 * ngw * sizeof(uint64_t), h->stream));
     if (n > 0) {
         for (int l = 0; l < W; ++l)
@@ -387,7 +388,7 @@ uint32_t next_epoch(pp_handle* h) {
     h->epoch = (h->epoch + 1) & LB_EPOCH_MASK;
     if (h->epoch == 0) {
         // wrapped: words of old launches could match again, so clear them
-        hipMemsetAsync(h->bufs.states, 0, 2 * (size_t)h->bufs.lb_tiles * sizeof(uint64_t), h->stream);
+        hipMemsetAsync(h->bufs.states, 0, 3 * (size_t)h->bufs.lb_tiles * sizeof(uint64_t), h->stream);
         hipMemsetAsync(h->bufs.gstates, 0, 2 * (size_t)h->bufs.lb_groups * sizeof(uint64_t), h->stream);
         h->epoch = 1;
     }
