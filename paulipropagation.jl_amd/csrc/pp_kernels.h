@@ -221,6 +221,108 @@ __device__ __forceinline__ uint64_t lb_prefix(Ctl* ctl, const Launch& L, const B
     return e[0];
 }
 
+// Prefix of a tile's counts for look-back kernels, called by every thread of
+// the workgroup; counts are packed (second chain << 32 | first chain), each
+// tile's below 1 << 16.
+//
+// Static tiles (tile = blockIdx.x + k * G, G = gridDim.x): the workgroup keeps
+// its inclusive prefix after its previous tile in `incl`; the exclusive prefix
+// of `tile` is that plus the counts of the G - 1 tiles between, which run at
+// the same time: every thread reads up to four of their words per poll (one
+// round trip for G <= 1025), where a chained look-back needs one per level.
+// A word carries the launch epoch, so words of older launches read as not
+// ready. Dynamic tiles use the two-level look-back (NV chains).
+__device__ __forceinline__ uint64_t* win_words(const Bufs& bf) { return bf.states + 2 * (size_t)bf.lb_tiles; }
+
+template <int NV>
+__device__ __forceinline__ void tile_publish(const Launch& L, const Bufs& bf, uint32_t tile, uint64_t agg) {
+    const int t = threadIdx.x;
+    if (L.dynamic) {
+        if (t < 64) {
+            uint64_t a[NV];
+            a[0] = agg & 0xffffffffull;
+            if (NV > 1) a[NV - 1] = agg >> 32;
+            lb_publish_n<NV>(L, bf, tile, a);
+        }
+    } else if (t == 0) {
+        const uint64_t ep = (uint64_t)(L.epoch & LB_EPOCH_MASK) << 40;
+        lb_store(win_words(bf) + tile, (1ull << 63) | ep | ((agg >> 32) << 16) | (agg & 0xffffull));
+    }
+}
+
+template <int NV>
+__device__ uint64_t tile_resolve(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile, uint64_t agg,
+                                 uint64_t& incl, uint64_t* s_wacc) {
+    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    if (L.dynamic) {
+        if (wave == 0) {
+            uint64_t a[NV], e[NV];
+            a[0] = agg & 0xffffffffull;
+            if (NV > 1) a[NV - 1] = agg >> 32;
+            lb_resolve_n<NV>(ctl, L, bf, tile, a, e);
+            if (lane == 0) s_wacc[0] = NV > 1 ? (e[0] | (e[NV - 1] << 32)) : e[0];
+        }
+        __syncthreads();
+        const uint64_t ex = s_wacc[0];
+        __syncthreads();
+        return ex;
+    }
+    const uint32_t ep32 = L.epoch & LB_EPOCH_MASK;
+    const uint32_t G = gridDim.x;
+    const uint32_t base = tile >= G ? tile - G + 1 : 0u;
+    const uint32_t cnt = tile - base;
+    const uint64_t* __restrict__ wb = win_words(bf) + base;
+    uint64_t acc = 0;
+    uint32_t spins = 0;
+    bool stop = false;
+    for (uint32_t b0 = 0; b0 < cnt && !stop; b0 += 4 * WG) {
+        uint32_t pend = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pend |= (b0 + t + (uint32_t)u * WG < cnt) ? (1u << u) : 0u;
+        while (true) {
+            uint64_t w[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) w[u] = ((pend >> u) & 1u) ? lb_load(wb + (b0 + t + (uint32_t)u * WG)) : 0ull;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (((pend >> u) & 1u) && (w[u] >> 63) && ((uint32_t)(w[u] >> 40) & LB_EPOCH_MASK) == ep32) {
+                    acc += (((w[u] >> 16) & 0xffffull) << 32) | (w[u] & 0xffffull);
+                    pend &= ~(1u << u);
+                }
+            }
+            if (!__syncthreads_or(pend != 0u)) break;
+            if (L.spin_limit && ++spins > L.spin_limit) {
+                // stop; the host runs the gate again with tiles claimed in start order
+                if (t == 0) {
+                    ctl->overflow_seq = L.seq;
+                    __threadfence();
+                    atomicMax(&ctl->overflow, 2u);
+                }
+                stop = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    acc = wave_sum_u64_dpp(acc);
+    if (lane == 0) s_wacc[wave] = acc;
+    __syncthreads();
+    uint64_t tot = 0;
+#pragma unroll
+    for (int v = 0; v < NWAVES; ++v) tot += s_wacc[v];
+    __syncthreads();  // s_wacc is free again
+    const uint64_t ex = incl + tot;
+    incl = ex + agg;
+    return ex;
+}
+
+template <int NV>
+__device__ __forceinline__ uint64_t tile_prefix(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile,
+                                                uint64_t agg, uint64_t& incl, uint64_t* s_wacc) {
+    tile_publish<NV>(L, bf, tile, agg);
+    return tile_resolve<NV>(ctl, L, bf, tile, agg, incl, s_wacc);
+}
+
 // a[i] for a small register array and a runtime index, without spilling it
 template <int N>
 __device__ __forceinline__ double pick(const double (&a)[N], int i) {
@@ -376,6 +478,8 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
     __shared__ uint32_t s_cnt[K1_ITEMS][NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -436,9 +540,9 @@ __device__ __forceinline__ void branch_scan(const Bufs& bf, Ctl* ctl, const Laun
             before[k] = agg + row_before;
             agg += row;
         }
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) fin((uint32_t)(excl + agg));
             }
@@ -1167,7 +1271,7 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint64_t s_excl, s_excl_t;
     uint32_t* __restrict__ rhead = REC ? bf.rec + c.s.roff : nullptr;
     uint32_t* __restrict__ rcreated = REC ? bf.rec + c.s.roff + n + na : nullptr;
-    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const int t = threadIdx.x;
     uint32_t* __restrict__ next_tail = bf.tail[0];
     // a tile's bounds from its splits; an inconsistent partition (a bug) is
     // reported, and the tile then writes nothing but still publishes
@@ -1206,8 +1310,6 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
     // static tiles: splits loaded two tiles ahead (s1: tile + G), and the
     // workgroup's inclusive prefix, packed as (touched << 32) | kept
     const uint32_t G = gridDim.x;
-    uint64_t* __restrict__ win = bf.states + 2 * (size_t)bf.lb_tiles;
-    const uint32_t ep32 = L.epoch & LB_EPOCH_MASK;
     uint64_t incl = 0;
     uint32_t s1a = 0, s1b = 0;
     if (!L.dynamic) {
@@ -1329,18 +1431,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             }
         }
         const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
-        if (L.dynamic) {
-            if (wave == 0) {
-                if (EMIT) {
-                    lb_publish_n<2>(L, bf, tile, a2);
-                } else {
-                    const uint64_t a1[1] = {a2[0]};
-                    lb_publish_n<1>(L, bf, tile, a1);
-                }
-            }
-        } else if (t == 0) {
-            lb_store(win + tile, (1ull << 63) | ((uint64_t)ep32 << 40) | ((uint64_t)(agg >> 16) << 16) | (agg & 0xffffu));
-        }
+        const uint64_t aggp = a2[0] | (a2[1] << 32);
+        tile_publish<EMIT ? 2 : 1>(L, bf, tile, aggp);
         // The next tile's loads go out before this tile's wait (always, from
         // clamped addresses after the last tile: a conditional fetch would
         // keep the consumed registers live through the merge).
@@ -1355,70 +1447,8 @@ __global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<
             nt = 0;
         }
         merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
-        uint64_t ek = 0, et = 0;  // exclusive prefixes: kept outputs, touched outputs
-        if (L.dynamic) {
-            if (wave == 0) {
-                if (EMIT) {
-                    uint64_t e2[2];
-                    lb_resolve_n<2>(ctl, L, bf, tile, a2, e2);
-                    ek = e2[0];
-                    et = e2[1];
-                } else {
-                    const uint64_t a1[1] = {a2[0]};
-                    uint64_t e1[1];
-                    lb_resolve_n<1>(ctl, L, bf, tile, a1, e1);
-                    ek = e1[0];
-                }
-            }
-        } else {
-            // the window [tile - G + 1, tile): four words a thread per poll
-            // (G <= 1025 in one poll, larger grids in 256-word batches after it)
-            const uint32_t base = tile >= G ? tile - G + 1 : 0u;
-            const uint32_t cnt = tile - base;
-            const uint64_t* __restrict__ wb = win + base;
-            uint64_t acc = 0;
-            uint32_t spins = 0;
-            bool stop = false;
-            for (uint32_t b0 = 0; b0 < cnt && !stop; b0 += 4 * WG) {
-                uint32_t pend = 0;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) pend |= (b0 + t + (uint32_t)u * WG < cnt) ? (1u << u) : 0u;
-                while (true) {
-                    uint64_t w[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) w[u] = ((pend >> u) & 1u) ? lb_load(wb + (b0 + t + (uint32_t)u * WG)) : 0ull;
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        if (((pend >> u) & 1u) && (w[u] >> 63) && ((uint32_t)(w[u] >> 40) & LB_EPOCH_MASK) == ep32) {
-                            acc += (((w[u] >> 16) & 0xffffull) << 32) | (w[u] & 0xffffull);
-                            pend &= ~(1u << u);
-                        }
-                    }
-                    if (!__syncthreads_or(pend != 0u)) break;
-                    if (L.spin_limit && ++spins > L.spin_limit) {
-                        // stop; the host runs the gate again with tiles claimed in start order
-                        if (t == 0) {
-                            ctl->overflow_seq = L.seq;
-                            __threadfence();
-                            atomicMax(&ctl->overflow, 2u);
-                        }
-                        stop = true;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-            acc = wave_sum_u64_dpp(acc);
-            if (lane == 0) s_wacc[wave] = acc;
-            __syncthreads();
-            uint64_t tot = 0;
-#pragma unroll
-            for (int v = 0; v < NWAVES; ++v) tot += s_wacc[v];
-            const uint64_t ex = incl + tot;
-            incl = ex + (((uint64_t)(agg >> 16)) << 32) + (agg & 0xffffu);
-            ek = ex & 0xffffffffull;
-            et = ex >> 32;
-        }
+        const uint64_t ex = tile_resolve<EMIT ? 2 : 1>(ctl, L, bf, tile, aggp, incl, s_wacc);
+        const uint64_t ek = ex & 0xffffffffull, et = ex >> 32;
         if (t == 0) {
             s_excl = ek;
             s_excl_t = et;
@@ -1534,6 +1564,8 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint64_t s_excl;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -1712,9 +1744,9 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         const uint32_t mine = (t < TB) ? s_keep[t] : 0u;
         uint32_t agg;
         const uint32_t pre = block_excl_sum_small<1>(mine, s_w, agg);
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(excl + agg);
@@ -1787,6 +1819,8 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
     __shared__ uint32_t s_cnt[ITEMS][NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -1827,9 +1861,9 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
             before[k] = agg + rb;
             agg += row;
         }
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(excl + agg);
@@ -1898,6 +1932,8 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -1961,9 +1997,9 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         uint32_t agg;
         const uint32_t pre = block_excl_sum_small<1>(mine, s_w, agg);
         if (t < KNB_ROWS) s_pos[t] = pre;
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) {
                     const uint32_t nout = (uint32_t)(excl + agg);
@@ -2367,6 +2403,8 @@ __global__ __launch_bounds__(WG) void k_intersect(Bufs bf, Ctl* ctl, Launch L, B
     __shared__ uint32_t s_cnt[KI_ITEMS][NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -2404,9 +2442,9 @@ __global__ __launch_bounds__(WG) void k_intersect(Bufs bf, Ctl* ctl, Launch L, B
             before[k] = agg + rb;
             agg += row;
         }
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) set_state(ctl, p ^ 1, (uint32_t)(excl + agg), cur ^ 1);
             }

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;
-    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const int t = threadIdx.x;
     if (n == 0) {
         if (blockIdx.x == 0 && t == 0) {
             set_state(ctl, p ^ 1, 0u, cur ^ 1);
@@ -195,6 +195,8 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl, s_excl_s;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -229,11 +231,11 @@ __global__ __launch_bounds__(WG) void k_clifford_map(Bufs bf, Ctl* ctl, Launch L
         uint32_t agg;
         static_assert(KC_ITEMS < 8, "per-thread counts fit 3 bits");
         const uint32_t pre = block_excl_sum_small<3>(mine, s_w, agg);
-        if (wave == 0) {
+        {
             const uint64_t a2[2] = {agg & 0xffffu, agg >> 16};
-            uint64_t e2[2];
-            lb_prefix_n<2>(ctl, L, bf, tile, a2, e2);
-            if (lane == 0) {
+            const uint64_t ex2 = tile_prefix<2>(ctl, L, bf, tile, a2[0] | (a2[1] << 32), lb_incl, s_wacc);
+            const uint64_t e2[2] = {ex2 & 0xffffffffull, ex2 >> 32};
+            if (t == 0) {
                 s_excl = e2[0];
                 s_excl_s = e2[1];
                 if (tile == ntiles - 1) {
@@ -346,7 +348,7 @@ __global__ __launch_bounds__(WG) void k_reduce_runs(Bufs bf, Ctl* ctl, Launch L,
     uint64_t* __restrict__ out_keys = bf.keys[1];
     double* __restrict__ out_coeffs = bf.coeffs[1];
     const size_t stride = bf.cap;
-    const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    const int t = threadIdx.x;
     if (n == 0) {
         if (blockIdx.x == 0 && t == 0) set_state(ctl, p ^ 1, 0u, 1u);
         return;
@@ -356,6 +358,8 @@ __global__ __launch_bounds__(WG) void k_reduce_runs(Bufs bf, Ctl* ctl, Launch L,
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
+    uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
+    __shared__ uint64_t s_wacc[NWAVES];
     while (true) {
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
@@ -386,9 +390,9 @@ __global__ __launch_bounds__(WG) void k_reduce_runs(Bufs bf, Ctl* ctl, Launch L,
         uint32_t agg;
         static_assert(KR_ITEMS < 8, "per-thread counts fit 3 bits");
         const uint32_t pre = block_excl_sum_small<3>((uint32_t)__popc(keep), s_w, agg);
-        if (wave == 0) {
-            const uint64_t excl = lb_prefix(ctl, L, bf, tile, agg);
-            if (lane == 0) {
+        {
+            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            if (t == 0) {
                 s_excl = excl;
                 if (tile == ntiles - 1) set_state(ctl, p ^ 1, (uint32_t)(excl + agg), 1u);
             }
