@@ -17,9 +17,12 @@ compared bit for bit.
 
 roofline: the kernel with the most device time, its algorithmic bytes per
 launch ((K + 8B)(n_in + n_out) for a merge: one read and one write of the live
-set) over its average launch time, from start/stop events that every merge
-dispatch of the last timed step records on the library's stream; traffic is
-the HBM bytes per launch from the committed rocprof counter summary.
+set) over its average launch time. Every merge dispatch of the last timed step
+stamps its first workgroup's start and its last workgroup's end on the
+device's constant-rate clock (100 MHz s_memrealtime); start/stop HIP events on
+each dispatch were measured to add the host-visible completion of every timed
+dispatch (about 7 us a merge, and the step slows by as much). traffic is the
+HBM bytes per launch from the committed rocprof counter summary.
 
 cpu_baseline: the zone-parallel C restatement of the reference's MultiPauliSum
 on all host cores (OpenMP), timed on the heaviest layer of the same run from
@@ -117,22 +120,25 @@ def main() -> int:
         t = time.perf_counter()
         step()
         log(f"warmup {w}: {time.perf_counter() - t:.3f} s, {len(psum)} terms")
-    # HIP events bracket the merge kernels of the last timed step only: recording
-    # events around every launch of every step would slow the steps themselves
     psum.profile_control(timing=0, reset=True)
     psum.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        if k == args.steps - 1:
-            psum.profile_control(timing=1, reset=True)
         step()
         log(f"step {k}: {time.perf_counter() - t0:.3f} s cumulative")
     psum.synchronize()
     elapsed = time.perf_counter() - t0
+    prof_t = psum.profile()
+    # one more step, after the timed region, in which every merge kernel stamps
+    # its device-clock start and end (profile mode 1: the stamps and their
+    # per-gate reduction would otherwise add to the timed steps)
+    psum.profile_control(timing=1, reset=True)
+    step()
+    psum.synchronize()
     prof = psum.profile()
     psum.profile_control(timing=0)
     counts = pc.counts[:pc.ngates].copy()
-    apps = int(prof["term_applications"]) * B * args.steps
+    apps = int(prof_t["term_applications"]) * B
     value = apps / elapsed
     overlap = psum.overlapwithzero()
     final_terms = len(psum)
@@ -147,7 +153,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_source": traffic_src,
                 "kernel": f"k_{kname}",
-                "avg_launch_us": round(avg_ms * 1e3, 2), "avg_algorithmic_bytes": round(avg_bytes),
+                "avg_launch_us": round(avg_ms * 1e3, 2), "timing": "device clock, first workgroup start to last workgroup end of each dispatch", "avg_algorithmic_bytes": round(avg_bytes),
                 "launches_per_step": kst["launches"]}
     kernels = {k: {"launches": v["launches"], "ms_timed_step": round(v["ms"], 2),
                    "GBps": round(v["bytes"] / v["ms"] / 1e6, 1) if v["ms"] > 0 else None}

@@ -333,8 +333,11 @@ typedef struct pp_profile {
     int64_t identity_gates;      /* rotations left out as touching no term (see pp_tune) */
 } pp_profile;
 
-/* Event timing: 0 off, 1 around the merge and noise kernels, 2 around every
- * kernel, -1 leave as is; reset != 0 zeroes the counters. No counterpart in the reference; its analogue is the per-gate
+/* Kernel timing: 0 off; 1 the merge and noise kernels of pp_run_circuit, each
+ * stamping its first workgroup's start and last workgroup's end on the
+ * device's constant-rate clock (no events, so the timed run is not slowed);
+ * 2 HIP events around every kernel (the events' completion adds to each timed
+ * dispatch); -1 leave as is; reset != 0 zeroes the counters. No counterpart in the reference; its analogue is the per-gate
  * term counter of @countpaulis (src/Base/countterms.jl:41-66). */
 int pp_profile_control(pp_handle *h, int enable_timing, int reset);
 int pp_profile_read(const pp_handle *h, pp_profile *out);
@@ -345,6 +348,9 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * polls; a gate whose wait runs out is run again with tiles claimed through an
  * atomic counter, which needs no residency (counted in pp_profile.fallbacks).
  * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always.
+ * PP_TUNE_FORCE_RERUN != 0 makes every look-back kernel with static tiles
+ * report its wait as run out once it has finished, so that every gate is run
+ * again with atomic tile claims (a test of that path; results are identical).
  * PP_TUNE_CHAIN_TAILS != 0 (the default) lets the merge of a rotation in
  * pp_run_circuit list the terms the next rotation touches, so that rotation
  * skips its own pass over the sum; 0 turns this off (results are identical).
@@ -376,7 +382,8 @@ enum pp_tune_key {
     PP_TUNE_BATCH_ROWS = 8,
     PP_TUNE_RADIX_BITS = 9,
     PP_TUNE_GRID_PER_CU = 10,
-    PP_TUNE_RUN_CHUNK = 11
+    PP_TUNE_RUN_CHUNK = 11,
+    PP_TUNE_FORCE_RERUN = 12
 };
 
 /* Set a tuning value. No counterpart in the reference. */

@@ -149,9 +149,67 @@ struct Launch {
     uint32_t slot;        // tile counter slot (dynamic tiles)
     uint32_t dynamic;     // 1: tiles from the atomic counter; 0: tile = blockIdx.x + k * gridDim.x
     uint32_t spin_limit;  // bound of one look-back wait with static tiles
-    uint32_t pad;
+    uint32_t flags;       // bit 0: report a static-tile wait as run out (tests the rerun path)
     uint32_t* glog;       // n_in, n_touched, n_out, roff, coff of this gate, or null
+    uint64_t* ktime;      // profiling: earliest workgroup start, latest workgroup end (device clock), or null
 };
+
+// With L.ktime set, every workgroup of a kernel stores its start and its end
+// on the device's constant-rate clock (s_memrealtime) at L.ktime[2 * blockIdx.x],
+// and k_ktime_reduce takes the earliest start and the latest end: the kernel's
+// execution time, free of the host-visible completion that event timestamps
+// add to every timed dispatch (plain stores to separate words, so the timing
+// adds no contention). Every return of a kernel that holds one is
+// workgroup-uniform (the end stamp waits at a barrier).
+struct KTimer {
+    uint64_t* kt;
+    __device__ __forceinline__ explicit KTimer(uint64_t* k) : kt(k) {
+        if (kt && threadIdx.x == 0) kt[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
+    }
+    __device__ __forceinline__ ~KTimer() {
+        if (kt) {
+            __syncthreads();
+            if (threadIdx.x == 0) kt[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+        }
+    }
+};
+
+// out[0] = earliest start, out[1] = latest end over the grid's stamps (one workgroup)
+__global__ __launch_bounds__(WG) void k_ktime_reduce(const uint64_t* __restrict__ st, unsigned grid,
+                                                     uint64_t* __restrict__ out) {
+    __shared__ uint64_t s_lo[NWAVES], s_hi[NWAVES];
+    uint64_t lo = ~0ull, hi = 0ull;
+    for (unsigned b = threadIdx.x; b < grid; b += WG) {
+        lo = min(lo, st[2 * b]);
+        hi = max(hi, st[2 * b + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        lo = min(lo, (uint64_t)__shfl_xor((unsigned long long)lo, o, 64));
+        hi = max(hi, (uint64_t)__shfl_xor((unsigned long long)hi, o, 64));
+    }
+    if (lane_id() == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NWAVES; ++w) {
+            lo = min(lo, s_lo[w]);
+            hi = max(hi, s_hi[w]);
+        }
+        out[0] = lo;
+        out[1] = hi;
+    }
+}
+
+__global__ void k_ktime_init(uint64_t* kt, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kt[2 * i] = ~0ull;
+        kt[2 * i + 1] = 0ull;
+    }
+}
 
 constexpr int LOG_WORDS = 5;
 
@@ -303,6 +361,11 @@ __device__ uint64_t tile_resolve(Ctl* ctl, const Launch& L, const Bufs& bf, uint
             }
             __builtin_amdgcn_s_sleep(1);
         }
+    }
+    if ((L.flags & 1u) && t == 0) {
+        ctl->overflow_seq = L.seq;
+        __threadfence();
+        atomicMax(&ctl->overflow, 2u);
     }
     acc = wave_sum_u64_dpp(acc);
     if (lane == 0) s_wacc[wave] = acc;
@@ -1237,8 +1300,11 @@ __device__ __forceinline__ void merge_put(const MergeRegs<W, TILE, REC>& f, uint
 // per level. Dynamic tiles (the fallback after a wait runs out) use the
 // two-level look-back.
 template <int W, bool EMIT, bool REC, int ITEMS = K3Cfg<W>::ITEMS>
-__global__ __launch_bounds__(WG) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
+// (at least three waves per SIMD for tiles of up to 768 diagonals, the register
+// budget of three resident workgroups per compute unit)
+__global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
                                                Key<W> nmask) {
+    const KTimer ktimer(L.ktime);
     constexpr int TILE = WG * ITEMS;
     static_assert(TILE < 32768, "kept positions carry a sign in bit 15; counts are packed in 16 bits");
     // the first static tile's splits go out with the control block (tile =
@@ -1532,6 +1598,7 @@ template <int W, int TB>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
                                                int damp_q, double fxy, int comm, const double* __restrict__ norm) {
+    const KTimer ktimer(L.ktime);
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (!lb_start(ctl, L, c)) return;
     const int p = L.seq & 1;
@@ -1790,6 +1857,7 @@ struct KNCfg {
 template <int W>
 __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int qubit, uint32_t damp_codes,
                                                double factor, int max_weight, double min_abs) {
+    const KTimer ktimer(L.ktime);
     constexpr int ITEMS = KNCfg<W>::ITEMS, TILE = KNCfg<W>::TILE;
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (!lb_start(ctl, L, c)) return;
@@ -1903,6 +1971,7 @@ constexpr int KNB_ROWS = 64;
 template <int W>
 __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int B, int qubit, uint32_t damp_codes,
                                                const double* __restrict__ factor, int max_weight, double min_abs) {
+    const KTimer ktimer(L.ktime);
     if (!lb_start(ctl, L)) return;
     const int p = L.seq & 1;
     const uint32_t n = ctl->st[p].n, cur = ctl->st[p].cur;

@@ -109,6 +109,7 @@ struct pp_handle {
     std::unordered_map<const void*, unsigned> resident;  // co-resident workgroups per kernel
     uint32_t spin_limit = 1u << 16;  // bound of one look-back wait with static tiles
     bool force_dynamic = false;      // take tiles from the atomic counter always
+    bool force_rerun = false;        // static-tile waits report running out (tests the rerun)
     bool chain_tails = true;         // merges list the next rotation's touched terms
     bool prev_lists = false;         // the last gate enqueued lists the next rotation's touched terms
     bool skip_identity = true;       // leave out rotations the Pauli presence shows touch nothing
@@ -161,6 +162,13 @@ struct pp_handle {
     // profile
     int prof_mode = 0;             // 0 off, 1 merge/noise kernels, 2 every kernel
     std::vector<hipEvent_t> ev;    // 6 per gate slot
+    uint64_t* d_ktime = nullptr;   // profile mode 1: device-clock start/end of each gate's merge or noise kernel
+    std::vector<uint64_t> h_ktime;
+    size_t ktime_cap = 0;
+    uint64_t* kt = nullptr;        // the enqueued gate's d_ktime slot (mode 1), else null
+    uint64_t* d_wgstamp = nullptr; // mode 1: start and end of every workgroup of the timed launch
+    unsigned last_grid = 0;        // grid of the last launch()
+    double clock_khz = 100000.0;   // rate of that clock
     int64_t gates = 0, term_apps = 0;
     int64_t launches[PP_KERNEL_KINDS] = {};
     double ms[PP_KERNEL_KINDS] = {};
@@ -402,8 +410,23 @@ Launch lb_launch(pp_handle* h, uint32_t* glog) {
     L.slot = (h->lb_count++) & 1u;
     L.dynamic = (h->force_dynamic || h->seq < h->safe_until) ? 1u : 0u;
     L.spin_limit = h->spin_limit;
+    L.flags = (h->force_rerun && !L.dynamic) ? 1u : 0u;
     L.glog = glog;
+    L.ktime = nullptr;
     return L;
+}
+
+// lb_launch of a gate's merge or noise kernel: in profile mode 1 it stamps
+// its device-clock start and end into the gate's slot
+Launch kt_launch(pp_handle* h, uint32_t* glog) {
+    Launch L = lb_launch(h, glog);
+    L.ktime = h->kt ? h->d_wgstamp : nullptr;
+    return L;
+}
+
+// after a kt_launch kernel: fold its workgroups' stamps into the gate's slot
+void ktime_finish(pp_handle* h) {
+    if (h->kt) hipLaunchKernelGGL(k_ktime_reduce, dim3(1), dim3(WG), 0, h->stream, h->d_wgstamp, h->last_grid, h->kt);
 }
 
 // Workgroups of `fn` the whole chip holds at once (occupancy x compute units).
@@ -458,6 +481,16 @@ int ensure_run(pp_handle* h, size_t ngates) {
         HIP_TRY(hipMalloc(&h->d_trig, c * sizeof(double)));
         HIP_TRY(hipHostMalloc(&h->h_trig, c * sizeof(double), hipHostMallocDefault));
         h->trig_cap = c;
+    }
+    if (h->prof_mode == 1 && h->ktime_cap < ngates) {
+        if (h->d_ktime) hipFree(h->d_ktime);
+        h->d_ktime = nullptr;
+        h->ktime_cap = 0;
+        HIP_TRY(hipMalloc(&h->d_ktime, 2 * ngates * sizeof(uint64_t)));
+        if (!h->d_wgstamp)
+            HIP_TRY(hipMalloc(&h->d_wgstamp, 2 * (size_t)std::max(64, MAX_GRID_PER_CU * h->cus) * sizeof(uint64_t)));
+        h->h_ktime.resize(2 * ngates);
+        h->ktime_cap = ngates;
     }
     if (h->prof_mode && h->ev.size() < 6 * ngates) {
         const size_t old = h->ev.size();
@@ -539,6 +572,7 @@ int check_mask(const pp_handle* h, const uint64_t* mask) {
 // them at its start and end (no marker packets between kernels).
 template <typename F, typename... Args>
 void launch(pp_handle* h, hipEvent_t e0, hipEvent_t e1, F kernel, unsigned grid, size_t shm, Args... args) {
+    h->last_grid = grid;
     if (e0 || e1)
         hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(WG), (uint32_t)shm, h->stream, e0, e1, 0u, args...);
     else
@@ -580,14 +614,15 @@ void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, in
            LP);
     if (variant == 2)
         launch(h, e4, e5, k_merge1<W, false, true, MI>, lb_grid(h, (const void*)k_merge1<W, false, true, MI>, 0, expect, TILE),
-               0, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, mask);
+               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
     else if (variant == 1)
         launch(h, e4, e5, k_merge1<W, true, false, MI>, lb_grid(h, (const void*)k_merge1<W, true, false, MI>, 0, expect, TILE),
-               0, h->bufs, h->d_ctl, lb_launch(h, glog), mask, rp, nmask);
+               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
     else
         launch(h, e4, e5, k_merge1<W, false, false, MI>,
                lb_grid(h, (const void*)k_merge1<W, false, false, MI>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-               lb_launch(h, glog), mask, rp, mask);
+               kt_launch(h, glog), mask, rp, mask);
+    ktime_finish(h);
 }
 
 template <int W>
@@ -611,7 +646,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     Levels lv;
     mask_levels(mask_in, W, &lv);
     const bool all = h->prof_mode == 2 && ev;
-    const bool mrg = h->prof_mode >= 1 && ev;
+    const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
     // Chained rotations (B = 1, not recording): every gate lists the next
     // one's touched terms (its merge, or its permutation kernel when it touches
@@ -645,7 +680,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const int B = h->B;
         const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
-        const Launch L3 = lb_launch(h, glog);
+        const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 0, nullptr)
@@ -657,6 +692,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         default: LAUNCH_MB(16); break;
         }
 #undef LAUNCH_MB
+        ktime_finish(h);
     }
     h->prev_lists = chained && next_mask;
     h->seq += 1;
@@ -673,7 +709,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
     Levels lv;
     mask_levels(mask_in, W, &lv);
     const bool all = h->prof_mode == 2 && ev;
-    const bool mrg = h->prof_mode >= 1 && ev;
+    const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
     launch(h, E(0, all), E(1, all), k_branch<W, false, true>,
            lb_grid(h, (const void*)k_branch<W, false, true>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
@@ -697,7 +733,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
         const int B = h->B;
         const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
-        const Launch L3 = lb_launch(h, glog);
+        const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 1, norm)
@@ -709,6 +745,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
         default: LAUNCH_MB(16); break;
         }
 #undef LAUNCH_MB
+        ktime_finish(h);
     }
     h->seq += 1;
 }
@@ -725,7 +762,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
     Levels lv;
     mask_levels(m, W, &lv);
     const bool all = h->prof_mode == 2 && ev;
-    const bool mrg = h->prof_mode >= 1 && ev;
+    const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
     launch(h, E(0, all), E(1, all), k_branch<W, true>,
            lb_grid(h, (const void*)k_branch<W, true>, 0, h->cap, K1_TILE), 0, h->bufs, h->d_ctl,
@@ -743,7 +780,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
         const int B = h->B;
         const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
-        const Launch L3 = lb_launch(h, glog);
+        const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy, 0, nullptr)
@@ -755,6 +792,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
         default: LAUNCH_MB(16); break;
         }
 #undef LAUNCH_MB
+        ktime_finish(h);
     }
     h->seq += 1;
 }
@@ -762,15 +800,16 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
 template <int W>
 void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const double* d_factor, int max_weight,
                double min_abs, uint32_t* glog, hipEvent_t* ev) {
-    const bool mrg = h->prof_mode >= 1 && ev;
+    const bool mrg = h->prof_mode == 2 && ev;
     hipEvent_t e0 = mrg ? ev[4] : nullptr, e1 = mrg ? ev[5] : nullptr;
     if (h->B == 1) {
         launch(h, e0, e1, k_noise1<W>, lb_grid(h, (const void*)k_noise1<W>, 0, h->cap, KNCfg<W>::TILE), 0, h->bufs,
-               h->d_ctl, lb_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
+               h->d_ctl, kt_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
     } else {
         launch(h, e0, e1, k_noiseB<W>, lb_grid(h, (const void*)k_noiseB<W>, 0, h->cap, KNB_ROWS), 0, h->bufs,
-               h->d_ctl, lb_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
+               h->d_ctl, kt_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
     }
+    ktime_finish(h);
     h->seq += 1;
 }
 
@@ -833,7 +872,7 @@ int enq_radix_sort(pp_handle* h, const uint32_t* n_ptr, int c, int nbits) {
 // map + truncate + compact, then re-sort inside segments (see pp_sort.h).
 template <int W>
 void enq_clifford(pp_handle* h, const HostGate& g, int max_weight, double min_abs, uint32_t* glog, hipEvent_t* ev) {
-    const bool mrg = h->prof_mode >= 1 && ev;
+    const bool mrg = h->prof_mode == 2 && ev;
     const int q0 = g.qubit, q1 = g.nq == 2 ? g.q1 : g.qubit;
     const int qmin = std::min(q0, q1), qmax = std::max(q0, q1);
     CliffArgs a{};
@@ -883,6 +922,7 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
             const uint64_t* next_mask) {
     uint32_t* glog = h->d_log + LOG_WORDS * slot;
     hipEvent_t* ev = h->prof_mode ? &h->ev[6 * slot] : nullptr;
+    h->kt = (h->prof_mode == 1 && h->d_ktime && slot < h->ktime_cap) ? h->d_ktime + 2 * slot : nullptr;
     const double* dt = h->B > 1 ? h->d_trig + slot * 2 * (size_t)h->B : nullptr;
     if (g.kind != PP_GATE_ROTATION) h->prev_lists = false;
     if (g.kind == PP_GATE_CLIFFORD) {
@@ -918,6 +958,7 @@ int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double
         default: enq_noise<4>(h, g.qubit, g.damp_codes, g.a, dt, max_weight, min_abs, glog, ev); break;
         }
     }
+    h->kt = nullptr;
     HIP_TRY(hipGetLastError());
     return PP_OK;
 }
@@ -1063,6 +1104,11 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
     const bool skip = h->skip_identity && !h->recording && last - first > 1;
     std::vector<uint8_t> skipped(last - first, 0);
     std::vector<size_t> order;  // gates of the chunk that are enqueued
+    if (h->prof_mode == 1 && h->d_ktime && last > first && last <= h->ktime_cap) {
+        const size_t nk = last - first;
+        hipLaunchKernelGGL(k_ktime_init, dim3((unsigned)((nk + WG - 1) / WG)), dim3(WG), 0, h->stream,
+                           h->d_ktime + 2 * first, nk);
+    }
     order.reserve(h->tun.run_chunk);
     bool pres_ok = false;
     if (skip && first < last) {
@@ -1145,6 +1191,10 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
     if (first >= last) return PP_OK;
     HIP_TRY(hipMemcpyAsync(h->h_log + LOG_WORDS * first, h->d_log + LOG_WORDS * first,
                            LOG_WORDS * (last - first) * sizeof(uint32_t), hipMemcpyDeviceToHost, h->stream));
+    const bool ktimed = h->prof_mode == 1 && h->d_ktime && last <= h->ktime_cap;
+    if (ktimed)
+        HIP_TRY(hipMemcpyAsync(h->h_ktime.data() + 2 * first, h->d_ktime + 2 * first,
+                               2 * (last - first) * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     // a gate left out passes the sum on: its size is the next run gate's input
     // size, or the final size
@@ -1173,6 +1223,11 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         }
         hipEvent_t* ev = h->prof_mode ? &h->ev[6 * i] : nullptr;
         auto span = [&](int a) -> double {
+            if (a == 4 && ktimed) {
+                // the gate's merge or noise kernel on the device clock (0 when it did not run)
+                const uint64_t t0 = h->h_ktime[2 * i], t1 = h->h_ktime[2 * i + 1];
+                return t1 > t0 ? (double)(t1 - t0) / h->clock_khz : 0.0;
+            }
             float ms = 0.f;
             if (ev && hipEventElapsedTime(&ms, ev[a], ev[a + 1]) == hipSuccess) return ms;
             return 0.0;
@@ -2133,6 +2188,9 @@ int create_handle(int nqubits, int key_words, int batch, size_t capacity, hipStr
     (void)scratch_setup(dev, &h->scratch);
     h->cus = std::max(1, cus);
     h->gmax = (unsigned)std::max(64, h->tun.grid_per_cu * h->cus);
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+        h->clock_khz = (double)khz;
     if (shared) {
         h->stream = shared;
         h->own_stream = false;
@@ -2170,6 +2228,8 @@ int pp_destroy(pp_handle* h) {
     if (h->h_ctl) hipHostFree(h->h_ctl);
     if (h->d_log) hipFree(h->d_log);
     if (h->h_log) hipHostFree(h->h_log);
+    if (h->d_ktime) hipFree(h->d_ktime);
+    if (h->d_wgstamp) hipFree(h->d_wgstamp);
     if (h->d_trig) hipFree(h->d_trig);
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
@@ -2206,6 +2266,7 @@ int pp_clone(pp_handle* h, pp_handle** out) {
     if (int rc = create_handle(h->nqubits, h->W, h->B, std::max<size_t>(n, 1024), nullptr, &c)) return rc;
     c->spin_limit = h->spin_limit;
     c->force_dynamic = h->force_dynamic;
+    c->force_rerun = h->force_rerun;
     c->chain_tails = h->chain_tails;
     c->skip_identity = h->skip_identity;
     c->adj_defer_bytes = h->adj_defer_bytes;
@@ -2587,6 +2648,9 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         return PP_OK;
     case PP_TUNE_DYNAMIC_TILES:
         h->force_dynamic = value != 0;
+        return PP_OK;
+    case PP_TUNE_FORCE_RERUN:
+        h->force_rerun = value != 0;
         return PP_OK;
     case PP_TUNE_CHAIN_TAILS:
         h->chain_tails = value != 0;

@@ -285,13 +285,17 @@ class HIPPauliSum:
         _lib.check(_lib.lib().pp_synchronize(self._h))
 
     def profile_control(self, timing: int = -1, reset: bool = False) -> None:
-        """timing: 1 records HIP events around every kernel, 0 stops, -1 leaves it."""
+        """timing: 1 stamps the merge and noise kernels of run() on the device clock, 2 records HIP
+        events around every kernel, 0 stops, -1 leaves it."""
         _lib.check(_lib.lib().pp_profile_control(self._h, int(timing), int(bool(reset))))
 
     def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None,
-             skip_identity: bool = None, adj_defer_bytes: int = None, **shapes) -> None:
+             skip_identity: bool = None, adj_defer_bytes: int = None, force_rerun: bool = None,
+             **shapes) -> None:
         """Bound of a look-back wait (polls); whether tiles are always claimed
-        atomically; whether a merge lists the next rotation's touched terms;
+        atomically; whether static-tile waits report running out (every gate
+        then runs again with atomic claims: a test of that path); whether a
+        merge lists the next rotation's touched terms;
         whether circuits leave out the rotations that touch no term; the device
         memory the adjoint sweep may keep per-rotation partial gradients in.
         Launch shapes by name (merge_items, perm_items, batch_rows, radix_bits,
@@ -310,6 +314,8 @@ class HIPPauliSum:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SPIN_LIMIT, int(spin_limit)))
         if dynamic_tiles is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_DYNAMIC_TILES, int(bool(dynamic_tiles))))
+        if force_rerun is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_FORCE_RERUN, int(bool(force_rerun))))
         if chain_tails is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_CHAIN_TAILS, int(bool(chain_tails))))
 

@@ -106,15 +106,18 @@ def test_empty_sum_and_errors():
         dev.upload(keys, c)
 
 
-@pytest.mark.parametrize("mode", ["tiny_spin_limit", "dynamic_tiles"])
+@pytest.mark.parametrize("mode", ["tiny_spin_limit", "forced_rerun", "dynamic_tiles"])
 def test_tile_claim_modes_are_bit_identical(mode):
-    # static tiles with a one-poll wait force look-back waits to run out, so gates
-    # are stopped and run again with atomic tile claims; both must match the oracle
+    # static tiles whose waits run out (a one-poll limit, where timing decides;
+    # or reported as run out by every static-tile launch) stop the gate, which is
+    # run again with atomic tile claims; every mode must match the oracle
     print(f"tile mode {mode}", flush=True)
     nq = 64
     dev, ref = _pair(nq, 200000, seed=11)
     if mode == "tiny_spin_limit":
         dev.tune(spin_limit=1)
+    elif mode == "forced_rerun":
+        dev.tune(force_rerun=True)
     else:
         dev.tune(dynamic_tiles=True)
     W = key_words(nq)
@@ -126,10 +129,10 @@ def test_tile_claim_modes_are_bit_identical(mode):
     dev.apply_pauli_noise(7, 0b1110, 0.2, max_weight=10, min_abs_coeff=1e-3)
     ref.pauli_noise(6, 0b1110, 0.8, max_weight=10, min_abs=1e-3)
     assert_bitwise(*dev.download(), *ref.get(), f"{mode} noise")
-    if mode == "tiny_spin_limit":
-        fb = dev.profile()["fallbacks"]
-        print("fallbacks", fb, flush=True)
-        assert fb > 0, "a one-poll wait limit never sent a gate to the atomic-claim rerun"
+    fb = dev.profile()["fallbacks"]
+    print("fallbacks", fb, flush=True)
+    if mode == "forced_rerun":
+        assert fb > 0, "no gate was sent to the atomic-claim rerun"
 
 
 @pytest.mark.parametrize("nq,qubit", [(32, 1), (32, 17), (64, 33), (127, 64), (127, 127)])
