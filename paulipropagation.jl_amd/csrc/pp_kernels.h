@@ -1725,27 +1725,30 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         }
         __syncthreads();
         if (B <= 64) {
-            // one column per lane: the next row's coefficients are loaded
-            // while this row is finished, so a wave keeps two rows' loads in
-            // flight (same values and operations as the loop below)
+            // one column per lane; a wave loads the coefficients of RD of its
+            // rows at once and then finishes them, so RD rows' loads are in
+            // flight together (same values and operations as the loop below)
+            constexpr int RD = TB / NWAVES < 8 ? TB / NWAVES : 8;
             const int b = lane;
             const bool col = b < B;
-            auto load_row = [&](int e, double& hv, double& tv) {
-                hv = 0.0;
-                tv = 0.0;
-                if (e < TB && s_keep[e] && col) {
-                    const int head = s_head[e];
-                    const uint32_t tl = s_tl[e];
-                    if (head != -1) hv = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + b];
-                    if (tl != ~0u) tv = coeffs[(size_t)(tl & SRC_MASK) * B + b];
+            for (int e0 = wave; e0 < TB; e0 += NWAVES * RD) {
+                double hv[RD], tv[RD];
+#pragma unroll
+                for (int d = 0; d < RD; ++d) {
+                    const int e = e0 + d * NWAVES;
+                    hv[d] = 0.0;
+                    tv[d] = 0.0;
+                    if (e < TB && s_keep[e] && col) {
+                        const int head = s_head[e];
+                        const uint32_t tl = s_tl[e];
+                        if (head != -1) hv[d] = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + b];
+                        if (tl != ~0u) tv[d] = coeffs[(size_t)(tl & SRC_MASK) * B + b];
+                    }
                 }
-            };
-            double ch, ct;
-            load_row(wave, ch, ct);
-            for (int e = wave; e < TB; e += NWAVES) {
-                double nh, nt;
-                load_row(e + NWAVES, nh, nt);
-                if (s_keep[e]) {
+#pragma unroll
+                for (int d = 0; d < RD; ++d) {
+                    const int e = e0 + d * NWAVES;
+                    if (e >= TB || !s_keep[e]) continue;
                     const int head = s_head[e];
                     const uint32_t tl = s_tl[e];
                     double m = 0.0;
@@ -1754,15 +1757,15 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                         bool have = false;
                         if (head != -1) {
                             if (head >= 0) {
-                                c = ch;
+                                c = hv[d];
                                 if (s_hxy[e]) c = c * fxy;
                             } else {
-                                c = ch * cosb[b];
+                                c = hv[d] * cosb[b];
                             }
                             have = true;
                         }
                         if (tl != ~0u) {
-                            double tc = ct * sinb[b];
+                            double tc = tv[d] * sinb[b];
                             if (tl & SRC_NEG) tc = -tc;
                             c = have ? c + tc : tc;
                         }
@@ -1773,8 +1776,6 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     m = wave_max_nonneg_f64(m);
                     if (lane == 0 && m < min_abs) s_keep[e] = 0u;
                 }
-                ch = nh;
-                ct = nt;
             }
         } else
         for (int e = wave; e < TB; e += NWAVES) {
