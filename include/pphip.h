@@ -331,6 +331,7 @@ typedef struct pp_profile {
     double milliseconds[PP_KERNEL_KINDS];
     double algorithmic_bytes[PP_KERNEL_KINDS];
     int64_t identity_gates;      /* rotations left out as touching no term (see pp_tune) */
+    int64_t persistent_gates;    /* gates applied by shared-memory runs of many gates (see pp_tune) */
 } pp_profile;
 
 /* Kernel timing: 0 off; 1 the merge and noise kernels of pp_run_circuit, each
@@ -348,6 +349,12 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * polls; a gate whose wait runs out is run again with tiles claimed through an
  * atomic counter, which needs no residency (counted in pp_profile.fallbacks).
  * PP_TUNE_DYNAMIC_TILES != 0 claims tiles through the counter always.
+ * PP_TUNE_LDS_RUNS != 0 (the default) lets pp_run_circuit apply the
+ * rotations and Pauli noise gates of one coefficient column to a sum of at
+ * most a few thousand terms (2944, 1984, 1152 for 64-, 128-, 256-bit keys, and
+ * the capacity) in one launch of one workgroup that holds the sum in shared
+ * memory, gate after gate, until the sum outgrows it (counted in
+ * pp_profile.persistent_gates); 0 turns this off (results are identical).
  * PP_TUNE_FORCE_RERUN != 0 makes every look-back kernel with static tiles
  * report its wait as run out once it has finished, so that every gate is run
  * again with atomic tile claims (a test of that path; results are identical).
@@ -383,7 +390,8 @@ enum pp_tune_key {
     PP_TUNE_RADIX_BITS = 9,
     PP_TUNE_GRID_PER_CU = 10,
     PP_TUNE_RUN_CHUNK = 11,
-    PP_TUNE_FORCE_RERUN = 12
+    PP_TUNE_FORCE_RERUN = 12,
+    PP_TUNE_LDS_RUNS = 13
 };
 
 /* Set a tuning value. No counterpart in the reference. */

@@ -98,7 +98,7 @@ struct Ctl {
     // ssamp holds every 64th entry of the input of gate ssamp_for - 1 (0: of
     // no gate's): a merge writes them for the next gate as it writes its output
     uint32_t ssamp_for;
-    uint32_t pad2;
+    uint32_t lds_done;       // gates the last shared-memory run (k_lds_run) applied
 };
 static_assert(sizeof(Ctl) % 16 == 0, "the control block is read in 16-byte pieces");
 

@@ -38,6 +38,7 @@
 #include "pp_kernels.h"
 #include "pp_sort.h"
 #include "pp_mc.h"
+#include "pp_lds.h"
 
 using namespace pp;
 
@@ -114,6 +115,11 @@ struct pp_handle {
     bool prev_lists = false;         // the last gate enqueued lists the next rotation's touched terms
     bool skip_identity = true;       // leave out rotations the Pauli presence shows touch nothing
     int64_t identity_gates = 0;      // rotations left out that way
+    int64_t lds_gates = 0;           // gates applied by shared-memory runs (k_lds_run)
+    bool lds_runs = true;            // run small sums' rotations and noise in shared memory
+    LdsGate* d_ldsg = nullptr;       // their descriptors, and a pinned staging copy
+    LdsGate* h_ldsg = nullptr;
+    size_t ldsg_cap = 0;
     uint32_t safe_until = 0;         // gates before this sequence number use dynamic tiles
     int64_t fallbacks = 0;           // gates run again with dynamic tiles
     size_t adj_defer_bytes = (size_t)2 << 30;  // largest store of per-rotation partial gradients (adjoint)
@@ -1088,6 +1094,95 @@ Presence read_presence(const pp_handle* h) {
     return P;
 }
 
+// Terms a shared-memory run may hold on this handle.
+uint32_t lds_limit(const pp_handle* h) {
+    const size_t nl = h->W == 1 ? LdsCfg<1>::NL : (h->W == 2 ? LdsCfg<2>::NL : LdsCfg<4>::NL);
+    return (uint32_t)std::min(nl, h->cap);
+}
+
+// Whether gate i may go to a shared-memory run: a rotation or Pauli noise on
+// one coefficient column, not recorded.
+bool lds_gate(const pp_handle* h, const HostGate& q) {
+    return h->lds_runs && h->B == 1 && !h->recording &&
+           (q.kind == PP_GATE_ROTATION || q.kind == PP_GATE_PAULI_NOISE);
+}
+
+// Gates sel[0 .. cnt) (all lds_gate, gate indices = log slots) in one
+// k_lds_run launch; *done receives how many it applied before the sum outgrew
+// shared memory. Reads the state back.
+int lds_run(pp_handle* h, const std::vector<HostGate>& gs, const std::vector<size_t>& sel, int max_weight,
+            double min_abs, size_t* done) {
+    *done = 0;
+    const size_t cnt = sel.size();
+    if (cnt > h->ldsg_cap) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        if (h->d_ldsg) hipFree(h->d_ldsg);
+        if (h->h_ldsg) hipHostFree(h->h_ldsg);
+        h->d_ldsg = nullptr;
+        h->h_ldsg = nullptr;
+        h->ldsg_cap = 0;
+        const size_t c = std::max<size_t>(cnt, 1024);
+        HIP_TRY(hipMalloc(&h->d_ldsg, c * sizeof(LdsGate)));
+        HIP_TRY(hipHostMalloc(&h->h_ldsg, c * sizeof(LdsGate), hipHostMallocDefault));
+        h->ldsg_cap = c;
+    }
+    for (size_t k = 0; k < cnt; ++k) {
+        const HostGate& q = gs[sel[k]];
+        LdsGate d{};
+        d.slot = (uint32_t)sel[k];
+        if (q.kind == PP_GATE_ROTATION) {
+            d.kind = 0;
+            uint64_t m4[4] = {0, 0, 0, 0};
+            for (int l = 0; l < h->W; ++l) m4[l] = q.mask[l];
+            for (int l = 0; l < 4; ++l) d.mask[l] = m4[l];
+            d.a = q.a;
+            d.b = q.b;
+            Levels lv;
+            if (mask_levels(m4, h->W, &lv)) return fail(PP_ERR_UNSUPPORTED, "rotation generator with too many runs");
+            d.nlev = lv.n;
+            d.top_runs = lv.top_runs;
+            for (int v = 0; v < lv.n; ++v) {
+                d.bit[v] = (int16_t)lv.bit[v];
+                d.sgn[v] = (int16_t)lv.sgn[v];
+            }
+        } else {
+            d.kind = 1;
+            d.qubit = q.qubit;
+            d.damp_codes = q.damp_codes;
+            d.a = q.a;
+        }
+        h->h_ldsg[k] = d;
+    }
+    HIP_TRY(hipMemcpyAsync(h->d_ldsg, h->h_ldsg, cnt * sizeof(LdsGate), hipMemcpyHostToDevice, h->stream));
+    const uint32_t lim = lds_limit(h);
+    uint32_t* glog = h->d_log;
+    switch (h->W) {
+    case 1:
+        hipLaunchKernelGGL(k_lds_run<1>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, h->d_ldsg,
+                           (int)cnt, lim, max_weight, min_abs, glog);
+        break;
+    case 2:
+        hipLaunchKernelGGL(k_lds_run<2>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, h->d_ldsg,
+                           (int)cnt, lim, max_weight, min_abs, glog);
+        break;
+    default:
+        hipLaunchKernelGGL(k_lds_run<4>, dim3(1), dim3(WG), 0, h->stream, h->bufs, h->d_ctl, h->seq, h->d_ldsg,
+                           (int)cnt, lim, max_weight, min_abs, glog);
+        break;
+    }
+    HIP_TRY(hipGetLastError());
+    if (int rc = sync_ctl(h)) return rc;
+    if (h->h_ctl->err)
+        return fail(PP_ERR_STATE, "internal consistency check failed in the shared-memory run (bits " +
+                                      std::to_string(h->h_ctl->err) + ")");
+    const size_t d = h->h_ctl->lds_done;
+    if (d > cnt) return fail(PP_ERR_STATE, "shared-memory run reports more gates than it was given");
+    h->seq += (uint32_t)d;
+    h->prev_lists = false;
+    *done = d;
+    return PP_OK;
+}
+
 // Run resolved gates first .. last-1 (trig staged at d_trig for B > 1, slots
 // are gate indices), resuming after an overflow; then account the gate log
 // into the profile. counts[i - first] receives gate i's input size.
@@ -1110,13 +1205,62 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
                            h->d_ktime + 2 * first, nk);
     }
     order.reserve(h->tun.run_chunk);
+    std::vector<uint8_t> in_lds(last - first, 0);  // applied by a shared-memory run
     bool pres_ok = false;
+    // h_ctl holds the current state when fresh (after every synchronization
+    // that no host-side repair followed)
+    bool fresh = false;
     if (skip && first < last) {
         if (int rc = enq_presence(h)) return rc;
         if (int rc = sync_ctl(h)) return rc;
         pres_ok = h->h_ctl->overflow == 0;
+        fresh = true;
+    } else if (last - first > 1 && lds_gate(h, gs[first])) {
+        if (int rc = sync_ctl(h)) return rc;
+        fresh = true;
     }
     while (g < ng) {
+        // a small sum: the rotations and noise from here run in shared memory,
+        // one launch for as many gates as the sum stays small (with the
+        // presence, rotations that touch nothing are left out as below)
+        if (fresh && h->h_ctl->overflow == 0 && lds_gate(h, gs[g]) &&
+            h->h_ctl->st[h->seq & 1].n <= lds_limit(h)) {
+            Presence P{};
+            if (pres_ok) P = read_presence(h);
+            order.clear();
+            size_t e = g;
+            while (e < ng && e - g < (1u << 16) && lds_gate(h, gs[e])) {
+                const HostGate& q = gs[e];
+                if (pres_ok && q.kind == PP_GATE_ROTATION) {
+                    if (!may_touch(P, q.mask, h->W)) {
+                        skipped[e - first] = 1;
+                        ++e;
+                        continue;
+                    }
+                    widen_rotation(P, q.mask, h->W);
+                }
+                order.push_back(e);
+                ++e;
+            }
+            size_t done = 0;
+            if (!order.empty())
+                if (int rc = lds_run(h, gs, order, max_weight, min_abs, &done)) return rc;
+            if (done > 0 || order.empty()) {
+                for (size_t k = 0; k < done; ++k) in_lds[order[k] - first] = 1;
+                // the rest, from the gate the run stopped at, goes to the pipeline
+                const size_t stop = done < order.size() ? order[done] : e;
+                for (size_t i = stop; i < e; ++i) skipped[i - first] = 0;
+                g = stop;
+                pres_ok = false;
+                if (skip && g < ng) {
+                    if (int rc = enq_presence(h)) return rc;
+                    if (int rc = sync_ctl(h)) return rc;
+                    pres_ok = h->h_ctl->overflow == 0;
+                }
+                continue;
+            }
+            for (size_t i = g; i < e; ++i) skipped[i - first] = 0;
+        }
         const size_t end = std::min(ng, g + (size_t)h->tun.run_chunk);
         const uint32_t seq0 = h->seq;
         order.clear();
@@ -1158,6 +1302,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
             return fail(PP_ERR_STATE, "internal consistency check failed in the rotation kernels (bits " +
                                           std::to_string(h->h_ctl->err) + ")");
         pres_ok = skip && h->h_ctl->overflow == 0;
+        fresh = h->h_ctl->overflow == 0;
         if (h->h_ctl->overflow) {
             const uint32_t fseq = h->h_ctl->overflow_seq;
             const size_t gi = order[fseq - seq0];
@@ -1219,6 +1364,10 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         h->term_apps += nin;
         if (skipped[i - first]) {
             h->identity_gates += 1;
+            continue;
+        }
+        if (in_lds[i - first]) {
+            h->lds_gates += 1;
             continue;
         }
         hipEvent_t* ev = h->prof_mode ? &h->ev[6 * i] : nullptr;
@@ -2230,6 +2379,8 @@ int pp_destroy(pp_handle* h) {
     if (h->h_log) hipHostFree(h->h_log);
     if (h->d_ktime) hipFree(h->d_ktime);
     if (h->d_wgstamp) hipFree(h->d_wgstamp);
+    if (h->d_ldsg) hipFree(h->d_ldsg);
+    if (h->h_ldsg) hipHostFree(h->h_ldsg);
     if (h->d_trig) hipFree(h->d_trig);
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
@@ -2267,6 +2418,7 @@ int pp_clone(pp_handle* h, pp_handle** out) {
     c->spin_limit = h->spin_limit;
     c->force_dynamic = h->force_dynamic;
     c->force_rerun = h->force_rerun;
+    c->lds_runs = h->lds_runs;
     c->chain_tails = h->chain_tails;
     c->skip_identity = h->skip_identity;
     c->adj_defer_bytes = h->adj_defer_bytes;
@@ -2629,7 +2781,7 @@ int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     if (!h) return fail(PP_ERR_ARG, "handle is NULL");
     if (enable_timing >= 0) h->prof_mode = std::min(enable_timing, 2);
     if (reset) {
-        h->gates = h->term_apps = h->fallbacks = h->identity_gates = 0;
+        h->gates = h->term_apps = h->fallbacks = h->identity_gates = h->lds_gates = 0;
         for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
             h->launches[k] = 0;
             h->ms[k] = 0.0;
@@ -2651,6 +2803,9 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         return PP_OK;
     case PP_TUNE_FORCE_RERUN:
         h->force_rerun = value != 0;
+        return PP_OK;
+    case PP_TUNE_LDS_RUNS:
+        h->lds_runs = value != 0;
         return PP_OK;
     case PP_TUNE_CHAIN_TAILS:
         h->chain_tails = value != 0;
@@ -2768,6 +2923,7 @@ int pp_profile_read(const pp_handle* h, pp_profile* out) {
     out->fallbacks = h->fallbacks;
     out->term_applications = h->term_apps;
     out->identity_gates = h->identity_gates;
+    out->persistent_gates = h->lds_gates;
     for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
         out->launches[k] = h->launches[k];
         out->milliseconds[k] = h->ms[k];

@@ -137,7 +137,7 @@ def test_listed_touched_terms_match_own_pass(mode):
     results = []
     for chain in (True, False):
         dev = pp.HIPPauliSum.from_paulisum(obs)
-        dev.tune(chain_tails=chain)
+        dev.tune(chain_tails=chain, lds_runs=False)  # the per-gate pipeline's chaining
         if mode == "tiny_spin_limit":
             dev.tune(spin_limit=1)
         pp.propagate(circ, dev, th, max_weight=8, min_abs_coeff=1e-6)
@@ -175,7 +175,7 @@ def test_identity_rotations_left_out(B):
     runs = []
     for skip in (True, False):
         dev = pp.HIPPauliSum.from_paulisum(obs, batch=B)
-        dev.tune(skip_identity=skip)
+        dev.tune(skip_identity=skip, lds_runs=False)  # the per-gate pipeline's presence test
         dev.profile_control(reset=True)
         _, cnt = pp.propagate(circ, dev, thb, max_weight=7, min_abs_coeff=1e-6, counts=True)
         runs.append((dev.download(), np.array(cnt), dev.profile()["identity_gates"]))

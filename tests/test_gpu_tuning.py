@@ -67,6 +67,7 @@ def _device(wl, table=None, **shapes):
     s = HIPPauliSum(wl["nqubits"], batch=B, capacity=1024)
     if table is not None:
         s.load_tuning(str(table))
+    s.tune(lds_runs=False)  # the shapes are the per-gate pipeline's
     s.tune(**shapes)
     k, c = wl["obs"].to_arrays(pc.W, B)
     s.upload(k, c)
@@ -179,7 +180,7 @@ def test_clone_of_tuned_sum_bitwise():
     wl = _tfim32()
     pc = PackedCircuit(wl["circuit"], wl["thetas"], 32)
     s = HIPPauliSum(32)
-    s.tune(merge_items=2, perm_items=4, grid_per_cu=1)
+    s.tune(merge_items=2, perm_items=4, grid_per_cu=1, lds_runs=False)
     k, c = wl["obs"].to_arrays(pc.W, 1)
     s.upload(k, c)
     c2 = s.clone()
