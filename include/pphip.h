@@ -332,6 +332,8 @@ typedef struct pp_profile {
     double algorithmic_bytes[PP_KERNEL_KINDS];
     int64_t identity_gates;      /* rotations left out as touching no term (see pp_tune) */
     int64_t persistent_gates;    /* gates applied by shared-memory runs of many gates (see pp_tune) */
+    int64_t graph_launches;      /* chunks launched as hipGraphs (see pp_tune), of which captured: */
+    int64_t graph_captures;
 } pp_profile;
 
 /* Kernel timing: 0 off; 1 the merge and noise kernels of pp_run_circuit, each
@@ -355,6 +357,11 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * the capacity) in one launch of one workgroup that holds the sum in shared
  * memory, gate after gate, until the sum outgrows it (counted in
  * pp_profile.persistent_gates); 0 turns this off (results are identical).
+ * PP_TUNE_GRAPHS != 0 captures every chunk of rotations and Pauli noise gates
+ * pp_run_circuit enqueues in a hipGraph and replays it when the same chunk
+ * comes again (same gates, angles, truncation, buffers and launch shapes: a
+ * repeated propagation); growing the buffers drops the graphs (counted in
+ * pp_profile.graph_launches; results are identical). Default 0.
  * PP_TUNE_FORCE_RERUN != 0 makes every look-back kernel with static tiles
  * report its wait as run out once it has finished, so that every gate is run
  * again with atomic tile claims (a test of that path; results are identical).
@@ -391,7 +398,8 @@ enum pp_tune_key {
     PP_TUNE_GRID_PER_CU = 10,
     PP_TUNE_RUN_CHUNK = 11,
     PP_TUNE_FORCE_RERUN = 12,
-    PP_TUNE_LDS_RUNS = 13
+    PP_TUNE_LDS_RUNS = 13,
+    PP_TUNE_GRAPHS = 14
 };
 
 /* Set a tuning value. No counterpart in the reference. */

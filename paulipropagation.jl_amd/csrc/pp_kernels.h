@@ -203,6 +203,12 @@ __global__ __launch_bounds__(WG) void k_ktime_reduce(const uint64_t* __restrict_
     }
 }
 
+// p[0 .. n) = v (grid-stride): clears look-back words inside a captured graph,
+// where a memset node is not used
+__global__ __launch_bounds__(WG) void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
+    for (size_t i = (size_t)blockIdx.x * WG + threadIdx.x; i < n; i += (size_t)gridDim.x * WG) p[i] = v;
+}
+
 __global__ void k_ktime_init(uint64_t* kt, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {

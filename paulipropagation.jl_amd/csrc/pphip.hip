@@ -117,6 +117,14 @@ struct pp_handle {
     int64_t identity_gates = 0;      // rotations left out that way
     int64_t lds_gates = 0;           // gates applied by shared-memory runs (k_lds_run)
     bool lds_runs = true;            // run small sums' rotations and noise in shared memory
+    bool use_graphs = false;         // capture chunks of gates in hipGraphs and replay them
+    struct ChunkGraph {
+        hipGraphExec_t exec = nullptr;
+        uint32_t seq_delta = 0, lb_delta = 0;
+        bool prev_lists = false;
+    };
+    std::unordered_map<uint64_t, ChunkGraph> graphs;
+    int64_t graph_launches = 0, graph_captures = 0;
     LdsGate* d_ldsg = nullptr;       // their descriptors, and a pinned staging copy
     LdsGate* h_ldsg = nullptr;
     size_t ldsg_cap = 0;
@@ -212,6 +220,12 @@ void free_terms(pp_handle* h) {
     h->bufs.splits = nullptr;
 }
 
+void drop_graphs(pp_handle* h) {
+    for (auto& kv : h->graphs)
+        if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+    h->graphs.clear();
+}
+
 int scratch_setup(int dev, double** out) {
     if (dev < 0 || dev >= MAX_DEVICES) return fail(PP_ERR_HIP, "device index beyond the scratch table");
     std::lock_guard<std::mutex> lock(g_scratch_mu);
@@ -297,6 +311,7 @@ int reset_ctl(pp_handle* h, uint32_t n, uint32_t cur) {
 // (Re)allocate every term buffer for `newcap` terms, keeping the live sum.
 int grow(pp_handle* h, size_t newcap) {
     if (newcap <= h->cap) return PP_OK;
+    drop_graphs(h);  // they hold the old buffers' addresses
     if (newcap > MAX_TERMS + 1) newcap = MAX_TERMS + 1;
     const int W = h->W, B = h->B;
     uint32_t n = 0, cur = 0;
@@ -1094,6 +1109,117 @@ Presence read_presence(const pp_handle* h) {
     return P;
 }
 
+// ---------------------------------------------------------------- chunks as graphs
+//
+// With PP_TUNE_GRAPHS a chunk of rotations and Pauli noise gates is captured
+// in a hipGraph the first time it is enqueued and replayed when the same chunk
+// comes again (same gates, angles, truncation, state parity, buffers and
+// launch shapes: a repeated propagation). The look-back words carry the epochs
+// of the capture, so the graph starts by clearing them (a fill kernel node);
+// the host's epoch counter has moved past them, so no later launch reads them
+// as its own.
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    }
+    template <typename T>
+    void val(const T& v) { add(&v, sizeof(T)); }
+};
+
+bool graph_ok(const pp_handle* h, const std::vector<HostGate>& gs, const std::vector<size_t>& order) {
+    if (!h->use_graphs || h->prof_mode != 0 || h->recording || order.empty()) return false;
+    if (h->force_dynamic || h->seq < h->safe_until) return false;
+    if ((size_t)h->epoch + 8 * order.size() + 64 >= (size_t)LB_EPOCH_MASK) return false;  // no wrap inside
+    for (size_t i : order)
+        if (gs[i].kind != PP_GATE_ROTATION && gs[i].kind != PP_GATE_PAULI_NOISE) return false;
+    return true;
+}
+
+int enqueue(pp_handle* h, const HostGate& g, size_t slot, int max_weight, double min_abs,
+            const uint64_t* next_mask);
+
+// The chunk's gates (order; chaining as in run_resolved), from the cache or
+// captured now; either way launched on the handle's stream.
+int enq_chunk_graph(pp_handle* h, const std::vector<HostGate>& gs, const std::vector<size_t>& order, size_t ng,
+                    int max_weight, double min_abs) {
+    Fnv f;
+    f.val(h->seq);  // baked into the launches (state parity, list and sample tags, overflow reports)
+    f.val(h->lb_count & 1u);
+    f.val(h->cap);
+    f.val(h->W);
+    f.val(h->B);
+    f.val(max_weight);
+    f.val(min_abs);
+    f.val(h->tun);
+    f.val(h->chain_tails);
+    f.val(h->spin_limit);
+    f.val(h->force_rerun);
+    for (size_t k = 0; k < order.size(); ++k) {
+        const size_t i = order[k];
+        const HostGate& q = gs[i];
+        f.val(i);
+        f.val(q.kind);
+        if (q.kind == PP_GATE_ROTATION) f.add(q.mask, sizeof(uint64_t) * h->W);
+        f.val(q.qubit);
+        f.val(q.damp_codes);
+        f.val(q.a);
+        f.val(q.b);
+        f.val(q.flags);
+    }
+    const size_t nl = order.back() + 1;
+    f.val(nl < ng ? gs[nl].kind : -1);  // the gate after the chunk (chaining of the last one)
+    if (nl < ng && gs[nl].kind == PP_GATE_ROTATION) f.add(gs[nl].mask, sizeof(uint64_t) * h->W);
+    const uint64_t key = f.h;
+    auto it = h->graphs.find(key);
+    if (it != h->graphs.end()) {
+        HIP_TRY(hipGraphLaunch(it->second.exec, h->stream));
+        h->seq += it->second.seq_delta;
+        h->lb_count += it->second.lb_delta;
+        h->prev_lists = it->second.prev_lists;
+        h->graph_launches += 1;
+        return PP_OK;
+    }
+    const uint32_t seq0 = h->seq, lb0 = h->lb_count;
+    HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = PP_OK;
+    const size_t nw = 3 * (size_t)h->bufs.lb_tiles, ngw = 2 * (size_t)h->bufs.lb_groups;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::min<size_t>((nw + WG - 1) / WG, 1024)), dim3(WG), 0, h->stream,
+                       h->bufs.states, nw, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)std::min<size_t>((ngw + WG - 1) / WG, 1024)), dim3(WG), 0,
+                       h->stream, h->bufs.gstates, ngw, 0ull);
+    for (size_t k = 0; k < order.size() && rc == PP_OK; ++k) {
+        const size_t i = order[k];
+        const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
+        const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
+        rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->stream, &graph);
+    if (rc != PP_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess || !graph) return fail(PP_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(ec));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(PP_ERR_HIP, std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+    if (h->graphs.size() >= 4096) drop_graphs(h);  // bounded cache
+    pp_handle::ChunkGraph cg;
+    cg.exec = exec;
+    cg.seq_delta = h->seq - seq0;
+    cg.lb_delta = h->lb_count - lb0;
+    cg.prev_lists = h->prev_lists;
+    h->graphs[key] = cg;
+    h->graph_captures += 1;
+    HIP_TRY(hipGraphLaunch(exec, h->stream));
+    h->graph_launches += 1;
+    return PP_OK;
+}
+
 // Terms a shared-memory run may hold on this handle.
 uint32_t lds_limit(const pp_handle* h) {
     const size_t nl = h->W == 1 ? LdsCfg<1>::NL : (h->W == 2 ? LdsCfg<2>::NL : LdsCfg<4>::NL);
@@ -1288,12 +1414,16 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
         // a chunk starts a new chain: the last gate of the previous one listed
         // the terms of the next gate of the circuit, which may have been left out
         h->prev_lists = false;
-        for (size_t k = 0; k < order.size(); ++k) {
-            const size_t i = order[k];
-            const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
-            const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng &&
-                               gs[nx].kind == PP_GATE_ROTATION;
-            if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr)) return rc;
+        if (graph_ok(h, gs, order)) {
+            if (int rc = enq_chunk_graph(h, gs, order, ng, max_weight, min_abs)) return rc;
+        } else {
+            for (size_t k = 0; k < order.size(); ++k) {
+                const size_t i = order[k];
+                const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
+                const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng &&
+                                   gs[nx].kind == PP_GATE_ROTATION;
+                if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr)) return rc;
+            }
         }
         if (skip && end < ng)
             if (int rc = enq_presence(h)) return rc;
@@ -2381,6 +2511,7 @@ int pp_destroy(pp_handle* h) {
     if (h->d_wgstamp) hipFree(h->d_wgstamp);
     if (h->d_ldsg) hipFree(h->d_ldsg);
     if (h->h_ldsg) hipHostFree(h->h_ldsg);
+    drop_graphs(h);
     if (h->d_trig) hipFree(h->d_trig);
     if (h->h_trig) hipHostFree(h->h_trig);
     if (h->d_partial) hipFree(h->d_partial);
@@ -2419,6 +2550,7 @@ int pp_clone(pp_handle* h, pp_handle** out) {
     c->force_dynamic = h->force_dynamic;
     c->force_rerun = h->force_rerun;
     c->lds_runs = h->lds_runs;
+    c->use_graphs = h->use_graphs;
     c->chain_tails = h->chain_tails;
     c->skip_identity = h->skip_identity;
     c->adj_defer_bytes = h->adj_defer_bytes;
@@ -2479,6 +2611,11 @@ int pp_upload(pp_handle* h, const uint64_t* keys, const double* coeffs, size_t n
     for (size_t i = 1; i < n; ++i)
         if (key_cmp(keys + idx[i] * W, keys + idx[i - 1] * W, W) == 0)
             return fail(PP_ERR_ARG, "duplicate Pauli string in upload");
+    // a new sum: gate numbers (which the kernels compare on the device) start
+    // again, so a repeated propagation repeats them and can replay its graphs
+    h->seq = 0;
+    h->safe_until = 0;
+    h->lb_count = 0;
     if (int rc = reset_ctl(h, 0, 0)) return rc;
     if (n > h->cap)
         if (int rc = grow(h, n)) return rc;
@@ -2782,6 +2919,7 @@ int pp_profile_control(pp_handle* h, int enable_timing, int reset) {
     if (enable_timing >= 0) h->prof_mode = std::min(enable_timing, 2);
     if (reset) {
         h->gates = h->term_apps = h->fallbacks = h->identity_gates = h->lds_gates = 0;
+        h->graph_launches = h->graph_captures = 0;
         for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
             h->launches[k] = 0;
             h->ms[k] = 0.0;
@@ -2806,6 +2944,10 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         return PP_OK;
     case PP_TUNE_LDS_RUNS:
         h->lds_runs = value != 0;
+        return PP_OK;
+    case PP_TUNE_GRAPHS:
+        h->use_graphs = value != 0;
+        if (!h->use_graphs) drop_graphs(h);
         return PP_OK;
     case PP_TUNE_CHAIN_TAILS:
         h->chain_tails = value != 0;
@@ -2924,6 +3066,8 @@ int pp_profile_read(const pp_handle* h, pp_profile* out) {
     out->term_applications = h->term_apps;
     out->identity_gates = h->identity_gates;
     out->persistent_gates = h->lds_gates;
+    out->graph_launches = h->graph_launches;
+    out->graph_captures = h->graph_captures;
     for (int k = 0; k < PP_KERNEL_KINDS; ++k) {
         out->launches[k] = h->launches[k];
         out->milliseconds[k] = h->ms[k];

@@ -33,7 +33,8 @@ class pp_profile(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_int64), ("term_applications", ctypes.c_int64), ("fallbacks", ctypes.c_int64),
                 ("launches", ctypes.c_int64 * 5), ("milliseconds", ctypes.c_double * 5),
                 ("algorithmic_bytes", ctypes.c_double * 5), ("identity_gates", ctypes.c_int64),
-                ("persistent_gates", ctypes.c_int64)]
+                ("persistent_gates", ctypes.c_int64), ("graph_launches", ctypes.c_int64),
+                ("graph_captures", ctypes.c_int64)]
 
 
 _lib = None
@@ -91,7 +92,7 @@ PP_TUNE_SPIN_LIMIT, PP_TUNE_DYNAMIC_TILES, PP_TUNE_CHAIN_TAILS, PP_TUNE_SKIP_IDE
 PP_TUNE_ADJ_DEFER_BYTES = 5
 PP_TUNE_MERGE_ITEMS, PP_TUNE_PERM_ITEMS, PP_TUNE_BATCH_ROWS, PP_TUNE_RADIX_BITS = 6, 7, 8, 9
 PP_TUNE_GRID_PER_CU, PP_TUNE_RUN_CHUNK = 10, 11
-PP_TUNE_FORCE_RERUN, PP_TUNE_LDS_RUNS = 12, 13
+PP_TUNE_FORCE_RERUN, PP_TUNE_LDS_RUNS, PP_TUNE_GRAPHS = 12, 13, 14
 TUNE_NAMES = {"merge_items": PP_TUNE_MERGE_ITEMS, "perm_items": PP_TUNE_PERM_ITEMS, "batch_rows": PP_TUNE_BATCH_ROWS,
               "radix_bits": PP_TUNE_RADIX_BITS, "grid_per_cu": PP_TUNE_GRID_PER_CU, "run_chunk": PP_TUNE_RUN_CHUNK}
 PP_OVERLAP_ZERO, PP_OVERLAP_PLUS, PP_OVERLAP_COMPUTATIONAL, PP_OVERLAP_MAXMIXED = 0, 1, 2, 3

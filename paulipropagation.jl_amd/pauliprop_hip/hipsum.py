@@ -291,13 +291,14 @@ class HIPPauliSum:
 
     def tune(self, spin_limit: int = None, dynamic_tiles: bool = None, chain_tails: bool = None,
              skip_identity: bool = None, adj_defer_bytes: int = None, force_rerun: bool = None,
-             lds_runs: bool = None, **shapes) -> None:
+             lds_runs: bool = None, graphs: bool = None, **shapes) -> None:
         """Bound of a look-back wait (polls); whether tiles are always claimed
         atomically; whether static-tile waits report running out (every gate
         then runs again with atomic claims: a test of that path); whether a
         merge lists the next rotation's touched terms; whether circuits run the
         rotations and noise of a small sum in shared memory, many gates per
-        launch;
+        launch; whether chunks of gates are captured in hipGraphs and replayed
+        when the same chunk comes again;
         whether circuits leave out the rotations that touch no term; the device
         memory the adjoint sweep may keep per-rotation partial gradients in.
         Launch shapes by name (merge_items, perm_items, batch_rows, radix_bits,
@@ -316,6 +317,8 @@ class HIPPauliSum:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_SPIN_LIMIT, int(spin_limit)))
         if dynamic_tiles is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_DYNAMIC_TILES, int(bool(dynamic_tiles))))
+        if graphs is not None:
+            _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_GRAPHS, int(bool(graphs))))
         if lds_runs is not None:
             _lib.check(L.pp_tune(self._h, _lib.PP_TUNE_LDS_RUNS, int(bool(lds_runs))))
         if force_rerun is not None:
@@ -336,6 +339,8 @@ class HIPPauliSum:
             "fallbacks": p.fallbacks,
             "identity_gates": p.identity_gates,
             "persistent_gates": p.persistent_gates,
+            "graph_launches": p.graph_launches,
+            "graph_captures": p.graph_captures,
             "kernels": {k: {"launches": p.launches[i], "ms": p.milliseconds[i], "bytes": p.algorithmic_bytes[i]}
                         for i, k in enumerate(_lib.KERNEL_KINDS)},
         }
