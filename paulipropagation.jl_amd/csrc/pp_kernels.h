@@ -1603,7 +1603,8 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
 template <int W, int TB>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
-                                               int damp_q, double fxy, int comm, const double* __restrict__ norm) {
+                                               int damp_q, double fxy, int comm, const double* __restrict__ norm,
+                                               int part, int emit, Key<W> nmask) {
     const KTimer ktimer(L.ktime);
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (!lb_start(ctl, L, c)) return;
@@ -1634,8 +1635,16 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint32_t s_split[2];
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_w[NWAVES];
-    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_excl, s_excl_t;
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
+    uint32_t* __restrict__ next_tail = bf.tail[0];
+    // with `part`, tiles take their splits from the partition kernel (launched
+    // before the merge, tiles of 64 rows or more); otherwise each tile
+    // searches its own (better on small sums, where the extra launch costs
+    // more than the searches)
+    const bool PART = (TB % 64 == 0) && part != 0;
+    bool have_next = false;
+    uint32_t nx0 = 0, nx1 = 0;
     TileSource ts;
     uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
     __shared__ uint64_t s_wacc[NWAVES];
@@ -1644,12 +1653,32 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         if (tile >= ntiles) break;
         const uint32_t d0 = tile * (uint32_t)TB;
         const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
-        if (wave < 2) {
-            const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
-            if (lane == 0) s_split[wave] = s;
+        uint32_t i0, i1;
+        if (PART) {
+            // splits from the partition kernel: this tile's were loaded during
+            // the previous tile (static tiles), the next tile's go out now
+            if (L.dynamic || !have_next) {
+                i0 = bf.splits[tile];
+                i1 = bf.splits[tile + 1];
+            } else {
+                i0 = nx0;
+                i1 = nx1;
+            }
+            const uint32_t nt2 = tile + gridDim.x;
+            have_next = !L.dynamic && nt2 < ntiles;
+            if (have_next) {
+                nx0 = bf.splits[nt2];
+                nx1 = bf.splits[nt2 + 1];
+            }
+        } else {
+            if (wave < 2) {
+                const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
+                if (lane == 0) s_split[wave] = s;
+            }
+            __syncthreads();
+            i0 = s_split[0];
+            i1 = s_split[1];
         }
-        __syncthreads();
-        const uint32_t i0 = s_split[0], i1 = s_split[1];
         const uint32_t j0 = d0 - i0, j1 = d1 - i1;
         int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
         if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
@@ -1815,16 +1844,32 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             if (lane == 0 && m < min_abs) s_keep[e] = 0u;
         }
         __syncthreads();
+        // kept rows; with `emit`, the kept rows the next rotation (nmask)
+        // touches are also listed for it, as its branch pass would list them
+        // (second prefix chain, counts in the high half)
         const uint32_t mine = (t < TB) ? s_keep[t] : 0u;
+        const bool touch = emit && mine && anticommutes<W>(nmask, okey);
         uint32_t agg;
-        const uint32_t pre = block_excl_sum_small<1>(mine, s_w, agg);
+        const uint32_t pre2 = block_excl_sum_small<1>(mine | (touch ? 1u << 16 : 0u), s_w, agg);
+        const uint32_t pre = pre2 & 0xffffu, pre_t = pre2 >> 16;
+        const uint64_t aggp = (uint64_t)(agg & 0xffffu) | ((uint64_t)(agg >> 16) << 32);
         {
-            const uint64_t excl = tile_prefix<1>(ctl, L, bf, tile, agg, lb_incl, s_wacc);
+            const uint64_t ex = emit ? tile_prefix<2>(ctl, L, bf, tile, aggp, lb_incl, s_wacc)
+                                     : tile_prefix<1>(ctl, L, bf, tile, aggp, lb_incl, s_wacc);
+            const uint64_t excl = ex & 0xffffffffull, excl_t = ex >> 32;
             if (t == 0) {
                 s_excl = excl;
+                s_excl_t = excl_t;
                 if (tile == ntiles - 1) {
-                    const uint32_t nout = (uint32_t)(excl + agg);
+                    const uint32_t nout = (uint32_t)(excl + (agg & 0xffffu));
                     set_state(ctl, p ^ 1, nout, cur ^ 1);
+                    if (emit) {
+                        ctl->st[p ^ 1].na = (uint32_t)(excl_t + (agg >> 16));
+#pragma unroll
+                        for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                        ctl->tail_seq = L.seq + 1;
+                        ctl->tail_ready = 1u;
+                    }
                     if (L.glog) L.glog[2] = nout;
                 }
             }
@@ -1833,6 +1878,13 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         if (t < TB && mine) {
             store_key<W>(out_keys, stride, s_excl + pre, okey);
             s_tl[t] = pre;
+            if (touch) {
+                const uint64_t j = s_excl_t + pre_t;
+                const bool neg = (sign_exponent<W>(nmask, okey) & 2) == 0;
+                next_tail[j] = (uint32_t)(s_excl + pre) | (neg ? SRC_NEG : 0u);
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + j] = okey.l[l];
+            }
         }
         if (bf.rec && t < TB) {
             const uint32_t at = mine ? (uint32_t)(s_excl + pre) : REC_NONE;

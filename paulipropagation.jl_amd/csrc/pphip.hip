@@ -621,6 +621,27 @@ void enq_perm(pp_handle* h, hipEvent_t e0, hipEvent_t e1, const Launch& L, bool 
     }
 }
 
+// The batched merge's partition (tiles of 64 rows or more; narrower tiles
+// search their own splits).
+// Sums of at least this many terms at the last synchronization take the
+// batched merge's splits from the partition kernel.
+constexpr uint32_t PART_B_MIN_TERMS = 1u << 18;
+
+int part_b(const pp_handle* h, int tb) {
+    return (tb % 64 == 0 && h->h_ctl->st[h->seq & 1].n >= PART_B_MIN_TERMS) ? 1 : 0;
+}
+
+template <int W, int TB>
+void enq_partition_b(pp_handle* h) {
+    if constexpr (TB % 64 == 0) {
+        if (!part_b(h, TB)) return;
+        Launch LP{};
+        LP.seq = h->seq;
+        launch(h, nullptr, nullptr, k_partition<W, TB>, grid_for(h, h->cap + h->cap / 2, (size_t)TB * NWAVES), 0,
+               h->bufs, h->d_ctl, LP);
+    }
+}
+
 // The one-column merge with the tuning table's tile (256 x MI diagonals): its
 // partition, then the merge; variant 0 plain, 1 listing the next rotation's
 // touched terms (nmask), 2 recording for the adjoint sweep.
@@ -669,11 +690,11 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
-    // Chained rotations (B = 1, not recording): every gate lists the next
+    // Chained rotations (not recording): every gate lists the next
     // one's touched terms (its merge, or its permutation kernel when it touches
     // nothing), so after the first gate of a chain k_branch is not launched and
     // the permutation kernel ends the branch step.
-    const bool chained = h->B == 1 && !h->recording && h->chain_tails && h->prof_mode != 2;
+    const bool chained = !h->recording && h->chain_tails && h->prof_mode != 2;
     const bool listed = chained && h->prev_lists;
     h->prev_lists = false;
     if (!listed)
@@ -702,9 +723,11 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const int TB = batch_rows(h);
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = kt_launch(h, glog);
+        const bool emitB = chained && next_mask;
 #define LAUNCH_MB(TBV)                                                                                         \
+    enq_partition_b<W, TBV>(h);                                                                                \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 0, nullptr)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 0, nullptr, part_b(h, TBV), emitB ? 1 : 0, emitB ? to_key<W>(next_mask) : mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -756,8 +779,9 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
+    enq_partition_b<W, TBV>(h);                                                                                \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 1, norm)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 1, norm, part_b(h, TBV), 0, mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -803,8 +827,9 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
         const size_t shm = (size_t)TB * B * sizeof(double);
         const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
+    enq_partition_b<W, TBV>(h);                                                                                \
     launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
-           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy, 0, nullptr)
+           shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy, 0, nullptr, part_b(h, TBV), 0, mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
         case 128: LAUNCH_MB(128); break;
@@ -1193,7 +1218,7 @@ int enq_chunk_graph(pp_handle* h, const std::vector<HostGate>& gs, const std::ve
     for (size_t k = 0; k < order.size() && rc == PP_OK; ++k) {
         const size_t i = order[k];
         const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
-        const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
+        const bool chain = h->chain_tails && !h->recording && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
         rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr);
     }
     hipGraph_t graph = nullptr;
@@ -1420,7 +1445,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
             for (size_t k = 0; k < order.size(); ++k) {
                 const size_t i = order[k];
                 const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
-                const bool chain = h->B == 1 && h->chain_tails && !h->recording && nx < ng &&
+                const bool chain = h->chain_tails && !h->recording && nx < ng &&
                                    gs[nx].kind == PP_GATE_ROTATION;
                 if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr)) return rc;
             }

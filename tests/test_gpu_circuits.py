@@ -115,13 +115,14 @@ def test_noisy_circuit():
     assert_bitwise(*dev.download(), *ref.get(), "noisy")
 
 
+@pytest.mark.parametrize("B", [1, 3, 64])
 @pytest.mark.parametrize("mode", ["default", "tiny_spin_limit"])
-def test_listed_touched_terms_match_own_pass(mode):
+def test_listed_touched_terms_match_own_pass(mode, B):
     # a merge lists the next rotation's touched terms; the result must equal the
     # run where every rotation lists them itself, and the oracle, bit for bit,
     # also when gates are stopped and run again (one-poll look-back waits) and
     # when noise gates break the chain of rotations
-    print(f"listed touched terms, {mode}", flush=True)
+    print(f"listed touched terms, {mode}, B={B}", flush=True)
     nq = 40
     circ = []
     for layer in range(4):
@@ -130,20 +131,22 @@ def test_listed_touched_terms_match_own_pass(mode):
         if layer == 1:
             pp.depolarizinglayer(circ, nq, 0.02)
     P = pp.countparameters(circ)
-    th = np.random.default_rng(9).uniform(0.1, 1.2, size=P)
+    th = np.random.default_rng(9).uniform(0.1, 1.2, size=(P, B))
+    if B == 1:
+        th = th[:, 0]
     obs = pp.PauliSum(nq)
     for q in (7, 20, 33):
         obs.add(["Z"], [q])
     results = []
     for chain in (True, False):
-        dev = pp.HIPPauliSum.from_paulisum(obs)
+        dev = pp.HIPPauliSum.from_paulisum(obs, batch=B)
         dev.tune(chain_tails=chain, lds_runs=False)  # the per-gate pipeline's chaining
         if mode == "tiny_spin_limit":
             dev.tune(spin_limit=1)
         pp.propagate(circ, dev, th, max_weight=8, min_abs_coeff=1e-6)
         results.append(dev.download())
     assert_bitwise(*results[0], *results[1], f"chained vs own pass ({mode})")
-    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=8, min_abs_coeff=1e-6)
+    ref, _ = oracle_propagate(circ, nq, th, obs.terms, max_weight=8, min_abs_coeff=1e-6, batch=B)
     assert_bitwise(*results[0], *ref.get(), f"chained vs oracle ({mode})")
 
 
