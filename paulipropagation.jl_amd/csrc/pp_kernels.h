@@ -2066,41 +2066,45 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)KNB_ROWS;
+        constexpr int RPW = KNB_ROWS / NWAVES;  // rows of a wave in a tile
+        constexpr int RG = 8;                    // rows loaded at once
+        double cvv[RPW];                         // B <= 64: the wave's rows, damped, kept for the stores
         if (B <= 64) {
-            // one column per lane: the next row's key and coefficient are
-            // loaded while this row is decided (same operations as below)
+            // one column per lane; a wave loads the keys and coefficients of
+            // RG of its rows at once, decides them, and keeps their values
+            // for the stores (same operations as the loop below)
             const int b = lane;
             const bool col = b < B;
-            auto load_row = [&](int r, Key<W>& k, double& c) {
-                const uint32_t i = base + r;
-                if (r < KNB_ROWS && i < n) {
-                    k = load_key<W>(keys, stride, i);
-                    c = col ? coeffs[(size_t)i * B + b] : 0.0;
+#pragma unroll
+            for (int g0 = 0; g0 < RPW; g0 += RG) {
+                Key<W> kk[RG];
+                double cc[RG];
+#pragma unroll
+                for (int d = 0; d < RG; ++d) {
+                    const uint32_t i = base + wave + (uint32_t)(g0 + d) * NWAVES;
+                    const uint32_t ic = min(i, n - 1);
+                    kk[d] = load_key<W>(keys, stride, ic);
+                    cc[d] = col ? coeffs[(size_t)ic * B + b] : 0.0;
                 }
-            };
-            Key<W> ck{};
-            double cc = 0.0;
-            load_row(wave, ck, cc);
-            for (int r = wave; r < KNB_ROWS; r += NWAVES) {
-                Key<W> nk{};
-                double nc = 0.0;
-                load_row(r + NWAVES, nk, nc);
-                const uint32_t i = base + r;
-                uint32_t keep = 0;
-                if (i < n) {
-                    const bool damped = (damp_codes >> code_at<W>(ck, qubit)) & 1u;
-                    double m = 0.0;
-                    if (col) {
-                        double c = cc;
-                        if (damped) c = c * factor[b];
-                        m = fabs(c);
+#pragma unroll
+                for (int d = 0; d < RG; ++d) {
+                    const int r = wave + (g0 + d) * NWAVES;
+                    const uint32_t i = base + r;
+                    uint32_t keep = 0;
+                    double c = cc[d];
+                    if (i < n) {
+                        const bool damped = (damp_codes >> code_at<W>(kk[d], qubit)) & 1u;
+                        double m = 0.0;
+                        if (col) {
+                            if (damped) c = c * factor[b];
+                            m = fabs(c);
+                        }
+                        m = wave_max_nonneg_f64(m);
+                        keep = !((max_weight >= 0 && weight<W>(kk[d]) > max_weight) || m < min_abs);
                     }
-                    m = wave_max_nonneg_f64(m);
-                    keep = !((max_weight >= 0 && weight<W>(ck) > max_weight) || m < min_abs);
+                    cvv[g0 + d] = c;
+                    if (lane == 0) s_keep[r] = keep;
                 }
-                if (lane == 0) s_keep[r] = keep;
-                ck = nk;
-                cc = nc;
             }
         } else
         for (int r = wave; r < KNB_ROWS; r += NWAVES) {
@@ -2145,6 +2149,19 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
             }
         }
         __syncthreads();
+        if (B <= 64 && !bf.rec) {
+            // the kept rows' values are in registers: keys copied, values stored
+#pragma unroll
+            for (int d = 0; d < RPW; ++d) {
+                const int r = wave + d * NWAVES;
+                if (!s_keep[r]) continue;
+                const uint32_t i = base + r;
+                const uint64_t pos = s_excl + s_pos[r];
+                if (lane == 0) store_key<W>(out_keys, stride, pos, load_key<W>(keys, stride, i));
+                if (lane < B) out_coeffs[pos * B + lane] = cvv[d];
+            }
+            continue;
+        }
         for (int r = wave; r < KNB_ROWS; r += NWAVES) {
             const uint32_t i = base + r;
             if (bf.rec && i < n && lane == 0) {
