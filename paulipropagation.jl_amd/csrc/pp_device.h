@@ -214,6 +214,39 @@ __device__ __forceinline__ double wave_max_nonneg_f64(double v) {
     return m;
 }
 
+// Maximum of values >= 0 over segments of 64 / rps lanes (rps = 1, 2 or 4
+// rows of a batch side by side in one wave): the same row steps, then each
+// lane takes the maximum of its segment's 16-lane rows. Every lane calls it.
+__device__ __forceinline__ double wave_seg_max_nonneg_f64(double v, int rps) {
+#define PP_ROW_STEP(CTRL)                                                                              \
+    {                                                                                                  \
+        const uint64_t u = (uint64_t)__double_as_longlong(v);                                          \
+        const uint32_t tl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, true); \
+        const uint32_t th = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true); \
+        v = fmax(v, __longlong_as_double((long long)(((uint64_t)th << 32) | tl)));                     \
+    }
+    PP_ROW_STEP(0x111)  // row_shr:1
+    PP_ROW_STEP(0x112)  // row_shr:2
+    PP_ROW_STEP(0x114)  // row_shr:4
+    PP_ROW_STEP(0x118)  // row_shr:8
+#undef PP_ROW_STEP
+    const uint64_t u = (uint64_t)__double_as_longlong(v);
+    double r[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t rl = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, 16 * k + 15);
+        const uint32_t rh = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), 16 * k + 15);
+        r[k] = __longlong_as_double((long long)(((uint64_t)rh << 32) | rl));
+    }
+    const int lane = lane_id();
+    if (rps == 4) {
+        const int q = lane >> 4;
+        return q == 0 ? r[0] : (q == 1 ? r[1] : (q == 2 ? r[2] : r[3]));
+    }
+    if (rps == 2) return lane < 32 ? fmax(r[0], r[1]) : fmax(r[2], r[3]);
+    return fmax(fmax(r[0], r[1]), fmax(r[2], r[3]));
+}
+
 __device__ __forceinline__ uint32_t wave_incl_sum_u32(uint32_t v) {
     const int lane = lane_id();
 #pragma unroll

@@ -1600,7 +1600,7 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
 // coefficient rows (lane b = column b), stages them in LDS and writes the kept
 // rows after the look-back. trig holds cos_b then sin_b of this gate.
 
-template <int W, int TB>
+template <int W, int TB, bool NARROW = false>
 __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
                                                int damp_q, double fxy, int comm, const double* __restrict__ norm,
@@ -1759,7 +1759,16 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             s_rt[t] = ~0u;
         }
         __syncthreads();
-        if (B <= 64) {
+        // B <= 64: one column per lane, rps rows of the batch side by side in a
+        // wave (segments of 64 / rps lanes); a wave loads the coefficients of
+        // RD slots of its rows at once and then finishes them (same values
+        // and operations as the loop for larger B below)
+        constexpr int RPW = TB / NWAVES;  // rows of a wave in a tile
+        // (NARROW: B <= 32, several rows per wave; otherwise one row per wave)
+        const int rps = !NARROW ? 1 : (B <= 16 ? 4 : 2);
+        const int sub = lane / (64 / rps), bcol = lane - sub * (64 / rps);
+        const bool col = bcol < B;
+        if (!NARROW && B <= 64) {
             // one column per lane; a wave loads the coefficients of RD of its
             // rows at once and then finishes them, so RD rows' loads are in
             // flight together (same values and operations as the loop below)
@@ -1810,6 +1819,58 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     }
                     m = wave_max_nonneg_f64(m);
                     if (lane == 0 && m < min_abs) s_keep[e] = 0u;
+                }
+            }
+        } else if (NARROW) {
+            constexpr int RD = RPW < 8 ? RPW : 8;
+            const int nslots = (RPW + rps - 1) / rps;
+            for (int q0 = 0; q0 < nslots; q0 += RD) {
+                double hv[RD], tv[RD];
+#pragma unroll
+                for (int d = 0; d < RD; ++d) {
+                    const int k = (q0 + d) * rps + sub;
+                    const int e = wave + k * NWAVES;
+                    hv[d] = 0.0;
+                    tv[d] = 0.0;
+                    if (k < RPW && s_keep[e] && col) {
+                        const int head = s_head[e];
+                        const uint32_t tl = s_tl[e];
+                        if (head != -1) hv[d] = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + bcol];
+                        if (tl != ~0u) tv[d] = coeffs[(size_t)(tl & SRC_MASK) * B + bcol];
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < RD; ++d) {
+                    if (q0 + d >= nslots) break;  // uniform: every lane's slot count is the same
+                    const int k = (q0 + d) * rps + sub;
+                    const int e = wave + k * NWAVES;
+                    const bool live = k < RPW && s_keep[e];
+                    double m = 0.0;
+                    if (live && col) {
+                        const int head = s_head[e];
+                        const uint32_t tl = s_tl[e];
+                        double c = 0.0;
+                        bool have = false;
+                        if (head != -1) {
+                            if (head >= 0) {
+                                c = hv[d];
+                                if (s_hxy[e]) c = c * fxy;
+                            } else {
+                                c = hv[d] * cosb[bcol];
+                            }
+                            have = true;
+                        }
+                        if (tl != ~0u) {
+                            double tc = tv[d] * sinb[bcol];
+                            if (tl & SRC_NEG) tc = -tc;
+                            c = have ? c + tc : tc;
+                        }
+                        if (norm) c = c * norm[bcol];
+                        s_rows[(size_t)e * B + bcol] = c;
+                        m = fabs(c);
+                    }
+                    m = wave_seg_max_nonneg_f64(m, rps);
+                    if (bcol == 0 && live && m < min_abs) s_keep[e] = 0u;
                 }
             }
         } else
@@ -1893,10 +1954,19 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
             if (s_rt[t] != ~0u) bf.rec[(size_t)roff + n + na + s_rt[t]] = at;
         }
         __syncthreads();
-        for (int e = wave; e < TB; e += NWAVES) {
-            if (!s_keep[e]) continue;
-            const uint64_t pos = s_excl + s_tl[e];
-            for (int b = lane; b < B; b += 64) out_coeffs[pos * B + b] = s_rows[(size_t)e * B + b];
+        if (NARROW) {
+            for (int k0 = 0; k0 < RPW; k0 += rps) {
+                const int k = k0 + sub;
+                const int e = wave + k * NWAVES;
+                if (k < RPW && col && s_keep[e])
+                    out_coeffs[(s_excl + s_tl[e]) * B + bcol] = s_rows[(size_t)e * B + bcol];
+            }
+        } else {
+            for (int e = wave; e < TB; e += NWAVES) {
+                if (!s_keep[e]) continue;
+                const uint64_t pos = s_excl + s_tl[e];
+                for (int b = lane; b < B; b += 64) out_coeffs[pos * B + b] = s_rows[(size_t)e * B + b];
+            }
         }
     }
 }

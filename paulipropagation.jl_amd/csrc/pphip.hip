@@ -621,6 +621,15 @@ void enq_perm(pp_handle* h, hipEvent_t e0, hipEvent_t e1, const Launch& L, bool 
     }
 }
 
+// The batched merge, with several rows per wave for B <= 32 (NARROW).
+template <int W, int TB, typename... Args>
+void launch_mb(pp_handle* h, hipEvent_t e0, hipEvent_t e1, unsigned grid, size_t shm, Args... args) {
+    if (h->B <= 32)
+        launch(h, e0, e1, k_mergeB<W, TB, true>, grid, shm, args...);
+    else
+        launch(h, e0, e1, k_mergeB<W, TB, false>, grid, shm, args...);
+}
+
 // The batched merge's partition (tiles of 64 rows or more; narrower tiles
 // search their own splits).
 // Sums of at least this many terms at the last synchronization take the
@@ -726,7 +735,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
         const bool emitB = chained && next_mask;
 #define LAUNCH_MB(TBV)                                                                                         \
     enq_partition_b<W, TBV>(h);                                                                                \
-    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+    launch_mb<W, TBV>(h, E(4, mrg), E(5, mrg), lb_grid(h, B <= 32 ? (const void*)k_mergeB<W, TBV, true> : (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 0, nullptr, part_b(h, TBV), emitB ? 1 : 0, emitB ? to_key<W>(next_mask) : mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
@@ -780,7 +789,7 @@ void enq_imag_rotation(pp_handle* h, const uint64_t* mask_in, double ch, double 
         const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     enq_partition_b<W, TBV>(h);                                                                                \
-    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+    launch_mb<W, TBV>(h, E(4, mrg), E(5, mrg), lb_grid(h, B <= 32 ? (const void*)k_mergeB<W, TBV, true> : (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, -1, 1.0, 1, norm, part_b(h, TBV), 0, mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
@@ -828,7 +837,7 @@ void enq_damping(pp_handle* h, int qubit, double gamma, double fxy, const double
         const Launch L3 = kt_launch(h, glog);
 #define LAUNCH_MB(TBV)                                                                                         \
     enq_partition_b<W, TBV>(h);                                                                                \
-    launch(h, E(4, mrg), E(5, mrg), k_mergeB<W, TBV>, lb_grid(h, (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
+    launch_mb<W, TBV>(h, E(4, mrg), E(5, mrg), lb_grid(h, B <= 32 ? (const void*)k_mergeB<W, TBV, true> : (const void*)k_mergeB<W, TBV>, shm, expect, TBV), \
            shm, h->bufs, h->d_ctl, L3, mask, B, d_trig, max_weight, min_abs, qubit, fxy, 0, nullptr, part_b(h, TBV), 0, mask)
         switch (TB) {
         case 256: LAUNCH_MB(256); break;
