@@ -2097,7 +2097,7 @@ __global__ __launch_bounds__(WG) void k_noise1(Bufs bf, Ctl* ctl, Launch L, int 
 // B columns: one wave per term row, 64 rows per tile
 constexpr int KNB_ROWS = 64;
 
-template <int W>
+template <int W, int RPS = 1>
 __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int B, int qubit, uint32_t damp_codes,
                                                const double* __restrict__ factor, int max_weight, double min_abs) {
     const KTimer ktimer(L.ktime);
@@ -2137,43 +2137,46 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         if (tile >= ntiles) break;
         const uint32_t base = tile * (uint32_t)KNB_ROWS;
         constexpr int RPW = KNB_ROWS / NWAVES;  // rows of a wave in a tile
-        constexpr int RG = 8;                    // rows loaded at once
-        double cvv[RPW];                         // B <= 64: the wave's rows, damped, kept for the stores
+        // B <= 64: one column per lane, RPS rows side by side in a wave
+        // (segments of 64 / RPS lanes; RPS > 1 only for B <= 64 / RPS); a wave
+        // loads the keys and coefficients of RG of its row slots at once,
+        // decides them, and keeps their damped values (cvv, one per slot) for
+        // the stores (same operations as the loop below)
+        constexpr int NSL = RPW / RPS;           // row slots of a wave
+        constexpr int RG = NSL < 8 ? NSL : 8;    // slots loaded at once
+        const int sub = RPS == 1 ? 0 : lane / (64 / RPS), bcol = lane - sub * (64 / RPS);
+        const bool col = bcol < B;
+        double cvv[NSL];
         if (B <= 64) {
-            // one column per lane; a wave loads the keys and coefficients of
-            // RG of its rows at once, decides them, and keeps their values
-            // for the stores (same operations as the loop below)
-            const int b = lane;
-            const bool col = b < B;
 #pragma unroll
-            for (int g0 = 0; g0 < RPW; g0 += RG) {
+            for (int g0 = 0; g0 < NSL; g0 += RG) {
                 Key<W> kk[RG];
                 double cc[RG];
 #pragma unroll
                 for (int d = 0; d < RG; ++d) {
-                    const uint32_t i = base + wave + (uint32_t)(g0 + d) * NWAVES;
+                    const int k = (g0 + d) * RPS + sub;
+                    const uint32_t i = base + wave + (uint32_t)k * NWAVES;
                     const uint32_t ic = min(i, n - 1);
                     kk[d] = load_key<W>(keys, stride, ic);
-                    cc[d] = col ? coeffs[(size_t)ic * B + b] : 0.0;
+                    cc[d] = col ? coeffs[(size_t)ic * B + bcol] : 0.0;
                 }
 #pragma unroll
                 for (int d = 0; d < RG; ++d) {
-                    const int r = wave + (g0 + d) * NWAVES;
+                    const int k = (g0 + d) * RPS + sub;
+                    const int r = wave + k * NWAVES;
                     const uint32_t i = base + r;
                     uint32_t keep = 0;
                     double c = cc[d];
-                    if (i < n) {
-                        const bool damped = (damp_codes >> code_at<W>(kk[d], qubit)) & 1u;
-                        double m = 0.0;
-                        if (col) {
-                            if (damped) c = c * factor[b];
-                            m = fabs(c);
-                        }
-                        m = wave_max_nonneg_f64(m);
-                        keep = !((max_weight >= 0 && weight<W>(kk[d]) > max_weight) || m < min_abs);
+                    double m = 0.0;
+                    const bool damped = (damp_codes >> code_at<W>(kk[d], qubit)) & 1u;
+                    if (i < n && col) {
+                        if (damped) c = c * factor[bcol];
+                        m = fabs(c);
                     }
+                    m = RPS == 1 ? wave_max_nonneg_f64(m) : wave_seg_max_nonneg_f64(m, RPS);
+                    if (i < n) keep = !((max_weight >= 0 && weight<W>(kk[d]) > max_weight) || m < min_abs);
                     cvv[g0 + d] = c;
-                    if (lane == 0) s_keep[r] = keep;
+                    if (bcol == 0) s_keep[r] = keep;
                 }
             }
         } else
@@ -2222,13 +2225,13 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
         if (B <= 64 && !bf.rec) {
             // the kept rows' values are in registers: keys copied, values stored
 #pragma unroll
-            for (int d = 0; d < RPW; ++d) {
-                const int r = wave + d * NWAVES;
+            for (int q = 0; q < NSL; ++q) {
+                const int r = wave + (q * RPS + sub) * NWAVES;
                 if (!s_keep[r]) continue;
                 const uint32_t i = base + r;
                 const uint64_t pos = s_excl + s_pos[r];
-                if (lane == 0) store_key<W>(out_keys, stride, pos, load_key<W>(keys, stride, i));
-                if (lane < B) out_coeffs[pos * B + lane] = cvv[d];
+                if (bcol == 0) store_key<W>(out_keys, stride, pos, load_key<W>(keys, stride, i));
+                if (col) out_coeffs[pos * B + bcol] = cvv[q];
             }
             continue;
         }

@@ -861,8 +861,14 @@ void enq_noise(pp_handle* h, int qubit, uint32_t damp_codes, double f1, const do
         launch(h, e0, e1, k_noise1<W>, lb_grid(h, (const void*)k_noise1<W>, 0, h->cap, KNCfg<W>::TILE), 0, h->bufs,
                h->d_ctl, kt_launch(h, glog), qubit, damp_codes, f1, max_weight, min_abs);
     } else {
-        launch(h, e0, e1, k_noiseB<W>, lb_grid(h, (const void*)k_noiseB<W>, 0, h->cap, KNB_ROWS), 0, h->bufs,
-               h->d_ctl, kt_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs);
+        // two or four rows side by side in a wave for narrow batches
+#define LAUNCH_NB(R)                                                                                          \
+    launch(h, e0, e1, k_noiseB<W, R>, lb_grid(h, (const void*)k_noiseB<W, R>, 0, h->cap, KNB_ROWS), 0, h->bufs, \
+           h->d_ctl, kt_launch(h, glog), h->B, qubit, damp_codes, d_factor, max_weight, min_abs)
+        if (h->B <= 16) LAUNCH_NB(4);
+        else if (h->B <= 32) LAUNCH_NB(2);
+        else LAUNCH_NB(1);
+#undef LAUNCH_NB
     }
     ktime_finish(h);
     h->seq += 1;
