@@ -315,7 +315,7 @@ __device__ __forceinline__ void tile_publish(const Launch& L, const Bufs& bf, ui
 }
 
 template <int NV>
-__device__ uint64_t tile_resolve(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile, uint64_t agg,
+__device__ __forceinline__ uint64_t tile_resolve(Ctl* ctl, const Launch& L, const Bufs& bf, uint32_t tile, uint64_t agg,
                                  uint64_t& incl, uint64_t* s_wacc) {
     const int t = threadIdx.x, lane = lane_id(), wave = t >> 6;
     if (L.dynamic) {
@@ -1591,6 +1591,303 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
         tile = nxt;
         s1a = s2a;
         s1b = s2b;
+    }
+}
+
+// The one-column merge with stores one tile behind (not recording). A tile's
+// kept outputs wait in a second shared-memory buffer while the workgroup
+// stages and merges its next tile and publishes that tile's counts; only then
+// does it read the prefix window of the waiting tile, issue the loads of the
+// tile after, and store the waiting tile. The window then holds a whole tile
+// of slack, so its polls -- issued ahead of the loads, whose data returns in
+// order -- find it complete, and the loads' latency overlaps the stores. Counts never depend on prefixes, so publishing a
+// tile before resolving the one before it adds no wait to any chain (static
+// windows and the dynamic two-level look-back alike). Outputs, the next
+// rotation's list and the partition samples are formed exactly as in k_merge1
+// and land at the same positions.
+template <int W, bool EMIT, int ITEMS>
+__global__ __launch_bounds__(WG, (ITEMS <= 2 ? 3 : (W == 4 && ITEMS == 4 ? 1 : 2))) void k_merge1d(Bufs bf, Ctl* ctl, Launch L, Key<W> mask,
+                                                                      RotParams rp, Key<W> nmask) {
+    const KTimer ktimer(L.ktime);
+    constexpr int TILE = WG * ITEMS;
+    constexpr bool REC = false;
+    static_assert(TILE < 32768, "kept positions carry a sign in bit 15; counts are packed in 16 bits");
+    const uint32_t pt = min((uint32_t)blockIdx.x, bf.lb_tiles - 2);
+    const uint32_t pre0 = bf.splits[pt], pre1 = bf.splits[pt + 1];
+    const CtlView c = load_ctl(ctl, L.seq & 1);
+    if (!lb_start(ctl, L, c)) return;
+    const int p = L.seq & 1;
+    const uint32_t na = c.s.na;
+    if (!(na || c.s.run)) return;
+    const uint32_t n = c.s.n, cur = c.s.cur;
+    const uint64_t* __restrict__ keys = bf.keys[cur];
+    const double* __restrict__ coeffs = bf.coeffs[cur];
+    uint64_t* __restrict__ out_keys = bf.keys[cur ^ 1];
+    double* __restrict__ out_coeffs = bf.coeffs[cur ^ 1];
+    const double inv = rp.norm ? *rp.norm : 1.0;
+    const size_t stride = bf.cap;
+    const uint32_t ntot = n + na;
+    const uint32_t ntiles = (ntot + TILE - 1) / TILE;
+    // staged inputs of the tile being merged
+    __shared__ uint64_t s_key[W][TILE + 2];
+    __shared__ double s_val[TILE + 2];
+    // kept outputs of the tile waiting for its prefix, in output order
+    __shared__ uint64_t o_key[W][TILE];
+    __shared__ double o_val[TILE];
+    __shared__ uint16_t o_tlp[EMIT ? TILE : 1];  // kept position of each touched output, bit 15: negative sign
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_w[NWAVES];
+    __shared__ uint64_t s_wacc[NWAVES];
+    __shared__ uint64_t s_excl, s_excl_t;
+    const int t = threadIdx.x;
+    uint32_t* __restrict__ next_tail = bf.tail[0];
+    auto bounds = [&](uint32_t tl, uint32_t a0, uint32_t a1, uint32_t& j0, int& ns, int& nt) {
+        const uint32_t d0 = tl * (uint32_t)TILE;
+        const uint32_t d1 = min(d0 + (uint32_t)TILE, ntot);
+        j0 = d0 - a0;
+        const uint32_t j1 = d1 - a1;
+        ns = (int)(a1 - a0);
+        nt = (int)(j1 - j0);
+        if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || a1 > n || j1 > na) {
+            if (t == 0) atomicOr(&ctl->err, 2u);
+            ns = 0;
+            nt = 0;
+        }
+    };
+    uint32_t tile;
+    if (L.dynamic) {
+        if (t == 0) s_tile = __hip_atomic_fetch_add(&ctl->counter[L.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        tile = s_tile;
+    } else {
+        tile = blockIdx.x;
+    }
+    if (tile >= ntiles) return;
+    uint32_t i0, j0;
+    int ns, nt;
+    {
+        const bool first = tile == pt && !L.dynamic;
+        i0 = first ? pre0 : bf.splits[tile];
+        const uint32_t i1 = first ? pre1 : bf.splits[tile + 1];
+        bounds(tile, i0, i1, j0, ns, nt);
+    }
+    MergeRegs<W, TILE, REC> F;
+    merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
+    const uint32_t G = gridDim.x;
+    uint64_t incl = 0;
+    uint32_t s1a = 0, s1b = 0;
+    if (!L.dynamic) {
+        const bool ok1 = tile + G < ntiles;
+        s1a = bf.splits[ok1 ? tile + G : 0u];
+        s1b = bf.splits[ok1 ? tile + G + 1 : 0u];
+    }
+    // the tile whose outputs wait in o_key / o_val, and its packed counts
+    bool waiting = false;
+    uint32_t wtile = 0;
+    uint64_t waggp = 0;
+    // the waiting tile's prefix (its window is almost always complete by now:
+    // the polls go out before the next tile's loads, so they do not wait
+    // behind them)
+    auto resolve = [&]() {
+        const uint64_t ex = tile_resolve<EMIT ? 2 : 1>(ctl, L, bf, wtile, waggp, incl, s_wacc);
+        const uint32_t nkept = (uint32_t)(waggp & 0xffffffffull), ntouch = (uint32_t)(waggp >> 32);
+        const uint64_t ek = ex & 0xffffffffull, et = ex >> 32;
+        if (t == 0) {
+            s_excl = ek;
+            s_excl_t = et;
+            if (wtile == ntiles - 1) {
+                const uint32_t nout = (uint32_t)(ek + nkept);
+                set_state(ctl, p ^ 1, nout, cur ^ 1);
+                if (EMIT) {
+                    ctl->st[p ^ 1].na = (uint32_t)(et + ntouch);
+#pragma unroll
+                    for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                    ctl->tail_seq = L.seq + 1;
+                    ctl->tail_ready = 1u;
+                }
+                ctl->ssamp_for = L.seq + 2u;
+                if (L.glog) L.glog[2] = nout;
+            }
+        }
+        __syncthreads();
+    };
+    // stores the waiting tile's outputs at that prefix
+    auto store = [&]() {
+        const uint32_t nkept = (uint32_t)(waggp & 0xffffffffull), ntouch = (uint32_t)(waggp >> 32);
+        const uint32_t obase = (uint32_t)s_excl, tbase = EMIT ? (uint32_t)s_excl_t : 0u;
+        // every 64th output is also the next gate's partition sample
+#pragma unroll 1
+        for (uint32_t j = t; j < nkept; j += WG) {
+            const uint32_t o = obase + j;
+#pragma unroll
+            for (int l = 0; l < W; ++l) out_keys[l * stride + o] = o_key[l][j];
+            out_coeffs[o] = o_val[j];
+            if ((o & 63u) == 0) {
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.ssamp[(size_t)(o >> 6) * W + l] = o_key[l][j];
+            }
+        }
+        if (EMIT) {
+            // the next rotation's list: positions, and the terms themselves
+#pragma unroll 1
+            for (uint32_t j = t; j < ntouch; j += WG) {
+                const uint32_t e = o_tlp[j];
+                const uint32_t lpj = e & 0x7fffu;
+                next_tail[tbase + j] = (obase + lpj) | ((e & 0x8000u) ? SRC_NEG : 0u);
+#pragma unroll
+                for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + tbase + j] = o_key[l][lpj];
+                bf.tl_coef[tbase + j] = o_val[lpj];
+            }
+        }
+    };
+    // each pass merges a tile (until done) and then stores the tile before it
+    bool done = false;
+    while (true) {
+        int slot[ITEMS];
+        double oc[ITEMS];
+        uint32_t keep = 0, touch = 0, neg = 0, pre = 0, nxt = 0, s2a = 0, s2b = 0, ni0 = 0, ni1 = 0, ci0 = 0, cj0 = 0;
+        int cns = 0, cnt = 0;
+        uint64_t aggp = 0;
+        bool more = false;
+        if (!done) {
+            __syncthreads();  // the previous tile's staged inputs are free again
+            if (L.dynamic) {
+                if (t == 0)
+                    s_tile = __hip_atomic_fetch_add(&ctl->counter[L.slot], 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                const bool ok2 = tile + 2 * G < ntiles;
+                s2a = bf.splits[ok2 ? tile + 2 * G : 0u];
+                s2b = bf.splits[ok2 ? tile + 2 * G + 1 : 0u];
+            }
+            merge_put<W, TILE, REC>(F, na, i0, j0, ns, nt, mask, rp.cosv, rp.damp_q, rp.fxy, rp.comm, s_key, s_val,
+                                    nullptr);
+            __syncthreads();
+            nxt = L.dynamic ? s_tile : tile + G;
+            more = nxt < ntiles;
+            ni0 = s1a;
+            ni1 = s1b;
+            if (L.dynamic) {
+                ni0 = bf.splits[more ? nxt : 0u];
+                ni1 = bf.splits[more ? nxt + 1 : 0u];
+            }
+            // this tile's bounds; the next tile's loads go out at once (always,
+            // from clamped addresses after the last tile), so they are in
+            // flight through this tile's merge, publish and stores
+            ci0 = i0;
+            cj0 = j0;
+            cns = ns;
+            cnt = nt;
+            if (more) {
+                i0 = ni0;
+                bounds(nxt, ni0, ni1, j0, ns, nt);
+            } else {
+                i0 = 0;
+                j0 = 0;
+                ns = 0;
+                nt = 0;
+            }
+            merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
+            auto S = [&](int x) {
+                Key<W> k;
+#pragma unroll
+                for (int l = 0; l < W; ++l) k.l[l] = s_key[l][1 + x];
+                return k;
+            };
+            auto T = [&](int y) {
+                Key<W> k;
+#pragma unroll
+                for (int l = 0; l < W; ++l) k.l[l] = s_key[l][cns + 1 + y];
+                return k;
+            };
+            const int dl = t * ITEMS;
+            const int dtot = cns + cnt;
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                slot[k] = 0;
+                oc[k] = 0.0;
+            }
+            if (dl < dtot) {
+                int lo = dl > cnt ? dl - cnt : 0, hi = min(dl, cns);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (key_le<W>(S(mid), T(dl - 1 - mid))) lo = mid + 1;
+                    else hi = mid;
+                }
+                int x = lo, y = dl - lo;
+#pragma unroll
+                for (int k = 0; k < ITEMS; ++k) {
+                    if (x + y >= dtot) continue;
+                    const bool hs = x < cns, ht = y < cnt;
+                    const Key<W> ks = S(hs ? x : 0), kt = T(y);
+                    const bool take_s = hs && (!ht || key_le<W>(ks, kt));
+                    bool live;
+                    Key<W> ko;
+                    if (take_s) {
+                        double cv = s_val[1 + x];
+                        const bool merged = cj0 + y < na && key_eq<W>(kt, ks);
+                        if (merged) cv = cv + s_val[cns + 1 + y];
+                        slot[k] = 1 + x;
+                        oc[k] = cv;
+                        ko = ks;
+                        live = true;
+                        ++x;
+                    } else {
+                        live = !((ci0 + x > 0) && key_eq<W>(S(x - 1), kt));
+                        slot[k] = cns + 1 + y;
+                        oc[k] = s_val[cns + 1 + y];
+                        ko = kt;
+                        ++y;
+                    }
+                    if (rp.norm) oc[k] = oc[k] * inv;
+                    if (live &&
+                        !((rp.max_weight >= 0 && weight<W>(ko) > rp.max_weight) || fabs(oc[k]) < rp.min_abs)) {
+                        keep |= 1u << k;
+                        if (EMIT && anticommutes<W>(nmask, ko)) {
+                            touch |= 1u << k;
+                            if ((sign_exponent<W>(nmask, ko) & 2) == 0) neg |= 1u << k;
+                        }
+                    }
+                }
+            }
+            uint32_t agg;
+            const uint32_t mine = (uint32_t)__popc(keep) | ((uint32_t)__popc(touch) << 16);
+            static_assert(ITEMS < 8, "per-thread counts fit 3 bits");
+            pre = block_excl_sum_small<3>(mine, s_w, agg);
+            aggp = (uint64_t)(agg & 0xffffu) | ((uint64_t)(agg >> 16) << 32);
+            tile_publish<EMIT ? 2 : 1>(L, bf, tile, aggp);
+        }
+        if (waiting) resolve();
+        if (waiting) store();
+        if (done) break;
+        __syncthreads();  // the waiting tile's buffer is free again
+        {
+            uint32_t lp = pre & 0xffffu, lt = pre >> 16;
+#pragma unroll
+            for (int k = 0; k < ITEMS; ++k) {
+                if ((keep >> k) & 1u) {
+#pragma unroll
+                    for (int l = 0; l < W; ++l) o_key[l][lp] = s_key[l][slot[k]];
+                    o_val[lp] = oc[k];
+                    if (EMIT && ((touch >> k) & 1u)) {
+                        o_tlp[lt] = (uint16_t)(lp | (((neg >> k) & 1u) ? 0x8000u : 0u));
+                        ++lt;
+                    }
+                    ++lp;
+                }
+            }
+        }
+        waiting = true;
+        wtile = tile;
+        waggp = aggp;
+        if (more) {
+            tile = nxt;
+            s1a = s2a;
+            s1b = s2b;
+        } else {
+            done = true;
+            __syncthreads();  // the last tile's outputs are in o_key / o_val
+        }
     }
 }
 

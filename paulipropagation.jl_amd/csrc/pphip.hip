@@ -84,6 +84,7 @@ int scratch_setup(int dev, double** out);
 // choice gives bit-identical results.
 struct Tuning {
     int merge_items = 0;   // one-column merge tile = 256 x this (2, 3, 4); 0: 3 for 256-bit keys, else 4
+    int merge_defer = 0;   // one-column merge (not recording) stores each tile one tile behind (k_merge1d)
     int perm_items = 2;    // tail permutation: list entries per thread (1, 2, 4)
     int batch_rows = 0;    // batched merge: term rows per tile (16 .. 256); 0: by B
     int radix_bits = 8;    // Clifford / load-merge radix sort: digit width (4 .. 8)
@@ -666,6 +667,12 @@ void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, in
     if (variant == 2)
         launch(h, e4, e5, k_merge1<W, false, true, MI>, lb_grid(h, (const void*)k_merge1<W, false, true, MI>, 0, expect, TILE),
                0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
+    else if (h->tun.merge_defer && variant == 1)
+        launch(h, e4, e5, k_merge1d<W, true, MI>, lb_grid(h, (const void*)k_merge1d<W, true, MI>, 0, expect, TILE), 0,
+               h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
+    else if (h->tun.merge_defer)
+        launch(h, e4, e5, k_merge1d<W, false, MI>, lb_grid(h, (const void*)k_merge1d<W, false, MI>, 0, expect, TILE), 0,
+               h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
     else if (variant == 1)
         launch(h, e4, e5, k_merge1<W, true, false, MI>, lb_grid(h, (const void*)k_merge1<W, true, false, MI>, 0, expect, TILE),
                0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
@@ -3004,6 +3011,10 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
             return fail(PP_ERR_ARG, "merge items must be 2, 3 or 4 (0: the key width's default)");
         h->tun.merge_items = (int)value;
         return PP_OK;
+    case PP_TUNE_MERGE_DEFER:
+        if (value != 0 && value != 1) return fail(PP_ERR_ARG, "merge defer must be 0 or 1");
+        h->tun.merge_defer = (int)value;
+        return PP_OK;
     case PP_TUNE_PERM_ITEMS:
         if (value != 1 && value != 2 && value != 4) return fail(PP_ERR_ARG, "permutation items must be 1, 2 or 4");
         h->tun.perm_items = (int)value;
@@ -3085,7 +3096,8 @@ int pp_load_tuning(pp_handle* h, const char* path) {
     static const struct {
         const char* name;
         int key;
-    } names[] = {{"merge_items", PP_TUNE_MERGE_ITEMS}, {"perm_items", PP_TUNE_PERM_ITEMS},
+    } names[] = {{"merge_items", PP_TUNE_MERGE_ITEMS}, {"merge_defer", PP_TUNE_MERGE_DEFER},
+                 {"perm_items", PP_TUNE_PERM_ITEMS},
                  {"batch_rows", PP_TUNE_BATCH_ROWS},   {"radix_bits", PP_TUNE_RADIX_BITS},
                  {"grid_per_cu", PP_TUNE_GRID_PER_CU}, {"run_chunk", PP_TUNE_RUN_CHUNK}};
     for (const Entry& e : es) {

@@ -13,7 +13,8 @@ writes the table that pp_create reads (pp_load_tuning, include/pphip.h):
     <key_words> <max_batch> <name> <value>
 
 Launch shapes: merge_items (the one-column merge's tile, 256 x 2..4
-diagonals, with its partition), perm_items (list entries per thread of the
+diagonals, with its partition), merge_defer (that merge storing each tile
+one tile behind), perm_items (list entries per thread of the
 tail permutation), batch_rows (term rows per tile of the batched merge),
 radix_bits (digit width of the Clifford re-sort), grid_per_cu (workgroups per
 compute unit of the persistent grids), run_chunk (gates enqueued between
@@ -37,6 +38,7 @@ from .paulis import PauliSum
 
 CANDIDATES = {
     "merge_items": [2, 3, 4],
+    "merge_defer": [0, 1],
     "perm_items": [1, 2, 4],
     "grid_per_cu": [2, 4, 6, 8],
     "run_chunk": [64, 128, 256, 512],
@@ -44,7 +46,7 @@ CANDIDATES = {
     "radix_bits": [4, 5, 6, 7, 8],
 }
 # the library's compiled defaults (0: chosen by key width or batch)
-DEFAULTS = {"merge_items": 0, "perm_items": 2, "batch_rows": 0, "radix_bits": 8, "grid_per_cu": 4, "run_chunk": 128}
+DEFAULTS = {"merge_items": 0, "merge_defer": 0, "perm_items": 2, "batch_rows": 0, "radix_bits": 8, "grid_per_cu": 4, "run_chunk": 128}
 
 
 class Case:
@@ -93,14 +95,14 @@ def _first_step_b64():
 # (key words, largest batch of the class, [(launch shape, workload)]); shapes a
 # class does not use (the batched merge's rows at B = 1, the one-column merge's
 # tile at B > 1) are left out
+ONE_COLUMN = ("merge_items", "merge_defer", "perm_items", "grid_per_cu", "run_chunk")
+BATCHED = ("batch_rows", "perm_items", "grid_per_cu", "run_chunk")
 CLASSES = [
-    (4, 1, [(n, workloads.kicked_ising_heavyhex_127q) for n in ("merge_items", "perm_items", "grid_per_cu", "run_chunk")]),
-    (2, 1, [(n, workloads.ansatz_line_48q) for n in ("merge_items", "perm_items", "grid_per_cu", "run_chunk")]
-     + [("radix_bits", lambda: clifford_ansatz(48))]),
-    (1, 1, [(n, workloads.tfim_brick_32q) for n in ("merge_items", "perm_items", "grid_per_cu", "run_chunk")]
-     + [("radix_bits", lambda: clifford_ansatz(24))]),
-    (2, 32, [(n, workloads.ansatz_line_noisy_48q_b32) for n in ("batch_rows", "perm_items", "grid_per_cu", "run_chunk")]),
-    (2, 64, [(n, _first_step_b64) for n in ("batch_rows", "perm_items", "grid_per_cu", "run_chunk")]),
+    (4, 1, [(n, workloads.kicked_ising_heavyhex_127q) for n in ONE_COLUMN]),
+    (2, 1, [(n, workloads.ansatz_line_48q) for n in ONE_COLUMN] + [("radix_bits", lambda: clifford_ansatz(48))]),
+    (1, 1, [(n, workloads.tfim_brick_32q) for n in ONE_COLUMN] + [("radix_bits", lambda: clifford_ansatz(24))]),
+    (2, 32, [(n, workloads.ansatz_line_noisy_48q_b32) for n in BATCHED]),
+    (2, 64, [(n, _first_step_b64) for n in BATCHED]),
 ]
 
 

@@ -117,7 +117,7 @@ def test_batch_rows_bitwise(B, rows, noise):
     assert_bitwise(*got, *_oracle(f"heis{B}{noise}", wl), f"B={B} rows={rows} noise={noise}")
 
 
-@pytest.mark.parametrize("shape,value", [("merge_items", 1), ("merge_items", 5), ("perm_items", 3),
+@pytest.mark.parametrize("shape,value", [("merge_items", 1), ("merge_items", 5), ("merge_defer", 2), ("perm_items", 3),
                                          ("batch_rows", 8), ("batch_rows", 48), ("batch_rows", 512),
                                          ("radix_bits", 3), ("radix_bits", 9), ("grid_per_cu", 0),
                                          ("grid_per_cu", 9), ("run_chunk", 8), ("run_chunk", 2048)])
@@ -195,3 +195,35 @@ def test_installed_table_loads_for_every_class():
         pytest.skip("no tuning table installed beside the library")
     for nq, B in [(30, 1), (60, 1), (120, 1), (60, 32), (60, 64), (60, 2), (60, 100)]:
         HIPPauliSum(nq, batch=B).load_tuning(path)
+
+
+@pytest.mark.parametrize("name", list(_WL))
+@pytest.mark.parametrize("defer,items", [(0, 2), (0, 3), (1, 2), (1, 3), (1, 4)])
+def test_deferred_merge_stores_bitwise(name, defer, items):
+    # the one-column merge with each tile's stores one tile behind (k_merge1d)
+    # and without, at every tile width
+    print(f"{name} merge_defer={defer} merge_items={items}", flush=True)
+    wl = _WL[name]()
+    got = _device(wl, merge_defer=defer, merge_items=items)
+    assert_bitwise(*got, *_oracle(name, wl), f"{name} merge_defer={defer} merge_items={items}")
+
+
+@pytest.mark.parametrize("defer", [0, 1])
+@pytest.mark.parametrize("mode", ["forced_rerun", "dynamic_tiles", "unchained"])
+def test_deferred_merge_tile_modes_bitwise(defer, mode):
+    # atomic tile claims (every gate rerun, or always) and merges that list
+    # nothing for the next rotation, with stores one tile behind and without
+    print(f"merge_defer={defer} {mode}", flush=True)
+    wl = _kicked127(4)
+    opts = {"forced_rerun": {"force_rerun": True}, "dynamic_tiles": {"dynamic_tiles": True},
+            "unchained": {"chain_tails": False}}[mode]
+    B = 1
+    pc = PackedCircuit(wl["circuit"], wl["thetas"], wl["nqubits"], batch=B)
+    s = HIPPauliSum(wl["nqubits"], batch=B, capacity=1024)
+    s.tune(lds_runs=False, merge_defer=defer, merge_items=2, **opts)
+    k, c = wl["obs"].to_arrays(pc.W, B)
+    s.upload(k, c)
+    pc.run(s, max_weight=wl["max_weight"], min_abs_coeff=wl["min_abs_coeff"])
+    assert_bitwise(*s.download(), *_oracle("kicked127_4", wl), f"merge_defer={defer} {mode}")
+    if mode == "forced_rerun":
+        assert s.profile()["fallbacks"] > 0, "no gate was sent to the atomic-claim rerun"
