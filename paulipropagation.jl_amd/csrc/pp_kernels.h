@@ -1923,11 +1923,14 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     __shared__ uint64_t s_key[W][TB + 2];
     __shared__ uint32_t s_src[TB + 1];
     __shared__ int s_head[TB];          // head index, -(index+2) when scaled by cos, -1 none, INT_MIN dead
-    __shared__ uint32_t s_tl[TB];       // tail src or ~0u; reused for the output offset
-    __shared__ uint32_t s_keep[TB];
+    // per tile (two, by tile parity: a tile's rows are stored after the next
+    // tile's merge paths have written its own): tail src or ~0u, reused for
+    // the output offset; whether the row is kept
+    __shared__ uint32_t s_tlb[2][TB];
+    __shared__ uint32_t s_keepb[2][TB];
     __shared__ uint32_t s_pj[TB + 1];    // recording: parent-order index of tail slot y
-    __shared__ uint32_t s_rh[TB];        // recording: head index | REC_FLAG if touched, or ~0u
-    __shared__ uint32_t s_rt[TB];        // recording: parent-order index of the consumed tail, or ~0u
+    __shared__ uint32_t s_rhb[2][TB];    // recording: head index | REC_FLAG if touched, or ~0u
+    __shared__ uint32_t s_rtb[2][TB];    // recording: parent-order index of the consumed tail, or ~0u
     __shared__ uint8_t s_hxy[TB];        // amplitude damping: head has X or Y on the qubit
     __shared__ uint32_t s_split[2];
     __shared__ uint32_t s_tile;
@@ -1945,126 +1948,204 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     TileSource ts;
     uint64_t lb_incl = 0;  // static tiles: this workgroup's inclusive prefix
     __shared__ uint64_t s_wacc[NWAVES];
+    constexpr int RPW = TB / NWAVES;  // rows of a wave in a tile
+    // (NARROW: B <= 32, several rows per wave; otherwise one row per wave)
+    const int rps = !NARROW ? 1 : (B <= 16 ? 4 : 2);
+    const int sub = lane / (64 / rps), bcol = lane - sub * (64 / rps);
+    const bool col = bcol < B;
+    // Tiles are software-pipelined by one: a workgroup merges tile k's paths
+    // (A), then stores tile k-1, whose counts it published before (C), then
+    // forms tile k's rows and publishes its counts (B). The wait for tile
+    // k-1's prefix thereby overlaps tile k's key loads and merge paths.
+    // Each pass runs the phases once, so each is inlined once.
+    uint32_t tile = 0, ptile = 0;  // tile k, tile k-1
+    bool cur_ok = false, prev_ok = false, fin = false;
+    int pb = 0;                    // tile k's metadata buffer
+    Key<W> okey, pokey;            // the output term of this thread's row, tiles k and k-1
+    uint32_t pmine = 0, ppre = 0, ppre_t = 0;
+    bool ptouch = false;
+    uint64_t paggp = 0;
     while (true) {
-        const uint32_t tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
-        if (tile >= ntiles) break;
-        const uint32_t d0 = tile * (uint32_t)TB;
-        const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
-        uint32_t i0, i1;
-        if (PART) {
-            // splits from the partition kernel: this tile's were loaded during
-            // the previous tile (static tiles), the next tile's go out now
-            if (L.dynamic || !have_next) {
-                i0 = bf.splits[tile];
-                i1 = bf.splits[tile + 1];
+        // ---- A: tile k's merge paths
+        if (!fin) {
+            tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
+            cur_ok = tile < ntiles;
+            fin = !cur_ok;
+        }
+        if (cur_ok) {
+            const uint32_t d0 = tile * (uint32_t)TB;
+            const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
+            uint32_t i0, i1;
+            if (PART) {
+                // splits from the partition kernel: this tile's were loaded during
+                // the previous tile (static tiles), the next tile's go out now
+                if (L.dynamic || !have_next) {
+                    i0 = bf.splits[tile];
+                    i1 = bf.splits[tile + 1];
+                } else {
+                    i0 = nx0;
+                    i1 = nx1;
+                }
+                const uint32_t nt2 = tile + gridDim.x;
+                have_next = !L.dynamic && nt2 < ntiles;
+                if (have_next) {
+                    nx0 = bf.splits[nt2];
+                    nx1 = bf.splits[nt2 + 1];
+                }
             } else {
-                i0 = nx0;
-                i1 = nx1;
+                if (wave < 2) {
+                    const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
+                    if (lane == 0) s_split[wave] = s;
+                }
+                __syncthreads();
+                i0 = s_split[0];
+                i1 = s_split[1];
             }
-            const uint32_t nt2 = tile + gridDim.x;
-            have_next = !L.dynamic && nt2 < ntiles;
-            if (have_next) {
-                nx0 = bf.splits[nt2];
-                nx1 = bf.splits[nt2 + 1];
+            const uint32_t j0 = d0 - i0, j1 = d1 - i1;
+            int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
+            if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
+                if (t == 0) atomicOr(&ctl->err, 2u);
+                ns = 0;
+                nt = 0;
             }
-        } else {
-            if (wave < 2) {
-                const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
-                if (lane == 0) s_split[wave] = s;
+            merge_load<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src, bf.tpj, s_pj);
+            __syncthreads();
+            auto S = [&](int x) {
+                Key<W> k;
+#pragma unroll
+                for (int l = 0; l < W; ++l) k.l[l] = s_key[l][1 + x];
+                return k;
+            };
+            auto T = [&](int y) {
+                Key<W> k;
+#pragma unroll
+                for (int l = 0; l < W; ++l) k.l[l] = s_key[l][ns + 1 + y];
+                return k;
+            };
+            const int dtot = ns + nt;
+            if (t < dtot) {
+                int lo = t > nt ? t - nt : 0, hi = min(t, ns);
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (key_le<W>(S(mid), T(t - 1 - mid))) lo = mid + 1;
+                    else hi = mid;
+                }
+                const int x = lo, y = t - lo;
+                bool take_s;
+                if (x >= ns) take_s = false;
+                else if (y >= nt) take_s = true;
+                else take_s = key_le<W>(S(x), T(y));
+                int head = -1;
+                uint32_t tl = ~0u;
+                bool live = false;
+                uint32_t rh = ~0u, rt = ~0u;
+                bool xy = false;
+                if (take_s) {
+                    okey = S(x);
+                    head = (int)(i0 + x);
+                    rh = (uint32_t)head;
+                    if (j0 + y < na && key_eq<W>(T(y), okey)) {
+                        tl = s_src[y];
+                        rt = s_pj[y];
+                    }
+                    live = true;
+                    if (damp_q >= 0) {
+                        const uint32_t code = code_at<W>(okey, damp_q);
+                        if (code == 3u) head = -(head + 2);
+                        xy = code == 1u || code == 2u;
+                    } else if (anticommutes<W>(mask, okey) != (comm != 0)) {
+                        head = -(head + 2);
+                        rh |= REC_FLAG;
+                    }
+                } else {
+                    okey = T(y);
+                    const bool dup = (i0 + x > 0) && key_eq<W>(S(x - 1), okey);
+                    if (!dup) {
+                        tl = s_src[y];
+                        rt = s_pj[y];
+                        live = true;
+                    }
+                }
+                s_rhb[pb][t] = rh;
+                s_rtb[pb][t] = rt;
+                s_hxy[t] = xy ? 1 : 0;
+                if (live && max_weight >= 0 && weight<W>(okey) > max_weight) live = false;
+                s_head[t] = live ? head : INT_MIN;
+                s_tlb[pb][t] = live ? tl : ~0u;
+                s_keepb[pb][t] = live ? 1u : 0u;
+            } else if (t < TB) {
+                s_keepb[pb][t] = 0u;
+                s_head[t] = INT_MIN;
+                s_rhb[pb][t] = ~0u;
+                s_rtb[pb][t] = ~0u;
             }
             __syncthreads();
-            i0 = s_split[0];
-            i1 = s_split[1];
         }
-        const uint32_t j0 = d0 - i0, j1 = d1 - i1;
-        int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
-        if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
-            if (t == 0) atomicOr(&ctl->err, 2u);
-            ns = 0;
-            nt = 0;
-        }
-        merge_load<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src, bf.tpj, s_pj);
-        __syncthreads();
-        auto S = [&](int x) {
-            Key<W> k;
+        // ---- C: tile k-1's prefix and stores
+        if (prev_ok) {
+            const int qb = pb ^ 1;
+            {
+                const uint64_t ex = emit ? tile_resolve<2>(ctl, L, bf, ptile, paggp, lb_incl, s_wacc)
+                                         : tile_resolve<1>(ctl, L, bf, ptile, paggp, lb_incl, s_wacc);
+                const uint64_t excl = ex & 0xffffffffull, excl_t = ex >> 32;
+                if (t == 0) {
+                    s_excl = excl;
+                    s_excl_t = excl_t;
+                    if (ptile == ntiles - 1) {
+                        const uint32_t nout = (uint32_t)(excl + (paggp & 0xffffffffull));
+                        set_state(ctl, p ^ 1, nout, cur ^ 1);
+                        if (emit) {
+                            ctl->st[p ^ 1].na = (uint32_t)(excl_t + (paggp >> 32));
 #pragma unroll
-            for (int l = 0; l < W; ++l) k.l[l] = s_key[l][1 + x];
-            return k;
-        };
-        auto T = [&](int y) {
-            Key<W> k;
-#pragma unroll
-            for (int l = 0; l < W; ++l) k.l[l] = s_key[l][ns + 1 + y];
-            return k;
-        };
-        const int dtot = ns + nt;
-        Key<W> okey;
-        if (t < dtot) {
-            int lo = t > nt ? t - nt : 0, hi = min(t, ns);
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (key_le<W>(S(mid), T(t - 1 - mid))) lo = mid + 1;
-                else hi = mid;
-            }
-            const int x = lo, y = t - lo;
-            bool take_s;
-            if (x >= ns) take_s = false;
-            else if (y >= nt) take_s = true;
-            else take_s = key_le<W>(S(x), T(y));
-            int head = -1;
-            uint32_t tl = ~0u;
-            bool live = false;
-            uint32_t rh = ~0u, rt = ~0u;
-            bool xy = false;
-            if (take_s) {
-                okey = S(x);
-                head = (int)(i0 + x);
-                rh = (uint32_t)head;
-                if (j0 + y < na && key_eq<W>(T(y), okey)) {
-                    tl = s_src[y];
-                    rt = s_pj[y];
+                            for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
+                            ctl->tail_seq = L.seq + 1;
+                            ctl->tail_ready = 1u;
+                        }
+                        if (L.glog) L.glog[2] = nout;
+                    }
                 }
-                live = true;
-                if (damp_q >= 0) {
-                    const uint32_t code = code_at<W>(okey, damp_q);
-                    if (code == 3u) head = -(head + 2);
-                    xy = code == 1u || code == 2u;
-                } else if (anticommutes<W>(mask, okey) != (comm != 0)) {
-                    head = -(head + 2);
-                    rh |= REC_FLAG;
+            }
+            __syncthreads();
+            if (t < TB && pmine) {
+                store_key<W>(out_keys, stride, s_excl + ppre, pokey);
+                s_tlb[qb][t] = ppre;
+                if (ptouch) {
+                    const uint64_t j = s_excl_t + ppre_t;
+                    const bool neg = (sign_exponent<W>(nmask, pokey) & 2) == 0;
+                    next_tail[j] = (uint32_t)(s_excl + ppre) | (neg ? SRC_NEG : 0u);
+#pragma unroll
+                    for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + j] = pokey.l[l];
+                }
+            }
+            if (bf.rec && t < TB) {
+                const uint32_t at = pmine ? (uint32_t)(s_excl + ppre) : REC_NONE;
+                const uint32_t roff = c.s.roff;
+                if (s_rhb[qb][t] != ~0u) bf.rec[(size_t)roff + (s_rhb[qb][t] & SRC_MASK)] = (s_rhb[qb][t] & REC_FLAG) | at;
+                if (s_rtb[qb][t] != ~0u) bf.rec[(size_t)roff + n + na + s_rtb[qb][t]] = at;
+            }
+            __syncthreads();
+            if (NARROW) {
+                for (int k0 = 0; k0 < RPW; k0 += rps) {
+                    const int k = k0 + sub;
+                    const int e = wave + k * NWAVES;
+                    if (k < RPW && col && s_keepb[qb][e])
+                        out_coeffs[(s_excl + s_tlb[qb][e]) * B + bcol] = s_rows[(size_t)e * B + bcol];
                 }
             } else {
-                okey = T(y);
-                const bool dup = (i0 + x > 0) && key_eq<W>(S(x - 1), okey);
-                if (!dup) {
-                    tl = s_src[y];
-                    rt = s_pj[y];
-                    live = true;
+                for (int e = wave; e < TB; e += NWAVES) {
+                    if (!s_keepb[qb][e]) continue;
+                    const uint64_t pos = s_excl + s_tlb[qb][e];
+                    for (int b = lane; b < B; b += 64) out_coeffs[pos * B + b] = s_rows[(size_t)e * B + b];
                 }
             }
-            s_rh[t] = rh;
-            s_rt[t] = rt;
-            s_hxy[t] = xy ? 1 : 0;
-            if (live && max_weight >= 0 && weight<W>(okey) > max_weight) live = false;
-            s_head[t] = live ? head : INT_MIN;
-            s_tl[t] = live ? tl : ~0u;
-            s_keep[t] = live ? 1u : 0u;
-        } else if (t < TB) {
-            s_keep[t] = 0u;
-            s_head[t] = INT_MIN;
-            s_rh[t] = ~0u;
-            s_rt[t] = ~0u;
+            __syncthreads();  // tile k-1's rows are out: the row buffer is free
         }
-        __syncthreads();
+        if (!cur_ok) break;
+        // ---- B: tile k's rows, counts and publication
         // B <= 64: one column per lane, rps rows of the batch side by side in a
         // wave (segments of 64 / rps lanes); a wave loads the coefficients of
         // RD slots of its rows at once and then finishes them (same values
         // and operations as the loop for larger B below)
-        constexpr int RPW = TB / NWAVES;  // rows of a wave in a tile
-        // (NARROW: B <= 32, several rows per wave; otherwise one row per wave)
-        const int rps = !NARROW ? 1 : (B <= 16 ? 4 : 2);
-        const int sub = lane / (64 / rps), bcol = lane - sub * (64 / rps);
-        const bool col = bcol < B;
         if (!NARROW && B <= 64) {
             // one column per lane; a wave loads the coefficients of RD of its
             // rows at once and then finishes them, so RD rows' loads are in
@@ -2079,9 +2160,9 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     const int e = e0 + d * NWAVES;
                     hv[d] = 0.0;
                     tv[d] = 0.0;
-                    if (e < TB && s_keep[e] && col) {
+                    if (e < TB && s_keepb[pb][e] && col) {
                         const int head = s_head[e];
-                        const uint32_t tl = s_tl[e];
+                        const uint32_t tl = s_tlb[pb][e];
                         if (head != -1) hv[d] = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + b];
                         if (tl != ~0u) tv[d] = coeffs[(size_t)(tl & SRC_MASK) * B + b];
                     }
@@ -2089,9 +2170,9 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
 #pragma unroll
                 for (int d = 0; d < RD; ++d) {
                     const int e = e0 + d * NWAVES;
-                    if (e >= TB || !s_keep[e]) continue;
+                    if (e >= TB || !s_keepb[pb][e]) continue;
                     const int head = s_head[e];
-                    const uint32_t tl = s_tl[e];
+                    const uint32_t tl = s_tlb[pb][e];
                     double m = 0.0;
                     if (col) {
                         double c = 0.0;
@@ -2115,7 +2196,7 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                         m = fabs(c);
                     }
                     m = wave_max_nonneg_f64(m);
-                    if (lane == 0 && m < min_abs) s_keep[e] = 0u;
+                    if (lane == 0 && m < min_abs) s_keepb[pb][e] = 0u;
                 }
             }
         } else if (NARROW) {
@@ -2129,9 +2210,9 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     const int e = wave + k * NWAVES;
                     hv[d] = 0.0;
                     tv[d] = 0.0;
-                    if (k < RPW && s_keep[e] && col) {
+                    if (k < RPW && s_keepb[pb][e] && col) {
                         const int head = s_head[e];
-                        const uint32_t tl = s_tl[e];
+                        const uint32_t tl = s_tlb[pb][e];
                         if (head != -1) hv[d] = coeffs[(size_t)(head >= 0 ? head : -head - 2) * B + bcol];
                         if (tl != ~0u) tv[d] = coeffs[(size_t)(tl & SRC_MASK) * B + bcol];
                     }
@@ -2141,11 +2222,11 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                     if (q0 + d >= nslots) break;  // uniform: every lane's slot count is the same
                     const int k = (q0 + d) * rps + sub;
                     const int e = wave + k * NWAVES;
-                    const bool live = k < RPW && s_keep[e];
+                    const bool live = k < RPW && s_keepb[pb][e];
                     double m = 0.0;
                     if (live && col) {
                         const int head = s_head[e];
-                        const uint32_t tl = s_tl[e];
+                        const uint32_t tl = s_tlb[pb][e];
                         double c = 0.0;
                         bool have = false;
                         if (head != -1) {
@@ -2167,14 +2248,14 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                         m = fabs(c);
                     }
                     m = wave_seg_max_nonneg_f64(m, rps);
-                    if (bcol == 0 && live && m < min_abs) s_keep[e] = 0u;
+                    if (bcol == 0 && live && m < min_abs) s_keepb[pb][e] = 0u;
                 }
             }
         } else
         for (int e = wave; e < TB; e += NWAVES) {
-            if (!s_keep[e]) continue;
+            if (!s_keepb[pb][e]) continue;
             const int head = s_head[e];
-            const uint32_t tl = s_tl[e];
+            const uint32_t tl = s_tlb[pb][e];
             double m = 0.0;
             for (int b = lane; b < B; b += 64) {
                 double c = 0.0;
@@ -2199,72 +2280,29 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
                 m = fmax(m, fabs(c));
             }
             m = wave_max_nonneg_f64(m);
-            if (lane == 0 && m < min_abs) s_keep[e] = 0u;
+            if (lane == 0 && m < min_abs) s_keepb[pb][e] = 0u;
         }
         __syncthreads();
         // kept rows; with `emit`, the kept rows the next rotation (nmask)
         // touches are also listed for it, as its branch pass would list them
         // (second prefix chain, counts in the high half)
-        const uint32_t mine = (t < TB) ? s_keep[t] : 0u;
+        const uint32_t mine = (t < TB) ? s_keepb[pb][t] : 0u;
         const bool touch = emit && mine && anticommutes<W>(nmask, okey);
         uint32_t agg;
         const uint32_t pre2 = block_excl_sum_small<1>(mine | (touch ? 1u << 16 : 0u), s_w, agg);
         const uint32_t pre = pre2 & 0xffffu, pre_t = pre2 >> 16;
         const uint64_t aggp = (uint64_t)(agg & 0xffffu) | ((uint64_t)(agg >> 16) << 32);
-        {
-            const uint64_t ex = emit ? tile_prefix<2>(ctl, L, bf, tile, aggp, lb_incl, s_wacc)
-                                     : tile_prefix<1>(ctl, L, bf, tile, aggp, lb_incl, s_wacc);
-            const uint64_t excl = ex & 0xffffffffull, excl_t = ex >> 32;
-            if (t == 0) {
-                s_excl = excl;
-                s_excl_t = excl_t;
-                if (tile == ntiles - 1) {
-                    const uint32_t nout = (uint32_t)(excl + (agg & 0xffffu));
-                    set_state(ctl, p ^ 1, nout, cur ^ 1);
-                    if (emit) {
-                        ctl->st[p ^ 1].na = (uint32_t)(excl_t + (agg >> 16));
-#pragma unroll
-                        for (int l = 0; l < W; ++l) ctl->tail_mask[l] = nmask.l[l];
-                        ctl->tail_seq = L.seq + 1;
-                        ctl->tail_ready = 1u;
-                    }
-                    if (L.glog) L.glog[2] = nout;
-                }
-            }
-        }
-        __syncthreads();
-        if (t < TB && mine) {
-            store_key<W>(out_keys, stride, s_excl + pre, okey);
-            s_tl[t] = pre;
-            if (touch) {
-                const uint64_t j = s_excl_t + pre_t;
-                const bool neg = (sign_exponent<W>(nmask, okey) & 2) == 0;
-                next_tail[j] = (uint32_t)(s_excl + pre) | (neg ? SRC_NEG : 0u);
-#pragma unroll
-                for (int l = 0; l < W; ++l) bf.tl_keys[l * stride + j] = okey.l[l];
-            }
-        }
-        if (bf.rec && t < TB) {
-            const uint32_t at = mine ? (uint32_t)(s_excl + pre) : REC_NONE;
-            const uint32_t roff = c.s.roff;
-            if (s_rh[t] != ~0u) bf.rec[(size_t)roff + (s_rh[t] & SRC_MASK)] = (s_rh[t] & REC_FLAG) | at;
-            if (s_rt[t] != ~0u) bf.rec[(size_t)roff + n + na + s_rt[t]] = at;
-        }
-        __syncthreads();
-        if (NARROW) {
-            for (int k0 = 0; k0 < RPW; k0 += rps) {
-                const int k = k0 + sub;
-                const int e = wave + k * NWAVES;
-                if (k < RPW && col && s_keep[e])
-                    out_coeffs[(s_excl + s_tl[e]) * B + bcol] = s_rows[(size_t)e * B + bcol];
-            }
-        } else {
-            for (int e = wave; e < TB; e += NWAVES) {
-                if (!s_keep[e]) continue;
-                const uint64_t pos = s_excl + s_tl[e];
-                for (int b = lane; b < B; b += 64) out_coeffs[pos * B + b] = s_rows[(size_t)e * B + b];
-            }
-        }
+        if (emit) tile_publish<2>(L, bf, tile, aggp);
+        else tile_publish<1>(L, bf, tile, aggp);
+        ptile = tile;
+        prev_ok = true;
+        pokey = okey;
+        pmine = mine;
+        ptouch = touch;
+        ppre = pre;
+        ppre_t = pre_t;
+        paggp = aggp;
+        pb ^= 1;
     }
 }
 

@@ -227,3 +227,23 @@ def test_deferred_merge_tile_modes_bitwise(defer, mode):
     assert_bitwise(*s.download(), *_oracle("kicked127_4", wl), f"merge_defer={defer} {mode}")
     if mode == "forced_rerun":
         assert s.profile()["fallbacks"] > 0, "no gate was sent to the atomic-claim rerun"
+
+
+@pytest.mark.parametrize("B", [5, 64])
+@pytest.mark.parametrize("mode", ["forced_rerun", "dynamic_tiles", "tiny_spin_limit", "unchained"])
+def test_batched_tile_modes_bitwise(B, mode):
+    # the batched merge (stores one tile behind its merge paths) under atomic
+    # tile claims, reruns and merges that list nothing for the next rotation
+    print(f"B={B} {mode}", flush=True)
+    wl = _heis(B, noise=True)
+    opts = {"forced_rerun": {"force_rerun": True}, "dynamic_tiles": {"dynamic_tiles": True},
+            "tiny_spin_limit": {"spin_limit": 1}, "unchained": {"chain_tails": False}}[mode]
+    pc = PackedCircuit(wl["circuit"], wl["thetas"], wl["nqubits"], batch=B)
+    s = HIPPauliSum(wl["nqubits"], batch=B, capacity=1024)
+    s.tune(lds_runs=False, batch_rows=16, **opts)
+    k, c = wl["obs"].to_arrays(pc.W, B)
+    s.upload(k, c)
+    pc.run(s, max_weight=wl["max_weight"], min_abs_coeff=wl["min_abs_coeff"])
+    assert_bitwise(*s.download(), *_oracle(f"heis{B}True", wl), f"B={B} {mode}")
+    if mode == "forced_rerun":
+        assert s.profile()["fallbacks"] > 0, "no gate was sent to the atomic-claim rerun"
