@@ -1089,25 +1089,31 @@ __device__ __forceinline__ uint32_t merge_split(const uint64_t* __restrict__ key
     });
 }
 
-// Loads the tile's head range S[i0-1 .. i1) and tail range T[j0 .. j1] into
-// LDS key slots: [0] = S[i0-1], [1 .. ns] = S[i0 .. i1), [ns+1 .. ns+1+nt] =
-// T[j0 .. j1]. Returns false (and reports) on an inconsistent partition.
+// A batched merge tile's head range S[i0-1 .. i1) and tail range T[j0 .. j1]
+// (keys, tail sources, recorded parent positions) held in registers between
+// their loads and their shared-memory staging: slot x = threadIdx.x + r * WG.
 template <int W, int TILE>
-__device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, size_t stride,
-                                           const uint32_t* __restrict__ tail, const uint64_t* __restrict__ tkeys,
-                                           uint32_t na, uint32_t i0, uint32_t j0, int ns, int nt,
-                                           uint64_t (*s_key)[TILE + 2], uint32_t* s_src,
-                                           const uint32_t* __restrict__ tpj, uint32_t* s_pj) {
-    const int t = threadIdx.x;
-    constexpr int R = (TILE + 1 + WG - 1) / WG;
-    // head slots x = 0 .. ns (global index i0 + x - 1) and tail slots y = 0 .. nt
-    // (global index j0 + y); indices are clamped so every load is issued unconditionally
+struct LoadRegs {
+    static constexpr int R = (TILE + 1 + WG - 1) / WG;
     uint64_t hv[R][W], tv[R][W];
     uint32_t sv[R], pv[R];
+};
+
+// Issues those loads without waiting for them; indices are clamped so every
+// load is issued unconditionally.
+template <int W, int TILE>
+__device__ __forceinline__ void merge_load_fetch(const uint64_t* __restrict__ keys, size_t stride,
+                                                 const uint32_t* __restrict__ tail,
+                                                 const uint64_t* __restrict__ tkeys, uint32_t na, uint32_t i0,
+                                                 uint32_t j0, int ns, const uint32_t* __restrict__ tpj,
+                                                 LoadRegs<W, TILE>& f) {
+    const int t = threadIdx.x;
+    // head slots x = 0 .. ns (global index i0 + x - 1) and tail slots y = 0 .. nt
+    // (global index j0 + y)
     const size_t hmax = (size_t)i0 + ns > 0 ? (size_t)i0 + ns - 1 : 0;
     const uint32_t tmax = na > 0 ? na - 1 : 0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < LoadRegs<W, TILE>::R; ++r) {
         const int x = t + r * WG;
         size_t gi = (size_t)i0 + x;
         gi = gi > 0 ? gi - 1 : 0;
@@ -1115,24 +1121,33 @@ __device__ __forceinline__ void merge_load(const uint64_t* __restrict__ keys, si
         const uint32_t gj = min(j0 + (uint32_t)x, tmax);
 #pragma unroll
         for (int l = 0; l < W; ++l) {
-            hv[r][l] = keys[l * stride + gi];
-            tv[r][l] = tkeys[l * stride + gj];
+            f.hv[r][l] = keys[l * stride + gi];
+            f.tv[r][l] = tkeys[l * stride + gj];
         }
-        sv[r] = tail[gj];
-        if (tpj) pv[r] = tpj[gj];
+        f.sv[r] = tail[gj];
+        if (tpj) f.pv[r] = tpj[gj];
     }
+}
+
+// Stages fetched entries in LDS key slots: [0] = S[i0-1], [1 .. ns] =
+// S[i0 .. i1), [ns+1 .. ns+1+nt] = T[j0 .. j1].
+template <int W, int TILE>
+__device__ __forceinline__ void merge_load_put(const LoadRegs<W, TILE>& f, uint32_t na, uint32_t i0, uint32_t j0,
+                                               int ns, int nt, uint64_t (*s_key)[TILE + 2], uint32_t* s_src,
+                                               const uint32_t* __restrict__ tpj, uint32_t* s_pj) {
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+    for (int r = 0; r < LoadRegs<W, TILE>::R; ++r) {
         const int x = t + r * WG;
         if (x < ns + 1 && (x > 0 || i0 > 0)) {
 #pragma unroll
-            for (int l = 0; l < W; ++l) s_key[l][x] = hv[r][l];
+            for (int l = 0; l < W; ++l) s_key[l][x] = f.hv[r][l];
         }
         if (x < nt + 1 && j0 + (uint32_t)x < na) {
-            s_src[x] = sv[r];
-            if (tpj) s_pj[x] = pv[r];
+            s_src[x] = f.sv[r];
+            if (tpj) s_pj[x] = f.pv[r];
 #pragma unroll
-            for (int l = 0; l < W; ++l) s_key[l][ns + 1 + x] = tv[r][l];
+            for (int l = 0; l < W; ++l) s_key[l][ns + 1 + x] = f.tv[r][l];
         }
     }
 }
@@ -1898,7 +1913,7 @@ __global__ __launch_bounds__(WG, (ITEMS <= 2 ? 3 : (W == 4 && ITEMS == 4 ? 1 : 2
 // rows after the look-back. trig holds cos_b then sin_b of this gate.
 
 template <int W, int TB, bool NARROW = false>
-__global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
+__global__ __launch_bounds__(WG, (W <= 2 ? 4 : 3)) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, int B,
                                                const double* __restrict__ trig, int max_weight, double min_abs,
                                                int damp_q, double fxy, int comm, const double* __restrict__ norm,
                                                int part, int emit, Key<W> nmask) {
@@ -1956,10 +1971,30 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     // Tiles are software-pipelined by one: a workgroup merges tile k's paths
     // (A), then stores tile k-1, whose counts it published before (C), then
     // forms tile k's rows and publishes its counts (B). The wait for tile
-    // k-1's prefix thereby overlaps tile k's key loads and merge paths.
-    // Each pass runs the phases once, so each is inlined once.
+    // k-1's prefix thereby overlaps tile k's merge paths. With static tiles
+    // and partition splits, tile k+1's keys are fetched at the start of B, so
+    // A starts from registers. Each pass runs the phases once, so each is
+    // inlined once.
+    const bool PREF = PART && !L.dynamic;
     uint32_t tile = 0, ptile = 0;  // tile k, tile k-1
-    bool cur_ok = false, prev_ok = false, fin = false;
+    bool cur_ok = false, prev_ok = false, fin = false, started = false;
+    bool pf = false;               // tile k's keys were fetched during tile k-1
+    uint32_t pi0 = 0, pj0 = 0;     // and its bounds
+    int pns = 0, pnt = 0;
+    LoadRegs<W, TB> KR;
+    auto bounds = [&](uint32_t tl, uint32_t a0, uint32_t a1, uint32_t& j0, int& ns, int& nt) {
+        const uint32_t d0 = tl * (uint32_t)TB;
+        const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
+        j0 = d0 - a0;
+        const uint32_t j1 = d1 - a1;
+        ns = (int)(a1 - a0);
+        nt = (int)(j1 - j0);
+        if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || a1 > n || j1 > na) {
+            if (t == 0) atomicOr(&ctl->err, 2u);
+            ns = 0;
+            nt = 0;
+        }
+    };
     int pb = 0;                    // tile k's metadata buffer
     Key<W> okey, pokey;            // the output term of this thread's row, tiles k and k-1
     uint32_t pmine = 0, ppre = 0, ppre_t = 0;
@@ -1968,47 +2003,54 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
     while (true) {
         // ---- A: tile k's merge paths
         if (!fin) {
-            tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
+            if (PREF) {
+                __syncthreads();  // the previous tile's shared memory is free again
+                tile = started ? tile + gridDim.x : blockIdx.x;
+                started = true;
+            } else {
+                tile = ts.take(L, &ctl->counter[L.slot], &s_tile);
+            }
             cur_ok = tile < ntiles;
             fin = !cur_ok;
         }
         if (cur_ok) {
-            const uint32_t d0 = tile * (uint32_t)TB;
-            const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
-            uint32_t i0, i1;
-            if (PART) {
-                // splits from the partition kernel: this tile's were loaded during
-                // the previous tile (static tiles), the next tile's go out now
-                if (L.dynamic || !have_next) {
-                    i0 = bf.splits[tile];
-                    i1 = bf.splits[tile + 1];
+            uint32_t i0, j0;
+            int ns, nt;
+            if (pf) {
+                i0 = pi0;
+                j0 = pj0;
+                ns = pns;
+                nt = pnt;
+            } else {
+                const uint32_t d0 = tile * (uint32_t)TB;
+                const uint32_t d1 = min(d0 + (uint32_t)TB, ntot);
+                uint32_t i1;
+                if (PART) {
+                    // splits from the partition kernel (static tiles: loaded during the previous tile)
+                    i0 = (L.dynamic || !have_next) ? bf.splits[tile] : nx0;
+                    i1 = (L.dynamic || !have_next) ? bf.splits[tile + 1] : nx1;
                 } else {
-                    i0 = nx0;
-                    i1 = nx1;
+                    if (wave < 2) {
+                        const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
+                        if (lane == 0) s_split[wave] = s;
+                    }
+                    __syncthreads();
+                    i0 = s_split[0];
+                    i1 = s_split[1];
                 }
+                bounds(tile, i0, i1, j0, ns, nt);
+                merge_load_fetch<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, bf.tpj, KR);
+            }
+            if (PART) {
+                // the next static tile's splits go out now
                 const uint32_t nt2 = tile + gridDim.x;
                 have_next = !L.dynamic && nt2 < ntiles;
                 if (have_next) {
                     nx0 = bf.splits[nt2];
                     nx1 = bf.splits[nt2 + 1];
                 }
-            } else {
-                if (wave < 2) {
-                    const uint32_t s = merge_split<W>(keys, stride, n, bf.tkeys, na, wave == 0 ? d0 : d1);
-                    if (lane == 0) s_split[wave] = s;
-                }
-                __syncthreads();
-                i0 = s_split[0];
-                i1 = s_split[1];
             }
-            const uint32_t j0 = d0 - i0, j1 = d1 - i1;
-            int ns = (int)(i1 - i0), nt = (int)(j1 - j0);
-            if (ns < 0 || nt < 0 || ns + nt != (int)(d1 - d0) || i1 > n || j1 > na) {
-                if (t == 0) atomicOr(&ctl->err, 2u);
-                ns = 0;
-                nt = 0;
-            }
-            merge_load<W, TB>(keys, stride, tail, bf.tkeys, na, i0, j0, ns, nt, s_key, s_src, bf.tpj, s_pj);
+            merge_load_put<W, TB>(KR, na, i0, j0, ns, nt, s_key, s_src, bf.tpj, s_pj);
             __syncthreads();
             auto S = [&](int x) {
                 Key<W> k;
@@ -2142,6 +2184,13 @@ __global__ __launch_bounds__(WG) void k_mergeB(Bufs bf, Ctl* ctl, Launch L, Key<
         }
         if (!cur_ok) break;
         // ---- B: tile k's rows, counts and publication
+        // (static tiles with partition splits: the next tile's keys go out first)
+        pf = PREF && have_next;
+        if (pf) {
+            pi0 = nx0;
+            bounds(tile + gridDim.x, nx0, nx1, pj0, pns, pnt);
+            merge_load_fetch<W, TB>(keys, stride, tail, bf.tkeys, na, pi0, pj0, pns, bf.tpj, KR);
+        }
         // B <= 64: one column per lane, rps rows of the batch side by side in a
         // wave (segments of 64 / rps lanes); a wave loads the coefficients of
         // RD slots of its rows at once and then finishes them (same values
