@@ -2511,6 +2511,7 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_keep[KNB_ROWS];
     __shared__ uint32_t s_pos[KNB_ROWS];
+    __shared__ uint8_t s_dmp[KNB_ROWS];  // recording: the row's Pauli on the qubit is damped
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_excl;
     TileSource ts;
@@ -2560,7 +2561,10 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
                     m = RPS == 1 ? wave_max_nonneg_f64(m) : wave_seg_max_nonneg_f64(m, RPS);
                     if (i < n) keep = !((max_weight >= 0 && weight<W>(kk[d]) > max_weight) || m < min_abs);
                     cvv[g0 + d] = c;
-                    if (bcol == 0) s_keep[r] = keep;
+                    if (bcol == 0) {
+                        s_keep[r] = keep;
+                        s_dmp[r] = damped ? 1 : 0;
+                    }
                 }
             }
         } else
@@ -2606,13 +2610,18 @@ __global__ __launch_bounds__(WG) void k_noiseB(Bufs bf, Ctl* ctl, Launch L, int 
             }
         }
         __syncthreads();
-        if (B <= 64 && !bf.rec) {
-            // the kept rows' values are in registers: keys copied, values stored
+        if (B <= 64) {
+            // the kept rows' values are in registers: keys copied, values
+            // stored; recording, every row's output position (or none) and
+            // whether it was damped
 #pragma unroll
             for (int q = 0; q < NSL; ++q) {
                 const int r = wave + (q * RPS + sub) * NWAVES;
-                if (!s_keep[r]) continue;
                 const uint32_t i = base + r;
+                if (bf.rec && bcol == 0 && i < n)
+                    bf.rec[(size_t)roff + i] = (s_dmp[r] ? REC_FLAG : 0u) |
+                                               (s_keep[r] ? (uint32_t)(s_excl + s_pos[r]) : REC_NONE);
+                if (!s_keep[r]) continue;
                 const uint64_t pos = s_excl + s_pos[r];
                 if (bcol == 0) store_key<W>(out_keys, stride, pos, load_key<W>(keys, stride, i));
                 if (col) out_coeffs[pos * B + bcol] = cvv[q];

@@ -697,6 +697,11 @@ void enq_merge1(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int v
 // the merge then lists that gate's touched terms (B = 1).
 // Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
 // kernel timed, mode 2); merge [4, 5] (mode 1 and 2).
+// Whether a rotation's merge may list the next rotation's touched terms: the
+// batched merge lists them while recording too; the one-column merge that
+// records has no listing variant.
+bool chain_ok(const pp_handle* h) { return h->chain_tails && (!h->recording || h->B > 1); }
+
 template <int W>
 void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
                   double min_abs, uint32_t* glog, hipEvent_t* ev, const uint64_t* next_mask) {
@@ -706,11 +711,12 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
-    // Chained rotations (not recording): every gate lists the next
+    // Chained rotations (recording too when B > 1): every gate lists the next
     // one's touched terms (its merge, or its permutation kernel when it touches
     // nothing), so after the first gate of a chain k_branch is not launched and
-    // the permutation kernel ends the branch step.
-    const bool chained = !h->recording && h->chain_tails && h->prof_mode != 2;
+    // the permutation kernel ends the branch step. (The one-column merge that
+    // records has no listing variant.)
+    const bool chained = chain_ok(h) && h->prof_mode != 2;
     const bool listed = chained && h->prev_lists;
     h->prev_lists = false;
     if (!listed)
@@ -720,15 +726,18 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     Launch L2{};
     L2.seq = h->seq;
     L2.glog = glog;
-    if (h->recording)
-        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
-               h->d_ctl, L2, h->B);
     const bool part = h->B == 1;
     if (chained)
         enq_perm<W, true>(h, E(2, all), nullptr, lb_launch(h, glog), true, lv, mask,
                           next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u, listed ? 1u : 0u, s1);
     else
         enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, s1);
+    // the records of the touched terms, after the branch step has checked their
+    // room (k_branch, or the permutation kernel of a chain); the permutation
+    // leaves the list in tail[0] as it found it
+    if (h->recording)
+        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
+               h->d_ctl, L2, h->B);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
@@ -1240,7 +1249,7 @@ int enq_chunk_graph(pp_handle* h, const std::vector<HostGate>& gs, const std::ve
     for (size_t k = 0; k < order.size() && rc == PP_OK; ++k) {
         const size_t i = order[k];
         const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
-        const bool chain = h->chain_tails && !h->recording && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
+        const bool chain = chain_ok(h) && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
         rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr);
     }
     hipGraph_t graph = nullptr;
@@ -1467,8 +1476,7 @@ int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, si
             for (size_t k = 0; k < order.size(); ++k) {
                 const size_t i = order[k];
                 const size_t nx = k + 1 < order.size() ? order[k + 1] : i + 1;
-                const bool chain = h->chain_tails && !h->recording && nx < ng &&
-                                   gs[nx].kind == PP_GATE_ROTATION;
+                const bool chain = chain_ok(h) && nx < ng && gs[nx].kind == PP_GATE_ROTATION;
                 if (int rc = enqueue(h, gs[i], i, max_weight, min_abs, chain ? gs[nx].mask : nullptr)) return rc;
             }
         }
