@@ -654,7 +654,7 @@ void enq_partition_b(pp_handle* h) {
 
 // The one-column merge with the tuning table's tile (256 x MI diagonals): its
 // partition, then the merge; variant 0 plain, 1 listing the next rotation's
-// touched terms (nmask), 2 recording for the adjoint sweep.
+// touched terms (nmask), 2 recording for the adjoint sweep, 3 both.
 template <int W, int MI>
 void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int variant, const Key<W>& mask,
                    const RotParams& rp, const Key<W>& nmask, uint32_t* glog) {
@@ -664,7 +664,10 @@ void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, in
     LP.seq = h->seq;
     launch(h, nullptr, e3, k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs, h->d_ctl,
            LP);
-    if (variant == 2)
+    if (variant == 3)
+        launch(h, e4, e5, k_merge1<W, true, true, MI>, lb_grid(h, (const void*)k_merge1<W, true, true, MI>, 0, expect, TILE),
+               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
+    else if (variant == 2)
         launch(h, e4, e5, k_merge1<W, false, true, MI>, lb_grid(h, (const void*)k_merge1<W, false, true, MI>, 0, expect, TILE),
                0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
     else if (h->tun.merge_defer && variant == 1)
@@ -697,10 +700,9 @@ void enq_merge1(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int v
 // the merge then lists that gate's touched terms (B = 1).
 // Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
 // kernel timed, mode 2); merge [4, 5] (mode 1 and 2).
-// Whether a rotation's merge may list the next rotation's touched terms: the
-// batched merge lists them while recording too; the one-column merge that
-// records has no listing variant.
-bool chain_ok(const pp_handle* h) { return h->chain_tails && (!h->recording || h->B > 1); }
+// Whether a rotation's merge may list the next rotation's touched terms (the
+// merges that record list them too).
+bool chain_ok(const pp_handle* h) { return h->chain_tails; }
 
 template <int W>
 void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
@@ -711,11 +713,10 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const bool all = h->prof_mode == 2 && ev;
     const bool mrg = h->prof_mode == 2 && ev;
     auto E = [&](int k, bool on) -> hipEvent_t { return on ? ev[k] : nullptr; };
-    // Chained rotations (recording too when B > 1): every gate lists the next
-    // one's touched terms (its merge, or its permutation kernel when it touches
+    // Chained rotations (recording too): every gate lists the next one's
+    // touched terms (its merge, or its permutation kernel when it touches
     // nothing), so after the first gate of a chain k_branch is not launched and
-    // the permutation kernel ends the branch step. (The one-column merge that
-    // records has no listing variant.)
+    // the permutation kernel ends the branch step.
     const bool chained = chain_ok(h) && h->prof_mode != 2;
     const bool listed = chained && h->prev_lists;
     h->prev_lists = false;
@@ -741,7 +742,7 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
-        enq_merge1<W>(h, E(3, all), E(4, mrg), E(5, mrg), h->recording ? 2 : (next_mask ? 1 : 0), mask, rp,
+        enq_merge1<W>(h, E(3, all), E(4, mrg), E(5, mrg), h->recording ? (next_mask ? 3 : 2) : (next_mask ? 1 : 0), mask, rp,
                       next_mask ? to_key<W>(next_mask) : mask, glog);
     } else {
         const int B = h->B;
