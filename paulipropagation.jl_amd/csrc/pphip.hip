@@ -2851,11 +2851,8 @@ int pp_adjoint(pp_handle* h, const pp_gate* gates, size_t ngates, const double* 
     if (!loss || (nparams > 0 && !grad)) return fail(PP_ERR_ARG, "loss or grad is NULL");
     Readout rd;
     if (int rc = make_readout(h, overlap_kind, ones, &rd)) return rc;
-    const bool chain = h->chain_tails;
-    h->chain_tails = false;
     const int rc = adjoint_impl(h, gates, ngates, theta, nparams, lut_changes, lut_signs, nluts, max_weight,
                                 min_abs_coeff, checkpoint_gates, rd, loss, grad);
-    h->chain_tails = chain;
     set_recording(h, false);
     return rc;
 }

@@ -126,3 +126,32 @@ def test_adjoint_rejects_bad_arguments():
         pp.adjointgradient(circ, obs, np.zeros(P + 1))
     with pytest.raises(NotImplementedError):
         pp.rewindgradient(circ, obs, np.zeros(P), overlapfunc=lambda s: 0.0)
+
+
+@pytest.mark.parametrize("B,noise,checkpoint", [(1, False, 0), (1, False, 40), (5, True, 0), (5, True, 37)])
+def test_adjoint_chained_and_unchained_bitwise(B, noise, checkpoint):
+    # the recorded forward runs list each rotation's touched terms from the
+    # merge before it (chained) or branch every rotation (unchained): the
+    # records, and so the losses and gradients, are the same bit for bit
+    print(f"adjoint chained vs unchained B={B} noise={noise} checkpoint={checkpoint}", flush=True)
+    from pauliprop_hip.hipsum import HIPPauliSum, PackedCircuit
+    n = 10
+    circ, obs = _ansatz(n, 4, noise)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(11 + B).normal(0, 0.3, size=(P, B))
+    pc = PackedCircuit(circ, th if B > 1 else th[:, 0], n, batch=B)
+    k0, c0 = obs.to_arrays(pc.W, B)
+    out = []
+    for chain in (True, False):
+        s = HIPPauliSum(n, batch=B, capacity=1024)
+        s.tune(chain_tails=chain)
+        s.upload(k0, c0)
+        out.append(pc.adjoint(s, max_weight=6, min_abs_coeff=1e-6, checkpoint_gates=checkpoint))
+    (l1, g1), (l2, g2) = out
+    assert np.array_equal(l1.view(np.uint64), l2.view(np.uint64))
+    assert np.array_equal(g1.view(np.uint64), g2.view(np.uint64))
+    rloss, rgrad = oracle_adjoint(circ, n, th if B > 1 else th[:, 0], obs.terms, max_weight=6, min_abs_coeff=1e-6,
+                                  batch=B)
+    scale = max(1.0, float(np.abs(rgrad).max()))
+    # Heisenberg parameter rows run in reverse Schrodinger order
+    np.testing.assert_allclose(g1[::-1].reshape(rgrad.shape), rgrad, rtol=0, atol=1e-12 * scale)
