@@ -1371,15 +1371,16 @@ int lds_run(pp_handle* h, const std::vector<HostGate>& gs, const std::vector<siz
 // into the profile. counts[i - first] receives gate i's input size.
 //
 // Rotations that touch no term are the identity (xortailmerge.jl:121-123:
-// nothing branches, nothing is truncated). Unless recording, the host reads
-// which Paulis occur on each qubit at every synchronization and leaves out the
-// rotations of the next chunk that this presence, widened gate by gate, shows
-// to touch nothing; their log entries are filled in afterwards.
+// nothing branches, nothing is truncated). The host reads which Paulis occur
+// on each qubit at every synchronization and leaves out the rotations of the
+// next chunk that this presence, widened gate by gate, shows to touch nothing;
+// their log entries are filled in afterwards (no touched terms, so no records:
+// the adjoint sweep passes them as it passes an identity that ran).
 int run_resolved(pp_handle* h, const std::vector<HostGate>& gs, size_t first, size_t last, int max_weight,
                  double min_abs, int64_t* counts) {
     const size_t ng = last;
     size_t g = first;
-    const bool skip = h->skip_identity && !h->recording && last - first > 1;
+    const bool skip = h->skip_identity && last - first > 1;
     std::vector<uint8_t> skipped(last - first, 0);
     std::vector<size_t> order;  // gates of the chunk that are enqueued
     if (h->prof_mode == 1 && h->d_ktime && last > first && last <= h->ktime_cap) {

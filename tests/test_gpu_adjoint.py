@@ -131,8 +131,9 @@ def test_adjoint_rejects_bad_arguments():
 @pytest.mark.parametrize("B,noise,checkpoint", [(1, False, 0), (1, False, 40), (5, True, 0), (5, True, 37)])
 def test_adjoint_chained_and_unchained_bitwise(B, noise, checkpoint):
     # the recorded forward runs list each rotation's touched terms from the
-    # merge before it (chained) or branch every rotation (unchained): the
-    # records, and so the losses and gradients, are the same bit for bit
+    # merge before it and leave out the rotations that touch nothing (the
+    # default), or branch and launch every rotation: the records, and so the
+    # losses and gradients, are the same bit for bit
     print(f"adjoint chained vs unchained B={B} noise={noise} checkpoint={checkpoint}", flush=True)
     from pauliprop_hip.hipsum import HIPPauliSum, PackedCircuit
     n = 10
@@ -144,7 +145,7 @@ def test_adjoint_chained_and_unchained_bitwise(B, noise, checkpoint):
     out = []
     for chain in (True, False):
         s = HIPPauliSum(n, batch=B, capacity=1024)
-        s.tune(chain_tails=chain)
+        s.tune(chain_tails=chain, skip_identity=chain)
         s.upload(k0, c0)
         out.append(pc.adjoint(s, max_weight=6, min_abs_coeff=1e-6, checkpoint_gates=checkpoint))
     (l1, g1), (l2, g2) = out
