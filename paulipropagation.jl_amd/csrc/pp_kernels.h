@@ -863,6 +863,13 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
 
     // list entry x's key (the list holds the terms themselves in list order)
     const uint64_t* __restrict__ lkeys = bf.tl_keys;
+    // recording with one column: the list's records (its entries, and the
+    // source coefficients, which the list carries) are written here, when they
+    // fit, under the same bound the branch step checks (k_record_tails for B > 1)
+    const bool rec1 = bf.rec && bf.tcoef && (uint64_t)c.s.roff + c.s.n + 2ull * na <= bf.rec_words &&
+                      (uint64_t)c.s.coff + na <= bf.rec_rows;
+    uint32_t* __restrict__ rsrc = rec1 ? bf.rec + (size_t)c.s.roff + c.s.n : nullptr;
+    double* __restrict__ rcf = rec1 ? bf.rcoef + c.s.coff : nullptr;
     auto key_at = [&](uint32_t x) -> Key<W> { return load_key_from<W>(lkeys, stride, x, from_limb); };
 
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -880,6 +887,15 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
             src[i] = tail_in[x];
             key[i] = load_key<W>(lkeys, stride, x);
             cf[i] = bf.tl_coef ? bf.tl_coef[x] : 0.0;
+        }
+        if (rec1) {
+#pragma unroll
+            for (int i = 0; i < K2_ITEMS; ++i) {
+                if (j0 + i <= last) {
+                    rsrc[j0 + i] = src[i];
+                    rcf[j0 + i] = cf[i];
+                }
+            }
         }
         // the entries just before and after the tile, loaded with it for every level's edges
         const bool has_prev = base > 0, has_after = last + 1 < na;

@@ -733,10 +733,11 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
                           next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u, listed ? 1u : 0u, s1);
     else
         enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, s1);
-    // the records of the touched terms, after the branch step has checked their
-    // room (k_branch, or the permutation kernel of a chain); the permutation
-    // leaves the list in tail[0] as it found it
-    if (h->recording)
+    // the records of the touched terms (B = 1: the permutation kernel writes
+    // them), after the branch step has checked their room (k_branch, or the
+    // permutation kernel of a chain); the permutation leaves the list in
+    // tail[0] as it found it
+    if (h->recording && !part)
         launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
                h->d_ctl, L2, h->B);
     const size_t expect = h->cap + h->cap / 2;
