@@ -92,6 +92,7 @@ struct Tuning {
     int run_chunk = 128;   // gates enqueued between overflow checks (16 .. 1024)
 };
 constexpr int MAX_GRID_PER_CU = 8;
+constexpr unsigned ADJ_BLOCKS = 1024;      // blocks of the adjoint sweep's rotation kernels
 constexpr int PRES_WORDS = 12;             // X, Y, Z presence planes of up to 4 limbs
 
 }  // namespace
@@ -1994,7 +1995,9 @@ int adjoint_impl(pp_handle* h, const pp_gate* gates, size_t ngates, const double
     // of WG threads covering B in groups of WG columns
     const int lgBP = B <= 1 ? 0 : std::min(8, 32 - __builtin_clz((unsigned)(B - 1)));
     static_assert(ADJ_MAX_GROUPS * WG >= 1024, "the column groups cover the largest batch");
-    const unsigned gblocks = h->gmax;
+    // a fixed block count (not the tuned persistent grid), so the per-block
+    // partial gradients, and their sums, do not depend on a launch shape
+    const unsigned gblocks = ADJ_BLOCKS;
     // The blocks' partial gradients of every rotation of a segment are kept and
     // summed by one launch after the segment's sweep (one workgroup per
     // rotation, blocks in order: the sums a launch per gate would give), unless

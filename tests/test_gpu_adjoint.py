@@ -156,3 +156,26 @@ def test_adjoint_chained_and_unchained_bitwise(B, noise, checkpoint):
     scale = max(1.0, float(np.abs(rgrad).max()))
     # Heisenberg parameter rows run in reverse Schrodinger order
     np.testing.assert_allclose(g1[::-1].reshape(rgrad.shape), rgrad, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("B", [1, 5])
+def test_adjoint_independent_of_grid_per_cu(B):
+    # the sweep's per-block partial gradients use a fixed block count, so the
+    # gradients are the same bits whatever persistent grid the table chose
+    print(f"adjoint over grid_per_cu, B={B}", flush=True)
+    from pauliprop_hip.hipsum import HIPPauliSum, PackedCircuit
+    n = 10
+    circ, obs = _ansatz(n, 3, True)
+    P = pp.countparameters(circ)
+    th = np.random.default_rng(5 + B).normal(0, 0.3, size=(P, B))
+    pc = PackedCircuit(circ, th if B > 1 else th[:, 0], n, batch=B)
+    k0, c0 = obs.to_arrays(pc.W, B)
+    out = []
+    for g in (1, 4, 8):
+        s = HIPPauliSum(n, batch=B, capacity=1024)
+        s.tune(grid_per_cu=g)
+        s.upload(k0, c0)
+        out.append(pc.adjoint(s, max_weight=6, min_abs_coeff=1e-6))
+    for loss, grad in out[1:]:
+        assert np.array_equal(loss.view(np.uint64), out[0][0].view(np.uint64))
+        assert np.array_equal(grad.view(np.uint64), out[0][1].view(np.uint64))
