@@ -64,12 +64,15 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="kicked_ising_heavyhex_127q",
-                    choices=["tfim_brick_32q", "kicked_ising_heavyhex_127q", "heisenberg_square_64q_b64"])
+                    choices=["tfim_brick_32q", "kicked_ising_heavyhex_127q", "heisenberg_square_64q_b64",
+                             "ansatz_line_48q", "ansatz_line_noisy_48q_b32"])
     ap.add_argument("--layers", type=int, default=None,
                     help="heisenberg_square_64q_b64 only: its first LAYERS steps (the full ten outgrow one GPU)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the single-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
+    ap.add_argument("--adjoint", default=None, choices=["ansatz_line_48q", "ansatz_line_noisy_48q_b32"],
+                    help="measure the adjoint of this workload only (default both)")
     ap.add_argument("--size-log", default=None,
                     help="run the workload in chunks of 64 gates and write every gate's input size (and capacity) to "
                          "this file until it ends or a device allocation fails; no timing")
@@ -181,7 +184,7 @@ def main() -> int:
 
     adj = None
     if not args.no_adjoint:
-        adj = adjoint_detail(pp, PackedCircuit)
+        adj = adjoint_detail(pp, PackedCircuit, [args.adjoint] if args.adjoint else None)
 
     L = 127 + 144 if args.workload == "kicked_ising_heavyhex_127q" else None
     out = {
@@ -228,7 +231,7 @@ def measured_traffic(workload, kname):
     return round(float(m.group(1))), os.path.relpath(path, ROOT)
 
 
-def adjoint_detail(pp, PackedCircuit):
+def adjoint_detail(pp, PackedCircuit, names=None):
     """Gradients/s of the adjoint sweep against plain forward finite differences
     (2P forward runs per gradient, h = 1e-6) on the same GPU: the 48-qubit
     line ansatz (1716 parameters, B = 1) and the same ansatz with depolarizing
@@ -238,7 +241,7 @@ def adjoint_detail(pp, PackedCircuit):
     import numpy as np
     from pauliprop_hip import workloads
     out = {}
-    for name in ("ansatz_line_48q", "ansatz_line_noisy_48q_b32"):
+    for name in names or ("ansatz_line_48q", "ansatz_line_noisy_48q_b32"):
         wl = workloads.WORKLOADS[name]()
         nq, mw, mac = wl["nqubits"], wl["max_weight"], wl["min_abs_coeff"]
         th = np.asarray(wl["thetas"], dtype=np.float64)
@@ -426,6 +429,10 @@ def workload_text(name, wl, layers=None) -> str:
         "kicked_ising_heavyhex_127q": "kicked Ising on the 127-qubit heavy-hex graph, 20 x (RX(U[0,pi/2]) on 127 qubits + RZZ(-pi/2) on 144 "
                    "edges), Z63, UInt256 keys",
         "heisenberg_square_64q_b64": "Heisenberg on the 8x8 periodic square lattice (64 qubits), 10 steps, B=64, sum ZZ, UInt128 keys",
+        "ansatz_line_48q": "hardware-efficient ansatz on a 48-qubit line, 12 x (RY, RZ on every qubit, RZZ on the "
+                           "line), theta N(0, 0.3^2), sum Z_i Z_i+1, UInt128 keys",
+        "ansatz_line_noisy_48q_b32": "the 48-qubit line ansatz with DepolarizingNoise(0.01) on every qubit after "
+                                     "each layer, B=32, theta N(0, 0.3^2), sum Z_i Z_i+1, UInt128 keys",
     }[name]
 
 
