@@ -58,6 +58,7 @@ struct Bufs {
     double* rcoef;       // per-gate coefficient rows of the touched terms, from the gate's coff
     uint32_t rec_words;  // capacity of rec
     uint32_t rec_rows;   // capacity of rcoef, in rows of B
+    uint32_t batch;      // B, the coefficient columns
 };
 
 // A gate's records. Rotation touching na > 0 of n terms, at rec + roff:
@@ -863,13 +864,17 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
 
     // list entry x's key (the list holds the terms themselves in list order)
     const uint64_t* __restrict__ lkeys = bf.tl_keys;
-    // recording with one column: the list's records (its entries, and the
-    // source coefficients, which the list carries) are written here, when they
-    // fit, under the same bound the branch step checks (k_record_tails for B > 1)
-    const bool rec1 = bf.rec && bf.tcoef && (uint64_t)c.s.roff + c.s.n + 2ull * na <= bf.rec_words &&
-                      (uint64_t)c.s.coff + na <= bf.rec_rows;
-    uint32_t* __restrict__ rsrc = rec1 ? bf.rec + (size_t)c.s.roff + c.s.n : nullptr;
-    double* __restrict__ rcf = rec1 ? bf.rcoef + c.s.coff : nullptr;
+    // recording: the list's records (its entries and their source coefficient
+    // rows) are written here, when they fit, under the same bound the branch
+    // step checks. With one column the list carries the coefficients and each
+    // tile writes its entries'; with B columns every workgroup of the grid,
+    // most of which have no tile, copies a grid-strided share of the rows
+    // from the sum (after its tile work, if any)
+    const bool rec = bf.rec && (uint64_t)c.s.roff + c.s.n + 2ull * na <= bf.rec_words &&
+                     (uint64_t)c.s.coff + na <= bf.rec_rows;
+    const bool rec1 = rec && bf.tcoef;
+    uint32_t* __restrict__ rsrc = rec ? bf.rec + (size_t)c.s.roff + c.s.n : nullptr;
+    double* __restrict__ rcf = rec ? bf.rcoef + (size_t)c.s.coff * bf.batch : nullptr;
     auto key_at = [&](uint32_t x) -> Key<W> { return load_key_from<W>(lkeys, stride, x, from_limb); };
 
     for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -1038,6 +1043,17 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         }
         __syncthreads();
     }
+    if (rec && !rec1) {
+        const uint32_t B = bf.batch;
+        const double* __restrict__ sc = bf.coeffs[c.s.cur];
+        const uint64_t total = (uint64_t)na * B;
+        for (uint64_t e = (uint64_t)blockIdx.x * WG + t; e < total; e += (uint64_t)gridDim.x * WG) {
+            const uint32_t j = (uint32_t)(e / B), b = (uint32_t)(e - (uint64_t)j * B);
+            const uint32_t sj = tail_in[j];
+            if (b == 0) rsrc[j] = sj;
+            rcf[e] = sc[(size_t)(sj & SRC_MASK) * B + b];
+        }
+    }
     // every 64th entry of the sum, for the merge partition, unless the merge
     // that wrote the sum left them
     if (c.ssamp_for == L.seq + 1u) return;
@@ -1046,30 +1062,6 @@ __global__ __launch_bounds__(WG) void k_xorperm(Bufs bf, Ctl* ctl, Launch L, Lev
         const Key<W> kk = load_key<W>(keys, stride, 64ull * k);
 #pragma unroll
         for (int l = 0; l < W; ++l) bf.ssamp[(size_t)k * W + l] = kk.l[l];
-    }
-}
-
-// ============================================================ k_record_tails
-//
-// Recording only: copies the touched terms of this gate (parent order, with
-// their signs) and their coefficient rows into the gate's records, for the
-// adjoint sweep. One thread per (term, column), grid-strided.
-__global__ __launch_bounds__(WG) void k_record_tails(Bufs bf, Ctl* ctl, Launch L, int B) {
-    if (ctl->overflow) return;
-    const int p = L.seq & 1;
-    const uint32_t na = ctl->st[p].na;
-    if (na == 0) return;
-    const uint32_t n = ctl->st[p].n, roff = ctl->st[p].roff, coff = ctl->st[p].coff;
-    const double* __restrict__ coeffs = bf.coeffs[ctl->st[p].cur];
-    const uint32_t* __restrict__ tail = bf.tail[0];
-    uint32_t* __restrict__ rsrc = bf.rec + (size_t)roff + n;
-    double* __restrict__ rc = bf.rcoef + (size_t)coff * B;
-    const uint64_t total = (uint64_t)na * B;
-    for (uint64_t e = (uint64_t)blockIdx.x * WG + threadIdx.x; e < total; e += (uint64_t)gridDim.x * WG) {
-        const uint32_t j = (uint32_t)(e / B), b = (uint32_t)(e % B);
-        const uint32_t src = tail[j];
-        if (b == 0) rsrc[j] = src;
-        rc[e] = coeffs[(size_t)(src & SRC_MASK) * B + b];
     }
 }
 

@@ -399,6 +399,7 @@ int grow(pp_handle* h, size_t newcap) {
     h->bufs.ssamp = nss;
     h->bufs.tsamp = nts;
     h->bufs.cap = (uint32_t)newcap;
+    h->bufs.batch = (uint32_t)h->B;
     h->lb_cap = nlb;
     h->cap = newcap;
     h->epoch = 0;
@@ -734,13 +735,6 @@ void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, c
                           next_mask ? to_key<W>(next_mask) : mask, next_mask ? 1u : 0u, listed ? 1u : 0u, s1);
     else
         enq_perm<W, false>(h, E(2, all), part ? nullptr : E(3, all), L2, false, lv, mask, mask, 0u, 0u, s1);
-    // the records of the touched terms (B = 1: the permutation kernel writes
-    // them), after the branch step has checked their room (k_branch, or the
-    // permutation kernel of a chain); the permutation leaves the list in
-    // tail[0] as it found it
-    if (h->recording && !part)
-        launch(h, nullptr, nullptr, k_record_tails, grid_for(h, h->cap * (size_t)h->B, WG * 4), 0, h->bufs,
-               h->d_ctl, L2, h->B);
     const size_t expect = h->cap + h->cap / 2;
     if (part) {
         RotParams rp{c1, s1, max_weight, min_abs};
