@@ -1619,15 +1619,13 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
 
 // The one-column merge with stores one tile behind (not recording). A tile's
 // kept outputs wait in a second shared-memory buffer while the workgroup
-// stages and merges its next tile and publishes that tile's counts; only then
-// does it read the prefix window of the waiting tile, issue the loads of the
-// tile after, and store the waiting tile. The window then holds a whole tile
-// of slack, so its polls -- issued ahead of the loads, whose data returns in
-// order -- find it complete, and the loads' latency overlaps the stores. Counts never depend on prefixes, so publishing a
-// tile before resolving the one before it adds no wait to any chain (static
-// windows and the dynamic two-level look-back alike). Outputs, the next
-// rotation's list and the partition samples are formed exactly as in k_merge1
-// and land at the same positions.
+// stages its next tile, issues the loads of the tile after, merges and
+// publishes; only then does it read the waiting tile's prefix window and
+// store it. The window then has a whole tile of slack. Counts never depend on
+// prefixes, so publishing a tile before resolving the one before it adds no
+// wait to any chain (static windows and the dynamic two-level look-back
+// alike). Outputs, the next rotation's list and the partition samples are
+// formed exactly as in k_merge1 and land at the same positions.
 template <int W, bool EMIT, int ITEMS>
 __global__ __launch_bounds__(WG, (ITEMS <= 2 ? 3 : (W == 4 && ITEMS == 4 ? 1 : 2))) void k_merge1d(Bufs bf, Ctl* ctl, Launch L, Key<W> mask,
                                                                       RotParams rp, Key<W> nmask) {
@@ -1708,9 +1706,7 @@ __global__ __launch_bounds__(WG, (ITEMS <= 2 ? 3 : (W == 4 && ITEMS == 4 ? 1 : 2
     bool waiting = false;
     uint32_t wtile = 0;
     uint64_t waggp = 0;
-    // the waiting tile's prefix (its window is almost always complete by now:
-    // the polls go out before the next tile's loads, so they do not wait
-    // behind them)
+    // the waiting tile's prefix (its window is almost always complete by now)
     auto resolve = [&]() {
         const uint64_t ex = tile_resolve<EMIT ? 2 : 1>(ctl, L, bf, wtile, waggp, incl, s_wacc);
         const uint32_t nkept = (uint32_t)(waggp & 0xffffffffull), ntouch = (uint32_t)(waggp >> 32);

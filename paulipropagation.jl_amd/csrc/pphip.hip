@@ -698,14 +698,14 @@ void enq_merge1(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int v
     }
 }
 
-// next_mask: the mask of the rotation enqueued right after this one, if any;
-// the merge then lists that gate's touched terms (B = 1).
-// Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
-// kernel timed, mode 2); merge [4, 5] (mode 1 and 2).
 // Whether a rotation's merge may list the next rotation's touched terms (the
 // merges that record list them too).
 bool chain_ok(const pp_handle* h) { return h->chain_tails; }
 
+// next_mask: the mask of the rotation enqueued right after this one, if any;
+// the merge then lists that gate's touched terms.
+// Events (6 per gate): branch [0, 1], permutation + partition [2, 3] (every
+// kernel timed, mode 2); merge [4, 5] (mode 1 and 2).
 template <int W>
 void enq_rotation(pp_handle* h, const uint64_t* mask_in, double c1, double s1, const double* d_trig, int max_weight,
                   double min_abs, uint32_t* glog, hipEvent_t* ev, const uint64_t* next_mask) {
