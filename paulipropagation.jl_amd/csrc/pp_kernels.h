@@ -1171,8 +1171,8 @@ struct K3Cfg {
 };
 constexpr int MIN_MERGE_ITEMS = 2;  // the narrowest merge tile any table may choose
 
-// Merge-path split of every tile boundary of this gate's merge, one wave per
-// boundary: splits[b] = head entries among the first min(b * TILE, n + na).
+// Merge-path split of tile boundary b of this gate's merge, by one wave: the
+// head entries among the first min(b * TILE, n + na) merged entries.
 //
 // The split at diagonal d is the first x in [max(0, d - na), min(d, n)) with
 // sum[x] > tail[d - 1 - x], or min(d, n). For d a multiple of 64 and x a
@@ -1182,47 +1182,49 @@ constexpr int MIN_MERGE_ITEMS = 2;  // the narrowest merge tile any table may ch
 // The samples are small and shared by all boundaries, so the search reads
 // little beyond them.
 template <int W, int TILE>
-__global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
+__device__ __forceinline__ uint32_t tile_split(const Bufs& bf, const uint64_t* __restrict__ keys, uint32_t n,
+                                               uint32_t na, uint32_t b) {
     static_assert(TILE % 64 == 0, "tile boundaries must fall on sampled diagonals");
+    const uint32_t ntot = n + na;
+    if (b == 0) return 0;
+    if ((uint64_t)b * TILE >= ntot) return n;
+    const uint64_t* __restrict__ ss = bf.ssamp;
+    const uint64_t* __restrict__ ts = bf.tsamp;
+    const size_t stride = bf.cap;
+    const uint32_t d = b * (uint32_t)TILE;
+    const uint32_t xlo = d > na ? d - na : 0u;
+    const uint32_t xhi = min(d, n);
+    // sampled points 64a in [xlo, xhi): tail partner d - 1 - 64a = 64c + 63
+    const uint32_t aL = (xlo + 63) / 64, aR = (xhi + 63) / 64;
+    const uint32_t cbase = d / 64 - 1;
+    const uint32_t aT = wave_first_true(aL, aR, [&](uint32_t a) {
+        Key<W> x, y;
+#pragma unroll
+        for (int l = 0; l < W; ++l) {
+            x.l[l] = ss[(size_t)a * W + l];
+            y.l[l] = ts[(size_t)(cbase - a) * W + l];
+        }
+        return !key_le<W>(x, y);
+    });
+    const uint32_t lo = aT > aL ? 64 * (aT - 1) + 1 : xlo;
+    const uint32_t hi = aT < aR ? 64 * aT : xhi;
+    return wave_first_true(lo, hi, [&](uint32_t x) {
+        return !key_le<W>(load_key<W>(keys, stride, x), load_key<W>(bf.tkeys, stride, d - 1 - x));
+    });
+}
+
+// Every tile boundary's split, one wave per boundary, into splits[].
+template <int W, int TILE>
+__global__ __launch_bounds__(WG) void k_partition(Bufs bf, Ctl* ctl, Launch L) {
     const CtlView c = load_ctl(ctl, L.seq & 1);
     if (c.overflow) return;
     const uint32_t na = c.s.na;
     if (!(c.s.na || c.s.run)) return;
     const uint32_t n = c.s.n;
     const uint64_t* __restrict__ keys = bf.keys[c.s.cur];
-    const uint64_t* __restrict__ ss = bf.ssamp;
-    const uint64_t* __restrict__ ts = bf.tsamp;
-    const size_t stride = bf.cap;
-    const uint32_t ntot = n + na;
-    const uint32_t nb = (ntot + TILE - 1) / TILE + 1;
+    const uint32_t nb = (n + na + TILE - 1) / TILE + 1;
     for (uint32_t b = blockIdx.x * NWAVES + (threadIdx.x >> 6); b < nb; b += gridDim.x * NWAVES) {
-        uint32_t split;
-        if (b == 0) {
-            split = 0;
-        } else if ((uint64_t)b * TILE >= ntot) {
-            split = n;
-        } else {
-            const uint32_t d = b * (uint32_t)TILE;
-            const uint32_t xlo = d > na ? d - na : 0u;
-            const uint32_t xhi = min(d, n);
-            // sampled points 64a in [xlo, xhi): tail partner d - 1 - 64a = 64c + 63
-            const uint32_t aL = (xlo + 63) / 64, aR = (xhi + 63) / 64;
-            const uint32_t cbase = d / 64 - 1;
-            const uint32_t aT = wave_first_true(aL, aR, [&](uint32_t a) {
-                Key<W> x, y;
-#pragma unroll
-                for (int l = 0; l < W; ++l) {
-                    x.l[l] = ss[(size_t)a * W + l];
-                    y.l[l] = ts[(size_t)(cbase - a) * W + l];
-                }
-                return !key_le<W>(x, y);
-            });
-            const uint32_t lo = aT > aL ? 64 * (aT - 1) + 1 : xlo;
-            const uint32_t hi = aT < aR ? 64 * aT : xhi;
-            split = wave_first_true(lo, hi, [&](uint32_t x) {
-                return !key_le<W>(load_key<W>(keys, stride, x), load_key<W>(bf.tkeys, stride, d - 1 - x));
-            });
-        }
+        const uint32_t split = tile_split<W, TILE>(bf, keys, n, na, b);
         if (lane_id() == 0) bf.splits[b] = split;
     }
 }
@@ -1328,7 +1330,7 @@ __device__ __forceinline__ void merge_put(const MergeRegs<W, TILE, REC>& f, uint
 // (up to 8 words each) -- one round trip, where a chained look-back takes one
 // per level. Dynamic tiles (the fallback after a wait runs out) use the
 // two-level look-back.
-template <int W, bool EMIT, bool REC, int ITEMS = K3Cfg<W>::ITEMS>
+template <int W, bool EMIT, bool REC, int ITEMS = K3Cfg<W>::ITEMS, bool SELF = false>
 // (at least three waves per SIMD for tiles of up to 768 diagonals, the register
 // budget of three resident workgroups per compute unit)
 __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ctl* ctl, Launch L, Key<W> mask, RotParams rp,
@@ -1364,6 +1366,10 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
     __shared__ uint32_t s_w[NWAVES];
     __shared__ uint64_t s_wacc[NWAVES];
     __shared__ uint64_t s_excl, s_excl_t;
+    // SELF (static tiles only): no partition kernel ran; the tiles' splits are
+    // searched here, by tile_split as that kernel searches them
+    constexpr bool self = SELF;
+    __shared__ uint32_t s_split[SELF ? 4 : 1];
     uint32_t* __restrict__ rhead = REC ? bf.rec + c.s.roff : nullptr;
     uint32_t* __restrict__ rcreated = REC ? bf.rec + c.s.roff + n + na : nullptr;
     const int t = threadIdx.x;
@@ -1392,22 +1398,33 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
         tile = blockIdx.x;
     }
     if (tile >= ntiles) return;
+    const uint32_t G = gridDim.x;
+    if (self) {
+        // this tile's and the next static tile's boundaries, a wave each
+        const int wv = t >> 6;
+        const uint32_t bd = (wv < 2 ? tile : tile + G) + (uint32_t)(wv & 1);
+        const uint32_t sp = tile_split<W, TILE>(bf, keys, n, na, bd);
+        if ((t & 63) == 0) s_split[wv] = sp;
+        __syncthreads();
+    }
     uint32_t i0, j0;
     int ns, nt;
     {
         const bool first = tile == pt && !L.dynamic;
-        i0 = first ? pre0 : bf.splits[tile];
-        const uint32_t i1 = first ? pre1 : bf.splits[tile + 1];
+        i0 = self ? s_split[0] : first ? pre0 : bf.splits[tile];
+        const uint32_t i1 = self ? s_split[1] : first ? pre1 : bf.splits[tile + 1];
         bounds(tile, i0, i1, j0, ns, nt);
     }
     MergeRegs<W, TILE, REC> F;
     merge_fetch<W, TILE, REC>(bf, keys, coeffs, stride, na, i0, j0, ns, F);
     // static tiles: splits loaded two tiles ahead (s1: tile + G), and the
     // workgroup's inclusive prefix, packed as (touched << 32) | kept
-    const uint32_t G = gridDim.x;
     uint64_t incl = 0;
     uint32_t s1a = 0, s1b = 0;
-    if (!L.dynamic) {
+    if (self) {
+        s1a = s_split[2];
+        s1b = s_split[3];
+    } else if (!L.dynamic) {
         const bool ok1 = tile + G < ntiles;
         s1a = bf.splits[ok1 ? tile + G : 0u];
         s1b = bf.splits[ok1 ? tile + G + 1 : 0u];
@@ -1418,14 +1435,26 @@ __global__ __launch_bounds__(WG, (ITEMS <= 3 ? 3 : 2)) void k_merge1(Bufs bf, Ct
         if (L.dynamic) {
             if (t == 0)
                 s_tile = __hip_atomic_fetch_add(&ctl->counter[L.slot], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
+        } else if (!self) {
             const bool ok2 = tile + 2 * G < ntiles;
             s2a = bf.splits[ok2 ? tile + 2 * G : 0u];
             s2b = bf.splits[ok2 ? tile + 2 * G + 1 : 0u];
+        } else if (tile + 2 * G < ntiles) {
+            // searched splits of the static tile after next (a sum that grew
+            // past one round of tiles since the host chose this path)
+            const int wv = t >> 6;
+            if (wv < 2) {
+                const uint32_t sp = tile_split<W, TILE>(bf, keys, n, na, tile + 2 * G + (uint32_t)wv);
+                if ((t & 63) == 0) s_split[wv] = sp;
+            }
         }
         merge_put<W, TILE, REC>(F, na, i0, j0, ns, nt, mask, rp.cosv, rp.damp_q, rp.fxy, rp.comm, s_key, s_val,
                                 s_pj);
         __syncthreads();
+        if (self && !L.dynamic && tile + 2 * G < ntiles) {
+            s2a = s_split[0];
+            s2b = s_split[1];
+        }
         // the next tile and its splits (dynamic tiles: loaded while this tile merges)
         const uint32_t nxt = L.dynamic ? s_tile : tile + G;
         const bool more = nxt < ntiles;

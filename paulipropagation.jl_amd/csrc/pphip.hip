@@ -654,37 +654,61 @@ void enq_partition_b(pp_handle* h) {
     }
 }
 
+// Sums of fewer terms than this at the last synchronization have the
+// one-column merge search its tiles' splits itself (static tiles), so no
+// partition kernel is launched: on a sum of a few tiles per workgroup the
+// launch costs more than the searches.
+constexpr uint32_t SELF_SPLIT_MAX_TERMS = 1u << 16;
+
+// One one-column merge launch, with or without its own split searches.
+template <int W, bool EMIT, bool REC, int MI>
+void launch_merge1(pp_handle* h, hipEvent_t e4, hipEvent_t e5, bool self, const Launch& L, const Key<W>& mask,
+                   const RotParams& rp, const Key<W>& nmask) {
+    constexpr int TILE = WG * MI;
+    const size_t expect = h->cap + h->cap / 2;
+    if (self)
+        launch(h, e4, e5, k_merge1<W, EMIT, REC, MI, true>,
+               lb_grid(h, (const void*)k_merge1<W, EMIT, REC, MI, true>, 0, expect, TILE), 0, h->bufs, h->d_ctl, L,
+               mask, rp, nmask);
+    else
+        launch(h, e4, e5, k_merge1<W, EMIT, REC, MI, false>,
+               lb_grid(h, (const void*)k_merge1<W, EMIT, REC, MI, false>, 0, expect, TILE), 0, h->bufs, h->d_ctl, L,
+               mask, rp, nmask);
+}
+
 // The one-column merge with the tuning table's tile (256 x MI diagonals): its
-// partition, then the merge; variant 0 plain, 1 listing the next rotation's
-// touched terms (nmask), 2 recording for the adjoint sweep, 3 both.
+// partition (or, small sums, its own split searches), then the merge; variant
+// 0 plain, 1 listing the next rotation's touched terms (nmask), 2 recording
+// for the adjoint sweep, 3 both.
 template <int W, int MI>
 void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, int variant, const Key<W>& mask,
                    const RotParams& rp, const Key<W>& nmask, uint32_t* glog) {
     constexpr int TILE = WG * MI;
     const size_t expect = h->cap + h->cap / 2;
-    Launch LP{};
-    LP.seq = h->seq;
-    launch(h, nullptr, e3, k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs, h->d_ctl,
-           LP);
+    const Launch LM = kt_launch(h, glog);
+    // (event timing, profile mode 2, times the partition kernel: it keeps it)
+    const bool self = !LM.dynamic && !h->tun.merge_defer && h->prof_mode != 2 &&
+                      h->h_ctl->st[h->seq & 1].n < SELF_SPLIT_MAX_TERMS;
+    if (!self) {
+        Launch LP{};
+        LP.seq = h->seq;
+        launch(h, nullptr, e3, k_partition<W, TILE>, grid_for(h, expect, (size_t)TILE * NWAVES), 0, h->bufs,
+               h->d_ctl, LP);
+    }
     if (variant == 3)
-        launch(h, e4, e5, k_merge1<W, true, true, MI>, lb_grid(h, (const void*)k_merge1<W, true, true, MI>, 0, expect, TILE),
-               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
+        launch_merge1<W, true, true, MI>(h, e4, e5, self, LM, mask, rp, nmask);
     else if (variant == 2)
-        launch(h, e4, e5, k_merge1<W, false, true, MI>, lb_grid(h, (const void*)k_merge1<W, false, true, MI>, 0, expect, TILE),
-               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
+        launch_merge1<W, false, true, MI>(h, e4, e5, self, LM, mask, rp, mask);
     else if (h->tun.merge_defer && variant == 1)
         launch(h, e4, e5, k_merge1d<W, true, MI>, lb_grid(h, (const void*)k_merge1d<W, true, MI>, 0, expect, TILE), 0,
-               h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
+               h->bufs, h->d_ctl, LM, mask, rp, nmask);
     else if (h->tun.merge_defer)
         launch(h, e4, e5, k_merge1d<W, false, MI>, lb_grid(h, (const void*)k_merge1d<W, false, MI>, 0, expect, TILE), 0,
-               h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, mask);
+               h->bufs, h->d_ctl, LM, mask, rp, mask);
     else if (variant == 1)
-        launch(h, e4, e5, k_merge1<W, true, false, MI>, lb_grid(h, (const void*)k_merge1<W, true, false, MI>, 0, expect, TILE),
-               0, h->bufs, h->d_ctl, kt_launch(h, glog), mask, rp, nmask);
+        launch_merge1<W, true, false, MI>(h, e4, e5, self, LM, mask, rp, nmask);
     else
-        launch(h, e4, e5, k_merge1<W, false, false, MI>,
-               lb_grid(h, (const void*)k_merge1<W, false, false, MI>, 0, expect, TILE), 0, h->bufs, h->d_ctl,
-               kt_launch(h, glog), mask, rp, mask);
+        launch_merge1<W, false, false, MI>(h, e4, e5, self, LM, mask, rp, mask);
     ktime_finish(h);
 }
 
