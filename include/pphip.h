@@ -381,7 +381,9 @@ int pp_profile_read(const pp_handle *h, pp_profile *out);
  * keys, 4 otherwise); PP_TUNE_MERGE_DEFER != 0 makes that merge (outside
  * recorded runs) keep each tile's outputs in shared memory until it has merged
  * its next tile, and only then wait for the tile's prefix and store it (0 or
- * 1, default 0); PP_TUNE_PERM_ITEMS the tail permutation's list entries
+ * 1, default 0); PP_TUNE_SPLIT_BELOW the size (terms, at the last
+ * synchronization) below which that merge searches its tiles' splits itself
+ * instead of after a partition kernel (0..2^31, default 2^16); PP_TUNE_PERM_ITEMS the tail permutation's list entries
  * per thread (1, 2, 4; default 2); PP_TUNE_BATCH_ROWS the batched merge's term
  * rows per tile (a power of two 16..256 with rows x B x 8 bytes <= 64 KiB; 0:
  * by B); PP_TUNE_RADIX_BITS the radix sort's digit width (4..8, default 8);
@@ -403,14 +405,15 @@ enum pp_tune_key {
     PP_TUNE_FORCE_RERUN = 12,
     PP_TUNE_LDS_RUNS = 13,
     PP_TUNE_GRAPHS = 14,
-    PP_TUNE_MERGE_DEFER = 15
+    PP_TUNE_MERGE_DEFER = 15,
+    PP_TUNE_SPLIT_BELOW = 16
 };
 
 /* Set a tuning value. No counterpart in the reference. */
 int pp_tune(pp_handle *h, int key, int64_t value);
 
 /* Apply a tuning table: lines "<key_words> <max_batch> <name> <value>" ('#'
- * starts a comment), name one of merge_items, merge_defer, perm_items, batch_rows,
+ * starts a comment), name one of merge_items, merge_defer, split_below, perm_items, batch_rows,
  * radix_bits, grid_per_cu, run_chunk. The handle takes the lines of its key
  * width with the smallest max_batch not below its batch. path NULL: the file
  * named by the environment variable PPHIP_TUNE_FILE, else pphip_tune.txt in

@@ -85,6 +85,7 @@ int scratch_setup(int dev, double** out);
 struct Tuning {
     int merge_items = 0;   // one-column merge tile = 256 x this (2, 3, 4); 0: 3 for 256-bit keys, else 4
     int merge_defer = 0;   // one-column merge (not recording) stores each tile one tile behind (k_merge1d)
+    int64_t split_below = 1 << 16;  // one-column merge: sums below this many terms search their own splits
     int perm_items = 2;    // tail permutation: list entries per thread (1, 2, 4)
     int batch_rows = 0;    // batched merge: term rows per tile (16 .. 256); 0: by B
     int radix_bits = 8;    // Clifford / load-merge radix sort: digit width (4 .. 8)
@@ -654,12 +655,6 @@ void enq_partition_b(pp_handle* h) {
     }
 }
 
-// Sums of fewer terms than this at the last synchronization have the
-// one-column merge search its tiles' splits itself (static tiles), so no
-// partition kernel is launched: on a sum of a few tiles per workgroup the
-// launch costs more than the searches.
-constexpr uint32_t SELF_SPLIT_MAX_TERMS = 1u << 16;
-
 // One one-column merge launch, with or without its own split searches.
 template <int W, bool EMIT, bool REC, int MI>
 void launch_merge1(pp_handle* h, hipEvent_t e4, hipEvent_t e5, bool self, const Launch& L, const Key<W>& mask,
@@ -687,8 +682,12 @@ void enq_merge1_mi(pp_handle* h, hipEvent_t e3, hipEvent_t e4, hipEvent_t e5, in
     const size_t expect = h->cap + h->cap / 2;
     const Launch LM = kt_launch(h, glog);
     // (event timing, profile mode 2, times the partition kernel: it keeps it)
+    // sums of fewer terms than tun.split_below at the last synchronization
+    // search their tiles' splits in the merge (static tiles), so no partition
+    // kernel is launched: on a sum of a few tiles per workgroup the launch
+    // costs more than the searches
     const bool self = !LM.dynamic && !h->tun.merge_defer && h->prof_mode != 2 &&
-                      h->h_ctl->st[h->seq & 1].n < SELF_SPLIT_MAX_TERMS;
+                      (int64_t)h->h_ctl->st[h->seq & 1].n < h->tun.split_below;
     if (!self) {
         Launch LP{};
         LP.seq = h->seq;
@@ -3044,6 +3043,10 @@ int pp_tune(pp_handle* h, int key, int64_t value) {
         if (value != 0 && value != 1) return fail(PP_ERR_ARG, "merge defer must be 0 or 1");
         h->tun.merge_defer = (int)value;
         return PP_OK;
+    case PP_TUNE_SPLIT_BELOW:
+        if (value < 0 || value > (int64_t)1 << 31) return fail(PP_ERR_ARG, "split_below must be 0..2^31 terms");
+        h->tun.split_below = value;
+        return PP_OK;
     case PP_TUNE_PERM_ITEMS:
         if (value != 1 && value != 2 && value != 4) return fail(PP_ERR_ARG, "permutation items must be 1, 2 or 4");
         h->tun.perm_items = (int)value;
@@ -3126,6 +3129,7 @@ int pp_load_tuning(pp_handle* h, const char* path) {
         const char* name;
         int key;
     } names[] = {{"merge_items", PP_TUNE_MERGE_ITEMS}, {"merge_defer", PP_TUNE_MERGE_DEFER},
+                 {"split_below", PP_TUNE_SPLIT_BELOW},
                  {"perm_items", PP_TUNE_PERM_ITEMS},
                  {"batch_rows", PP_TUNE_BATCH_ROWS},   {"radix_bits", PP_TUNE_RADIX_BITS},
                  {"grid_per_cu", PP_TUNE_GRID_PER_CU}, {"run_chunk", PP_TUNE_RUN_CHUNK}};

@@ -93,8 +93,8 @@ PP_TUNE_ADJ_DEFER_BYTES = 5
 PP_TUNE_MERGE_ITEMS, PP_TUNE_PERM_ITEMS, PP_TUNE_BATCH_ROWS, PP_TUNE_RADIX_BITS = 6, 7, 8, 9
 PP_TUNE_GRID_PER_CU, PP_TUNE_RUN_CHUNK = 10, 11
 PP_TUNE_FORCE_RERUN, PP_TUNE_LDS_RUNS, PP_TUNE_GRAPHS = 12, 13, 14
-PP_TUNE_MERGE_DEFER = 15
-TUNE_NAMES = {"merge_items": PP_TUNE_MERGE_ITEMS, "merge_defer": PP_TUNE_MERGE_DEFER,
+PP_TUNE_MERGE_DEFER, PP_TUNE_SPLIT_BELOW = 15, 16
+TUNE_NAMES = {"merge_items": PP_TUNE_MERGE_ITEMS, "merge_defer": PP_TUNE_MERGE_DEFER, "split_below": PP_TUNE_SPLIT_BELOW,
               "perm_items": PP_TUNE_PERM_ITEMS, "batch_rows": PP_TUNE_BATCH_ROWS,
               "radix_bits": PP_TUNE_RADIX_BITS, "grid_per_cu": PP_TUNE_GRID_PER_CU, "run_chunk": PP_TUNE_RUN_CHUNK}
 PP_OVERLAP_ZERO, PP_OVERLAP_PLUS, PP_OVERLAP_COMPUTATIONAL, PP_OVERLAP_MAXMIXED = 0, 1, 2, 3

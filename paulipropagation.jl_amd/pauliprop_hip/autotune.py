@@ -14,7 +14,8 @@ writes the table that pp_create reads (pp_load_tuning, include/pphip.h):
 
 Launch shapes: merge_items (the one-column merge's tile, 256 x 2..4
 diagonals, with its partition), merge_defer (that merge storing each tile
-one tile behind), perm_items (list entries per thread of the
+one tile behind), split_below (sums, in terms, below which that merge searches
+its own tile splits), perm_items (list entries per thread of the
 tail permutation), batch_rows (term rows per tile of the batched merge),
 radix_bits (digit width of the Clifford re-sort), grid_per_cu (workgroups per
 compute unit of the persistent grids), run_chunk (gates enqueued between
@@ -39,6 +40,7 @@ from .paulis import PauliSum
 CANDIDATES = {
     "merge_items": [2, 3, 4],
     "merge_defer": [0, 1],
+    "split_below": [0, 1 << 14, 1 << 16, 1 << 18],
     "perm_items": [1, 2, 4],
     "grid_per_cu": [2, 4, 6, 8],
     "run_chunk": [64, 128, 256, 512],
@@ -46,7 +48,7 @@ CANDIDATES = {
     "radix_bits": [4, 5, 6, 7, 8],
 }
 # the library's compiled defaults (0: chosen by key width or batch)
-DEFAULTS = {"merge_items": 0, "merge_defer": 0, "perm_items": 2, "batch_rows": 0, "radix_bits": 8, "grid_per_cu": 4, "run_chunk": 128}
+DEFAULTS = {"merge_items": 0, "merge_defer": 0, "split_below": 1 << 16, "perm_items": 2, "batch_rows": 0, "radix_bits": 8, "grid_per_cu": 4, "run_chunk": 128}
 
 
 class Case:
@@ -95,7 +97,7 @@ def _first_step_b64():
 # (key words, largest batch of the class, [(launch shape, workload)]); shapes a
 # class does not use (the batched merge's rows at B = 1, the one-column merge's
 # tile at B > 1) are left out
-ONE_COLUMN = ("merge_items", "merge_defer", "perm_items", "grid_per_cu", "run_chunk")
+ONE_COLUMN = ("merge_items", "merge_defer", "split_below", "perm_items", "grid_per_cu", "run_chunk")
 BATCHED = ("batch_rows", "perm_items", "grid_per_cu", "run_chunk")
 CLASSES = [
     (4, 1, [(n, workloads.kicked_ising_heavyhex_127q) for n in ONE_COLUMN]),

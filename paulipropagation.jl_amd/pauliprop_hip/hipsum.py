@@ -301,7 +301,7 @@ class HIPPauliSum:
         when the same chunk comes again;
         whether circuits leave out the rotations that touch no term; the device
         memory the adjoint sweep may keep per-rotation partial gradients in.
-        Launch shapes by name (merge_items, merge_defer, perm_items, batch_rows, radix_bits,
+        Launch shapes by name (merge_items, merge_defer, split_below, perm_items, batch_rows, radix_bits,
         grid_per_cu, run_chunk; any value the library accepts gives identical
         results)."""
         L = _lib.lib()

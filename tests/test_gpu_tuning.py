@@ -117,7 +117,8 @@ def test_batch_rows_bitwise(B, rows, noise):
     assert_bitwise(*got, *_oracle(f"heis{B}{noise}", wl), f"B={B} rows={rows} noise={noise}")
 
 
-@pytest.mark.parametrize("shape,value", [("merge_items", 1), ("merge_items", 5), ("merge_defer", 2), ("perm_items", 3),
+@pytest.mark.parametrize("shape,value", [("merge_items", 1), ("merge_items", 5), ("merge_defer", 2), ("split_below", -1),
+                                         ("split_below", (1 << 31) + 1), ("perm_items", 3),
                                          ("batch_rows", 8), ("batch_rows", 48), ("batch_rows", 512),
                                          ("radix_bits", 3), ("radix_bits", 9), ("grid_per_cu", 0),
                                          ("grid_per_cu", 9), ("run_chunk", 8), ("run_chunk", 2048)])
@@ -247,3 +248,15 @@ def test_batched_tile_modes_bitwise(B, mode):
     assert_bitwise(*s.download(), *_oracle(f"heis{B}True", wl), f"B={B} {mode}")
     if mode == "forced_rerun":
         assert s.profile()["fallbacks"] > 0, "no gate was sent to the atomic-claim rerun"
+
+
+@pytest.mark.parametrize("name", ["kicked127", "ansatz48"])
+@pytest.mark.parametrize("below", [0, 1 << 31])
+def test_self_split_merge_bitwise(name, below):
+    # every one-column merge searching its own tile splits (whatever the sum's
+    # size; one workgroup per CU, so large sums span several rounds of tiles
+    # and the merge searches the splits of its later tiles too), or none
+    print(f"{name} split_below={below}", flush=True)
+    wl = _WL[name]()
+    got = _device(wl, split_below=below, grid_per_cu=1)
+    assert_bitwise(*got, *_oracle(name, wl), f"{name} split_below={below}")

@@ -156,7 +156,8 @@ def test_tuning_table_and_autotuner_candidates():
     # whose values the library accepts (pp_tune's ranges, include/pphip.h)
     import os
     from pauliprop_hip import _lib, autotune
-    ok = {"merge_items": {0, 2, 3, 4}, "merge_defer": {0, 1}, "perm_items": {1, 2, 4},
+    ok = {"merge_items": {0, 2, 3, 4}, "merge_defer": {0, 1}, "split_below": set(range(0, (1 << 31) + 1, 1 << 14)),
+          "perm_items": {1, 2, 4},
           "batch_rows": {0, 16, 32, 64, 128, 256},
           "radix_bits": set(range(4, 9)), "grid_per_cu": set(range(1, 9)), "run_chunk": set(range(16, 1025))}
     assert set(ok) == set(_lib.TUNE_NAMES) == set(autotune.CANDIDATES) == set(autotune.DEFAULTS)
