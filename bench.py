@@ -65,7 +65,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="kicked_ising_heavyhex_127q",
                     choices=["tfim_brick_32q", "kicked_ising_heavyhex_127q", "heisenberg_square_64q_b64",
-                             "ansatz_line_48q", "ansatz_line_noisy_48q_b32"])
+                             "ansatz_line_48q", "ansatz_line_noisy_48q_b32", "tfim_tilted_6x6_36q"])
     ap.add_argument("--layers", type=int, default=None,
                     help="heisenberg_square_64q_b64 only: its first LAYERS steps (the full ten outgrow one GPU)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the single-thread CPU sample")
@@ -201,7 +201,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload_text(args.workload, wl, args.layers), "nqubits": nq, "key_bits": 64 * pc.W, "batch": B,
-                   "gates": pc.ngates, "max_weight": mw, "min_abs_coeff": mac},
+                   "gates": pc.ngates, "max_weight": None if mw == math.inf else mw, "min_abs_coeff": mac},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "detail": {"term_applications_per_step": apps // args.steps, "peak_terms": int(counts.max()),
@@ -355,18 +355,13 @@ def published_anchor(pp, PackedCircuit):
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
-    nq = 36
-    layer = []
-    pp.rxlayer(layer, nq)
-    pp.rzlayer(layer, nq)
-    pp.rzzlayer(layer, pp.rectangletopology(6, 6))
-    th = np.ones(len(layer)) * 0.05 * 2
-    th[2 * nq:] *= 0.5
-    nlay, L = 14, len(layer)
-    pc = PackedCircuit(layer * nlay, np.tile(th, nlay), nq)
-    obs = pp.PauliSum.from_string(pp.PauliString(nq, ["Z", "Z"], [21, 22]))
-    k0, c0 = obs.to_arrays(pc.W, 1)
-    mac = 2.0 ** -20
+    from pauliprop_hip import workloads
+    wl = workloads.tfim_tilted_6x6_36q()
+    nq, nlay = wl["nqubits"], 14
+    L = len(wl["circuit"]) // nlay
+    pc = PackedCircuit(wl["circuit"], wl["thetas"], nq)
+    k0, c0 = wl["obs"].to_arrays(pc.W, 1)
+    mac = wl["min_abs_coeff"]
     dev = pp.HIPPauliSum(nq, capacity=1 << 21)
 
     def gpu_run():
@@ -433,6 +428,8 @@ def workload_text(name, wl, layers=None) -> str:
                            "line), theta N(0, 0.3^2), sum Z_i Z_i+1, UInt128 keys",
         "ansatz_line_noisy_48q_b32": "the 48-qubit line ansatz with DepolarizingNoise(0.01) on every qubit after "
                                      "each layer, B=32, theta N(0, 0.3^2), sum Z_i Z_i+1, UInt128 keys",
+        "tfim_tilted_6x6_36q": "tilted-field Ising on the 6x6 open square lattice (36 qubits), 14 x (RX(0.1), RZ(0.1) on "
+                               "every qubit, RZZ(0.05) on 60 edges), Z21 Z22, min_abs_coeff 2^-20, UInt128 keys",
     }[name]
 
 

@@ -40,7 +40,7 @@ from .paulis import PauliSum
 CANDIDATES = {
     "merge_items": [2, 3, 4],
     "merge_defer": [0, 1],
-    "split_below": [0, 1 << 14, 1 << 16, 1 << 18],
+    "split_below": [0, 1 << 14, 1 << 16, 1 << 18, 1 << 20, 1 << 22],
     "perm_items": [1, 2, 4],
     "grid_per_cu": [2, 4, 6, 8],
     "run_chunk": [64, 128, 256, 512],
@@ -101,7 +101,10 @@ ONE_COLUMN = ("merge_items", "merge_defer", "split_below", "perm_items", "grid_p
 BATCHED = ("batch_rows", "perm_items", "grid_per_cu", "run_chunk")
 CLASSES = [
     (4, 1, [(n, workloads.kicked_ising_heavyhex_127q) for n in ONE_COLUMN]),
-    (2, 1, [(n, workloads.ansatz_line_48q) for n in ONE_COLUMN] + [("radix_bits", lambda: clifford_ansatz(48))]),
+    # split_below only acts on sums above 2^16 terms, which the 48-qubit ansatz
+    # never reaches (67234 terms at most): the 6x6 TFIM grows to 1.04 M
+    (2, 1, [(n, workloads.ansatz_line_48q) for n in ONE_COLUMN if n != "split_below"] +
+     [("split_below", workloads.tfim_tilted_6x6_36q), ("radix_bits", lambda: clifford_ansatz(48))]),
     (1, 1, [(n, workloads.tfim_brick_32q) for n in ONE_COLUMN] + [("radix_bits", lambda: clifford_ansatz(24))]),
     (2, 32, [(n, workloads.ansatz_line_noisy_48q_b32) for n in BATCHED]),
     (2, 64, [(n, _first_step_b64) for n in BATCHED]),

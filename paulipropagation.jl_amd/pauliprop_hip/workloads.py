@@ -22,6 +22,11 @@ device run reads identical parameter bits.
   ansatz_line_noisy_48q_b32  the same ansatz plus DepolarizingNoise(q, 0.01) on
                              every qubit after each layer, B = 32 columns
                              (seed 4802)
+  tfim_tilted_6x6_36q        36 qubits, the reference's published CPU
+                             benchmark (examples/advanced_performance.ipynb:
+                             92-118): 14 layers of RX(0.1) and RZ(0.1) on every
+                             qubit then RZZ(0.05) on rectangletopology(6, 6),
+                             Z21 Z22, no weight truncation, min_abs_coeff 2^-20
 """
 from __future__ import annotations
 
@@ -148,5 +153,18 @@ def ansatz_line_noisy_48q_b32(B: int = 32):
                 batch=B)
 
 
+def tfim_tilted_6x6_36q(nlayers: int = 14):
+    n = 36
+    layer = []
+    C.rxlayer(layer, n)
+    C.rzlayer(layer, n)
+    C.rzzlayer(layer, C.rectangletopology(6, 6))
+    th = np.full(len(layer), 0.1)  # 2 dt h with dt 0.05, h 1
+    th[2 * n:] *= 0.5  # 2 dt J with J 0.5
+    obs = PauliSum.from_string(PauliString(n, ["Z", "Z"], [21, 22]))
+    return dict(name="tfim_tilted_6x6_36q", nqubits=n, circuit=layer * nlayers, thetas=np.tile(th, nlayers), obs=obs,
+                max_weight=math.inf, min_abs_coeff=2.0 ** -20)
+
+
 WORKLOADS = {f.__name__: f for f in (tfim_brick_32q, kicked_ising_heavyhex_127q, heisenberg_square_64q_b64,
-                                     ansatz_line_48q, ansatz_line_noisy_48q_b32)}
+                                     ansatz_line_48q, ansatz_line_noisy_48q_b32, tfim_tilted_6x6_36q)}

@@ -250,13 +250,19 @@ def test_batched_tile_modes_bitwise(B, mode):
         assert s.profile()["fallbacks"] > 0, "no gate was sent to the atomic-claim rerun"
 
 
-@pytest.mark.parametrize("name", ["kicked127", "ansatz48"])
+def _tfim66():
+    from pauliprop_hip import workloads
+    return workloads.tfim_tilted_6x6_36q()
+
+
+@pytest.mark.parametrize("name", ["kicked127", "ansatz48", "tfim66"])
 @pytest.mark.parametrize("below", [0, 1 << 31])
 def test_self_split_merge_bitwise(name, below):
     # every one-column merge searching its own tile splits (whatever the sum's
     # size; one workgroup per CU, so large sums span several rounds of tiles
-    # and the merge searches the splits of its later tiles too), or none
+    # and the merge searches the splits of its later tiles too), or none; the
+    # 6x6 TFIM grows to 1.04 M terms with 128-bit keys
     print(f"{name} split_below={below}", flush=True)
-    wl = _WL[name]()
+    wl = {**_WL, "tfim66": _tfim66}[name]()
     got = _device(wl, split_below=below, grid_per_cu=1)
     assert_bitwise(*got, *_oracle(name, wl), f"{name} split_below={below}")
